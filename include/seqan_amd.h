@@ -1,0 +1,165 @@
+/*
+ * seqan_amd.h -- C ABI of the MI355X (gfx950) batched pairwise-alignment
+ * engine.  Plain C types only; every pointer is a plain address and every size
+ * an integer.  All entry points return an sqmi_status and never throw.
+ *
+ * The entry points replace these reference interfaces (SeqAn 2, paths relative
+ * to its source tree):
+ *
+ *   sqmi_align_score   <- globalAlignmentScore(ExecutionPolicy<Parallel,Vectorial>,
+ *                           setH, setV, score [, AlignConfig<T,L,R,B>])
+ *                           include/seqan/align_parallel/parallel_align_interface.h:332-343
+ *                         localAlignmentScore(ExecutionPolicy<Parallel,Vectorial>,
+ *                           setH, setV, score)
+ *                           include/seqan/align_parallel/parallel_align_interface.h:354-365
+ *   sqmi_align_counts  <- globalAlignment(ExecutionPolicy<Parallel,Vectorial>,
+ *                           gapsSetH, gapsSetV, score)
+ *                           include/seqan/align_parallel/parallel_align_interface.h:288-299
+ *                         followed by computeAlignmentStats on the rows
+ *                           include/seqan/align/evaluate_alignment.h:215-300
+ *
+ * Semantics follow the reference exactly: affine gaps where a gap of k
+ * characters costs gap_open + (k-1)*gap_extend (Score<int,Simple> constructor
+ * order match, mismatch, gapExtend, gapOpen, include/seqan/score/score_simple.h:94-97),
+ * free end gaps per AlignConfig<TOP,LEFT,RIGHT,BOTTOM> (dp_setup.h:57-182),
+ * and an empty sequence yields INT32_MIN (dp_algorithm_impl.h:1399-1400).
+ */
+#ifndef SEQAN_AMD_H
+#define SEQAN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SQMI_ABI_VERSION 1
+
+typedef enum sqmi_status {
+    SQMI_OK = 0,
+    SQMI_ERR_INVALID_ARGUMENT = 1,  /* null pointer, bad offsets, bad scoring    */
+    SQMI_ERR_LENGTH_MISMATCH = 2,   /* |setH| != |setV|; the reference asserts,
+                                       parallel_align_interface.h:111           */
+    SQMI_ERR_UNSUPPORTED = 3,       /* request outside the library's scope      */
+    SQMI_ERR_NO_DEVICE = 4,         /* no usable gfx950 device                  */
+    SQMI_ERR_DEVICE = 5,            /* HIP runtime error (see sqmi_last_error)  */
+    SQMI_ERR_OUT_OF_MEMORY = 6
+} sqmi_status;
+
+/* Alignment mode: AlignConfig<TOP, LEFT, RIGHT, BOTTOM> bits, or LOCAL. */
+enum {
+    SQMI_FREE_TOP = 1,      /* first row free: leading gaps in V are free     */
+    SQMI_FREE_LEFT = 2,     /* first column free: leading gaps in H are free  */
+    SQMI_FREE_RIGHT = 4,    /* last column free: trailing gaps in H are free  */
+    SQMI_FREE_BOTTOM = 8,   /* last row free: trailing gaps in V are free     */
+    SQMI_LOCAL = 16         /* Smith-Waterman (localAlignmentScore)           */
+};
+
+/* Call flags. */
+enum {
+    SQMI_RESIDUES_ON_DEVICE = 1,  /* residues pointers are device addresses   */
+    SQMI_SCORES_ON_DEVICE = 2,    /* scores/counts pointers are device addresses */
+    SQMI_FORCE_INT32 = 4,         /* skip the packed int16 kernels             */
+    SQMI_PLAIN_PER_PAIR = 8       /* one kernel launch per pair (baseline form) */
+};
+
+/* A set of sequences: concatenated residue ordinals (Dna: A0 C1 G2 T3;
+ * AminoAcid: SeqAn ordinal order ABCDEFGHIJKLMNOPQRSTUVWYZX*) and count+1
+ * offsets.  Offsets always live in host memory; residues in host or device
+ * memory (SQMI_RESIDUES_ON_DEVICE).  Residues must be < alphabet_size. */
+typedef struct sqmi_sequences {
+    const uint8_t *residues;
+    const int64_t *offsets;
+    int64_t count;
+} sqmi_sequences;
+
+/* Scoring scheme: a substitution table over the alphabet (row = residue of
+ * the horizontal sequence H, column = residue of V), plus affine gap costs.
+ * Score<int,Simple>(match, mismatch, ext, open) is the table with match on the
+ * diagonal and mismatch elsewhere; Blosum62(ext, open) is sqmi_blosum62(). */
+typedef struct sqmi_scoring {
+    int32_t alphabet_size;      /* 1..32                                        */
+    int32_t gap_open;           /* cost of the first gap character              */
+    int32_t gap_extend;         /* cost of every further gap character          */
+    const int32_t *table;       /* alphabet_size^2 entries, host memory         */
+} sqmi_scoring;
+
+/* Counts layout of sqmi_align_counts: per pair, alphabet_size^2 substitution
+ * counts (index h*alphabet_size + v), then gap opens, then gap extensions.
+ * For each pair sum(count * parameter) equals the score. */
+#define SQMI_COUNTS_STRIDE(alpha) ((alpha) * (alpha) + 2)
+
+/* Per-call statistics (filled by the last call on the context). */
+typedef struct sqmi_stats {
+    int64_t pairs;              /* pairs in the call                            */
+    int64_t cells;              /* sum of |H_i| * |V_i|                         */
+    int64_t pairs_int16;        /* pairs computed by the packed int16 kernels   */
+    int64_t pairs_int32;        /* pairs computed in int32 (range guard)        */
+    int64_t work_items;         /* wavefront work items launched                */
+    int64_t padded_cells;       /* cells computed including padding            */
+    double host_plan_ms;        /* host planning (bucketing) time               */
+    double device_ms;           /* device time of the kernels (HIP events)      */
+    double total_ms;            /* wall time of the call                        */
+} sqmi_stats;
+
+typedef struct sqmi_context sqmi_context;
+
+/* Creates a context on HIP device `device` (owns scratch, stream, tuning). */
+sqmi_status sqmi_create(int device, sqmi_context **ctx);
+sqmi_status sqmi_destroy(sqmi_context *ctx);
+
+/* Use the given hipStream_t for subsequent calls (NULL: the context's own). */
+sqmi_status sqmi_set_stream(sqmi_context *ctx, void *hip_stream);
+
+/* Optimal alignment score of every pair (setH[i], setV[i]).  Blocks until the
+ * scores are written.  scores: count int32 (host, or device with
+ * SQMI_SCORES_ON_DEVICE). */
+sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *set_h,
+                             const sqmi_sequences *set_v, const sqmi_scoring *scoring,
+                             uint32_t mode, int32_t *scores, uint32_t flags);
+
+/* Score plus the exact subgradient of the score with respect to the scoring
+ * parameters: substitution counts, gap opens and gap extensions along the
+ * optimal path chosen by the reference traceback (single trace, gaps left,
+ * gaps preferred at the end cell).  Global alignment (mode 0) only. */
+sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *set_h,
+                              const sqmi_sequences *set_v, const sqmi_scoring *scoring,
+                              uint32_t mode, int32_t *scores, int32_t *counts,
+                              uint32_t flags);
+
+sqmi_status sqmi_get_stats(const sqmi_context *ctx, sqmi_stats *out);
+
+/* The reference's Blosum62 table (27 x 27, AminoAcid ordinal order). */
+const int32_t *sqmi_blosum62(void);
+
+const char *sqmi_status_string(sqmi_status status);
+/* Text of the last error on this context ("" when none). */
+const char *sqmi_last_error(const sqmi_context *ctx);
+int sqmi_abi_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Synthetic workloads (benchmarking and tests; no counterpart in the        */
+/* reference).  Pairs are generated independently from (seed, index), so any */
+/* window [first, first+count) is reproducible on its own.                   */
+/* ------------------------------------------------------------------------ */
+enum {
+    SQMI_WL_DNA150 = 1,         /* 150 bp, 5% subst, 1% single-base indels     */
+    SQMI_WL_DNA_LOGNORMAL = 2,  /* log-normal 100..10k bp, 8% subst, 2% indels */
+    SQMI_WL_PROTEIN = 3,        /* 50..1000 aa vs 50..2000 aa, 30% homologs    */
+    SQMI_WL_DNA1K = 4,          /* 1 kbp, 10% subst, 2% indels                 */
+    SQMI_WL_DNA100K = 5         /* 100 kbp, 12% subst, 3% geometric indels     */
+};
+typedef struct sqmi_workload sqmi_workload;
+/* length_override > 0 replaces the configured H length (DNA configs). */
+sqmi_status sqmi_workload_create(int config, uint64_t seed, int64_t first, int64_t count,
+                                 int64_t length_override, sqmi_workload **out);
+sqmi_status sqmi_workload_sizes(const sqmi_workload *w, int64_t *h_total, int64_t *v_total);
+sqmi_status sqmi_workload_copy(const sqmi_workload *w, uint8_t *h_res, int64_t *h_off,
+                               uint8_t *v_res, int64_t *v_off);
+sqmi_status sqmi_workload_destroy(sqmi_workload *w);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SEQAN_AMD_H */

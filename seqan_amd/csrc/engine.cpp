@@ -1,0 +1,590 @@
+// engine.cpp -- host side of the C ABI: validation, exact int16/int32
+// routing, length bucketing into wavefront work items, launches.
+//
+// Behaviour mirrors the reference entry points:
+//   globalAlignmentScore / localAlignmentScore (ExecutionPolicy<Parallel,Vectorial>)
+//     include/seqan/align_parallel/parallel_align_interface.h:100-145, 332-365
+// The reference splits the set into chunks of <= 256 pairs for OpenMP threads
+// and each chunk into SIMD batches of 8/16 pairs padded to the batch maximum
+// (include/seqan/align/dp_align_simd_helper.h:147-221, 277-347).  Here pairs
+// are instead grouped by length across the whole set, so that the pairs that
+// share a wavefront have nearly the same shape.
+#include "kernels.hpp"
+#include "seqan_amd.h"
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace sqmi;
+
+namespace {
+
+struct DevBuf
+{
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace
+
+struct sqmi_context
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t ownStream = nullptr;
+    int numCUs = 0;
+    std::string lastError;
+    sqmi_stats stats{};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf hRes, vRes, hOff, vOff, order, items, queue, rowbuf, scores, tTable, dnaProf, ids32, scratchOff32,
+        scratch32, table32, emptyIds;
+};
+
+namespace {
+
+sqmi_status fail(sqmi_context *ctx, sqmi_status st, const std::string &msg)
+{
+    if (ctx)
+        ctx->lastError = msg;
+    return st;
+}
+
+sqmi_status hipFail(sqmi_context *ctx, hipError_t err, const char *what)
+{
+    std::string msg = std::string(what) + ": " + hipGetErrorString(err);
+    return fail(ctx, err == hipErrorOutOfMemory ? SQMI_ERR_OUT_OF_MEMORY : SQMI_ERR_DEVICE, msg);
+}
+
+hipError_t ensure(DevBuf &b, size_t bytes)
+{
+    if (bytes == 0)
+        bytes = 16;
+    if (b.cap >= bytes)
+        return hipSuccess;
+    if (b.p)
+        (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e == hipSuccess)
+        b.cap = bytes;
+    return e;
+}
+
+void release(DevBuf &b)
+{
+    if (b.p)
+        (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+struct ScoringInfo
+{
+    int alpha = 0;
+    int o = 0, e = 0;
+    int sMax = 0, sMin = 0;
+    bool dna = false;   // v_perm profile path possible
+};
+
+// Conservative range of every value a pair can produce inside the DP (H, E, F
+// and H + gapOpen), from the sizes alone.  H <= max(0, sMax) * min(n, m) since
+// gap costs are <= 0; H >= the cost of the all-gap path, 2*o + (n+m)*min(o,e).
+bool fitsInt16(const ScoringInfo &si, int64_t n, int64_t m, bool local)
+{
+    const int64_t up = (int64_t)std::max(0, si.sMax) * std::min(n, m);
+    const int64_t lowH = local ? 0 : 2LL * si.o + (n + m) * (int64_t)std::min(si.o, si.e);
+    const int64_t low = lowH + std::min({si.o, si.sMin, 0}) + std::min(si.e, 0);
+    return up <= 32767 - kGuard16 && low >= (int64_t)kNeg16 + kGuard16 && n <= 30000 && m <= 30000;
+}
+
+struct Plan16
+{
+    Variant16 var;
+    std::vector<int32_t> order;
+    std::vector<WorkItem> items;
+    int maxN = 0;
+};
+
+int pairsPerWave(const Variant16 &v) { return 2 * (64 / v.G); }
+
+// Kernel shape for a pair with m rows (V) and n columns (H).
+Variant16 chooseVariant(bool dna, uint32_t track, int64_t n, int64_t m)
+{
+    (void)n;
+    Variant16 v{16, 16, dna, track};
+    if (m + 1 <= 128)
+        v.R = 8;
+    return v;
+}
+
+// One pair per launch: one wavefront sweeps the pair with all 64 lanes.
+Variant16 plainVariant(bool dna, uint32_t track, int64_t m)
+{
+    Variant16 v{64, 16, dna, track};
+    if (m + 1 <= 256)
+        v.R = 4;
+    return v;
+}
+
+sqmi_status checkSet(sqmi_context *ctx, const sqmi_sequences *s, const char *name)
+{
+    if (!s)
+        return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, std::string(name) + " is null");
+    if (s->count < 0)
+        return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, std::string(name) + ": negative count");
+    if (s->count > 0 && (!s->offsets || !s->residues))
+        return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, std::string(name) + ": null offsets or residues");
+    if (s->count > INT32_MAX - 1)
+        return fail(ctx, SQMI_ERR_UNSUPPORTED, std::string(name) + ": more than 2^31-2 sequences");
+    if (s->count > 0 && s->offsets[0] < 0)
+        return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, std::string(name) + ": negative offset");
+    for (int64_t i = 0; i < s->count; ++i)
+        if (s->offsets[i + 1] < s->offsets[i])
+            return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, std::string(name) + ": offsets not monotone");
+    return SQMI_OK;
+}
+
+double msSince(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int sqmi_abi_version(void) { return SQMI_ABI_VERSION; }
+
+const char *sqmi_status_string(sqmi_status s)
+{
+    switch (s) {
+        case SQMI_OK: return "ok";
+        case SQMI_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case SQMI_ERR_LENGTH_MISMATCH: return "sequence sets differ in length";
+        case SQMI_ERR_UNSUPPORTED: return "unsupported request";
+        case SQMI_ERR_NO_DEVICE: return "no usable gfx950 device";
+        case SQMI_ERR_DEVICE: return "device error";
+        case SQMI_ERR_OUT_OF_MEMORY: return "out of device memory";
+    }
+    return "unknown status";
+}
+
+const char *sqmi_last_error(const sqmi_context *ctx) { return ctx ? ctx->lastError.c_str() : ""; }
+
+sqmi_status sqmi_create(int device, sqmi_context **out)
+{
+    if (!out)
+        return SQMI_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0 || device < 0 || device >= n)
+        return SQMI_ERR_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+        return SQMI_ERR_NO_DEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return SQMI_ERR_NO_DEVICE;
+    sqmi_context *ctx = new (std::nothrow) sqmi_context();
+    if (!ctx)
+        return SQMI_ERR_OUT_OF_MEMORY;
+    ctx->device = device;
+    ctx->numCUs = prop.multiProcessorCount;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->ownStream) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        delete ctx;
+        return SQMI_ERR_DEVICE;
+    }
+    ctx->stream = ctx->ownStream;
+    *out = ctx;
+    return SQMI_OK;
+}
+
+sqmi_status sqmi_destroy(sqmi_context *ctx)
+{
+    if (!ctx)
+        return SQMI_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (DevBuf *b : {&ctx->hRes, &ctx->vRes, &ctx->hOff, &ctx->vOff, &ctx->order, &ctx->items, &ctx->queue,
+                      &ctx->rowbuf, &ctx->scores, &ctx->tTable, &ctx->dnaProf, &ctx->ids32, &ctx->scratchOff32,
+                      &ctx->scratch32, &ctx->table32, &ctx->emptyIds})
+        release(*b);
+    if (ctx->ev0)
+        (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1)
+        (void)hipEventDestroy(ctx->ev1);
+    if (ctx->ownStream)
+        (void)hipStreamDestroy(ctx->ownStream);
+    delete ctx;
+    return SQMI_OK;
+}
+
+sqmi_status sqmi_set_stream(sqmi_context *ctx, void *stream)
+{
+    if (!ctx)
+        return SQMI_ERR_INVALID_ARGUMENT;
+    ctx->stream = stream ? reinterpret_cast<hipStream_t>(stream) : ctx->ownStream;
+    return SQMI_OK;
+}
+
+sqmi_status sqmi_get_stats(const sqmi_context *ctx, sqmi_stats *out)
+{
+    if (!ctx || !out)
+        return SQMI_ERR_INVALID_ARGUMENT;
+    *out = ctx->stats;
+    return SQMI_OK;
+}
+
+sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
+                             const sqmi_scoring *scoring, uint32_t mode, int32_t *scores, uint32_t flags)
+{
+    const auto tStart = std::chrono::steady_clock::now();
+    if (!ctx)
+        return SQMI_ERR_INVALID_ARGUMENT;
+    ctx->lastError.clear();
+    ctx->stats = sqmi_stats{};
+    sqmi_status st;
+    if ((st = checkSet(ctx, setH, "set_h")) != SQMI_OK || (st = checkSet(ctx, setV, "set_v")) != SQMI_OK)
+        return st;
+    if (setH->count != setV->count)
+        return fail(ctx, SQMI_ERR_LENGTH_MISMATCH, "set_h and set_v differ in length");
+    if (!scoring || !scoring->table || scoring->alphabet_size < 1 || scoring->alphabet_size > 32)
+        return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "scoring: alphabet_size must be 1..32 with a table");
+    if (mode & ~31u)
+        return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "mode: unknown bits");
+    if (mode & SQMI_LOCAL)
+        mode = SQMI_LOCAL;   // localAlignmentScore has no AlignConfig
+    const int64_t N = setH->count;
+    if (N == 0)
+        return SQMI_OK;
+    if (!scores)
+        return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "scores is null");
+    if (hipSetDevice(ctx->device) != hipSuccess)
+        return fail(ctx, SQMI_ERR_DEVICE, "hipSetDevice failed");
+    hipStream_t stream = ctx->stream;
+
+    ScoringInfo si;
+    si.alpha = scoring->alphabet_size;
+    si.o = scoring->gap_open;
+    si.e = scoring->gap_extend;
+    si.sMax = INT_MIN;
+    si.sMin = INT_MAX;
+    for (int k = 0; k < si.alpha * si.alpha; ++k) {
+        si.sMax = std::max(si.sMax, scoring->table[k]);
+        si.sMin = std::min(si.sMin, scoring->table[k]);
+    }
+    const bool local = (mode & SQMI_LOCAL) != 0;
+    const bool costsOk16 = si.o <= 0 && si.e <= 0 && si.o >= -1024 && si.e >= -1024 && si.sMax <= 1024 &&
+                           si.sMin >= -1024 && !(flags & SQMI_FORCE_INT32);
+    si.dna = si.alpha <= 4 && !local && si.sMin - si.o >= 0 && si.sMax - si.o <= 255;
+    uint32_t track;
+    if (local)
+        track = kTrackLocal;
+    else {
+        track = 0;
+        if (mode & SQMI_FREE_BOTTOM)
+            track |= kTrackBottom;
+        if (mode & SQMI_FREE_RIGHT)
+            track |= kTrackRight;
+        if (!track)
+            track = kTrackEnd;
+    }
+
+    // ---- host planning ---------------------------------------------------
+    const auto tPlan = std::chrono::steady_clock::now();
+    std::vector<int32_t> emptyIds, ids32;
+    std::vector<int64_t> scratchOff32;
+    struct Key
+    {
+        uint64_t key;
+        int32_t id;
+    };
+    std::vector<Key> keys16;
+    keys16.reserve(N);
+    int64_t cells = 0, scratch32 = 0;
+    for (int64_t i = 0; i < N; ++i) {
+        const int64_t n = setH->offsets[i + 1] - setH->offsets[i];
+        const int64_t m = setV->offsets[i + 1] - setV->offsets[i];
+        if (n == 0 || m == 0) {
+            emptyIds.push_back((int32_t)i);
+            continue;
+        }
+        cells += n * m;
+        if (costsOk16 && fitsInt16(si, n, m, local)) {
+            const Variant16 v = chooseVariant(si.dna, track, n, m);
+            const int64_t BR = (int64_t)v.G * v.R;
+            const uint64_t nblocks = (uint64_t)((m + 1 + BR - 1) / BR);
+            // key: variant (G, R), row blocks, then longest H first.
+            const uint64_t vkey = (uint64_t)(v.G * 64 + v.R);
+            const uint64_t key = (vkey << 48) | (nblocks << 24) | (uint64_t)(0xFFFFFF - std::min<int64_t>(n, 0xFFFFFF));
+            keys16.push_back({key, (int32_t)i});
+        } else {
+            ids32.push_back((int32_t)i);
+            scratchOff32.push_back(scratch32);
+            scratch32 += m + 1;
+        }
+    }
+    std::sort(keys16.begin(), keys16.end(),
+              [](const Key &a, const Key &b) { return a.key < b.key || (a.key == b.key && a.id < b.id); });
+    std::vector<Plan16> plans;
+    for (size_t k = 0; k < keys16.size();) {
+        const int32_t id0 = keys16[k].id;
+        const int64_t n0 = setH->offsets[id0 + 1] - setH->offsets[id0];
+        const int64_t m0 = setV->offsets[id0 + 1] - setV->offsets[id0];
+        const Variant16 v = chooseVariant(si.dna, track, n0, m0);
+        const uint64_t vkey = keys16[k].key >> 48;
+        if (plans.empty() || (uint64_t)(plans.back().var.G * 64 + plans.back().var.R) != vkey) {
+            plans.push_back(Plan16{});
+            plans.back().var = v;
+        }
+        Plan16 &pl = plans.back();
+        const int P = pairsPerWave(v);
+        const uint64_t kb = (keys16[k].key >> 24) & 0xFFFFFF;
+        size_t e = k;
+        WorkItem w{(int32_t)pl.order.size(), 0, 0, (int32_t)kb};
+        while (e < keys16.size() && (int)(e - k) < P && (keys16[e].key >> 24) == (keys16[k].key >> 24)) {
+            const int32_t id = keys16[e].id;
+            pl.order.push_back(id);
+            w.maxN = std::max<int32_t>(w.maxN, (int32_t)(setH->offsets[id + 1] - setH->offsets[id]));
+            ++e;
+        }
+        w.count = (int32_t)(e - k);
+        pl.maxN = std::max(pl.maxN, w.maxN);
+        pl.items.push_back(w);
+        k = e;
+    }
+    // Longest work first (by estimated cells) for the queue's tail balance.
+    for (Plan16 &pl : plans)
+        std::stable_sort(pl.items.begin(), pl.items.end(), [](const WorkItem &a, const WorkItem &b) {
+            return (int64_t)a.maxN * a.nblocks > (int64_t)b.maxN * b.nblocks;
+        });
+    ctx->stats.host_plan_ms = msSince(tPlan);
+    ctx->stats.pairs = N;
+    ctx->stats.cells = cells;
+    ctx->stats.pairs_int32 = (int64_t)ids32.size();
+    ctx->stats.pairs_int16 = (int64_t)keys16.size();
+
+    // ---- device buffers --------------------------------------------------
+    hipError_t err;
+    const int64_t hTot = setH->offsets[N], vTot = setV->offsets[N];
+    const uint8_t *dH = setH->residues, *dV = setV->residues;
+    if (!(flags & SQMI_RESIDUES_ON_DEVICE)) {
+        if ((err = ensure(ctx->hRes, hTot + 64)) != hipSuccess || (err = ensure(ctx->vRes, vTot + 64)) != hipSuccess)
+            return hipFail(ctx, err, "allocating residues");
+        if ((err = hipMemcpyAsync(ctx->hRes.p, setH->residues, hTot, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+            (err = hipMemcpyAsync(ctx->vRes.p, setV->residues, vTot, hipMemcpyHostToDevice, stream)) != hipSuccess)
+            return hipFail(ctx, err, "copying residues");
+        dH = static_cast<const uint8_t *>(ctx->hRes.p);
+        dV = static_cast<const uint8_t *>(ctx->vRes.p);
+    }
+    if ((err = ensure(ctx->hOff, 8 * (N + 1))) != hipSuccess || (err = ensure(ctx->vOff, 8 * (N + 1))) != hipSuccess)
+        return hipFail(ctx, err, "allocating offsets");
+    if ((err = hipMemcpyAsync(ctx->hOff.p, setH->offsets, 8 * (N + 1), hipMemcpyHostToDevice, stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(ctx->vOff.p, setV->offsets, 8 * (N + 1), hipMemcpyHostToDevice, stream)) != hipSuccess)
+        return hipFail(ctx, err, "copying offsets");
+    int32_t *dScores = scores;
+    if (!(flags & SQMI_SCORES_ON_DEVICE)) {
+        if ((err = ensure(ctx->scores, 4 * N)) != hipSuccess)
+            return hipFail(ctx, err, "allocating scores");
+        dScores = static_cast<int32_t *>(ctx->scores.p);
+    }
+
+    // Substitution tables: t(h, v) = s(h, v) - gapOpen with a padding code.
+    const int A = si.alpha, TW = A + 1;
+    std::vector<int16_t> tTab((size_t)TW * TW, -1024);
+    for (int h = 0; h < A; ++h)
+        for (int v = 0; v < A; ++v)
+            tTab[h * TW + v] = (int16_t)(scoring->table[h * A + v] - si.o);
+    uint32_t prof[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (si.dna)
+        for (int h = 0; h < A; ++h)
+            for (int v = 0; v < A; ++v)
+                prof[h] |= (uint32_t)(scoring->table[h * A + v] - si.o) << (8 * v);
+    if ((err = ensure(ctx->tTable, tTab.size() * 2)) != hipSuccess || (err = ensure(ctx->dnaProf, 32)) != hipSuccess ||
+        (err = ensure(ctx->table32, 4 * A * A)) != hipSuccess)
+        return hipFail(ctx, err, "allocating tables");
+    if ((err = hipMemcpyAsync(ctx->tTable.p, tTab.data(), tTab.size() * 2, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(ctx->dnaProf.p, prof, 32, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(ctx->table32.p, scoring->table, 4 * A * A, hipMemcpyHostToDevice, stream)) != hipSuccess)
+        return hipFail(ctx, err, "copying tables");
+
+    if ((err = hipEventRecord(ctx->ev0, stream)) != hipSuccess)
+        return hipFail(ctx, err, "event");
+
+    // ---- plain form: one launch per pair (baseline for the batched form) --
+    if (flags & SQMI_PLAIN_PER_PAIR) {
+        std::vector<int32_t> ids16;
+        std::vector<WorkItem> items;
+        int maxN = 0;
+        for (const Key &k : keys16) {
+            const int32_t id = k.id;
+            const int64_t n = setH->offsets[id + 1] - setH->offsets[id];
+            const int64_t m = setV->offsets[id + 1] - setV->offsets[id];
+            const Variant16 v = plainVariant(si.dna, track, m);
+            const int64_t BR = (int64_t)v.G * v.R;
+            items.push_back(WorkItem{(int32_t)ids16.size(), 1, (int32_t)n, (int32_t)((m + 1 + BR - 1) / BR)});
+            ids16.push_back(id);
+            maxN = std::max<int>(maxN, (int)n);
+        }
+        const size_t np = ids16.size();
+        if ((err = ensure(ctx->order, 4 * np)) != hipSuccess ||
+            (err = ensure(ctx->items, sizeof(WorkItem) * np)) != hipSuccess ||
+            (err = ensure(ctx->queue, 4 * np + 64)) != hipSuccess ||
+            (err = ensure(ctx->rowbuf, (size_t)(maxN + 66) * 2 * sizeof(uint32_t))) != hipSuccess)
+            return hipFail(ctx, err, "allocating plain work lists");
+        if (np && ((err = hipMemcpyAsync(ctx->order.p, ids16.data(), 4 * np, hipMemcpyHostToDevice, stream)) !=
+                       hipSuccess ||
+                   (err = hipMemcpyAsync(ctx->items.p, items.data(), sizeof(WorkItem) * np, hipMemcpyHostToDevice,
+                                         stream)) != hipSuccess ||
+                   (err = hipMemsetAsync(ctx->queue.p, 0, 4 * np, stream)) != hipSuccess))
+            return hipFail(ctx, err, "uploading plain work lists");
+        for (size_t k = 0; k < np; ++k) {
+            const int32_t id = ids16[k];
+            const Variant16 v = plainVariant(si.dna, track, setV->offsets[id + 1] - setV->offsets[id]);
+            FwdArgs a{};
+            a.hRes = dH;
+            a.vRes = dV;
+            a.hOff = static_cast<const int64_t *>(ctx->hOff.p);
+            a.vOff = static_cast<const int64_t *>(ctx->vOff.p);
+            a.order = static_cast<const int32_t *>(ctx->order.p);
+            a.items = static_cast<const WorkItem *>(ctx->items.p) + k;
+            a.nItems = 1;
+            a.queue = static_cast<uint32_t *>(ctx->queue.p) + k;
+            a.rowbuf = static_cast<uint32_t *>(ctx->rowbuf.p);
+            a.rowCols = maxN + 66;
+            a.scores = dScores;
+            a.gapOpen = si.o;
+            a.gapExtend = si.e;
+            a.mode = mode;
+            a.alpha = A;
+            a.tTable = static_cast<const int16_t *>(ctx->tTable.p);
+            a.dnaProfile = static_cast<const uint32_t *>(ctx->dnaProf.p);
+            if ((err = launchFwd16(v, a, 1, stream, 64)) != hipSuccess)
+                return hipFail(ctx, err, "launching plain kernel");
+        }
+        plans.clear();
+    }
+
+    // ---- packed int16 kernels -------------------------------------------
+    int64_t padded = 0, nItems = 0;
+    for (Plan16 &pl : plans) {
+        const Variant16 &v = pl.var;
+        if (!variantSupported(v))
+            return fail(ctx, SQMI_ERR_UNSUPPORTED, "kernel variant not compiled");
+        const int GROUPS = 64 / v.G;
+        const int wpb = wavesPerBlock16();
+        const int occ = occupancy16(v);
+        const int64_t nIt = (int64_t)pl.items.size();
+        int grid = (int)std::min<int64_t>((int64_t)ctx->numCUs * occ, (nIt + wpb - 1) / wpb);
+        grid = std::max(grid, 1);
+        const int rowCols = pl.maxN + v.G + 2;
+        const size_t rbBytes = (size_t)grid * wpb * GROUPS * rowCols * 2 * sizeof(uint32_t);
+        if ((err = ensure(ctx->order, 4 * pl.order.size())) != hipSuccess ||
+            (err = ensure(ctx->items, sizeof(WorkItem) * pl.items.size())) != hipSuccess ||
+            (err = ensure(ctx->queue, 64)) != hipSuccess || (err = ensure(ctx->rowbuf, rbBytes)) != hipSuccess)
+            return hipFail(ctx, err, "allocating work lists");
+        if ((err = hipMemcpyAsync(ctx->order.p, pl.order.data(), 4 * pl.order.size(), hipMemcpyHostToDevice,
+                                  stream)) != hipSuccess ||
+            (err = hipMemcpyAsync(ctx->items.p, pl.items.data(), sizeof(WorkItem) * pl.items.size(),
+                                  hipMemcpyHostToDevice, stream)) != hipSuccess ||
+            (err = hipMemsetAsync(ctx->queue.p, 0, 64, stream)) != hipSuccess)
+            return hipFail(ctx, err, "uploading work lists");
+        FwdArgs a{};
+        a.hRes = dH;
+        a.vRes = dV;
+        a.hOff = static_cast<const int64_t *>(ctx->hOff.p);
+        a.vOff = static_cast<const int64_t *>(ctx->vOff.p);
+        a.order = static_cast<const int32_t *>(ctx->order.p);
+        a.items = static_cast<const WorkItem *>(ctx->items.p);
+        a.nItems = (int32_t)nIt;
+        a.queue = static_cast<uint32_t *>(ctx->queue.p);
+        a.rowbuf = static_cast<uint32_t *>(ctx->rowbuf.p);
+        a.rowCols = rowCols;
+        a.scores = dScores;
+        a.gapOpen = si.o;
+        a.gapExtend = si.e;
+        a.mode = mode;
+        a.alpha = A;
+        a.tTable = static_cast<const int16_t *>(ctx->tTable.p);
+        a.dnaProfile = static_cast<const uint32_t *>(ctx->dnaProf.p);
+        if ((err = launchFwd16(v, a, grid, stream)) != hipSuccess)
+            return hipFail(ctx, err, "launching packed kernel");
+        for (const WorkItem &w : pl.items)
+            padded += 2LL * (64 / v.G) * (int64_t)w.maxN * w.nblocks * v.G * v.R;
+        nItems += nIt;
+        // The work lists are reused by the next variant: wait for this one.
+        if (plans.size() > 1 && (err = hipStreamSynchronize(stream)) != hipSuccess)
+            return hipFail(ctx, err, "packed kernel");
+    }
+    ctx->stats.work_items = nItems;
+    ctx->stats.padded_cells = padded;
+
+    // ---- int32 pairs -------------------------------------------------------
+    if (!ids32.empty()) {
+        const size_t n32 = ids32.size();
+        if ((err = ensure(ctx->ids32, 4 * n32)) != hipSuccess ||
+            (err = ensure(ctx->scratchOff32, 8 * n32)) != hipSuccess ||
+            (err = ensure(ctx->scratch32, (size_t)8 * scratch32)) != hipSuccess)
+            return hipFail(ctx, err, "allocating int32 lists");
+        if ((err = hipMemcpyAsync(ctx->ids32.p, ids32.data(), 4 * n32, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+            (err = hipMemcpyAsync(ctx->scratchOff32.p, scratchOff32.data(), 8 * n32, hipMemcpyHostToDevice, stream)) !=
+                hipSuccess)
+            return hipFail(ctx, err, "uploading int32 lists");
+        Simple32Args a{};
+        a.hRes = dH;
+        a.vRes = dV;
+        a.hOff = static_cast<const int64_t *>(ctx->hOff.p);
+        a.vOff = static_cast<const int64_t *>(ctx->vOff.p);
+        a.ids = static_cast<const int32_t *>(ctx->ids32.p);
+        a.scratchOff = static_cast<const int64_t *>(ctx->scratchOff32.p);
+        a.nIds = (int32_t)n32;
+        a.scratch = static_cast<int32_t *>(ctx->scratch32.p);
+        a.scores = dScores;
+        a.gapOpen = si.o;
+        a.gapExtend = si.e;
+        a.mode = mode;
+        a.alpha = A;
+        a.table = static_cast<const int32_t *>(ctx->table32.p);
+        if ((err = launchSimple32(a, stream)) != hipSuccess)
+            return hipFail(ctx, err, "launching int32 kernel");
+    }
+    if (!emptyIds.empty()) {
+        if ((err = ensure(ctx->emptyIds, 4 * emptyIds.size())) != hipSuccess)
+            return hipFail(ctx, err, "allocating empty list");
+        if ((err = hipMemcpyAsync(ctx->emptyIds.p, emptyIds.data(), 4 * emptyIds.size(), hipMemcpyHostToDevice,
+                                  stream)) != hipSuccess)
+            return hipFail(ctx, err, "uploading empty list");
+        const int ne = (int)emptyIds.size();
+        if ((err = launchFillScores(dScores, static_cast<const int32_t *>(ctx->emptyIds.p), ne, INT32_MIN,
+                                    stream)) != hipSuccess)
+            return hipFail(ctx, err, "launching fill kernel");
+    }
+    if ((err = hipEventRecord(ctx->ev1, stream)) != hipSuccess)
+        return hipFail(ctx, err, "event");
+    if (!(flags & SQMI_SCORES_ON_DEVICE)) {
+        if ((err = hipMemcpyAsync(scores, dScores, 4 * N, hipMemcpyDeviceToHost, stream)) != hipSuccess)
+            return hipFail(ctx, err, "copying scores");
+    }
+    if ((err = hipStreamSynchronize(stream)) != hipSuccess)
+        return hipFail(ctx, err, "alignment kernels");
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess)
+        ctx->stats.device_ms = ms;
+    ctx->stats.total_ms = msSince(tStart);
+    return SQMI_OK;
+}
+
+sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *, const sqmi_sequences *,
+                              const sqmi_scoring *, uint32_t, int32_t *, int32_t *, uint32_t)
+{
+    return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_counts: not available yet");
+}
+
+}  // extern "C"

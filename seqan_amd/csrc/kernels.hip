@@ -1,0 +1,531 @@
+// kernels.hip -- gfx950 kernels of the batched affine-gap alignment engine.
+//
+// The recurrence is SeqAn 2's Gotoh formula (include/seqan/align/
+// dp_formula_affine.h:62-126) with its borders (dp_formula_affine.h:244-330,
+// dp_formula.h:289-309) and result-cell rules (dp_meta_info.h:178-221,
+// dp_scout.h:169-210).  It is computed here in "G-space": every lane keeps
+// G = H + gapOpen instead of H, because H + gapOpen is the only H-derived term
+// both gap recurrences need:
+//
+//   E(i,j) = max(E(i,j-1) + e, G(i,j-1))
+//   F(i,j) = max(F(i-1,j) + e, G(i-1,j))
+//   H(i,j) = max(G(i-1,j-1) + t(h,v), E(i,j), F(i,j))   [local: max(.,0)]
+//   G(i,j) = H(i,j) + o          with t(h,v) = s(h,v) - o >= 0
+//
+// so one cell is 9 VALU ops: the score lookup (one v_perm_b32 for DNA), one
+// add for the diagonal, add+max for E, add+max for F, two maxes for H and
+// one add for G.  Two different pairs share every 32-bit register: the low
+// and high int16 halves (v_pk_add_u16 / v_pk_max_i16).
+//
+// Layout of the packed kernel.  A wavefront takes one work item: up to
+// 2*64/G pairs that need the same number of row blocks.  G consecutive lanes
+// (a "group") sweep one pair-couple; each lane owns R consecutive rows of a
+// block of G*R rows and the group walks the columns as a skewed wavefront:
+// at step s lane l computes column j = s - l, receiving the bottom cell of
+// lane l-1 (column j, one step old) through a DPP lane shift (row_shr:1 for
+// G = 16, wave_shr:1 for G = 64).  The first lane of the group reads the
+// bottom row of the previous block from a global row buffer that the last
+// lane writes.  Rows are bottom-aligned: row m of every pair is the last row
+// of the last block, so the rows above a pair's row 0 are "virtual" (floored
+// at the -inf sentinel) and the border row emerges from the recurrence
+// itself (TOP free: floored at 0).
+#include "kernels.hpp"
+#include "seqan_amd.h"
+
+namespace sqmi {
+namespace {
+
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ s16x2 as16(uint32_t x) { return __builtin_bit_cast(s16x2, x); }
+__device__ __forceinline__ uint32_t asu(s16x2 x) { return __builtin_bit_cast(uint32_t, x); }
+__device__ __forceinline__ s16x2 vmax(s16x2 a, s16x2 b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ s16x2 splat(int x)
+{
+    s16x2 r = {(short)x, (short)x};
+    return r;
+}
+__device__ __forceinline__ s16x2 pack(int lo, int hi)
+{
+    s16x2 r = {(short)lo, (short)hi};
+    return r;
+}
+__device__ __forceinline__ uint32_t bfi(uint32_t mask, uint32_t a, uint32_t b) { return (mask & a) | (~mask & b); }
+
+// Lane l of every group of G lanes receives v from lane l-1; lane 0 of the
+// group receives `first`.
+template <int G>
+__device__ __forceinline__ uint32_t shiftIn(uint32_t v, uint32_t first)
+{
+    if constexpr (G == 1) {
+        return first;
+    } else if constexpr (G == 16) {
+        return (uint32_t)__builtin_amdgcn_update_dpp((int)first, (int)v, 0x111, 0xf, 0xf, false);
+    } else if constexpr (G == 64) {
+        return (uint32_t)__builtin_amdgcn_update_dpp((int)first, (int)v, 0x138, 0xf, 0xf, false);
+    } else {
+        static_assert(G == 2 || G == 4 || G == 8, "unsupported group size");
+        const uint32_t s = (uint32_t)__builtin_amdgcn_update_dpp((int)first, (int)v, 0x111, 0xf, 0xf, false);
+        return ((threadIdx.x % G) == 0) ? first : s;
+    }
+}
+
+template <int G>
+__device__ __forceinline__ s16x2 groupMax(s16x2 x)
+{
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_xor((int)asu(x), d, G);
+        x = vmax(x, as16(y));
+    }
+    return x;
+}
+
+__device__ __forceinline__ uint64_t rbLoad(const uint32_t *p)
+{
+    return __hip_atomic_load(reinterpret_cast<const uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void rbStore(uint32_t *p, uint64_t v)
+{
+    __hip_atomic_store(reinterpret_cast<uint64_t *>(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Per-lane state of one pair couple (lo = pair A, hi = pair B).
+struct Couple {
+    const uint8_t *h[2];
+    const uint8_t *v[2];
+    int n[2], m[2], p[2], pid[2];
+};
+
+template <int G, int R, bool DNA, uint32_t TRK>
+struct Fwd16 {
+    static constexpr int BR = G * R;
+
+    // One block of rows: column 0 init plus the skewed sweep over columns.
+    template <bool HEAD, bool LAST>
+    __device__ __forceinline__ static void block(const FwdArgs &a, const Couple &c, int b, int Nw, int l,
+                                                 uint32_t *rowbuf, const uint32_t *sProf, const int16_t *sT,
+                                                 s16x2 &best)
+    {
+        const int o = a.gapOpen, e = a.gapExtend;
+        const s16x2 o2 = splat(o), e2 = splat(e), neg2 = splat(kNeg16);
+        const bool leftFree = (a.mode & (SQMI_FREE_LEFT | SQMI_LOCAL)) != 0;
+        const bool topFree = (a.mode & SQMI_FREE_TOP) != 0;
+        const int eo = max(e, o);   // first-column gap chain grows by max(e, o) per row
+        const int PAD = a.alpha;
+        const int TW = a.alpha + 1;
+        const int rowbase = b * BR + l * R;
+
+        uint32_t sel[R];
+        s16x2 Gk[R], Ek[R], floorK[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int r = rowbase + k;
+            int code[2], g0[2], fl[2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                const int i = r - c.p[x];
+                int cc = PAD;
+                if (i >= 1) {
+                    const int idx = min(i - 1, max(c.m[x] - 1, 0));
+                    cc = min((int)c.v[x][idx], PAD);
+                }
+                code[x] = cc;
+                // G at column 0: virtual rows -inf, border H(0,0)=0, first column.
+                g0[x] = (i < 0) ? kNeg16 : (i == 0) ? o : ((leftFree ? 0 : o + (i - 1) * eo) + o);
+                fl[x] = (i == 0 && topFree) ? 0 : kNeg16;
+            }
+            if constexpr (DNA) {
+                const uint32_t sa = code[0] < 4 ? (uint32_t)(code[0] | 0x0c00) : 0x0d0cu;
+                const uint32_t sb = code[1] < 4 ? (uint32_t)((code[1] + 4) | 0x0c00) : 0x0d0cu;
+                sel[k] = sa | (sb << 16);
+            } else {
+                sel[k] = (uint32_t)code[0] | ((uint32_t)code[1] << 16);
+            }
+            Gk[k] = pack(g0[0], g0[1]);
+            Ek[k] = neg2;
+            floorK[k] = pack(fl[0], fl[1]);
+        }
+
+        // Row-buffer entry for step 0 (read before this block's column-0 write).
+        uint64_t rbv = (b > 0) ? rbLoad(rowbuf) : ((uint64_t)asu(neg2) << 32 | asu(neg2));
+        if (!LAST && l == G - 1)
+            rbStore(rowbuf, ((uint64_t)asu(neg2) << 32) | asu(Gk[R - 1]));
+
+        if constexpr (LAST && (TRK & kTrackBottom)) {
+            if (l == G - 1)
+                best = vmax(best, Gk[R - 1]);   // H(m, 0) is a tracked cell
+        }
+
+        uint32_t botG = asu(Gk[R - 1]), botF = asu(neg2);
+        s16x2 upGprev = neg2;
+        const s16x2 remBase = pack(c.n[0] + l, c.n[1] + l);
+        const int steps = Nw + G;
+        for (int s = 0; s < steps; ++s) {
+            const int j = s - l;
+            const s16x2 upG = as16(shiftIn<G>(botG, (uint32_t)rbv));
+            const s16x2 upF = as16(shiftIn<G>(botF, (uint32_t)(rbv >> 32)));
+            // Prefetch the next row-buffer entry (column s+1 for the group head).
+            if (b > 0 && s + 1 <= Nw)
+                rbv = rbLoad(rowbuf + 2 * (s + 1));
+            const s16x2 diagTop = upGprev;
+            upGprev = upG;
+            if (j >= 1 && j <= Nw) {
+                int hc[2];
+#pragma unroll
+                for (int x = 0; x < 2; ++x) {
+                    const int idx = min(j, max(c.n[x], 1)) - 1;
+                    const int cc = min((int)c.h[x][idx], PAD);
+                    hc[x] = (j <= c.n[x]) ? cc : PAD;
+                }
+                uint32_t profA = 0, profB = 0;
+                int baseA = 0, baseB = 0;
+                if constexpr (DNA) {
+                    profA = sProf[hc[0]];
+                    profB = sProf[hc[1]];
+                } else {
+                    baseA = hc[0] * TW;
+                    baseB = hc[1] * TW;
+                }
+                s16x2 diag = diagTop, uG = upG, uF = upF;
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    uint32_t t;
+                    if constexpr (DNA) {
+                        t = __builtin_amdgcn_perm(profB, profA, sel[k]);
+                    } else {
+                        const uint32_t lo = (uint16_t)sT[baseA + (int)(sel[k] & 0xffffu)];
+                        const uint32_t hi = (uint16_t)sT[baseB + (int)(sel[k] >> 16)];
+                        t = lo | (hi << 16);
+                    }
+                    const s16x2 D = diag + as16(t);
+                    Ek[k] = vmax(Ek[k] + e2, Gk[k]);
+                    const s16x2 F = vmax(uF + e2, uG);
+                    s16x2 H = vmax(vmax(D, Ek[k]), F);
+                    if constexpr (TRK & kTrackLocal) {
+                        H = vmax(H, splat(0));
+                    } else if constexpr (HEAD) {
+                        H = vmax(H, floorK[k]);
+                    }
+                    diag = Gk[k];
+                    const s16x2 Gn = H + o2;
+                    Gk[k] = Gn;
+                    if constexpr (TRK & kTrackLocal)
+                        best = vmax(best, Gn);
+                    uG = Gn;
+                    uF = F;
+                }
+                botG = asu(uG);
+                botF = asu(uF);
+                if constexpr (TRK & (kTrackEnd | kTrackBottom | kTrackRight)) {
+                    const s16x2 rem = remBase - splat(s);   // n - j per half
+                    if constexpr (LAST && (TRK & kTrackBottom)) {
+                        const uint32_t after = asu(rem >> splat(15));   // 0xffff where j > n
+                        const s16x2 cand = as16(bfi(after, asu(neg2), asu(uG)));
+                        if (l == G - 1)
+                            best = vmax(best, cand);
+                    }
+                    if constexpr ((LAST && (TRK & kTrackEnd)) || (TRK & kTrackRight)) {
+                        // eq = 0xffff in halves where rem == 0.
+                        const uint16_t r0 = (uint16_t)rem.x, r1 = (uint16_t)rem.y;
+                        const uint32_t eq = (r0 == 0 ? 0xffffu : 0u) | (r1 == 0 ? 0xffff0000u : 0u);
+                        if constexpr (LAST && (TRK & kTrackEnd)) {
+                            if (l == G - 1)
+                                best = as16(bfi(eq, asu(uG), asu(best)));
+                        }
+                        if constexpr (TRK & kTrackRight) {
+                            if (eq != 0) {
+                                s16x2 cm = Gk[0];
+#pragma unroll
+                                for (int k = 1; k < R; ++k)
+                                    cm = vmax(cm, Gk[k]);
+                                best = vmax(best, as16(bfi(eq, asu(cm), asu(neg2))));
+                            }
+                        }
+                    }
+                }
+                if (!LAST && l == G - 1)
+                    rbStore(rowbuf + 2 * j, ((uint64_t)botF << 32) | botG);
+            }
+        }
+        // Make this block's row-buffer writes visible to the next block.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+
+    __device__ static void run(const FwdArgs &a)
+    {
+        constexpr int GROUPS = 64 / G;
+        __shared__ uint32_t sProf[8];
+        __shared__ int16_t sT[33 * 33];
+        if constexpr (DNA) {
+            if (threadIdx.x < 5)
+                sProf[threadIdx.x] = a.dnaProfile[threadIdx.x];
+        } else {
+            const int nT = (a.alpha + 1) * (a.alpha + 1);
+            for (int i = threadIdx.x; i < nT; i += blockDim.x)
+                sT[i] = a.tTable[i];
+        }
+        __syncthreads();
+
+        const int lane = threadIdx.x & 63;
+        const int l = lane % G;
+        const int grp = lane / G;
+        const int slot = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        uint32_t *rowbuf = a.rowbuf + ((size_t)slot * GROUPS + grp) * (size_t)a.rowCols * 2;
+
+        for (;;) {
+            int it = 0;
+            if (lane == 0)
+                it = (int)atomicAdd(a.queue, 1u);
+            it = __builtin_amdgcn_readfirstlane(it);
+            if (it >= a.nItems)
+                break;
+            const WorkItem w = a.items[it];
+            Couple c;
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                const int sidx = 2 * grp + x;
+                const int pid = sidx < w.count ? a.order[w.first + sidx] : -1;
+                int64_t ho = 0, vo = 0;
+                int n = 0, m = 0;
+                if (pid >= 0) {
+                    ho = a.hOff[pid];
+                    vo = a.vOff[pid];
+                    n = (int)(a.hOff[pid + 1] - ho);
+                    m = (int)(a.vOff[pid + 1] - vo);
+                }
+                c.h[x] = a.hRes + ho;
+                c.v[x] = a.vRes + vo;
+                c.n[x] = n;
+                c.m[x] = m;
+                c.pid[x] = pid;
+                c.p[x] = w.nblocks * BR - 1 - m;
+            }
+            s16x2 best = (TRK & kTrackLocal) ? splat(a.gapOpen) : splat(kNeg16);
+            const int Nw = w.maxN;
+            if (w.nblocks == 1) {
+                block<true, true>(a, c, 0, Nw, l, rowbuf, sProf, sT, best);
+            } else {
+                block<true, false>(a, c, 0, Nw, l, rowbuf, sProf, sT, best);
+                for (int b = 1; b < w.nblocks - 1; ++b)
+                    block<false, false>(a, c, b, Nw, l, rowbuf, sProf, sT, best);
+                block<false, true>(a, c, w.nblocks - 1, Nw, l, rowbuf, sProf, sT, best);
+            }
+            best = groupMax<G>(best);
+            if (l == 0) {
+                if (c.pid[0] >= 0)
+                    a.scores[c.pid[0]] = (int)best.x - a.gapOpen;
+                if (c.pid[1] >= 0)
+                    a.scores[c.pid[1]] = (int)best.y - a.gapOpen;
+            }
+        }
+    }
+};
+
+template <int G, int R, bool DNA, uint32_t TRK>
+__global__ __launch_bounds__(256) void kFwd16(FwdArgs a)
+{
+    Fwd16<G, R, DNA, TRK>::run(a);
+}
+
+// ---------------------------------------------------------------------------
+// int32, one thread per pair: the textbook column sweep with the reference's
+// border and tracking rules.  Used for pairs whose values could leave the
+// int16 window.
+// ---------------------------------------------------------------------------
+constexpr int kNeg32 = INT32_MIN / 2;   // DPCellDefaultInfinity (dp_cell.h:141-146)
+
+__device__ __forceinline__ bool tracked32(uint32_t mode, int col, int cell)
+{
+    if (mode & SQMI_LOCAL)
+        return true;
+    if (cell == 2)
+        return col == 2 || (mode & SQMI_FREE_BOTTOM);
+    return col == 2 && (mode & SQMI_FREE_RIGHT);
+}
+
+__global__ __launch_bounds__(256) void kSimple32(Simple32Args a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nIds)
+        return;
+    const int pid = a.ids[t];
+    const int64_t ho = a.hOff[pid], vo = a.vOff[pid];
+    const int64_t n = a.hOff[pid + 1] - ho, m = a.vOff[pid + 1] - vo;
+    const uint8_t *h = a.hRes + ho;
+    const uint8_t *v = a.vRes + vo;
+    int32_t *H = a.scratch + 2 * a.scratchOff[t];
+    int32_t *E = H + (m + 1);
+    const int32_t o = a.gapOpen, e = a.gapExtend;
+    const uint32_t mode = a.mode;
+    const bool local = (mode & SQMI_LOCAL) != 0;
+    const int A = a.alpha;
+    int32_t best = kNeg32;
+    {
+        int32_t F = kNeg32;
+        H[0] = 0;
+        E[0] = kNeg32;
+        if (tracked32(mode, 0, 0) && H[0] > best)
+            best = H[0];
+        for (int64_t i = 1; i <= m; ++i) {
+            if (local || (mode & SQMI_FREE_LEFT)) {
+                H[i] = 0;
+            } else {
+                F = max(F + e, H[i - 1] + o);
+                H[i] = F;
+            }
+            E[i] = kNeg32;
+            if (tracked32(mode, 0, i == m ? 2 : 1) && H[i] > best)
+                best = H[i];
+        }
+    }
+    for (int64_t j = 1; j <= n; ++j) {
+        const int col = (j == n) ? 2 : 1;
+        const int hj = min((int)h[j - 1], A - 1);
+        const int32_t *row = a.table + hj * A;
+        int32_t diag = H[0];
+        int32_t F = kNeg32;
+        if (local || (mode & SQMI_FREE_TOP)) {
+            H[0] = 0;
+        } else {
+            E[0] = max(E[0] + e, H[0] + o);
+            H[0] = E[0];
+        }
+        if (tracked32(mode, col, 0) && H[0] > best)
+            best = H[0];
+        int32_t up = H[0];
+        for (int64_t i = 1; i <= m; ++i) {
+            const int32_t d = diag + row[min((int)v[i - 1], A - 1)];
+            diag = H[i];
+            const int32_t ei = max(E[i] + e, H[i] + o);
+            E[i] = ei;
+            F = max(F + e, up + o);
+            int32_t x = max(d, max(F, ei));
+            if (local)
+                x = max(x, 0);
+            H[i] = x;
+            up = x;
+            if (tracked32(mode, col, i == m ? 2 : 1) && x > best)
+                best = x;
+        }
+    }
+    a.scores[pid] = best;
+}
+
+__global__ void kFillScores(int32_t *scores, const int32_t *ids, int n, int32_t value)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n)
+        scores[ids[t]] = value;
+}
+
+template <int G, int R, bool DNA>
+hipError_t launchTrk(const Variant16 &v, const FwdArgs &args, int gridBlocks, hipStream_t stream,
+                     int threadsPerBlock)
+{
+    dim3 grid(gridBlocks), block(threadsPerBlock);
+    switch (v.track) {
+        case kTrackEnd:
+            hipLaunchKernelGGL((kFwd16<G, R, DNA, kTrackEnd>), grid, block, 0, stream, args);
+            break;
+        case kTrackBottom:
+            hipLaunchKernelGGL((kFwd16<G, R, DNA, kTrackBottom>), grid, block, 0, stream, args);
+            break;
+        case kTrackRight:
+            hipLaunchKernelGGL((kFwd16<G, R, DNA, kTrackRight>), grid, block, 0, stream, args);
+            break;
+        case kTrackBottom | kTrackRight:
+            hipLaunchKernelGGL((kFwd16<G, R, DNA, kTrackBottom | kTrackRight>), grid, block, 0, stream, args);
+            break;
+        case kTrackLocal:
+            if constexpr (!DNA) {
+                hipLaunchKernelGGL((kFwd16<G, R, DNA, kTrackLocal>), grid, block, 0, stream, args);
+                break;
+            }
+            return hipErrorInvalidValue;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int G, int R, bool DNA>
+const void *kernelTrk(uint32_t track)
+{
+    switch (track) {
+        case kTrackEnd: return (const void *)&kFwd16<G, R, DNA, kTrackEnd>;
+        case kTrackBottom: return (const void *)&kFwd16<G, R, DNA, kTrackBottom>;
+        case kTrackRight: return (const void *)&kFwd16<G, R, DNA, kTrackRight>;
+        case kTrackBottom | kTrackRight: return (const void *)&kFwd16<G, R, DNA, kTrackBottom | kTrackRight>;
+        case kTrackLocal:
+            if constexpr (!DNA)
+                return (const void *)&kFwd16<G, R, DNA, kTrackLocal>;
+            return nullptr;
+        default: return nullptr;
+    }
+}
+
+// The compiled (G, R) family.
+#define SQMI_FOR_EACH_GR(X) X(16, 16) X(16, 8) X(64, 16) X(64, 4) X(1, 16)
+
+template <bool DNA>
+const void *kernelFor(const Variant16 &v)
+{
+#define X(g, r) if (v.G == g && v.R == r) return kernelTrk<g, r, DNA>(v.track);
+    SQMI_FOR_EACH_GR(X)
+#undef X
+    return nullptr;
+}
+
+}  // namespace
+
+int wavesPerBlock16() { return 4; }
+
+bool variantSupported(const Variant16 &v)
+{
+    return (v.dna ? kernelFor<true>(v) : kernelFor<false>(v)) != nullptr;
+}
+
+int occupancy16(const Variant16 &v)
+{
+    const void *k = v.dna ? kernelFor<true>(v) : kernelFor<false>(v);
+    if (!k)
+        return 0;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 256, 0) != hipSuccess)
+        return 1;
+    return blocks > 0 ? blocks : 1;
+}
+
+hipError_t launchFwd16(const Variant16 &v, const FwdArgs &args, int gridBlocks, hipStream_t stream,
+                       int threadsPerBlock)
+{
+#define X(g, r)                                                                      \
+    if (v.G == g && v.R == r)                                                        \
+        return v.dna ? launchTrk<g, r, true>(v, args, gridBlocks, stream, threadsPerBlock)   \
+                     : launchTrk<g, r, false>(v, args, gridBlocks, stream, threadsPerBlock);
+    SQMI_FOR_EACH_GR(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launchSimple32(const Simple32Args &args, hipStream_t stream)
+{
+    if (args.nIds <= 0)
+        return hipSuccess;
+    dim3 grid((args.nIds + 255) / 256), block(256);
+    hipLaunchKernelGGL(kSimple32, grid, block, 0, stream, args);
+    return hipGetLastError();
+}
+
+hipError_t launchFillScores(int32_t *scores, const int32_t *ids, int n, int32_t value, hipStream_t stream)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kFillScores, dim3((n + 255) / 256), dim3(256), 0, stream, scores, ids, n, value);
+    return hipGetLastError();
+}
+
+}  // namespace sqmi

@@ -1,0 +1,145 @@
+"""Writes the golden fixtures under tests/golden/ (run in a container that
+holds the reference tree and the oracle/_ref build):
+
+    make -C oracle && python tests/golden/make_golden.py
+
+* ``ref_*.npz``   -- inputs (residue ordinals + offsets) and the outputs of
+  the reference library itself (oracle/_ref: SeqAn 2 globalAlignmentScore /
+  localAlignmentScore / globalAlignment under ExecutionPolicy<Parallel,
+  Vectorial>, plus computeAlignmentStats on the rows).
+* ``pair_align.json`` -- sequences and expected scores / rows copied from the
+  reference's app tests (apps/pair_align/tests/*.fa, *.stdout, *_out.fa).
+* ``BLOSUM62.txt`` -- the reference's tests/score/BLOSUM62 data file.
+"""
+import json
+import os
+import re
+import shutil
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import pyoracle as po  # noqa: E402
+import seqan_amd as sa  # noqa: E402
+
+REF = "/root/reference"
+PA = os.path.join(REF, "apps", "pair_align", "tests")
+
+SIMPLE = dict(match=2, mismatch=-3, gap_extend=-1, gap_open=-5)
+
+
+def save(name, H, V, **extra):
+    np.savez_compressed(os.path.join(HERE, name), h_res=H.residues, h_off=H.offsets, v_res=V.residues,
+                        v_off=V.offsets, **extra)
+    print("wrote", name, {k: getattr(v, "shape", v) for k, v in extra.items()})
+
+
+def ref_scores(H, V, mode, **kw):
+    r = po.run_reference(po.OP_SCORE, H.residues, H.offsets, V.residues, V.offsets, mode, **kw)
+    return r["scores"]
+
+
+def random_pairs(rng, count, lo, hi, alpha, related=True):
+    hs, vs = [], []
+    for _ in range(count):
+        n = int(rng.integers(lo, hi + 1))
+        h = rng.integers(0, alpha, n).astype(np.uint8)
+        if related and rng.random() < 0.7:
+            v = h.copy()
+            mut = rng.random(n) < 0.2
+            v[mut] = rng.integers(0, alpha, int(mut.sum()))
+            if n > 4 and rng.random() < 0.5:
+                a = int(rng.integers(0, n - 2))
+                b = int(rng.integers(a + 1, min(n, a + 6)))
+                v = np.concatenate([v[:a], v[b:]]) if rng.random() < 0.5 else np.concatenate(
+                    [v[:a], rng.integers(0, alpha, b - a).astype(np.uint8), v[a:]])
+        else:
+            v = rng.integers(0, alpha, int(rng.integers(lo, hi + 1))).astype(np.uint8)
+        if len(v) == 0:
+            v = np.array([0], np.uint8)
+        hs.append(h)
+        vs.append(v)
+    return sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+
+
+def main():
+    os.makedirs(HERE, exist_ok=True)
+    rng = np.random.default_rng(20261202)
+
+    # --- reference data files -------------------------------------------
+    shutil.copyfile(os.path.join(REF, "tests", "score", "BLOSUM62"), os.path.join(HERE, "BLOSUM62.txt"))
+    pa = {"source": "apps/pair_align/tests (reference app tests)", "pairs": {}}
+    for name in ["1aab", "1ad2", "2trx", "adeno1", "adeno2", "adeno3"]:
+        txt = open(os.path.join(PA, name + ".fa")).read()
+        recs = [r for r in re.split(r"^>.*$", txt, flags=re.M) if r.strip()]
+        seqs = ["".join(r.split()) for r in recs]
+        entry = {"h": seqs[0], "v": seqs[1], "scores": {}, "rows": {}}
+        for f in sorted(os.listdir(PA)):
+            m = re.match(re.escape(name) + r"(\.[A-Za-z0-9-]+)?\.stdout$", f)
+            if not m:
+                continue
+            tag = (m.group(1) or ".default")[1:]
+            sm = re.search(r"score:\s*(-?\d+)", open(os.path.join(PA, f)).read())
+            if sm:
+                entry["scores"][tag] = int(sm.group(1))
+            rows_file = os.path.join(PA, f.replace(".stdout", "_out.fa"))
+            if os.path.exists(rows_file):
+                rt = open(rows_file).read()
+                rr = [r for r in re.split(r"^>.*$", rt, flags=re.M) if r.strip()]
+                entry["rows"][tag] = ["".join(r.split()) for r in rr]
+        pa["pairs"][name] = entry
+    with open(os.path.join(HERE, "pair_align.json"), "w") as f:
+        json.dump(pa, f, indent=1, sort_keys=True)
+    print("wrote pair_align.json")
+
+    # --- config 1: 4,096 x 150 bp, global, full output -------------------
+    H, V = sa.workload(sa.WL_DNA150, 0x5EED0001, 0, 4096)
+    save("ref_dna150_global.npz", H, V, scores=ref_scores(H, V, 0, alphabet="dna", **SIMPLE),
+         seed=0x5EED0001, mode=0)
+
+    # --- config 2 subset: semi-global AlignConfig<1,0,0,1> ---------------
+    H, V = sa.workload(sa.WL_DNA_LOGNORMAL, 0x5EED0002, 0, 1500)
+    save("ref_dna_lognormal_semiglobal.npz", H, V,
+         scores=ref_scores(H, V, 9, alphabet="dna", **SIMPLE), seed=0x5EED0002, mode=9)
+
+    # --- config 3 subset: protein local, Blosum62(-1, -11) ---------------
+    H, V = sa.workload(sa.WL_PROTEIN, 0x5EED0003, 0, 300)
+    save("ref_protein_local.npz", H, V,
+         scores=ref_scores(H, V, 16, alphabet="aa", matrix="blosum62", gap_extend=-1, gap_open=-11),
+         seed=0x5EED0003, mode=16)
+
+    # --- every AlignConfig and local on short random DNA and protein -----
+    H, V = random_pairs(rng, 400, 1, 90, 4)
+    modes = list(range(16)) + [16]
+    allm = np.stack([ref_scores(H, V, m, alphabet="dna", match=2, mismatch=-1, gap_extend=-1, gap_open=-3)
+                     for m in modes])
+    save("ref_dna_all_modes.npz", H, V, scores=allm, modes=np.array(modes), match=2, mismatch=-1,
+         gap_extend=-1, gap_open=-3)
+    H, V = random_pairs(rng, 200, 1, 120, 24)
+    allm = np.stack([ref_scores(H, V, m, alphabet="aa", matrix="blosum62", gap_extend=-2, gap_open=-4)
+                     for m in modes])
+    save("ref_protein_all_modes.npz", H, V, scores=allm, modes=np.array(modes), gap_extend=-2, gap_open=-4)
+
+    # --- config 4 subset: traceback counts (SIMD and scalar reference) ---
+    H, V = sa.workload(sa.WL_DNA1K, 0x5EED0004, 0, 96)
+    rs = po.run_reference(po.OP_COUNTS_SIMD, H.residues, H.offsets, V.residues, V.offsets, 0, alphabet="dna",
+                          **SIMPLE)
+    rc = po.run_reference(po.OP_COUNTS_SCALAR, H.residues, H.offsets, V.residues, V.offsets, 0, alphabet="dna",
+                          **SIMPLE)
+    assert np.array_equal(rs["counts"], rc["counts"]) and np.array_equal(rs["scores"], rc["scores"])
+    save("ref_dna1k_counts.npz", H, V, scores=rs["scores"], counts=rs["counts"], seed=0x5EED0004, mode=0)
+
+    # --- short random DNA traceback counts with many ties ----------------
+    H, V = random_pairs(rng, 300, 1, 60, 4)
+    rs = po.run_reference(po.OP_COUNTS_SIMD, H.residues, H.offsets, V.residues, V.offsets, 0, alphabet="dna",
+                          match=1, mismatch=-1, gap_extend=-1, gap_open=-2)
+    save("ref_dna_short_counts.npz", H, V, scores=rs["scores"], counts=rs["counts"], match=1, mismatch=-1,
+         gap_extend=-1, gap_open=-2)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,237 @@
+"""Pins the oracle (oracle/oracle.c) before it is trusted as the GPU checker.
+
+Known answers come from the reference's own tests (file:line cited), from its
+pair_align app tests, and from fixtures produced by the compiled reference
+(tests/golden/make_golden.py).  Brute-force enumeration and closed forms
+cover what those leave open.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+DNA5 = "ACGTN"
+AA = "ABCDEFGHIJKLMNOPQRSTUVWYZX*"
+
+
+def enc(s, alphabet=DNA5):
+    return np.array([alphabet.index(c) for c in s], dtype=np.uint8)
+
+
+def rows_from_ops(h, v, ops, alphabet=DNA5):
+    rh, rv, i, j = [], [], 0, 0
+    for op in ops:
+        if op == "M":
+            rh.append(alphabet[h[j]]); rv.append(alphabet[v[i]]); i += 1; j += 1
+        elif op == "H":
+            rh.append(alphabet[h[j]]); rv.append("-"); j += 1
+        else:
+            rh.append("-"); rv.append(alphabet[v[i]]); i += 1
+    return "".join(rh), "".join(rv)
+
+
+SC_2_1_1_3 = po.Scoring.simple(2, -1, -1, -3, alpha=5)   # Score<int,Simple>(2,-1,-1,-3)
+
+
+# tests/align/test_align_global_alignment_score.h:133-175
+def test_kat_global_alignment_score():
+    h, v = enc("ATGT"), enc("ATAGAT")
+    assert po.score(h, v, SC_2_1_1_3, 0) == 2
+    assert po.score(h, v, SC_2_1_1_3, 15) == 4
+
+
+# tests/align/test_alignment_algorithms_global.h:420-498 (scores and rows)
+@pytest.mark.parametrize("sh,sv,score,rh,rv", [
+    ("ATGT", "ATAGAT", 2, "AT-G-T", "ATAGAT"),
+    ("AAAAAGGGGTTTT", "AAAGTT", 1, "AAAAAGGGGTTTT", "AAA-----GTT--"),
+    ("AAAAAATTTTTGGG", "TTTTTTTTGGGGGGGG", 1, "AAAAAATTTTTGGG-----", "---TTTTTTTTGGGGGGGG"),
+])
+def test_kat_global_affine_rows(sh, sv, score, rh, rv):
+    h, v = enc(sh), enc(sv)
+    assert po.score(h, v, SC_2_1_1_3, 0) == score
+    s, counts, ops = po.traceback(h, v, SC_2_1_1_3, 0)
+    assert s == score
+    assert rows_from_ops(h, v, ops) == (rh, rv)
+
+
+# tests/align/test_alignment_algorithms_global.h:2007-2083 and :2318-2384
+# (AlignConfig<true,false,false,true>)
+@pytest.mark.parametrize("sh,sv,score", [
+    ("ATGT", "ATAGAT", 2), ("AAAAAGGGGTTTT", "AAAGTT", 7), ("AAAAAATTTTTGGG", "TTTTTTTTGGGGGGGG", 6)])
+def test_kat_semiglobal_1001(sh, sv, score):
+    assert po.score(enc(sh), enc(sv), SC_2_1_1_3, po.FREE_TOP | po.FREE_BOTTOM) == score
+
+
+# tests/align/test_alignment_algorithms_local.h:250-300
+def test_kat_local_affine():
+    assert po.score(enc("CACACTTAACTTCACAA"), enc("GGGGCTTGAGAGCTTGGGG"), SC_2_1_1_3, po.LOCAL) == 8
+
+
+def _pair_align():
+    with open(os.path.join(GOLDEN, "pair_align.json")) as f:
+        return json.load(f)["pairs"]
+
+
+def _blosum():
+    return po.Scoring(_blosum62_fixture(), -11, -1)
+
+
+def _blosum62_fixture():
+    lines = [l for l in open(os.path.join(GOLDEN, "BLOSUM62.txt")) if l.strip() and not l.startswith("#")]
+    cols = lines[0].split()
+    t = np.zeros((27, 27), np.int32)
+    for line in lines[1:]:
+        parts = line.split()
+        a = AA.index(parts[0])
+        for c, val in zip(cols, parts[1:]):
+            t[a, AA.index(c)] = int(val)
+    return t
+
+
+# apps/pair_align/tests/*.stdout: -c <TOP><LEFT><RIGHT><BOTTOM>, -m gotoh / sw,
+# -g / -e overrides; Blosum62 with gop -11, gex -1 (pair_align.cpp:59-67).
+@pytest.mark.parametrize("name", ["1aab", "1ad2", "2trx"])
+def test_kat_pair_align_protein(name):
+    p = _pair_align()[name]
+    h, v = enc(p["h"], AA), enc(p["v"], AA)
+    t = _blosum62_fixture()
+    sc = po.Scoring(t, -11, -1)
+    for tag, want in p["scores"].items():
+        if tag in ("default", "protein", "mgotoh", "lo5", "ms10", "mm-8"):
+            assert po.score(h, v, sc, 0) == want, tag
+        elif tag == "msw":
+            assert po.score(h, v, sc, po.LOCAL) == want, tag
+        elif tag.startswith("c") and len(tag) == 5:
+            bits = [c == "t" for c in tag[1:]]
+            mode = (po.FREE_TOP if bits[0] else 0) | (po.FREE_LEFT if bits[1] else 0) | (
+                po.FREE_RIGHT if bits[2] else 0) | (po.FREE_BOTTOM if bits[3] else 0)
+            assert po.score(h, v, sc, mode) == want, tag
+        elif tag == "g-20":
+            assert po.score(h, v, po.Scoring(t, -20, -1), 0) == want, tag
+        elif tag == "e-5":
+            assert po.score(h, v, po.Scoring(t, -11, -5), 0) == want, tag
+        elif tag == "mnw":   # linear gaps: every gap character costs gapExtend
+            assert po.score(h, v, po.Scoring(t, -1, -1), 0) == want, tag
+
+
+@pytest.mark.parametrize("name", ["adeno1", "adeno2", "adeno3"])
+def test_kat_pair_align_dna(name):
+    # Dna5, Score<int>(5, -4, -1, -11) (pair_align.cpp:59-67)
+    p = _pair_align()[name]
+    h, v = enc(p["h"]), enc(p["v"])
+    assert po.score(h, v, po.Scoring.simple(5, -4, -1, -11, alpha=5), 0) == p["scores"]["default"]
+
+
+def test_pair_align_rows_give_counts():
+    """The gotoh rows of pair_align, scored by per-row gap runs, equal the score;
+    our traceback walks to the same rows."""
+    t = _blosum62_fixture()
+    sc = po.Scoring(t, -11, -1)
+    for name in ["1aab", "1ad2", "2trx"]:
+        p = _pair_align()[name]
+        rh, rv = p["rows"]["mgotoh"]
+        h, v = enc(p["h"], AA), enc(p["v"], AA)
+        s, counts, ops = po.traceback(h, v, sc, 0)
+        assert rows_from_ops(h, v, ops, AA) == (rh, rv)
+        assert s == p["scores"]["mgotoh"]
+        assert int((counts[:729] * t.ravel()).sum() + counts[729] * -11 + counts[730] * -1) == s
+
+
+def test_blosum62_fixture_matches_library_preset():
+    import seqan_amd as sa
+    assert np.array_equal(sa.blosum62().table, _blosum62_fixture())
+
+
+def _load(name):
+    return np.load(os.path.join(GOLDEN, name))
+
+
+def test_oracle_matches_reference_dna150():
+    z = _load("ref_dna150_global.npz")
+    sc = po.Scoring.simple(2, -3, -1, -5)
+    got = po.score_batch(z["h_res"], z["h_off"], z["v_res"], z["v_off"], sc, 0)
+    assert np.array_equal(got, z["scores"])
+
+
+def test_oracle_matches_reference_lognormal_semiglobal():
+    z = _load("ref_dna_lognormal_semiglobal.npz")
+    sc = po.Scoring.simple(2, -3, -1, -5)
+    got = po.score_batch(z["h_res"], z["h_off"], z["v_res"], z["v_off"], sc, 9)
+    assert np.array_equal(got, z["scores"])
+
+
+def test_oracle_matches_reference_protein_local():
+    z = _load("ref_protein_local.npz")
+    sc = po.Scoring(_blosum62_fixture(), -11, -1)
+    got = po.score_batch(z["h_res"], z["h_off"], z["v_res"], z["v_off"], sc, 16)
+    assert np.array_equal(got, z["scores"])
+
+
+@pytest.mark.parametrize("fixture,make", [
+    ("ref_dna_all_modes.npz", lambda z: po.Scoring.simple(int(z["match"]), int(z["mismatch"]),
+                                                          int(z["gap_extend"]), int(z["gap_open"]))),
+    ("ref_protein_all_modes.npz", lambda z: po.Scoring(_blosum62_fixture(), int(z["gap_open"]),
+                                                       int(z["gap_extend"]))),
+])
+def test_oracle_matches_reference_all_modes(fixture, make):
+    z = _load(fixture)
+    sc = make(z)
+    for k, mode in enumerate(z["modes"]):
+        got = po.score_batch(z["h_res"], z["h_off"], z["v_res"], z["v_off"], sc, int(mode))
+        assert np.array_equal(got, z["scores"][k]), int(mode)
+
+
+@pytest.mark.parametrize("fixture", ["ref_dna1k_counts.npz", "ref_dna_short_counts.npz"])
+def test_oracle_traceback_counts_match_reference(fixture):
+    z = _load(fixture)
+    if "match" in z:
+        sc = po.Scoring.simple(int(z["match"]), int(z["mismatch"]), int(z["gap_extend"]), int(z["gap_open"]))
+    else:
+        sc = po.Scoring.simple(2, -3, -1, -5)
+    s, c = po.traceback_batch(z["h_res"], z["h_off"], z["v_res"], z["v_off"], sc, 0)
+    assert np.array_equal(s, z["scores"])
+    assert np.array_equal(c, z["counts"])
+    # sum(count * parameter) == score for every pair
+    par = np.concatenate([sc.table.ravel(), [sc.gap_open, sc.gap_extend]])
+    assert np.array_equal(c.astype(np.int64) @ par, s.astype(np.int64))
+
+
+@pytest.mark.parametrize("mode", list(range(16)) + [16])
+def test_oracle_vs_brute_force(mode):
+    rng = np.random.default_rng(mode)
+    for trial in range(60):
+        alpha = int(rng.integers(2, 5))
+        o = int(rng.integers(-6, 0)); e = int(rng.integers(o, 1)) if rng.random() < 0.7 else int(rng.integers(-6, 1))
+        t = rng.integers(-4, 5, size=(alpha, alpha)).astype(np.int32)
+        sc = po.Scoring(t, o, e)
+        n, m = int(rng.integers(1, 7)), int(rng.integers(1, 7))
+        h = rng.integers(0, alpha, n).astype(np.uint8)
+        v = rng.integers(0, alpha, m).astype(np.uint8)
+        want = po.brute_force(h, v, sc, mode)
+        assert po.score(h, v, sc, mode) == want, (mode, h, v, t, o, e)
+
+
+def test_oracle_closed_forms():
+    rng = np.random.default_rng(7)
+    sc = po.Scoring.simple(2, -3, -1, -5)
+    for L in (1, 5, 150, 1000):
+        h = rng.integers(0, 4, L).astype(np.uint8)
+        assert po.score(h, h, sc, 0) == 2 * L                       # identical: all matches
+        assert po.score(h, h[:0], sc, 0) == np.iinfo(np.int32).min  # empty -> INT_MIN
+    # a single gap run of k costs open + (k-1)*extend
+    h = np.zeros(10, np.uint8)
+    for k in range(1, 6):
+        assert po.score(h, h[:10 - k], sc, 0) == 2 * (10 - k) - 5 - (k - 1)
+    # local >= 0, symmetric under swapping H and V with a symmetric table
+    t = _blosum62_fixture()
+    bs = po.Scoring(t, -11, -1)
+    for _ in range(20):
+        a = rng.integers(0, 24, int(rng.integers(1, 80))).astype(np.uint8)
+        b = rng.integers(0, 24, int(rng.integers(1, 80))).astype(np.uint8)
+        s1, s2 = po.score(a, b, bs, po.LOCAL), po.score(b, a, bs, po.LOCAL)
+        assert s1 == s2 >= 0
+        assert po.score(a, b, bs, 15) >= po.score(a, b, bs, 9) >= po.score(a, b, bs, 0)
