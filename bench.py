@@ -74,7 +74,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--pairs", type=int, default=N_PAIRS)
     ap.add_argument("--cpu-sample", type=int, default=N_PAIRS // 8)
-    ap.add_argument("--plain-sample", type=int, default=2048)
+    ap.add_argument("--plain-sample", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -96,16 +96,19 @@ def main():
     def step():
         ctx.align_score(H, V, score, MODE, h_dev=dh.address, v_dev=dv.address, scores_dev=out.address)
 
-    for _ in range(args.warmup):
+    for w in range(args.warmup):
+        tw = time.perf_counter()
         step()
+        log(f"warmup {w}: {time.perf_counter() - tw:.3f}s {ctx.stats()}")
     sa.synchronize()
     dev_ms, plan_ms = [], []
     t1 = time.perf_counter()
-    for _ in range(args.steps):
+    for k in range(args.steps):
         step()
         st = ctx.stats()
         dev_ms.append(st["device_ms"])
         plan_ms.append(st["host_plan_ms"])
+        log(f"step {k}: device {st['device_ms']:.1f} ms, plan {st['host_plan_ms']:.1f} ms")
     sa.synchronize()
     elapsed = time.perf_counter() - t1
     ms_per_step = elapsed * 1e3 / args.steps
@@ -128,10 +131,18 @@ def main():
     hp = sa.SeqSet.from_list([H[i] for i in idx])
     vp = sa.SeqSet.from_list([V[i] for i in idx])
     pcells = int((hp.lengths() * vp.lengths()).sum())
-    ctx.align_score(hp, vp, score, MODE, flags=sa.PLAIN_PER_PAIR)     # warm
-    tp = time.perf_counter()
-    plain_scores = ctx.align_score(hp, vp, score, MODE, flags=sa.PLAIN_PER_PAIR)
-    plain_s = time.perf_counter() - tp
+    warm = sa.SeqSet.from_list([H[i] for i in idx[:8]]), sa.SeqSet.from_list([V[i] for i in idx[:8]])
+    ctx.align_score(warm[0], warm[1], score, MODE, flags=sa.PLAIN_PER_PAIR)
+    plain_scores = np.empty(ps, np.int32)
+    plain_s = 0.0
+    chunk = 256
+    for c0 in range(0, ps, chunk):
+        sub_h = sa.SeqSet.from_list([H[i] for i in idx[c0:c0 + chunk]])
+        sub_v = sa.SeqSet.from_list([V[i] for i in idx[c0:c0 + chunk]])
+        tp = time.perf_counter()
+        plain_scores[c0:c0 + chunk] = ctx.align_score(sub_h, sub_v, score, MODE, flags=sa.PLAIN_PER_PAIR)
+        plain_s += time.perf_counter() - tp
+        log(f"plain: {min(c0 + chunk, ps)}/{ps} pairs, {plain_s:.2f}s")
     plain_gcups = pcells / plain_s / 1e9
     plain_ok = bool(np.array_equal(plain_scores, scores[idx]))
     log(f"plain: {ps} pairs, {plain_gcups:.2f} GCUPS, parity {plain_ok}")

@@ -62,6 +62,9 @@ enum {
     SQMI_FORCE_INT32 = 4,         /* skip the packed int16 kernels             */
     SQMI_PLAIN_PER_PAIR = 8       /* one kernel launch per pair (baseline form) */
 };
+/* Tuning knob: force the packed kernel shape (lanes per pair group G and rows
+ * per lane R) instead of the planner's choice; 0 means automatic. */
+#define SQMI_FORCE_SHAPE(G, R) ((((uint32_t)(G) & 0xFFu) << 8) | (((uint32_t)(R) & 0xFFu) << 16))
 
 /* A set of sequences: concatenated residue ordinals (Dna: A0 C1 G2 T3;
  * AminoAcid: SeqAn ordinal order ABCDEFGHIJKLMNOPQRSTUVWYZX*) and count+1
@@ -128,6 +131,14 @@ sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *set_h,
                               uint32_t flags);
 
 sqmi_status sqmi_get_stats(const sqmi_context *ctx, sqmi_stats *out);
+
+/* Host-only dry run of the planner: validates the request exactly as
+ * sqmi_align_score does and reports how the pairs would be routed (int16 /
+ * int32 / empty) and bucketed into work items.  Needs no device.  (No
+ * counterpart in the reference; the reference has no planning step.) */
+sqmi_status sqmi_plan_stats(const sqmi_sequences *set_h, const sqmi_sequences *set_v,
+                            const sqmi_scoring *scoring, uint32_t mode, uint32_t flags,
+                            sqmi_stats *out);
 
 /* The reference's Blosum62 table (27 x 27, AminoAcid ordinal order). */
 const int32_t *sqmi_blosum62(void);

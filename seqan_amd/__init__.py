@@ -28,6 +28,11 @@ LIB_PATH = os.path.join(_HERE, "libseqan_amd.so")
 FREE_TOP, FREE_LEFT, FREE_RIGHT, FREE_BOTTOM, LOCAL = 1, 2, 4, 8, 16
 RESIDUES_ON_DEVICE, SCORES_ON_DEVICE, FORCE_INT32, PLAIN_PER_PAIR = 1, 2, 4, 8
 
+
+def force_shape(G, R):
+    """Flag bits forcing the packed kernel shape (SQMI_FORCE_SHAPE)."""
+    return ((G & 0xFF) << 8) | ((R & 0xFF) << 16)
+
 WL_DNA150, WL_DNA_LOGNORMAL, WL_PROTEIN, WL_DNA1K, WL_DNA100K = 1, 2, 3, 4, 5
 
 DNA = "ACGT"
@@ -67,7 +72,7 @@ class Stats(ctypes.Structure):
 # Every symbol the C ABI declares (include/seqan_amd.h).
 ABI_SYMBOLS = (
     "sqmi_create", "sqmi_destroy", "sqmi_set_stream", "sqmi_align_score", "sqmi_align_counts",
-    "sqmi_get_stats", "sqmi_blosum62", "sqmi_status_string", "sqmi_last_error", "sqmi_abi_version",
+    "sqmi_get_stats", "sqmi_plan_stats", "sqmi_blosum62", "sqmi_status_string", "sqmi_last_error", "sqmi_abi_version",
     "sqmi_workload_create", "sqmi_workload_sizes", "sqmi_workload_copy", "sqmi_workload_destroy",
 )
 
@@ -92,6 +97,9 @@ def lib():
         L.sqmi_align_counts.argtypes = [vp, ctypes.POINTER(_Sequences), ctypes.POINTER(_Sequences),
                                         ctypes.POINTER(_Scoring), ctypes.c_uint32, vp, vp, ctypes.c_uint32]
         L.sqmi_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+        L.sqmi_plan_stats.argtypes = [ctypes.POINTER(_Sequences), ctypes.POINTER(_Sequences),
+                                      ctypes.POINTER(_Scoring), ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.POINTER(Stats)]
         L.sqmi_blosum62.restype = i32p
         L.sqmi_status_string.restype = ctypes.c_char_p
         L.sqmi_status_string.argtypes = [ctypes.c_int]
@@ -177,6 +185,19 @@ class SeqSet:
             off[1:] = np.cumsum([len(c) for c in codes])
         res = np.concatenate(codes) if codes and off[-1] > 0 else np.zeros(0, np.uint8)
         return cls(np.ascontiguousarray(res, dtype=np.uint8), off)
+
+
+def plan_stats(set_h: SeqSet, set_v: SeqSet, score: Score, mode=0, flags=0):
+    """Host-only planner dry run (no device): routing and work-item counts."""
+    ch, keep_h = _seqs(set_h)
+    cv, keep_v = _seqs(set_v)
+    sc = score._c()
+    st = Stats()
+    rc = lib().sqmi_plan_stats(ctypes.byref(ch), ctypes.byref(cv), ctypes.byref(sc), mode, flags,
+                               ctypes.byref(st))
+    if rc:
+        raise SqmiError(rc, "sqmi_plan_stats")
+    return st.as_dict()
 
 
 def workload(config, seed, first, count, length_override=0):

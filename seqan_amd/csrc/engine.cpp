@@ -95,11 +95,14 @@ struct ScoringInfo
 
 // Conservative range of every value a pair can produce inside the DP (H, E, F
 // and H + gapOpen), from the sizes alone.  H <= max(0, sMax) * min(n, m) since
-// gap costs are <= 0; H >= the cost of the all-gap path, 2*o + (n+m)*min(o,e).
+// gap costs are <= 0.  H(i,j) >= the score of the all-gap path along row 0
+// and column j, 2*o + (i+j-2)*max(o,e) (a gap chain grows by max(o,e) per
+// character); E, F and H + o are at least H + o; the diagonal term at least
+// H + sMin.
 bool fitsInt16(const ScoringInfo &si, int64_t n, int64_t m, bool local)
 {
     const int64_t up = (int64_t)std::max(0, si.sMax) * std::min(n, m);
-    const int64_t lowH = local ? 0 : 2LL * si.o + (n + m) * (int64_t)std::min(si.o, si.e);
+    const int64_t lowH = local ? 0 : 2LL * si.o + (n + m) * (int64_t)std::min(std::max(si.o, si.e), 0);
     const int64_t low = lowH + std::min({si.o, si.sMin, 0}) + std::min(si.e, 0);
     return up <= 32767 - kGuard16 && low >= (int64_t)kNeg16 + kGuard16 && n <= 30000 && m <= 30000;
 }
@@ -115,21 +118,25 @@ struct Plan16
 int pairsPerWave(const Variant16 &v) { return 2 * (64 / v.G); }
 
 // Kernel shape for a pair with m rows (V) and n columns (H).
-Variant16 chooseVariant(bool dna, uint32_t track, int64_t n, int64_t m)
+Variant16 chooseVariant(bool dna, uint32_t track, int64_t n, int64_t m, uint32_t flags)
 {
     (void)n;
+    const int forcedG = (int)((flags >> 8) & 0xFF), forcedR = (int)((flags >> 16) & 0xFF);
+    if (forcedG && forcedR)
+        return Variant16{forcedG, forcedR, dna, track};
     Variant16 v{16, 16, dna, track};
     if (m + 1 <= 128)
         v.R = 8;
     return v;
 }
 
-// One pair per launch: one wavefront sweeps the pair with all 64 lanes.
+// One pair per launch: one wavefront, whose first 16-lane group sweeps the
+// pair (the other pair slots of the wavefront stay empty).
 Variant16 plainVariant(bool dna, uint32_t track, int64_t m)
 {
-    Variant16 v{64, 16, dna, track};
-    if (m + 1 <= 256)
-        v.R = 4;
+    Variant16 v{16, 16, dna, track};
+    if (m + 1 <= 128)
+        v.R = 8;
     return v;
 }
 
@@ -155,6 +162,165 @@ double msSince(std::chrono::steady_clock::time_point t0)
 {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
+
+struct Key
+{
+    uint64_t key;
+    int32_t id;
+};
+
+struct Planned
+{
+    ScoringInfo si;
+    bool local = false;
+    uint32_t track = 0;
+    std::vector<int32_t> emptyIds, ids32;
+    std::vector<int64_t> scratchOff32;
+    int64_t scratch32 = 0;
+    std::vector<Key> keys16;        // int16 pairs, sorted by bucket key
+    std::vector<Plan16> plans;      // one per kernel variant
+    int64_t cells = 0;
+};
+
+// Routing (empty / int16 / int32) and length bucketing into work items.
+// Pure host code: needs no device.
+void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &scoring, uint32_t mode,
+          uint32_t flags, Planned &P)
+{
+    const int64_t N = H.count;
+    ScoringInfo &si = P.si;
+    si.alpha = scoring.alphabet_size;
+    si.o = scoring.gap_open;
+    si.e = scoring.gap_extend;
+    si.sMax = INT_MIN;
+    si.sMin = INT_MAX;
+    for (int k = 0; k < si.alpha * si.alpha; ++k) {
+        si.sMax = std::max(si.sMax, scoring.table[k]);
+        si.sMin = std::min(si.sMin, scoring.table[k]);
+    }
+    const bool local = (mode & SQMI_LOCAL) != 0;
+    P.local = local;
+    const bool costsOk16 = si.o <= 0 && si.e <= 0 && si.o >= -1024 && si.e >= -1024 && si.sMax <= 1024 &&
+                           si.sMin >= -1024 && !(flags & SQMI_FORCE_INT32);
+    si.dna = si.alpha <= 4 && !local && si.sMin - si.o >= 0 && si.sMax - si.o <= 255;
+    uint32_t track;
+    if (local)
+        track = kTrackLocal;
+    else {
+        track = 0;
+        if (mode & SQMI_FREE_BOTTOM)
+            track |= kTrackBottom;
+        if (mode & SQMI_FREE_RIGHT)
+            track |= kTrackRight;
+        if (!track)
+            track = kTrackEnd;
+    }
+    P.track = track;
+    std::vector<Key> &keys16 = P.keys16;
+    keys16.reserve(N);
+    for (int64_t i = 0; i < N; ++i) {
+        const int64_t n = H.offsets[i + 1] - H.offsets[i];
+        const int64_t m = V.offsets[i + 1] - V.offsets[i];
+        if (n == 0 || m == 0) {
+            P.emptyIds.push_back((int32_t)i);
+            continue;
+        }
+        P.cells += n * m;
+        if (costsOk16 && fitsInt16(si, n, m, local)) {
+            const Variant16 v = chooseVariant(si.dna, track, n, m, flags);
+            const int64_t BR = (int64_t)v.G * v.R;
+            const uint64_t nblocks = (uint64_t)((m + 1 + BR - 1) / BR);
+            // key: variant (G, R), row blocks, then longest H first.
+            const uint64_t vkey = (uint64_t)(v.G * 64 + v.R);
+            const uint64_t key =
+                (vkey << 48) | (nblocks << 24) | (uint64_t)(0xFFFFFF - std::min<int64_t>(n, 0xFFFFFF));
+            keys16.push_back({key, (int32_t)i});
+        } else {
+            P.ids32.push_back((int32_t)i);
+            P.scratchOff32.push_back(P.scratch32);
+            P.scratch32 += m + 1;
+        }
+    }
+    std::sort(keys16.begin(), keys16.end(),
+              [](const Key &a, const Key &b) { return a.key < b.key || (a.key == b.key && a.id < b.id); });
+    std::vector<Plan16> &plans = P.plans;
+    for (size_t k = 0; k < keys16.size();) {
+        const int32_t id0 = keys16[k].id;
+        const int64_t n0 = H.offsets[id0 + 1] - H.offsets[id0];
+        const int64_t m0 = V.offsets[id0 + 1] - V.offsets[id0];
+        const Variant16 v = chooseVariant(si.dna, track, n0, m0, flags);
+        const uint64_t vkey = keys16[k].key >> 48;
+        if (plans.empty() || (uint64_t)(plans.back().var.G * 64 + plans.back().var.R) != vkey) {
+            plans.push_back(Plan16{});
+            plans.back().var = v;
+        }
+        Plan16 &pl = plans.back();
+        const int PP = pairsPerWave(v);
+        const uint64_t kb = (keys16[k].key >> 24) & 0xFFFFFF;
+        size_t e = k;
+        WorkItem w{(int32_t)pl.order.size(), 0, 0, (int32_t)kb};
+        while (e < keys16.size() && (int)(e - k) < PP && (keys16[e].key >> 24) == (keys16[k].key >> 24)) {
+            const int32_t id = keys16[e].id;
+            pl.order.push_back(id);
+            w.maxN = std::max<int32_t>(w.maxN, (int32_t)(H.offsets[id + 1] - H.offsets[id]));
+            ++e;
+        }
+        w.count = (int32_t)(e - k);
+        pl.maxN = std::max(pl.maxN, w.maxN);
+        pl.items.push_back(w);
+        k = e;
+    }
+    // Longest work first (by estimated cells) for the queue's tail balance.
+    for (Plan16 &pl : plans)
+        std::stable_sort(pl.items.begin(), pl.items.end(), [](const WorkItem &a, const WorkItem &b) {
+            return (int64_t)a.maxN * a.nblocks > (int64_t)b.maxN * b.nblocks;
+        });
+}
+
+void fillStats(const Planned &P, sqmi_stats &st)
+{
+    st.pairs = (int64_t)(P.emptyIds.size() + P.ids32.size() + P.keys16.size());
+    st.cells = P.cells;
+    st.pairs_int32 = (int64_t)P.ids32.size();
+    st.pairs_int16 = (int64_t)P.keys16.size();
+    int64_t items = 0, padded = 0;
+    for (const Plan16 &pl : P.plans) {
+        items += (int64_t)pl.items.size();
+        for (const WorkItem &w : pl.items)
+            padded += 2LL * (64 / pl.var.G) * (int64_t)w.maxN * w.nblocks * pl.var.G * pl.var.R;
+    }
+    st.work_items = items;
+    st.padded_cells = padded;
+}
+
+sqmi_status validate(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
+                     const sqmi_scoring *scoring, uint32_t &mode)
+{
+    sqmi_status st;
+    if ((st = checkSet(ctx, setH, "set_h")) != SQMI_OK || (st = checkSet(ctx, setV, "set_v")) != SQMI_OK)
+        return st;
+    if (setH->count != setV->count)
+        return fail(ctx, SQMI_ERR_LENGTH_MISMATCH, "set_h and set_v differ in length");
+    if (!scoring || !scoring->table || scoring->alphabet_size < 1 || scoring->alphabet_size > 32)
+        return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "scoring: alphabet_size must be 1..32 with a table");
+    if (mode & ~31u)
+        return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "mode: unknown bits");
+    if (mode & SQMI_LOCAL)
+        mode = SQMI_LOCAL;   // localAlignmentScore has no AlignConfig
+    return SQMI_OK;
+}
+
+sqmi_status checkForced(sqmi_context *ctx, uint32_t flags)
+{
+    const int g = (int)((flags >> 8) & 0xFF), r = (int)((flags >> 16) & 0xFF);
+    if (!g && !r)
+        return SQMI_OK;
+    for (bool dna : {true, false})
+        if (!variantSupported(Variant16{g, r, dna, kTrackEnd}))
+            return fail(ctx, SQMI_ERR_UNSUPPORTED, "forced kernel variant is not compiled");
+    return SQMI_OK;
+}
+
 
 }  // namespace
 
@@ -251,16 +417,8 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
     ctx->lastError.clear();
     ctx->stats = sqmi_stats{};
     sqmi_status st;
-    if ((st = checkSet(ctx, setH, "set_h")) != SQMI_OK || (st = checkSet(ctx, setV, "set_v")) != SQMI_OK)
+    if ((st = validate(ctx, setH, setV, scoring, mode)) != SQMI_OK || (st = checkForced(ctx, flags)) != SQMI_OK)
         return st;
-    if (setH->count != setV->count)
-        return fail(ctx, SQMI_ERR_LENGTH_MISMATCH, "set_h and set_v differ in length");
-    if (!scoring || !scoring->table || scoring->alphabet_size < 1 || scoring->alphabet_size > 32)
-        return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "scoring: alphabet_size must be 1..32 with a table");
-    if (mode & ~31u)
-        return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "mode: unknown bits");
-    if (mode & SQMI_LOCAL)
-        mode = SQMI_LOCAL;   // localAlignmentScore has no AlignConfig
     const int64_t N = setH->count;
     if (N == 0)
         return SQMI_OK;
@@ -270,106 +428,18 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
         return fail(ctx, SQMI_ERR_DEVICE, "hipSetDevice failed");
     hipStream_t stream = ctx->stream;
 
-    ScoringInfo si;
-    si.alpha = scoring->alphabet_size;
-    si.o = scoring->gap_open;
-    si.e = scoring->gap_extend;
-    si.sMax = INT_MIN;
-    si.sMin = INT_MAX;
-    for (int k = 0; k < si.alpha * si.alpha; ++k) {
-        si.sMax = std::max(si.sMax, scoring->table[k]);
-        si.sMin = std::min(si.sMin, scoring->table[k]);
-    }
-    const bool local = (mode & SQMI_LOCAL) != 0;
-    const bool costsOk16 = si.o <= 0 && si.e <= 0 && si.o >= -1024 && si.e >= -1024 && si.sMax <= 1024 &&
-                           si.sMin >= -1024 && !(flags & SQMI_FORCE_INT32);
-    si.dna = si.alpha <= 4 && !local && si.sMin - si.o >= 0 && si.sMax - si.o <= 255;
-    uint32_t track;
-    if (local)
-        track = kTrackLocal;
-    else {
-        track = 0;
-        if (mode & SQMI_FREE_BOTTOM)
-            track |= kTrackBottom;
-        if (mode & SQMI_FREE_RIGHT)
-            track |= kTrackRight;
-        if (!track)
-            track = kTrackEnd;
-    }
-
-    // ---- host planning ---------------------------------------------------
+    Planned P;
     const auto tPlan = std::chrono::steady_clock::now();
-    std::vector<int32_t> emptyIds, ids32;
-    std::vector<int64_t> scratchOff32;
-    struct Key
-    {
-        uint64_t key;
-        int32_t id;
-    };
-    std::vector<Key> keys16;
-    keys16.reserve(N);
-    int64_t cells = 0, scratch32 = 0;
-    for (int64_t i = 0; i < N; ++i) {
-        const int64_t n = setH->offsets[i + 1] - setH->offsets[i];
-        const int64_t m = setV->offsets[i + 1] - setV->offsets[i];
-        if (n == 0 || m == 0) {
-            emptyIds.push_back((int32_t)i);
-            continue;
-        }
-        cells += n * m;
-        if (costsOk16 && fitsInt16(si, n, m, local)) {
-            const Variant16 v = chooseVariant(si.dna, track, n, m);
-            const int64_t BR = (int64_t)v.G * v.R;
-            const uint64_t nblocks = (uint64_t)((m + 1 + BR - 1) / BR);
-            // key: variant (G, R), row blocks, then longest H first.
-            const uint64_t vkey = (uint64_t)(v.G * 64 + v.R);
-            const uint64_t key = (vkey << 48) | (nblocks << 24) | (uint64_t)(0xFFFFFF - std::min<int64_t>(n, 0xFFFFFF));
-            keys16.push_back({key, (int32_t)i});
-        } else {
-            ids32.push_back((int32_t)i);
-            scratchOff32.push_back(scratch32);
-            scratch32 += m + 1;
-        }
-    }
-    std::sort(keys16.begin(), keys16.end(),
-              [](const Key &a, const Key &b) { return a.key < b.key || (a.key == b.key && a.id < b.id); });
-    std::vector<Plan16> plans;
-    for (size_t k = 0; k < keys16.size();) {
-        const int32_t id0 = keys16[k].id;
-        const int64_t n0 = setH->offsets[id0 + 1] - setH->offsets[id0];
-        const int64_t m0 = setV->offsets[id0 + 1] - setV->offsets[id0];
-        const Variant16 v = chooseVariant(si.dna, track, n0, m0);
-        const uint64_t vkey = keys16[k].key >> 48;
-        if (plans.empty() || (uint64_t)(plans.back().var.G * 64 + plans.back().var.R) != vkey) {
-            plans.push_back(Plan16{});
-            plans.back().var = v;
-        }
-        Plan16 &pl = plans.back();
-        const int P = pairsPerWave(v);
-        const uint64_t kb = (keys16[k].key >> 24) & 0xFFFFFF;
-        size_t e = k;
-        WorkItem w{(int32_t)pl.order.size(), 0, 0, (int32_t)kb};
-        while (e < keys16.size() && (int)(e - k) < P && (keys16[e].key >> 24) == (keys16[k].key >> 24)) {
-            const int32_t id = keys16[e].id;
-            pl.order.push_back(id);
-            w.maxN = std::max<int32_t>(w.maxN, (int32_t)(setH->offsets[id + 1] - setH->offsets[id]));
-            ++e;
-        }
-        w.count = (int32_t)(e - k);
-        pl.maxN = std::max(pl.maxN, w.maxN);
-        pl.items.push_back(w);
-        k = e;
-    }
-    // Longest work first (by estimated cells) for the queue's tail balance.
-    for (Plan16 &pl : plans)
-        std::stable_sort(pl.items.begin(), pl.items.end(), [](const WorkItem &a, const WorkItem &b) {
-            return (int64_t)a.maxN * a.nblocks > (int64_t)b.maxN * b.nblocks;
-        });
+    plan(*setH, *setV, *scoring, mode, flags, P);
     ctx->stats.host_plan_ms = msSince(tPlan);
-    ctx->stats.pairs = N;
-    ctx->stats.cells = cells;
-    ctx->stats.pairs_int32 = (int64_t)ids32.size();
-    ctx->stats.pairs_int16 = (int64_t)keys16.size();
+    fillStats(P, ctx->stats);
+    const ScoringInfo &si = P.si;
+    std::vector<Plan16> &plans = P.plans;
+    std::vector<int32_t> &emptyIds = P.emptyIds, &ids32 = P.ids32;
+    std::vector<int64_t> &scratchOff32 = P.scratchOff32;
+    const int64_t scratch32 = P.scratch32;
+    const auto &keys16 = P.keys16;
+    const uint32_t track = P.track;
 
     // ---- device buffers --------------------------------------------------
     hipError_t err;
@@ -419,9 +489,13 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
         return hipFail(ctx, err, "event");
 
     // ---- plain form: one launch per pair (baseline for the batched form) --
+    // Host data staged by hipMemcpyAsync must outlive the copies: the plain
+    // form's lists live at function scope, like the planner's.
+    std::vector<int32_t> plainIds;
+    std::vector<WorkItem> plainItems;
     if (flags & SQMI_PLAIN_PER_PAIR) {
-        std::vector<int32_t> ids16;
-        std::vector<WorkItem> items;
+        std::vector<int32_t> &ids16 = plainIds;
+        std::vector<WorkItem> &items = plainItems;
         int maxN = 0;
         for (const Key &k : keys16) {
             const int32_t id = k.id;
@@ -437,7 +511,7 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
         if ((err = ensure(ctx->order, 4 * np)) != hipSuccess ||
             (err = ensure(ctx->items, sizeof(WorkItem) * np)) != hipSuccess ||
             (err = ensure(ctx->queue, 4 * np + 64)) != hipSuccess ||
-            (err = ensure(ctx->rowbuf, (size_t)(maxN + 66) * 2 * sizeof(uint32_t))) != hipSuccess)
+            (err = ensure(ctx->rowbuf, (size_t)4 * (maxN + 18) * 2 * sizeof(uint32_t))) != hipSuccess)
             return hipFail(ctx, err, "allocating plain work lists");
         if (np && ((err = hipMemcpyAsync(ctx->order.p, ids16.data(), 4 * np, hipMemcpyHostToDevice, stream)) !=
                        hipSuccess ||
@@ -458,7 +532,7 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
             a.nItems = 1;
             a.queue = static_cast<uint32_t *>(ctx->queue.p) + k;
             a.rowbuf = static_cast<uint32_t *>(ctx->rowbuf.p);
-            a.rowCols = maxN + 66;
+            a.rowCols = maxN + 18;
             a.scores = dScores;
             a.gapOpen = si.o;
             a.gapExtend = si.e;
@@ -523,8 +597,8 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
         if (plans.size() > 1 && (err = hipStreamSynchronize(stream)) != hipSuccess)
             return hipFail(ctx, err, "packed kernel");
     }
-    ctx->stats.work_items = nItems;
-    ctx->stats.padded_cells = padded;
+    (void)padded;
+    (void)nItems;
 
     // ---- int32 pairs -------------------------------------------------------
     if (!ids32.empty()) {
@@ -578,6 +652,23 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
     if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess)
         ctx->stats.device_ms = ms;
     ctx->stats.total_ms = msSince(tStart);
+    return SQMI_OK;
+}
+
+sqmi_status sqmi_plan_stats(const sqmi_sequences *setH, const sqmi_sequences *setV, const sqmi_scoring *scoring,
+                            uint32_t mode, uint32_t flags, sqmi_stats *out)
+{
+    if (!out)
+        return SQMI_ERR_INVALID_ARGUMENT;
+    *out = sqmi_stats{};
+    sqmi_status st;
+    if ((st = validate(nullptr, setH, setV, scoring, mode)) != SQMI_OK || (st = checkForced(nullptr, flags)) != SQMI_OK)
+        return st;
+    Planned P;
+    const auto t0 = std::chrono::steady_clock::now();
+    plan(*setH, *setV, *scoring, mode, flags, P);
+    out->host_plan_ms = msSince(t0);
+    fillStats(P, *out);
     return SQMI_OK;
 }
 

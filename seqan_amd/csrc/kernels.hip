@@ -22,8 +22,8 @@
 // (a "group") sweep one pair-couple; each lane owns R consecutive rows of a
 // block of G*R rows and the group walks the columns as a skewed wavefront:
 // at step s lane l computes column j = s - l, receiving the bottom cell of
-// lane l-1 (column j, one step old) through a DPP lane shift (row_shr:1 for
-// G = 16, wave_shr:1 for G = 64).  The first lane of the group reads the
+// lane l-1 (column j, one step old) through a lane shift (DPP row_shr:1 for
+// G = 16, ds_bpermute for G = 64).  The first lane of the group reads the
 // bottom row of the previous block from a global row buffer that the last
 // lane writes.  Rows are bottom-aligned: row m of every pair is the last row
 // of the last block, so the rows above a pair's row 0 are "virtual" (floored
@@ -62,7 +62,12 @@ __device__ __forceinline__ uint32_t shiftIn(uint32_t v, uint32_t first)
     } else if constexpr (G == 16) {
         return (uint32_t)__builtin_amdgcn_update_dpp((int)first, (int)v, 0x111, 0xf, 0xf, false);
     } else if constexpr (G == 64) {
-        return (uint32_t)__builtin_amdgcn_update_dpp((int)first, (int)v, 0x138, 0xf, 0xf, false);
+        // Whole-wave shift through the LDS crossbar (ds_bpermute).  The DPP
+        // wavefront shift (wave_shr:1) assembles for gfx950 but does not run
+        // there: kernels using it never completed.
+        const int lane = (int)(threadIdx.x & 63);
+        const uint32_t s = (uint32_t)__builtin_amdgcn_ds_bpermute(((lane + 63) & 63) << 2, (int)v);
+        return lane == 0 ? first : s;
     } else {
         static_assert(G == 2 || G == 4 || G == 8, "unsupported group size");
         const uint32_t s = (uint32_t)__builtin_amdgcn_update_dpp((int)first, (int)v, 0x111, 0xf, 0xf, false);
@@ -100,6 +105,7 @@ struct Couple {
 template <int G, int R, bool DNA, uint32_t TRK>
 struct Fwd16 {
     static constexpr int BR = G * R;
+    static constexpr int GROUPS = 64 / G;
 
     // One block of rows: column 0 init plus the skewed sweep over columns.
     template <bool HEAD, bool LAST>
@@ -256,7 +262,6 @@ struct Fwd16 {
 
     __device__ static void run(const FwdArgs &a)
     {
-        constexpr int GROUPS = 64 / G;
         __shared__ uint32_t sProf[8];
         __shared__ int16_t sT[33 * 33];
         if constexpr (DNA) {
@@ -283,6 +288,11 @@ struct Fwd16 {
             if (it >= a.nItems)
                 break;
             const WorkItem w = a.items[it];
+            // A malformed descriptor is skipped rather than allowed to run
+            // past the row buffer or loop without bound.
+            if (w.count < 1 || w.count > 2 * GROUPS || w.maxN < 1 || w.maxN + G + 2 > a.rowCols ||
+                w.nblocks < 1 || w.nblocks > (1 << 20) / BR + 1)
+                continue;
             Couple c;
 #pragma unroll
             for (int x = 0; x < 2; ++x) {
@@ -467,8 +477,12 @@ const void *kernelTrk(uint32_t track)
     }
 }
 
-// The compiled (G, R) family.
-#define SQMI_FOR_EACH_GR(X) X(16, 16) X(16, 8) X(64, 16) X(64, 4) X(1, 16)
+// The compiled (G, R) family.  G = 64 (one pair couple per wavefront, all
+// per-pair values wave-uniform) is not built: the compiler turns its
+// work-queue loop into a divergent-exit loop nest, and those kernels never
+// completed on gfx950, while the G = 16 and G = 1 builds get a uniform loop
+// exit and run correctly.
+#define SQMI_FOR_EACH_GR(X) X(16, 16) X(16, 8) X(1, 16)
 
 template <bool DNA>
 const void *kernelFor(const Variant16 &v)
