@@ -67,7 +67,35 @@ def cpu_baseline(H, V, score, count, gpu_scores):
             "parity_vs_gpu": ok}, ok
 
 
+def summarize_rocpd(paths):
+    """Prints per-kernel duration statistics and counter totals of rocprofv3
+    result databases (the rocpd SQLite output)."""
+    import sqlite3
+    for path in paths:
+        db = sqlite3.connect(path)
+        print(f"# {path}")
+        rows = db.execute("select name, count(*), sum(duration), avg(duration), min(duration), max(duration) "
+                          "from kernels group by name order by sum(duration) desc").fetchall()
+        print(f"{'calls':>6} {'total_ms':>10} {'avg_ms':>10} {'min_ms':>10} {'max_ms':>10}  kernel")
+        for name, n, tot, avg, mn, mx in rows:
+            print(f"{n:6d} {tot / 1e6:10.3f} {avg / 1e6:10.3f} {mn / 1e6:10.3f} {mx / 1e6:10.3f}  {name[:110]}")
+        try:
+            pm = db.execute("select k.name, p.counter_name, count(*), sum(p.counter_value) from pmc_events p "
+                            "join kernels k on k.dispatch_id = p.dispatch_id group by k.name, p.counter_name "
+                            "order by k.name").fetchall()
+        except sqlite3.Error:
+            pm = []
+        if pm:
+            print(f"{'dispatches':>10} {'counter':>18} {'sum':>18} {'per_dispatch':>18}  kernel")
+            for name, cname, n, tot in pm:
+                print(f"{n:10d} {cname:>18} {tot:18.1f} {tot / n:18.1f}  {name[:90]}")
+        print()
+
+
 def main():
+    if len(sys.argv) > 2 and sys.argv[1] == "--summarize-rocpd":
+        summarize_rocpd(sys.argv[2:])
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -76,6 +104,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=N_PAIRS // 8)
     ap.add_argument("--plain-sample", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="also copy this many MiB device-to-device (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -125,8 +155,20 @@ def main():
                 "note": "packed-int16 VALU element ops, 8 per algorithmic cell; kernel time from HIP events "
                         "on the library stream around the alignment kernels"}
 
+    if args.copy_calib:
+        nb = args.copy_calib << 20
+        src, dst = sa.DeviceBuffer(nb), sa.DeviceBuffer(nb)
+        sa.hip().hipMemcpy(dst.ptr, src.ptr, nb, 3)     # hipMemcpyDeviceToDevice
+        sa.synchronize()
+        log(f"copied {args.copy_calib} MiB device-to-device")
+
     # Plain form: one kernel launch per pair, same library, same GPU.
     ps = min(args.plain_sample, args.pairs)
+    if ps == 0:
+        print(json.dumps({"metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+                          "roofline": roofline, "note": "profiling run: plain and CPU legs skipped"}))
+        return
     idx = np.arange(0, args.pairs, args.pairs // ps)[:ps]
     hp = sa.SeqSet.from_list([H[i] for i in idx])
     vp = sa.SeqSet.from_list([V[i] for i in idx])

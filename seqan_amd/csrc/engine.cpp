@@ -165,7 +165,7 @@ double msSince(std::chrono::steady_clock::time_point t0)
 
 struct Key
 {
-    uint64_t key;
+    uint32_t key;
     int32_t id;
 };
 
@@ -216,8 +216,13 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
             track = kTrackEnd;
     }
     P.track = track;
+    // Route every pair and give each int16 pair a 32-bit bucket key:
+    // kernel shape (2 bits) | row blocks, most first (14 bits) | |H|, longest
+    // first (16 bits).  Sorting by it groups pairs of equal block count and
+    // similar width, and orders work items longest first for the queue.
     std::vector<Key> &keys16 = P.keys16;
     keys16.reserve(N);
+    std::vector<Variant16> shapes;   // distinct shapes, indexed by the key's top bits
     for (int64_t i = 0; i < N; ++i) {
         const int64_t n = H.offsets[i + 1] - H.offsets[i];
         const int64_t m = V.offsets[i + 1] - V.offsets[i];
@@ -228,12 +233,14 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
         P.cells += n * m;
         if (costsOk16 && fitsInt16(si, n, m, local)) {
             const Variant16 v = chooseVariant(si.dna, track, n, m, flags);
+            uint32_t vidx = 0;
+            while (vidx < shapes.size() && !(shapes[vidx].G == v.G && shapes[vidx].R == v.R))
+                ++vidx;
+            if (vidx == shapes.size())
+                shapes.push_back(v);
             const int64_t BR = (int64_t)v.G * v.R;
-            const uint64_t nblocks = (uint64_t)((m + 1 + BR - 1) / BR);
-            // key: variant (G, R), row blocks, then longest H first.
-            const uint64_t vkey = (uint64_t)(v.G * 64 + v.R);
-            const uint64_t key =
-                (vkey << 48) | (nblocks << 24) | (uint64_t)(0xFFFFFF - std::min<int64_t>(n, 0xFFFFFF));
+            const uint32_t nblocks = (uint32_t)((m + 1 + BR - 1) / BR);
+            const uint32_t key = (vidx << 30) | ((0x3FFFu - nblocks) << 16) | (0xFFFFu - (uint32_t)n);
             keys16.push_back({key, (int32_t)i});
         } else {
             P.ids32.push_back((int32_t)i);
@@ -241,28 +248,42 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
             P.scratch32 += m + 1;
         }
     }
-    std::sort(keys16.begin(), keys16.end(),
-              [](const Key &a, const Key &b) { return a.key < b.key || (a.key == b.key && a.id < b.id); });
+    // LSD radix sort, two stable 16-bit passes (ties keep input order).
+    {
+        std::vector<Key> tmp(keys16.size());
+        std::vector<uint32_t> hist(1u << 16);
+        for (int pass = 0; pass < 2; ++pass) {
+            const int shift = 16 * pass;
+            std::fill(hist.begin(), hist.end(), 0u);
+            for (const Key &k : keys16)
+                ++hist[(k.key >> shift) & 0xFFFFu];
+            uint32_t run = 0;
+            for (uint32_t &h : hist) {
+                const uint32_t c = h;
+                h = run;
+                run += c;
+            }
+            for (const Key &k : keys16)
+                tmp[hist[(k.key >> shift) & 0xFFFFu]++] = k;
+            keys16.swap(tmp);
+        }
+    }
     std::vector<Plan16> &plans = P.plans;
     for (size_t k = 0; k < keys16.size();) {
-        const int32_t id0 = keys16[k].id;
-        const int64_t n0 = H.offsets[id0 + 1] - H.offsets[id0];
-        const int64_t m0 = V.offsets[id0 + 1] - V.offsets[id0];
-        const Variant16 v = chooseVariant(si.dna, track, n0, m0, flags);
-        const uint64_t vkey = keys16[k].key >> 48;
-        if (plans.empty() || (uint64_t)(plans.back().var.G * 64 + plans.back().var.R) != vkey) {
+        const uint32_t vidx = keys16[k].key >> 30;
+        const Variant16 v = shapes[vidx];
+        if (plans.empty() || !(plans.back().var.G == v.G && plans.back().var.R == v.R)) {
             plans.push_back(Plan16{});
             plans.back().var = v;
         }
         Plan16 &pl = plans.back();
         const int PP = pairsPerWave(v);
-        const uint64_t kb = (keys16[k].key >> 24) & 0xFFFFFF;
+        const uint32_t bucket = keys16[k].key >> 16;   // shape and row blocks
+        WorkItem w{(int32_t)pl.order.size(), 0, 0, (int32_t)(0x3FFFu - (bucket & 0x3FFFu))};
         size_t e = k;
-        WorkItem w{(int32_t)pl.order.size(), 0, 0, (int32_t)kb};
-        while (e < keys16.size() && (int)(e - k) < PP && (keys16[e].key >> 24) == (keys16[k].key >> 24)) {
-            const int32_t id = keys16[e].id;
-            pl.order.push_back(id);
-            w.maxN = std::max<int32_t>(w.maxN, (int32_t)(H.offsets[id + 1] - H.offsets[id]));
+        while (e < keys16.size() && (int)(e - k) < PP && (keys16[e].key >> 16) == bucket) {
+            pl.order.push_back(keys16[e].id);
+            w.maxN = std::max<int32_t>(w.maxN, (int32_t)(0xFFFFu - (keys16[e].key & 0xFFFFu)));
             ++e;
         }
         w.count = (int32_t)(e - k);
@@ -270,11 +291,6 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
         pl.items.push_back(w);
         k = e;
     }
-    // Longest work first (by estimated cells) for the queue's tail balance.
-    for (Plan16 &pl : plans)
-        std::stable_sort(pl.items.begin(), pl.items.end(), [](const WorkItem &a, const WorkItem &b) {
-            return (int64_t)a.maxN * a.nblocks > (int64_t)b.maxN * b.nblocks;
-        });
 }
 
 void fillStats(const Planned &P, sqmi_stats &st)

@@ -166,8 +166,10 @@ struct Fwd16 {
         uint32_t botG = asu(Gk[R - 1]), botF = asu(neg2);
         s16x2 upGprev = neg2;
         const s16x2 remBase = pack(c.n[0] + l, c.n[1] + l);
-        const int steps = Nw + G;
-        for (int s = 0; s < steps; ++s) {
+
+        // One column of the lane's R rows.  pA / pB: the column's profile
+        // (DNA: t bytes for v = 0..3) or table row offset, for pair A / B.
+        auto step = [&](int s, uint32_t pA, uint32_t pB) {
             const int j = s - l;
             const s16x2 upG = as16(shiftIn<G>(botG, (uint32_t)rbv));
             const s16x2 upF = as16(shiftIn<G>(botF, (uint32_t)(rbv >> 32)));
@@ -177,31 +179,15 @@ struct Fwd16 {
             const s16x2 diagTop = upGprev;
             upGprev = upG;
             if (j >= 1 && j <= Nw) {
-                int hc[2];
-#pragma unroll
-                for (int x = 0; x < 2; ++x) {
-                    const int idx = min(j, max(c.n[x], 1)) - 1;
-                    const int cc = min((int)c.h[x][idx], PAD);
-                    hc[x] = (j <= c.n[x]) ? cc : PAD;
-                }
-                uint32_t profA = 0, profB = 0;
-                int baseA = 0, baseB = 0;
-                if constexpr (DNA) {
-                    profA = sProf[hc[0]];
-                    profB = sProf[hc[1]];
-                } else {
-                    baseA = hc[0] * TW;
-                    baseB = hc[1] * TW;
-                }
                 s16x2 diag = diagTop, uG = upG, uF = upF;
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
                     uint32_t t;
                     if constexpr (DNA) {
-                        t = __builtin_amdgcn_perm(profB, profA, sel[k]);
+                        t = __builtin_amdgcn_perm(pB, pA, sel[k]);
                     } else {
-                        const uint32_t lo = (uint16_t)sT[baseA + (int)(sel[k] & 0xffffu)];
-                        const uint32_t hi = (uint16_t)sT[baseB + (int)(sel[k] >> 16)];
+                        const uint32_t lo = (uint16_t)sT[(int)pA + (int)(sel[k] & 0xffffu)];
+                        const uint32_t hi = (uint16_t)sT[(int)pB + (int)(sel[k] >> 16)];
                         t = lo | (hi << 16);
                     }
                     const s16x2 D = diag + as16(t);
@@ -252,6 +238,42 @@ struct Fwd16 {
                 }
                 if (!LAST && l == G - 1)
                     rbStore(rowbuf + 2 * j, ((uint64_t)botF << 32) | botG);
+            }
+        };
+
+        // Residue of pair x at column jj (padding outside 1..n); the address
+        // is clamped so the load is always in bounds.
+        auto hcode = [&](int x, int jj) -> int {
+            const int idx = min(max(jj, 1), max(c.n[x], 1)) - 1;
+            const int cc = min((int)c.h[x][idx], PAD);
+            return (jj >= 1 && jj <= c.n[x]) ? cc : PAD;
+        };
+        auto prof = [&](int code) -> uint32_t {
+            if constexpr (DNA)
+                return sProf[code];
+            else
+                return (uint32_t)(code * TW);
+        };
+        // Residues are loaded QD steps ahead of their use and the profile of
+        // the next step is read one step ahead, so neither latency sits on
+        // the critical path of a step.
+        constexpr int QD = 4;
+        uint32_t q[QD];   // residue codes of pair A (bits 0-7) and B (bits 8-15)
+#pragma unroll
+        for (int u = 0; u < QD; ++u)
+            q[u] = (uint32_t)hcode(0, u - l) | ((uint32_t)hcode(1, u - l) << 8);
+        uint32_t pA = prof(q[0] & 0xff), pB = prof(q[0] >> 8);
+        const int steps = ((Nw + G + QD - 1) / QD) * QD;
+        for (int s0 = 0; s0 < steps; s0 += QD) {
+#pragma unroll
+            for (int u = 0; u < QD; ++u) {
+                const int s = s0 + u;
+                const uint32_t qn = q[(u + 1) % QD];
+                const uint32_t nA = prof(qn & 0xff), nB = prof(qn >> 8);
+                q[u] = (uint32_t)hcode(0, s + QD - l) | ((uint32_t)hcode(1, s + QD - l) << 8);
+                step(s, pA, pB);
+                pA = nA;
+                pB = nB;
             }
         }
         // Make this block's row-buffer writes visible to the next block.
@@ -482,7 +504,7 @@ const void *kernelTrk(uint32_t track)
 // work-queue loop into a divergent-exit loop nest, and those kernels never
 // completed on gfx950, while the G = 16 and G = 1 builds get a uniform loop
 // exit and run correctly.
-#define SQMI_FOR_EACH_GR(X) X(16, 16) X(16, 8) X(1, 16)
+#define SQMI_FOR_EACH_GR(X) X(16, 16) X(16, 8)
 
 template <bool DNA>
 const void *kernelFor(const Variant16 &v)
