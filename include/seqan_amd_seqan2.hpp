@@ -1,0 +1,257 @@
+/*
+ * seqan_amd_seqan2.hpp -- drop-in overloads for SeqAn 2 callers.
+ *
+ * Including this header after (or instead of) <seqan/align_parallel.h> routes
+ *
+ *   globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial>, setH, setV, score [, AlignConfig<T,L,R,B>])
+ *   localAlignmentScore(ExecutionPolicy<Parallel, Vectorial>, setH, setV, score)
+ *
+ * to the MI355X library (libseqan_amd.so, C ABI in seqan_amd.h) whenever
+ *   - setH and setV are StringSets over one of Dna, Dna5, Rna, Rna5, AminoAcid,
+ *     both over the same alphabet, and
+ *   - score is Score<int, Simple> or Score<int, ScoreMatrix<A, Spec>> with A
+ *     the sets' alphabet (Blosum62 and the other built-in matrices included).
+ * Every other combination (another policy, another alphabet, a non-int score,
+ * a banded call, an algorithm tag) resolves to the reference templates as
+ * before.  The results are the reference's, value for value.
+ *
+ * The reference entry points these overloads take precedence over:
+ *   globalAlignmentScore(ExecutionPolicy<TPar,TVec> const &, TArgs && ...)
+ *       include/seqan/align_parallel/parallel_align_interface.h:332-343
+ *   localAlignmentScore(ExecutionPolicy<TPar,TVec> const &, TArgs && ...)
+ *       include/seqan/align_parallel/parallel_align_interface.h:354-365
+ * Both take forwarding references; so do the overloads below, which makes the
+ * conversion sequences of the two candidates identical for const, non-const
+ * and rvalue arguments alike.  Partial ordering then prefers these overloads
+ * because their first parameter, ExecutionPolicy<Parallel, Vectorial>, is
+ * more specialised than ExecutionPolicy<TPar, TVec>.
+ *
+ * Differences a caller can observe:
+ *   - errors throw seqan_amd::Error (the reference asserts in debug builds
+ *     and is undefined otherwise, e.g. on |setH| != |setV|,
+ *     parallel_align_interface.h:111);
+ *   - numThreads(ex) is not used; each calling thread owns one library
+ *     context (device seqan_amd::device(), stream and scratch), created on its
+ *     first call and destroyed when the thread ends.
+ *
+ * Needs C++17.  Link with -lseqan_amd.  Define SEQAN_AMD_NO_SEQAN2_OVERLOADS to keep the
+ * helpers of namespace seqan_amd without the overloads.  Define
+ * SEQAN_AMD_SEQAN2_PROBE to declare the overloads deleted instead: a call
+ * that resolves to one of them is then ill-formed, which lets a test detect
+ * at compile time which candidate overload resolution picks.
+ */
+#ifndef SEQAN_AMD_SEQAN2_HPP
+#define SEQAN_AMD_SEQAN2_HPP
+
+#include <seqan/align.h>
+#include <seqan/align_parallel.h>
+#include <seqan/score.h>
+#include <seqan/sequence.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "seqan_amd.h"
+
+namespace seqan_amd {
+
+/* Raised for every status other than SQMI_OK. */
+class Error : public std::runtime_error
+{
+public:
+    Error(sqmi_status status, std::string const & what) : std::runtime_error(what), status_(status) {}
+    sqmi_status status() const noexcept { return status_; }
+
+private:
+    sqmi_status status_;
+};
+
+/* HIP device used by contexts created after the call (default 0). */
+inline int & device_slot() { static int dev = 0; return dev; }
+inline int device() { return device_slot(); }
+inline void set_device(int dev) { device_slot() = dev; }
+
+namespace detail {
+
+inline void check(sqmi_status st, sqmi_context const * ctx)
+{
+    if (st == SQMI_OK)
+        return;
+    std::string msg = std::string("seqan_amd: ") + sqmi_status_string(st);
+    if (ctx != nullptr && sqmi_last_error(ctx)[0] != '\0')
+        msg += std::string(": ") + sqmi_last_error(ctx);
+    throw Error(st, msg);
+}
+
+struct ThreadContext
+{
+    sqmi_context * ctx = nullptr;
+    ~ThreadContext() { if (ctx != nullptr) sqmi_destroy(ctx); }
+};
+
+inline sqmi_context * thread_context()
+{
+    thread_local ThreadContext holder;
+    if (holder.ctx == nullptr)
+        check(sqmi_create(device(), &holder.ctx), nullptr);
+    return holder.ctx;
+}
+
+/* Alphabets the library packs: one byte per residue, value == ordinal. */
+template <typename TAlphabet> struct IsGpuAlphabet : std::false_type {};
+template <> struct IsGpuAlphabet<seqan2::Dna> : std::true_type {};
+template <> struct IsGpuAlphabet<seqan2::Dna5> : std::true_type {};
+template <> struct IsGpuAlphabet<seqan2::Rna> : std::true_type {};
+template <> struct IsGpuAlphabet<seqan2::Rna5> : std::true_type {};
+template <> struct IsGpuAlphabet<seqan2::AminoAcid> : std::true_type {};
+
+/* Alphabet of a StringSet<String<A, ...>, ...>; void for anything else. */
+template <typename T> struct SetAlphabet { using Type = void; };
+template <typename TString, typename TSpec>
+struct SetAlphabet<seqan2::StringSet<TString, TSpec>>
+{
+    using Type = typename seqan2::Value<TString>::Type;
+};
+
+/* Score schemes the library evaluates exactly like the reference. */
+template <typename TScore, typename TAlphabet> struct IsGpuScore : std::false_type {};
+template <typename TAlphabet>
+struct IsGpuScore<seqan2::Score<int, seqan2::Simple>, TAlphabet> : std::true_type {};
+template <typename TAlphabet, typename TSpec>
+struct IsGpuScore<seqan2::Score<int, seqan2::ScoreMatrix<TAlphabet, TSpec>>, TAlphabet> : std::true_type {};
+
+template <typename T> struct IsAlignConfig : std::false_type {};
+template <bool TOP, bool LEFT, bool RIGHT, bool BOTTOM, typename TSpec>
+struct IsAlignConfig<seqan2::AlignConfig<TOP, LEFT, RIGHT, BOTTOM, TSpec>> : std::true_type
+{
+    static constexpr uint32_t mode = (TOP ? SQMI_FREE_TOP : 0u) | (LEFT ? SQMI_FREE_LEFT : 0u) |
+                                     (RIGHT ? SQMI_FREE_RIGHT : 0u) | (BOTTOM ? SQMI_FREE_BOTTOM : 0u);
+};
+
+template <typename TSetH, typename TSetV, typename TScore>
+struct IsGpuCall
+{
+    using AlphaH = typename SetAlphabet<std::decay_t<TSetH>>::Type;
+    using AlphaV = typename SetAlphabet<std::decay_t<TSetV>>::Type;
+    static constexpr bool value = std::is_same<AlphaH, AlphaV>::value && IsGpuAlphabet<AlphaH>::value &&
+                                  IsGpuScore<std::decay_t<TScore>, AlphaH>::value;
+};
+
+/* Concatenated residue ordinals and count+1 offsets of a StringSet. */
+template <typename TSet>
+void pack(TSet const & set, std::vector<uint8_t> & residues, std::vector<int64_t> & offsets)
+{
+    using TAlphabet = typename SetAlphabet<TSet>::Type;
+    int64_t const count = static_cast<int64_t>(seqan2::length(set));
+    offsets.resize(count + 1);
+    offsets[0] = 0;
+    for (int64_t i = 0; i < count; ++i)
+        offsets[i + 1] = offsets[i] + static_cast<int64_t>(seqan2::length(set[i]));
+    residues.resize(static_cast<size_t>(offsets[count]));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int64_t i = 0; i < count; ++i)
+    {
+        auto const & seq = set[i];
+        uint8_t * out = residues.data() + offsets[i];
+        int64_t const len = offsets[i + 1] - offsets[i];
+        using TIter = typename seqan2::Iterator<std::decay_t<decltype(seq)> const, seqan2::Standard>::Type;
+        if constexpr (std::is_pointer<TIter>::value && sizeof(TAlphabet) == 1)
+        {
+            if (len > 0)
+                std::memcpy(out, &*seqan2::begin(seq, seqan2::Standard()), static_cast<size_t>(len));
+        }
+        else
+        {
+            for (int64_t k = 0; k < len; ++k)
+                out[k] = static_cast<uint8_t>(seqan2::ordValue(static_cast<TAlphabet>(seq[k])));
+        }
+    }
+}
+
+/* Substitution table over the alphabet, row = H residue, column = V residue;
+ * entries are the reference's own score(scheme, h, v) values. */
+template <typename TAlphabet, typename TScore>
+sqmi_scoring make_scoring(TScore const & scheme, std::vector<int32_t> & table)
+{
+    int const alpha = static_cast<int>(seqan2::ValueSize<TAlphabet>::VALUE);
+    table.resize(static_cast<size_t>(alpha) * alpha);
+    for (int h = 0; h < alpha; ++h)
+        for (int v = 0; v < alpha; ++v)
+            table[h * alpha + v] = seqan2::score(scheme, TAlphabet(h), TAlphabet(v));
+    sqmi_scoring sc;
+    sc.alphabet_size = alpha;
+    sc.gap_open = seqan2::scoreGapOpen(scheme);
+    sc.gap_extend = seqan2::scoreGapExtend(scheme);
+    sc.table = table.data();
+    return sc;
+}
+
+template <typename TSetH, typename TSetV, typename TScore>
+seqan2::String<int> align_scores(TSetH const & setH, TSetV const & setV, TScore const & scheme, uint32_t mode)
+{
+    using TAlphabet = typename SetAlphabet<TSetH>::Type;
+    if (seqan2::length(setH) != seqan2::length(setV))
+        throw Error(SQMI_ERR_LENGTH_MISMATCH, "seqan_amd: setH and setV differ in length");
+    seqan2::String<int> result;
+    if (seqan2::length(setH) == 0)
+        return result;
+    std::vector<uint8_t> resH, resV;
+    std::vector<int64_t> offH, offV;
+    pack(setH, resH, offH);
+    pack(setV, resV, offV);
+    std::vector<int32_t> table;
+    sqmi_scoring const sc = make_scoring<TAlphabet>(scheme, table);
+    sqmi_sequences const h{resH.data(), offH.data(), static_cast<int64_t>(offH.size()) - 1};
+    sqmi_sequences const v{resV.data(), offV.data(), static_cast<int64_t>(offV.size()) - 1};
+    seqan2::resize(result, seqan2::length(setH));
+    sqmi_context * ctx = thread_context();
+    check(sqmi_align_score(ctx, &h, &v, &sc, mode, &result[0], 0u), ctx);
+    return result;
+}
+
+}  // namespace detail
+}  // namespace seqan_amd
+
+#ifndef SEQAN_AMD_NO_SEQAN2_OVERLOADS
+#ifdef SEQAN_AMD_SEQAN2_PROBE
+#define SEQAN_AMD_OVERLOAD_BODY(...) = delete;
+#else
+#define SEQAN_AMD_OVERLOAD_BODY(...) { return __VA_ARGS__; }
+#endif
+namespace seqan2 {
+
+template <typename TSetH, typename TSetV, typename TScore,
+          std::enable_if_t<seqan_amd::detail::IsGpuCall<TSetH, TSetV, TScore>::value, int> = 0>
+inline String<int>
+globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
+                     TSetH && setH, TSetV && setV, TScore && scoringScheme)
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores(setH, setV, scoringScheme, 0u))
+
+template <typename TSetH, typename TSetV, typename TScore, typename TConfig,
+          std::enable_if_t<seqan_amd::detail::IsGpuCall<TSetH, TSetV, TScore>::value &&
+                           seqan_amd::detail::IsAlignConfig<std::decay_t<TConfig>>::value, int> = 0>
+inline String<int>
+globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
+                     TSetH && setH, TSetV && setV, TScore && scoringScheme, TConfig && /*alignConfig*/)
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores(setH, setV, scoringScheme,
+                                                        seqan_amd::detail::IsAlignConfig<std::decay_t<TConfig>>::mode))
+
+template <typename TSetH, typename TSetV, typename TScore,
+          std::enable_if_t<seqan_amd::detail::IsGpuCall<TSetH, TSetV, TScore>::value, int> = 0>
+inline String<int>
+localAlignmentScore(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
+                    TSetH && setH, TSetV && setV, TScore && scoringScheme)
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores(setH, setV, scoringScheme, static_cast<uint32_t>(SQMI_LOCAL)))
+
+}  // namespace seqan2
+#undef SEQAN_AMD_OVERLOAD_BODY
+#endif  // SEQAN_AMD_NO_SEQAN2_OVERLOADS
+
+#endif  // SEQAN_AMD_SEQAN2_HPP

@@ -1,0 +1,147 @@
+// Runs the SeqAn 2 drop-in overloads (include/seqan_amd_seqan2.hpp) against
+// the reference's own CPU path in the same program.
+//
+// For each case it prints one line "case <name> <status>", where status is
+//   match     the overload ran on the GPU and equals the reference's scores,
+//   mismatch  it ran and differs (the first differing pair is printed),
+//   nodevice  the overload was taken and reported SQMI_ERR_NO_DEVICE,
+//   error     any other failure.
+// The reference side is the same call under ExecutionPolicy<Parallel, Serial>,
+// which the overloads leave to the reference templates
+// (parallel_align_interface.h:332-365).
+#include "seqan_amd_seqan2.hpp"
+
+#include <cstdio>
+#include <random>
+#include <string>
+
+using namespace seqan2;
+
+namespace {
+
+template <typename TAlphabet>
+void randomPairs(StringSet<String<TAlphabet>> & setH, StringSet<String<TAlphabet>> & setV, unsigned count,
+                 unsigned lo, unsigned hi, unsigned seed)
+{
+    std::mt19937 rng(seed);
+    unsigned const alpha = ValueSize<TAlphabet>::VALUE;
+    for (unsigned i = 0; i < count; ++i)
+    {
+        String<TAlphabet> h, v;
+        unsigned const n = lo + rng() % (hi - lo + 1);
+        for (unsigned k = 0; k < n; ++k)
+            appendValue(h, TAlphabet(rng() % alpha));
+        for (unsigned k = 0; k < n; ++k)
+        {
+            unsigned const r = rng() % 100;
+            if (r < 8)
+                appendValue(v, TAlphabet(rng() % alpha));
+            else if (r < 10)
+                continue;
+            else if (r < 12)
+            {
+                appendValue(v, TAlphabet(rng() % alpha));
+                appendValue(v, h[k]);
+            }
+            else
+                appendValue(v, h[k]);
+        }
+        if (i % 17 == 3)
+            clear(v);  // empty sequences score INT_MIN in both paths
+        appendValue(setH, h);
+        appendValue(setV, v);
+    }
+}
+
+template <typename TFn, typename TRef>
+void runCase(char const * name, TFn && gpu, TRef && ref)
+{
+    String<int> expect = ref();
+    try
+    {
+        String<int> got = gpu();
+        if (length(got) != length(expect))
+        {
+            std::printf("case %s mismatch length %zu vs %zu\n", name, (size_t)length(got), (size_t)length(expect));
+            return;
+        }
+        for (size_t i = 0; i < length(got); ++i)
+            if (got[i] != expect[i])
+            {
+                std::printf("case %s mismatch pair %zu: %d vs %d\n", name, i, got[i], expect[i]);
+                return;
+            }
+        std::printf("case %s match %zu\n", name, (size_t)length(got));
+    }
+    catch (seqan_amd::Error const & e)
+    {
+        std::printf("case %s %s (%s)\n", name, e.status() == SQMI_ERR_NO_DEVICE ? "nodevice" : "error", e.what());
+    }
+}
+
+}  // namespace
+
+int main()
+{
+    ExecutionPolicy<Parallel, Vectorial> ex;
+    ExecutionPolicy<Parallel, Serial> cpu;
+    setNumThreads(cpu, 4);
+
+    StringSet<String<Dna>> dh, dv;
+    randomPairs(dh, dv, 300, 1, 400, 7);
+    Score<int, Simple> dna(2, -3, -1, -5);
+    AlignConfig<true, false, false, true> semi;
+    AlignConfig<false, true, true, false> overlap;
+
+    runCase("dna_global_nonconst", [&] { return globalAlignmentScore(ex, dh, dv, dna); },
+            [&] { return globalAlignmentScore(cpu, dh, dv, dna); });
+    StringSet<String<Dna>> const & cdh = dh;
+    StringSet<String<Dna>> const & cdv = dv;
+    runCase("dna_semiglobal_const", [&] { return globalAlignmentScore(ex, cdh, cdv, dna, semi); },
+            [&] { return globalAlignmentScore(cpu, cdh, cdv, dna, semi); });
+    runCase("dna_overlap_rvalue",
+            [&] { return globalAlignmentScore(ex, StringSet<String<Dna>>(dh), StringSet<String<Dna>>(dv),
+                                              Score<int, Simple>(2, -3, -1, -5), AlignConfig<false, true, true, false>()); },
+            [&] { return globalAlignmentScore(cpu, dh, dv, dna, overlap); });
+    runCase("dna_local", [&] { return localAlignmentScore(ex, dh, dv, dna); },
+            [&] { return localAlignmentScore(cpu, dh, dv, dna); });
+
+    StringSet<String<Dna>, Owner<ConcatDirect<>>> ch, cv;
+    for (size_t i = 0; i < length(dh); ++i)
+    {
+        appendValue(ch, dh[i]);
+        appendValue(cv, dv[i]);
+    }
+    runCase("dna_concat_direct", [&] { return globalAlignmentScore(ex, ch, cv, dna, semi); },
+            [&] { return globalAlignmentScore(cpu, dh, dv, dna, semi); });
+
+    StringSet<String<Dna5>> nh, nv;
+    randomPairs(nh, nv, 200, 1, 300, 11);
+    Score<int, Simple> lin(1, -1, -1, -1);  // gapOpen == gapExtend: the reference's linear-gap dispatch
+    runCase("dna5_linear", [&] { return globalAlignmentScore(ex, nh, nv, lin); },
+            [&] { return globalAlignmentScore(cpu, nh, nv, lin); });
+
+    StringSet<String<AminoAcid>> ph, pv;
+    randomPairs(ph, pv, 200, 1, 500, 13);
+    Blosum62 b62(-1, -11);
+    runCase("protein_local_blosum62", [&] { return localAlignmentScore(ex, ph, pv, b62); },
+            [&] { return localAlignmentScore(cpu, ph, pv, b62); });
+    runCase("protein_global_blosum62", [&] { return globalAlignmentScore(ex, ph, pv, b62); },
+            [&] { return globalAlignmentScore(cpu, ph, pv, b62); });
+    runCase("protein_semiglobal_blosum62", [&] { return globalAlignmentScore(ex, ph, pv, b62, semi); },
+            [&] { return globalAlignmentScore(cpu, ph, pv, b62, semi); });
+
+    // |setH| != |setV| raises instead of asserting.
+    StringSet<String<Dna>> shorter = dv;
+    eraseBack(shorter);
+    try
+    {
+        globalAlignmentScore(ex, dh, shorter, dna);
+        std::printf("case length_mismatch error (no exception)\n");
+    }
+    catch (seqan_amd::Error const & e)
+    {
+        std::printf("case length_mismatch %s\n", e.status() == SQMI_ERR_LENGTH_MISMATCH ? "match 0" : "error");
+    }
+    return 0;
+}
