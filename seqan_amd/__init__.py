@@ -294,10 +294,24 @@ class Context:
         self._check(st, "sqmi_align_score")
         return out
 
-    def align_counts(self, set_h: SeqSet, set_v: SeqSet, score: Score, mode=0, flags=0):
-        ch, keep_h = _seqs(set_h)
-        cv, keep_v = _seqs(set_v)
+    def align_counts(self, set_h: SeqSet, set_v: SeqSet, score: Score, mode=0, flags=0,
+                     h_dev=None, v_dev=None, scores_dev=None, counts_dev=None):
+        """Scores and counts (alpha^2 substitution counts, gap opens, gap
+        extensions) of all pairs.  Device addresses as in align_score; with
+        scores_dev, counts_dev must be given too and nothing is returned."""
+        ch, keep_h = _seqs(set_h, h_dev)
+        cv, keep_v = _seqs(set_v, v_dev)
+        if h_dev is not None or v_dev is not None:
+            assert h_dev is not None and v_dev is not None
+            flags |= RESIDUES_ON_DEVICE
         sc = score._c()
+        if scores_dev is not None:
+            assert counts_dev is not None
+            flags |= SCORES_ON_DEVICE
+            st = self._L.sqmi_align_counts(self._h, ctypes.byref(ch), ctypes.byref(cv), ctypes.byref(sc),
+                                           mode, scores_dev, counts_dev, flags)
+            self._check(st, "sqmi_align_counts")
+            return None
         n = len(set_h.offsets) - 1
         scores = np.empty(n, np.int32)
         A = score.alphabet_size

@@ -16,6 +16,7 @@
 #include <chrono>
 #include <climits>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -44,6 +45,8 @@ struct sqmi_context
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf hRes, vRes, hOff, vOff, order, items, queue, rowbuf, scores, tTable, dnaProf, ids32, scratchOff32,
         scratch32, table32, emptyIds;
+    // Traceback (sqmi_align_counts).
+    DevBuf trace, traceOff, locs, endEF, counts;
 };
 
 namespace {
@@ -185,7 +188,7 @@ struct Planned
 // Routing (empty / int16 / int32) and length bucketing into work items.
 // Pure host code: needs no device.
 void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &scoring, uint32_t mode,
-          uint32_t flags, Planned &P)
+          uint32_t flags, Planned &P, bool traceShape = false)
 {
     const int64_t N = H.count;
     ScoringInfo &si = P.si;
@@ -232,7 +235,8 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
         }
         P.cells += n * m;
         if (costsOk16 && fitsInt16(si, n, m, local)) {
-            const Variant16 v = chooseVariant(si.dna, track, n, m, flags);
+            const Variant16 v = traceShape ? Variant16{kTraceLanes, kTraceRows / kTraceLanes, si.dna, track}
+                                           : chooseVariant(si.dna, track, n, m, flags);
             uint32_t vidx = 0;
             while (vidx < shapes.size() && !(shapes[vidx].G == v.G && shapes[vidx].R == v.R))
                 ++vidx;
@@ -245,7 +249,9 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
         } else {
             P.ids32.push_back((int32_t)i);
             P.scratchOff32.push_back(P.scratch32);
-            P.scratch32 += m + 1;
+            // score: H and E columns (int32 pairs); traceback: the columns
+            // plus one trace byte per cell, 16-byte aligned.
+            P.scratch32 += traceShape ? ((8 * (m + 1) + (n + 1) * (m + 1) + 15) & ~(int64_t)15) : m + 1;
         }
     }
     // LSD radix sort, two stable 16-bit passes (ties keep input order).
@@ -338,6 +344,88 @@ sqmi_status checkForced(sqmi_context *ctx, uint32_t flags)
 }
 
 
+// Bytes the traceback may hold in trace records or int32 trace scratch at
+// once: 8 GiB, or SQMI_TRACE_BUDGET_MB from the environment.
+int64_t traceBudget()
+{
+    const char *env = std::getenv("SQMI_TRACE_BUDGET_MB");
+    if (env && *env) {
+        const long long mb = std::atoll(env);
+        if (mb > 0)
+            return (int64_t)mb << 20;
+    }
+    return (int64_t)8 << 30;
+}
+
+// Device copies of what every entry point needs: residues (unless already on
+// the device), offsets, the output scores buffer (unless the caller's is on
+// the device) and the substitution tables: t(h, v) = s(h, v) - gapOpen with a
+// padding code (int16 kernels), its byte profile (DNA path) and the plain
+// table (int32 kernels).  The host staging data of the asynchronous copies
+// lives in the struct, which the caller keeps until its stream is synchronised.
+struct Inputs
+{
+    const uint8_t *dH = nullptr, *dV = nullptr;
+    int32_t *dScores = nullptr;
+    std::vector<int16_t> tTab;
+    uint32_t prof[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+sqmi_status uploadInputs(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
+                         const sqmi_scoring *scoring, const ScoringInfo &si, int32_t *scores, uint32_t flags,
+                         Inputs &in)
+{
+    hipStream_t stream = ctx->stream;
+    const int64_t N = setH->count;
+    hipError_t err;
+    const int64_t hTot = setH->offsets[N], vTot = setV->offsets[N];
+    const uint8_t *dH = setH->residues, *dV = setV->residues;
+    if (!(flags & SQMI_RESIDUES_ON_DEVICE)) {
+        if ((err = ensure(ctx->hRes, hTot + 64)) != hipSuccess || (err = ensure(ctx->vRes, vTot + 64)) != hipSuccess)
+            return hipFail(ctx, err, "allocating residues");
+        if ((err = hipMemcpyAsync(ctx->hRes.p, setH->residues, hTot, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+            (err = hipMemcpyAsync(ctx->vRes.p, setV->residues, vTot, hipMemcpyHostToDevice, stream)) != hipSuccess)
+            return hipFail(ctx, err, "copying residues");
+        dH = static_cast<const uint8_t *>(ctx->hRes.p);
+        dV = static_cast<const uint8_t *>(ctx->vRes.p);
+    }
+    if ((err = ensure(ctx->hOff, 8 * (N + 1))) != hipSuccess || (err = ensure(ctx->vOff, 8 * (N + 1))) != hipSuccess)
+        return hipFail(ctx, err, "allocating offsets");
+    if ((err = hipMemcpyAsync(ctx->hOff.p, setH->offsets, 8 * (N + 1), hipMemcpyHostToDevice, stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(ctx->vOff.p, setV->offsets, 8 * (N + 1), hipMemcpyHostToDevice, stream)) != hipSuccess)
+        return hipFail(ctx, err, "copying offsets");
+    int32_t *dScores = scores;
+    if (!(flags & SQMI_SCORES_ON_DEVICE)) {
+        if ((err = ensure(ctx->scores, 4 * N)) != hipSuccess)
+            return hipFail(ctx, err, "allocating scores");
+        dScores = static_cast<int32_t *>(ctx->scores.p);
+    }
+
+    // Substitution tables: t(h, v) = s(h, v) - gapOpen with a padding code.
+    const int A = si.alpha, TW = A + 1;
+    std::vector<int16_t> &tTab = in.tTab;
+    tTab.assign((size_t)TW * TW, -1024);
+    for (int h = 0; h < A; ++h)
+        for (int v = 0; v < A; ++v)
+            tTab[h * TW + v] = (int16_t)(scoring->table[h * A + v] - si.o);
+    uint32_t *prof = in.prof;
+    if (si.dna)
+        for (int h = 0; h < A; ++h)
+            for (int v = 0; v < A; ++v)
+                prof[h] |= (uint32_t)(scoring->table[h * A + v] - si.o) << (8 * v);
+    if ((err = ensure(ctx->tTable, tTab.size() * 2)) != hipSuccess || (err = ensure(ctx->dnaProf, 32)) != hipSuccess ||
+        (err = ensure(ctx->table32, 4 * A * A)) != hipSuccess)
+        return hipFail(ctx, err, "allocating tables");
+    if ((err = hipMemcpyAsync(ctx->tTable.p, tTab.data(), tTab.size() * 2, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(ctx->dnaProf.p, prof, 32, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(ctx->table32.p, scoring->table, 4 * A * A, hipMemcpyHostToDevice, stream)) != hipSuccess)
+        return hipFail(ctx, err, "copying tables");
+    in.dH = dH;
+    in.dV = dV;
+    in.dScores = dScores;
+    return SQMI_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -396,7 +484,8 @@ sqmi_status sqmi_destroy(sqmi_context *ctx)
     (void)hipStreamSynchronize(ctx->stream);
     for (DevBuf *b : {&ctx->hRes, &ctx->vRes, &ctx->hOff, &ctx->vOff, &ctx->order, &ctx->items, &ctx->queue,
                       &ctx->rowbuf, &ctx->scores, &ctx->tTable, &ctx->dnaProf, &ctx->ids32, &ctx->scratchOff32,
-                      &ctx->scratch32, &ctx->table32, &ctx->emptyIds})
+                      &ctx->scratch32, &ctx->table32, &ctx->emptyIds, &ctx->trace, &ctx->traceOff, &ctx->locs,
+                      &ctx->endEF, &ctx->counts})
         release(*b);
     if (ctx->ev0)
         (void)hipEventDestroy(ctx->ev0);
@@ -459,48 +548,12 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
 
     // ---- device buffers --------------------------------------------------
     hipError_t err;
-    const int64_t hTot = setH->offsets[N], vTot = setV->offsets[N];
-    const uint8_t *dH = setH->residues, *dV = setV->residues;
-    if (!(flags & SQMI_RESIDUES_ON_DEVICE)) {
-        if ((err = ensure(ctx->hRes, hTot + 64)) != hipSuccess || (err = ensure(ctx->vRes, vTot + 64)) != hipSuccess)
-            return hipFail(ctx, err, "allocating residues");
-        if ((err = hipMemcpyAsync(ctx->hRes.p, setH->residues, hTot, hipMemcpyHostToDevice, stream)) != hipSuccess ||
-            (err = hipMemcpyAsync(ctx->vRes.p, setV->residues, vTot, hipMemcpyHostToDevice, stream)) != hipSuccess)
-            return hipFail(ctx, err, "copying residues");
-        dH = static_cast<const uint8_t *>(ctx->hRes.p);
-        dV = static_cast<const uint8_t *>(ctx->vRes.p);
-    }
-    if ((err = ensure(ctx->hOff, 8 * (N + 1))) != hipSuccess || (err = ensure(ctx->vOff, 8 * (N + 1))) != hipSuccess)
-        return hipFail(ctx, err, "allocating offsets");
-    if ((err = hipMemcpyAsync(ctx->hOff.p, setH->offsets, 8 * (N + 1), hipMemcpyHostToDevice, stream)) != hipSuccess ||
-        (err = hipMemcpyAsync(ctx->vOff.p, setV->offsets, 8 * (N + 1), hipMemcpyHostToDevice, stream)) != hipSuccess)
-        return hipFail(ctx, err, "copying offsets");
-    int32_t *dScores = scores;
-    if (!(flags & SQMI_SCORES_ON_DEVICE)) {
-        if ((err = ensure(ctx->scores, 4 * N)) != hipSuccess)
-            return hipFail(ctx, err, "allocating scores");
-        dScores = static_cast<int32_t *>(ctx->scores.p);
-    }
-
-    // Substitution tables: t(h, v) = s(h, v) - gapOpen with a padding code.
-    const int A = si.alpha, TW = A + 1;
-    std::vector<int16_t> tTab((size_t)TW * TW, -1024);
-    for (int h = 0; h < A; ++h)
-        for (int v = 0; v < A; ++v)
-            tTab[h * TW + v] = (int16_t)(scoring->table[h * A + v] - si.o);
-    uint32_t prof[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (si.dna)
-        for (int h = 0; h < A; ++h)
-            for (int v = 0; v < A; ++v)
-                prof[h] |= (uint32_t)(scoring->table[h * A + v] - si.o) << (8 * v);
-    if ((err = ensure(ctx->tTable, tTab.size() * 2)) != hipSuccess || (err = ensure(ctx->dnaProf, 32)) != hipSuccess ||
-        (err = ensure(ctx->table32, 4 * A * A)) != hipSuccess)
-        return hipFail(ctx, err, "allocating tables");
-    if ((err = hipMemcpyAsync(ctx->tTable.p, tTab.data(), tTab.size() * 2, hipMemcpyHostToDevice, stream)) != hipSuccess ||
-        (err = hipMemcpyAsync(ctx->dnaProf.p, prof, 32, hipMemcpyHostToDevice, stream)) != hipSuccess ||
-        (err = hipMemcpyAsync(ctx->table32.p, scoring->table, 4 * A * A, hipMemcpyHostToDevice, stream)) != hipSuccess)
-        return hipFail(ctx, err, "copying tables");
-
+    Inputs in;
+    if ((st = uploadInputs(ctx, setH, setV, scoring, si, scores, flags, in)) != SQMI_OK)
+        return st;
+    const uint8_t *dH = in.dH, *dV = in.dV;
+    int32_t *dScores = in.dScores;
+    const int A = si.alpha;
     if ((err = hipEventRecord(ctx->ev0, stream)) != hipSuccess)
         return hipFail(ctx, err, "event");
 
@@ -688,10 +741,246 @@ sqmi_status sqmi_plan_stats(const sqmi_sequences *setH, const sqmi_sequences *se
     return SQMI_OK;
 }
 
-sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *, const sqmi_sequences *,
-                              const sqmi_scoring *, uint32_t, int32_t *, int32_t *, uint32_t)
+// Traceback: the packed forward pass writes 4 trace bits per cell into a
+// buffer of at most traceBudget() bytes, so the work items run in chunks;
+// after each chunk one thread per pair walks its path.  Pairs outside the
+// int16 window run the int32 kernel with one trace byte per cell, chunked by
+// the same budget.
+sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
+                              const sqmi_scoring *scoring, uint32_t mode, int32_t *scores, int32_t *counts,
+                              uint32_t flags)
 {
-    return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_counts: not available yet");
+    const auto tStart = std::chrono::steady_clock::now();
+    if (!ctx)
+        return SQMI_ERR_INVALID_ARGUMENT;
+    ctx->lastError.clear();
+    ctx->stats = sqmi_stats{};
+    sqmi_status st;
+    if ((st = validate(ctx, setH, setV, scoring, mode)) != SQMI_OK)
+        return st;
+    if (mode != 0)
+        return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_counts: global alignment (mode 0) only");
+    if (flags & (SQMI_PLAIN_PER_PAIR | 0x00FFFF00u))
+        return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_counts: no plain form or forced shapes");
+    const int64_t N = setH->count;
+    if (N == 0)
+        return SQMI_OK;
+    if (!scores || !counts)
+        return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "scores or counts is null");
+    if (hipSetDevice(ctx->device) != hipSuccess)
+        return fail(ctx, SQMI_ERR_DEVICE, "hipSetDevice failed");
+    hipStream_t stream = ctx->stream;
+
+    Planned P;
+    const auto tPlan = std::chrono::steady_clock::now();
+    plan(*setH, *setV, *scoring, mode, flags, P, true);
+    ctx->stats.host_plan_ms = msSince(tPlan);
+    fillStats(P, ctx->stats);
+    const ScoringInfo &si = P.si;
+    const int A = si.alpha, S = A * A + 2;
+    const int64_t budget = traceBudget();
+
+    // Host-side chunking of the int16 items and of the int32 pairs.
+    const Plan16 *pl = P.plans.empty() ? nullptr : &P.plans[0];
+    std::vector<int64_t> itemOff;           // per item: first record within its chunk (units of 64 records)
+    std::vector<size_t> chunkEnd;           // item index after each chunk
+    std::vector<TraceLoc> locs;             // per pair in plan order
+    int64_t traceRecs = 0;                  // largest chunk, in units of 64 records
+    if (pl) {
+        const int64_t recBytes = 64 * 16;
+        itemOff.resize(pl->items.size());
+        locs.resize(pl->order.size());
+        int64_t run = 0;
+        for (size_t k = 0; k < pl->items.size(); ++k) {
+            const WorkItem &w = pl->items[k];
+            const int steps = traceSteps(w.maxN);
+            const int64_t recs = (int64_t)w.nblocks * steps;
+            if (run > 0 && (run + recs) * recBytes > budget) {
+                chunkEnd.push_back(k);
+                run = 0;
+            }
+            itemOff[k] = run;
+            for (int x = 0; x < w.count; ++x) {
+                const int32_t pid = pl->order[w.first + x];
+                const int64_t m = setV->offsets[pid + 1] - setV->offsets[pid];
+                locs[w.first + x] = TraceLoc{run, steps, (int32_t)((int64_t)w.nblocks * kTraceRows - 1 - m),
+                                             (x / 2) * kTraceLanes, x % 2};
+            }
+            run += recs;
+            traceRecs = std::max(traceRecs, run);
+        }
+        chunkEnd.push_back(pl->items.size());
+    }
+    std::vector<size_t> chunk32End;         // int32 pairs per chunk
+    std::vector<int64_t> off32(P.ids32.size());
+    int64_t scratch32 = 0;
+    {
+        int64_t run = 0;
+        for (size_t k = 0; k < P.ids32.size(); ++k) {
+            const int64_t need = (k + 1 < P.ids32.size() ? P.scratchOff32[k + 1] : P.scratch32) - P.scratchOff32[k];
+            if (need > std::max<int64_t>(budget, (int64_t)4 << 30))
+                return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_counts: pair too large for the int32 traceback");
+            if (run > 0 && run + need > budget) {
+                chunk32End.push_back(k);
+                run = 0;
+            }
+            off32[k] = run;
+            run += need;
+            scratch32 = std::max(scratch32, run);
+        }
+        if (!P.ids32.empty())
+            chunk32End.push_back(P.ids32.size());
+    }
+
+    // ---- device buffers --------------------------------------------------
+    hipError_t err;
+    Inputs in;
+    if ((st = uploadInputs(ctx, setH, setV, scoring, si, scores, flags, in)) != SQMI_OK)
+        return st;
+    int32_t *dCounts = counts;
+    if (!(flags & SQMI_SCORES_ON_DEVICE)) {
+        if ((err = ensure(ctx->counts, (size_t)4 * S * N)) != hipSuccess)
+            return hipFail(ctx, err, "allocating counts");
+        dCounts = static_cast<int32_t *>(ctx->counts.p);
+    }
+    if ((err = hipMemsetAsync(dCounts, 0, (size_t)4 * S * N, stream)) != hipSuccess)
+        return hipFail(ctx, err, "clearing counts");
+    if ((err = hipEventRecord(ctx->ev0, stream)) != hipSuccess)
+        return hipFail(ctx, err, "event");
+
+    if (pl) {
+        const size_t nIt = pl->items.size(), np = pl->order.size();
+        const int wpb = wavesPerBlock16();
+        const int occ = occupancyTrace16(si.dna);
+        const int rowCols = pl->maxN + kTraceLanes + 2;
+        int64_t maxChunkItems = 0;
+        for (size_t c = 0, b = 0; c < chunkEnd.size(); b = chunkEnd[c], ++c)
+            maxChunkItems = std::max<int64_t>(maxChunkItems, (int64_t)(chunkEnd[c] - b));
+        const int grid = std::max(1, (int)std::min<int64_t>((int64_t)ctx->numCUs * occ, (maxChunkItems + wpb - 1) / wpb));
+        const size_t rbBytes = (size_t)grid * wpb * (64 / kTraceLanes) * rowCols * 2 * sizeof(uint32_t);
+        if ((err = ensure(ctx->order, 4 * np)) != hipSuccess ||
+            (err = ensure(ctx->items, sizeof(WorkItem) * nIt)) != hipSuccess ||
+            (err = ensure(ctx->traceOff, 8 * nIt)) != hipSuccess ||
+            (err = ensure(ctx->locs, sizeof(TraceLoc) * np)) != hipSuccess ||
+            (err = ensure(ctx->queue, 4 * chunkEnd.size() + 64)) != hipSuccess ||
+            (err = ensure(ctx->rowbuf, rbBytes)) != hipSuccess ||
+            (err = ensure(ctx->endEF, (size_t)8 * N)) != hipSuccess ||
+            (err = ensure(ctx->trace, (size_t)traceRecs * 64 * 16)) != hipSuccess)
+            return hipFail(ctx, err, "allocating traceback buffers");
+        if ((err = hipMemcpyAsync(ctx->order.p, pl->order.data(), 4 * np, hipMemcpyHostToDevice, stream)) !=
+                hipSuccess ||
+            (err = hipMemcpyAsync(ctx->items.p, pl->items.data(), sizeof(WorkItem) * nIt, hipMemcpyHostToDevice,
+                                  stream)) != hipSuccess ||
+            (err = hipMemcpyAsync(ctx->traceOff.p, itemOff.data(), 8 * nIt, hipMemcpyHostToDevice, stream)) !=
+                hipSuccess ||
+            (err = hipMemcpyAsync(ctx->locs.p, locs.data(), sizeof(TraceLoc) * np, hipMemcpyHostToDevice, stream)) !=
+                hipSuccess ||
+            (err = hipMemsetAsync(ctx->queue.p, 0, 4 * chunkEnd.size() + 64, stream)) != hipSuccess)
+            return hipFail(ctx, err, "uploading traceback work lists");
+        const WorkItem *dItems = static_cast<const WorkItem *>(ctx->items.p);
+        const int32_t *dOrder = static_cast<const int32_t *>(ctx->order.p);
+        size_t b = 0;
+        for (size_t c = 0; c < chunkEnd.size(); ++c) {
+            const size_t e = chunkEnd[c];
+            FwdArgs a{};
+            a.hRes = in.dH;
+            a.vRes = in.dV;
+            a.hOff = static_cast<const int64_t *>(ctx->hOff.p);
+            a.vOff = static_cast<const int64_t *>(ctx->vOff.p);
+            a.order = dOrder;
+            a.items = dItems + b;
+            a.nItems = (int32_t)(e - b);
+            a.queue = static_cast<uint32_t *>(ctx->queue.p) + c;
+            a.rowbuf = static_cast<uint32_t *>(ctx->rowbuf.p);
+            a.rowCols = rowCols;
+            a.scores = in.dScores;
+            a.gapOpen = si.o;
+            a.gapExtend = si.e;
+            a.mode = mode;
+            a.alpha = A;
+            a.tTable = static_cast<const int16_t *>(ctx->tTable.p);
+            a.dnaProfile = static_cast<const uint32_t *>(ctx->dnaProf.p);
+            a.trace = static_cast<uint4 *>(ctx->trace.p);
+            a.traceOff = static_cast<const int64_t *>(ctx->traceOff.p) + b;
+            a.endEF = static_cast<int32_t *>(ctx->endEF.p);
+            const int cg = std::max(1, std::min(grid, (int)((e - b + wpb - 1) / wpb)));
+            if ((err = launchFwdTrace16(si.dna, a, cg, stream)) != hipSuccess)
+                return hipFail(ctx, err, "launching trace kernel");
+            const int32_t p0 = pl->items[b].first;
+            const int32_t p1 = pl->items[e - 1].first + pl->items[e - 1].count;
+            WalkArgs wa{};
+            wa.hRes = in.dH;
+            wa.vRes = in.dV;
+            wa.hOff = a.hOff;
+            wa.vOff = a.vOff;
+            wa.ids = dOrder + p0;
+            wa.locs = static_cast<const TraceLoc *>(ctx->locs.p) + p0;
+            wa.nPairs = p1 - p0;
+            wa.trace = a.trace;
+            wa.scores = in.dScores;
+            wa.endEF = a.endEF;
+            wa.counts = dCounts;
+            wa.alpha = A;
+            if ((err = launchWalk16(wa, stream)) != hipSuccess)
+                return hipFail(ctx, err, "launching walk kernel");
+            b = e;
+        }
+    }
+    if (!P.ids32.empty()) {
+        const size_t n32 = P.ids32.size();
+        if ((err = ensure(ctx->ids32, 4 * n32)) != hipSuccess || (err = ensure(ctx->scratchOff32, 8 * n32)) != hipSuccess ||
+            (err = ensure(ctx->scratch32, (size_t)scratch32)) != hipSuccess)
+            return hipFail(ctx, err, "allocating int32 traceback buffers");
+        if ((err = hipMemcpyAsync(ctx->ids32.p, P.ids32.data(), 4 * n32, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+            (err = hipMemcpyAsync(ctx->scratchOff32.p, off32.data(), 8 * n32, hipMemcpyHostToDevice, stream)) !=
+                hipSuccess)
+            return hipFail(ctx, err, "uploading int32 traceback lists");
+        size_t b = 0;
+        for (size_t e : chunk32End) {
+            Counts32Args a{};
+            a.hRes = in.dH;
+            a.vRes = in.dV;
+            a.hOff = static_cast<const int64_t *>(ctx->hOff.p);
+            a.vOff = static_cast<const int64_t *>(ctx->vOff.p);
+            a.ids = static_cast<const int32_t *>(ctx->ids32.p) + b;
+            a.scratchOff = static_cast<const int64_t *>(ctx->scratchOff32.p) + b;
+            a.nIds = (int32_t)(e - b);
+            a.scratch = static_cast<uint8_t *>(ctx->scratch32.p);
+            a.scores = in.dScores;
+            a.counts = dCounts;
+            a.gapOpen = si.o;
+            a.gapExtend = si.e;
+            a.alpha = A;
+            a.table = static_cast<const int32_t *>(ctx->table32.p);
+            if ((err = launchCounts32(a, stream)) != hipSuccess)
+                return hipFail(ctx, err, "launching int32 traceback kernel");
+            b = e;
+        }
+    }
+    if (!P.emptyIds.empty()) {
+        if ((err = ensure(ctx->emptyIds, 4 * P.emptyIds.size())) != hipSuccess)
+            return hipFail(ctx, err, "allocating empty list");
+        if ((err = hipMemcpyAsync(ctx->emptyIds.p, P.emptyIds.data(), 4 * P.emptyIds.size(), hipMemcpyHostToDevice,
+                                  stream)) != hipSuccess)
+            return hipFail(ctx, err, "uploading empty list");
+        if ((err = launchFillScores(in.dScores, static_cast<const int32_t *>(ctx->emptyIds.p), (int)P.emptyIds.size(),
+                                    INT32_MIN, stream)) != hipSuccess)
+            return hipFail(ctx, err, "launching fill kernel");
+    }
+    if ((err = hipEventRecord(ctx->ev1, stream)) != hipSuccess)
+        return hipFail(ctx, err, "event");
+    if (!(flags & SQMI_SCORES_ON_DEVICE)) {
+        if ((err = hipMemcpyAsync(scores, in.dScores, 4 * N, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+            (err = hipMemcpyAsync(counts, dCounts, (size_t)4 * S * N, hipMemcpyDeviceToHost, stream)) != hipSuccess)
+            return hipFail(ctx, err, "copying results");
+    }
+    if ((err = hipStreamSynchronize(stream)) != hipSuccess)
+        return hipFail(ctx, err, "traceback kernels");
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess)
+        ctx->stats.device_ms = ms;
+    ctx->stats.total_ms = msSince(tStart);
+    return SQMI_OK;
 }
 
 }  // extern "C"

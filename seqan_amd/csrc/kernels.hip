@@ -36,6 +36,7 @@ namespace sqmi {
 namespace {
 
 typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ s16x2 as16(uint32_t x) { return __builtin_bit_cast(s16x2, x); }
 __device__ __forceinline__ uint32_t asu(s16x2 x) { return __builtin_bit_cast(uint32_t, x); }
@@ -51,6 +52,17 @@ __device__ __forceinline__ s16x2 pack(int lo, int hi)
     return r;
 }
 __device__ __forceinline__ uint32_t bfi(uint32_t mask, uint32_t a, uint32_t b) { return (mask & a) | (~mask & b); }
+// Saturating a - b per half: its sign bit is exactly (a < b), whatever the
+// magnitudes (v_pk_sub_i16 with clamp).
+__device__ __forceinline__ s16x2 subSat(s16x2 a, s16x2 b) { return __builtin_elementwise_sub_sat(a, b); }
+// Shifts the accumulator right by one bit per half and inserts the sign bits
+// of t at bits 15 and 31: after 16 rows, row k's bit sits at bit k (pair A)
+// and bit 16 + k (pair B).
+__device__ __forceinline__ uint32_t pushSign(uint32_t acc, s16x2 t)
+{
+    const u16x2 sh = __builtin_bit_cast(u16x2, acc) >> (unsigned short)1;
+    return __builtin_bit_cast(uint32_t, sh) | (asu(t) & 0x80008000u);
+}
 
 // Lane l of every group of G lanes receives v from lane l-1; lane 0 of the
 // group receives `first`.
@@ -102,16 +114,30 @@ struct Couple {
     int n[2], m[2], p[2], pid[2];
 };
 
-template <int G, int R, bool DNA, uint32_t TRK>
+// TB: also write the trace bits of every cell (global alignment only):
+// per lane and column step one 16-byte record of four dwords, each holding
+// one bit per row (bit k: pair A, bit 16 + k: pair B):
+//   x  E(i,j) opened a gap:    E(i,j-1) + e <  G(i,j-1)   (TraceBitMap_ HORIZONTAL_OPEN)
+//   y  F(i,j) opened a gap:    F(i-1,j) + e <  G(i-1,j)   (VERTICAL_OPEN)
+//   z  H(i,j) is not diagonal: D < max(E, F)              (not DIAGONAL)
+//   w  E beats F:              F < E                      (MAX_FROM_HORIZONTAL_MATRIX)
+// which is what dp_formula_affine.h:62-126 and _maxScore (dp_formula.h:165-201)
+// record, ties resolved the same way.  Records of block b, step s of a work
+// item sit at trace[(traceOff[item] + b * steps + s) * 64 + lane].
+template <int G, int R, bool DNA, uint32_t TRK, bool TB = false>
 struct Fwd16 {
     static constexpr int BR = G * R;
     static constexpr int GROUPS = 64 / G;
+    static constexpr int QD = 4;   // residue prefetch depth (steps)
+    static_assert(!TB || (R == 16 && TRK == kTrackEnd), "trace records hold 16 rows; global alignment only");
+
+    __host__ __device__ __forceinline__ static int stepsFor(int Nw) { return ((Nw + G + QD - 1) / QD) * QD; }
 
     // One block of rows: column 0 init plus the skewed sweep over columns.
     template <bool HEAD, bool LAST>
     __device__ __forceinline__ static void block(const FwdArgs &a, const Couple &c, int b, int Nw, int l,
                                                  uint32_t *rowbuf, const uint32_t *sProf, const int16_t *sT,
-                                                 s16x2 &best)
+                                                 s16x2 &best, uint4 *trItem, s16x2 &endE, s16x2 &endF)
     {
         const int o = a.gapOpen, e = a.gapExtend;
         const s16x2 o2 = splat(o), e2 = splat(e), neg2 = splat(kNeg16);
@@ -166,6 +192,10 @@ struct Fwd16 {
         uint32_t botG = asu(Gk[R - 1]), botF = asu(neg2);
         s16x2 upGprev = neg2;
         const s16x2 remBase = pack(c.n[0] + l, c.n[1] + l);
+        const int steps = stepsFor(Nw);
+        uint4 *trBlock = nullptr;
+        if constexpr (TB)
+            trBlock = trItem + (size_t)b * steps * 64;
 
         // One column of the lane's R rows.  pA / pB: the column's profile
         // (DNA: t bytes for v = 0..3) or table row offset, for pair A / B.
@@ -180,6 +210,7 @@ struct Fwd16 {
             upGprev = upG;
             if (j >= 1 && j <= Nw) {
                 s16x2 diag = diagTop, uG = upG, uF = upF;
+                uint32_t accE = 0, accF = 0, accD = 0, accH = 0;
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
                     uint32_t t;
@@ -191,9 +222,22 @@ struct Fwd16 {
                         t = lo | (hi << 16);
                     }
                     const s16x2 D = diag + as16(t);
-                    Ek[k] = vmax(Ek[k] + e2, Gk[k]);
-                    const s16x2 F = vmax(uF + e2, uG);
-                    s16x2 H = vmax(vmax(D, Ek[k]), F);
+                    s16x2 F, H;
+                    if constexpr (TB) {
+                        const s16x2 Eext = Ek[k] + e2, Fext = uF + e2;
+                        accE = pushSign(accE, subSat(Eext, Gk[k]));
+                        accF = pushSign(accF, subSat(Fext, uG));
+                        Ek[k] = vmax(Eext, Gk[k]);
+                        F = vmax(Fext, uG);
+                        const s16x2 fe = vmax(Ek[k], F);
+                        accD = pushSign(accD, subSat(D, fe));
+                        accH = pushSign(accH, subSat(F, Ek[k]));
+                        H = vmax(D, fe);
+                    } else {
+                        Ek[k] = vmax(Ek[k] + e2, Gk[k]);
+                        F = vmax(uF + e2, uG);
+                        H = vmax(vmax(D, Ek[k]), F);
+                    }
                     if constexpr (TRK & kTrackLocal) {
                         H = vmax(H, splat(0));
                     } else if constexpr (HEAD) {
@@ -222,8 +266,13 @@ struct Fwd16 {
                         const uint16_t r0 = (uint16_t)rem.x, r1 = (uint16_t)rem.y;
                         const uint32_t eq = (r0 == 0 ? 0xffffu : 0u) | (r1 == 0 ? 0xffff0000u : 0u);
                         if constexpr (LAST && (TRK & kTrackEnd)) {
-                            if (l == G - 1)
+                            if (l == G - 1) {
                                 best = as16(bfi(eq, asu(uG), asu(best)));
+                                if constexpr (TB) {
+                                    endE = as16(bfi(eq, asu(Ek[R - 1]), asu(endE)));
+                                    endF = as16(bfi(eq, asu(uF), asu(endF)));
+                                }
+                            }
                         }
                         if constexpr (TRK & kTrackRight) {
                             if (eq != 0) {
@@ -238,6 +287,8 @@ struct Fwd16 {
                 }
                 if (!LAST && l == G - 1)
                     rbStore(rowbuf + 2 * j, ((uint64_t)botF << 32) | botG);
+                if constexpr (TB)
+                    trBlock[(size_t)s * 64] = make_uint4(accE, accF, accD, accH);
             }
         };
 
@@ -257,13 +308,11 @@ struct Fwd16 {
         // Residues are loaded QD steps ahead of their use and the profile of
         // the next step is read one step ahead, so neither latency sits on
         // the critical path of a step.
-        constexpr int QD = 4;
         uint32_t q[QD];   // residue codes of pair A (bits 0-7) and B (bits 8-15)
 #pragma unroll
         for (int u = 0; u < QD; ++u)
             q[u] = (uint32_t)hcode(0, u - l) | ((uint32_t)hcode(1, u - l) << 8);
         uint32_t pA = prof(q[0] & 0xff), pB = prof(q[0] >> 8);
-        const int steps = ((Nw + G + QD - 1) / QD) * QD;
         for (int s0 = 0; s0 < steps; s0 += QD) {
 #pragma unroll
             for (int u = 0; u < QD; ++u) {
@@ -336,14 +385,32 @@ struct Fwd16 {
                 c.p[x] = w.nblocks * BR - 1 - m;
             }
             s16x2 best = (TRK & kTrackLocal) ? splat(a.gapOpen) : splat(kNeg16);
+            s16x2 endE = splat(kNeg16), endF = splat(kNeg16);
             const int Nw = w.maxN;
+            uint4 *trItem = nullptr;
+            if constexpr (TB)
+                trItem = a.trace + (size_t)a.traceOff[it] * 64 + lane;
             if (w.nblocks == 1) {
-                block<true, true>(a, c, 0, Nw, l, rowbuf, sProf, sT, best);
+                block<true, true>(a, c, 0, Nw, l, rowbuf, sProf, sT, best, trItem, endE, endF);
             } else {
-                block<true, false>(a, c, 0, Nw, l, rowbuf, sProf, sT, best);
+                block<true, false>(a, c, 0, Nw, l, rowbuf, sProf, sT, best, trItem, endE, endF);
                 for (int b = 1; b < w.nblocks - 1; ++b)
-                    block<false, false>(a, c, b, Nw, l, rowbuf, sProf, sT, best);
-                block<false, true>(a, c, w.nblocks - 1, Nw, l, rowbuf, sProf, sT, best);
+                    block<false, false>(a, c, b, Nw, l, rowbuf, sProf, sT, best, trItem, endE, endF);
+                block<false, true>(a, c, w.nblocks - 1, Nw, l, rowbuf, sProf, sT, best, trItem, endE, endF);
+            }
+            if constexpr (TB) {
+                // E and F of the end cell (m, n), for the end-cell rule of
+                // the traceback (dp_algorithm_impl.h:1190-1207).
+                if (l == G - 1) {
+                    if (c.pid[0] >= 0) {
+                        a.endEF[2 * c.pid[0]] = (int)endE.x;
+                        a.endEF[2 * c.pid[0] + 1] = (int)endF.x;
+                    }
+                    if (c.pid[1] >= 0) {
+                        a.endEF[2 * c.pid[1]] = (int)endE.y;
+                        a.endEF[2 * c.pid[1] + 1] = (int)endF.y;
+                    }
+                }
             }
             best = groupMax<G>(best);
             if (l == 0) {
@@ -360,6 +427,13 @@ template <int G, int R, bool DNA, uint32_t TRK>
 __global__ __launch_bounds__(256) void kFwd16(FwdArgs a)
 {
     Fwd16<G, R, DNA, TRK>::run(a);
+}
+
+// Forward pass with trace records (global alignment, G = 16, R = 16).
+template <bool DNA>
+__global__ __launch_bounds__(256) void kFwdTrace16(FwdArgs a)
+{
+    Fwd16<16, 16, DNA, kTrackEnd, true>::run(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -451,6 +525,199 @@ __global__ void kFillScores(int32_t *scores, const int32_t *ids, int n, int32_t 
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n)
         scores[ids[t]] = value;
+}
+
+// ---------------------------------------------------------------------------
+// Traceback.  The walk follows _computeTraceback / _doTracebackGoDiagonal /
+// _doTracebackGoVertical / _doTracebackGoHorizontal (dp_traceback_impl.h:
+// 115-546) over trace bits with the encoding of the trace records (bit 0: E
+// opened, 1: F opened, 2: not diagonal, 3: E beats F), and counts the
+// alignment the way computeAlignmentStats reads its rows
+// (evaluate_alignment.h:215-300): one substitution count per aligned pair
+// (index h * alpha + v), and per maximal run of gaps in one row one open and
+// run-1 extensions.  A run is maximal in either reading direction, so the
+// counts can be taken while walking backwards.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kBitEOpen = 1, kBitFOpen = 2, kBitNotDiag = 4, kBitHori = 8;
+
+__device__ __forceinline__ int dirOf(uint32_t bits)   // 0 diagonal, 1 vertical, 2 horizontal
+{
+    return (bits & kBitNotDiag) ? ((bits & kBitHori) ? 2 : 1) : 0;
+}
+
+template <class Bits>
+__device__ void walkCounts(int n, int m, const uint8_t *h, const uint8_t *v, int A, int score, int endE,
+                           int endF, Bits &bits, int32_t *cnt)
+{
+    int i = m, j = n;
+    int prev = 0;   // last emitted operation: 0 none / aligned pair, 1 gap in H (vertical), 2 gap in V
+    int opens = 0, exts = 0;
+    // End cell: F, then E, win ties against the recorded direction
+    // (_correctTraceValue, dp_algorithm_impl.h:1190-1207), and a gap start
+    // there follows the gap matrix (PreferGapsAtEnd_, dp_traceback_impl.h:115-122).
+    int dir;
+    {
+        const uint32_t b0 = bits(i, j);
+        dir = (endF == score) ? 1 : (endE == score) ? 2 : dirOf(b0);
+    }
+    while (i > 0 && j > 0) {
+        if (dir == 0) {
+            const int hc = min((int)h[j - 1], A - 1), vc = min((int)v[i - 1], A - 1);
+            cnt[hc * A + vc] += 1;
+            prev = 0;
+            --i;
+            --j;
+        } else if (dir == 1) {
+            for (;;) {
+                const bool open = (bits(i, j) & kBitFOpen) != 0;
+                if (prev == 1) ++exts; else ++opens;
+                prev = 1;
+                const bool stop = open || i == 1;
+                --i;
+                if (stop)
+                    break;
+            }
+        } else {
+            for (;;) {
+                const bool open = (bits(i, j) & kBitEOpen) != 0;
+                if (prev == 2) ++exts; else ++opens;
+                prev = 2;
+                const bool stop = open || j == 1;
+                --j;
+                if (stop)
+                    break;
+            }
+        }
+        if (i > 0 && j > 0)
+            dir = dirOf(bits(i, j));
+    }
+    // Leading gaps down to the origin: first the column, then the row
+    // (dp_traceback_impl.h:522-545).
+    if (i > 0) {
+        if (prev == 1) exts += i; else { ++opens; exts += i - 1; }
+        prev = 1;
+    }
+    if (j > 0) {
+        if (prev == 2) exts += j; else { ++opens; exts += j - 1; }
+    }
+    cnt[A * A] += opens;
+    cnt[A * A + 1] += exts;
+}
+
+// Trace bits of cell (i, j) of one pair from the packed records, with the
+// last record read kept (vertical moves inside a lane reuse it).
+struct Bits16 {
+    const uint4 *trace;
+    TraceLoc loc;
+    int64_t key = -1;
+    uint4 w = make_uint4(0, 0, 0, 0);
+    __device__ __forceinline__ uint32_t operator()(int i, int j)
+    {
+        const int r = loc.p + i;
+        const int b = r / kTraceRows, rr = r % kTraceRows, l = rr / 16, k = rr % 16;
+        const int64_t idx = (loc.rec0 + (int64_t)b * loc.steps + (j + l)) * 64 + loc.lane0 + l;
+        if (idx != key) {
+            w = trace[idx];
+            key = idx;
+        }
+        const int sh = k + 16 * loc.half;
+        return ((w.x >> sh) & 1u) | (((w.y >> sh) & 1u) << 1) | (((w.z >> sh) & 1u) << 2) | (((w.w >> sh) & 1u) << 3);
+    }
+};
+
+// Counts live in LDS while walking when alpha^2 + 2 <= kWalkLdsStride.
+constexpr int kWalkLdsStride = 51;
+
+__global__ __launch_bounds__(256) void kWalk16(WalkArgs a)
+{
+    extern __shared__ int32_t sCnt[];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nPairs)
+        return;
+    const int pid = a.ids[t];
+    const int A = a.alpha, S = A * A + 2;
+    const int64_t ho = a.hOff[pid], vo = a.vOff[pid];
+    const int n = (int)(a.hOff[pid + 1] - ho), m = (int)(a.vOff[pid + 1] - vo);
+    int32_t *out = a.counts + (int64_t)pid * S;
+    const bool lds = S <= kWalkLdsStride;
+    int32_t *cnt = lds ? sCnt + threadIdx.x * S : out;
+    for (int k = 0; k < S; ++k)
+        cnt[k] = 0;
+    Bits16 bits{a.trace, a.locs[t]};
+    walkCounts(n, m, a.hRes + ho, a.vRes + vo, A, a.scores[pid], a.endEF[2 * pid], a.endEF[2 * pid + 1], bits, cnt);
+    if (lds)
+        for (int k = 0; k < S; ++k)
+            out[k] = cnt[k];
+}
+
+// int32 forward with trace bytes (same encoding), column-major with column
+// stride m + 1, followed by the same walk.  For pairs outside the int16 window.
+__global__ __launch_bounds__(256) void kCounts32(Counts32Args a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nIds)
+        return;
+    const int pid = a.ids[t];
+    const int A = a.alpha, S = A * A + 2;
+    const int64_t ho = a.hOff[pid], vo = a.vOff[pid];
+    const int n = (int)(a.hOff[pid + 1] - ho), m = (int)(a.vOff[pid + 1] - vo);
+    const uint8_t *h = a.hRes + ho;
+    const uint8_t *v = a.vRes + vo;
+    uint8_t *base = a.scratch + a.scratchOff[t];
+    int32_t *H = reinterpret_cast<int32_t *>(base);
+    int32_t *E = H + (m + 1);
+    uint8_t *T = base + 8 * (int64_t)(m + 1);
+    const int64_t CS = m + 1;
+    const int32_t o = a.gapOpen, e = a.gapExtend;
+    int32_t *cnt = a.counts + (int64_t)pid * S;
+    for (int k = 0; k < S; ++k)
+        cnt[k] = 0;
+    {
+        int32_t F = kNeg32;
+        H[0] = 0;
+        E[0] = kNeg32;
+        for (int i = 1; i <= m; ++i) {
+            F = max(F + e, H[i - 1] + o);
+            H[i] = F;
+            E[i] = kNeg32;
+        }
+    }
+    int32_t lastF = kNeg32;
+    for (int j = 1; j <= n; ++j) {
+        const int hj = min((int)h[j - 1], A - 1);
+        const int32_t *row = a.table + hj * A;
+        int32_t diag = H[0];
+        int32_t F = kNeg32;
+        E[0] = max(E[0] + e, H[0] + o);
+        H[0] = E[0];
+        int32_t up = H[0];
+        uint8_t *Tc = T + (int64_t)j * CS;
+        for (int i = 1; i <= m; ++i) {
+            const int32_t d = diag + row[min((int)v[i - 1], A - 1)];
+            diag = H[i];
+            const int32_t extE = E[i] + e, opnE = H[i] + o;
+            uint32_t tb = extE < opnE ? kBitEOpen : 0u;
+            const int32_t ei = max(extE, opnE);
+            E[i] = ei;
+            const int32_t extF = F + e, opnF = up + o;
+            tb |= extF < opnF ? kBitFOpen : 0u;
+            F = max(extF, opnF);
+            const int32_t fe = F < ei ? ei : F;
+            int32_t x = d;
+            if (d < fe) {
+                x = fe;
+                tb |= kBitNotDiag | (F < ei ? kBitHori : 0u);
+            }
+            Tc[i] = (uint8_t)tb;
+            H[i] = x;
+            up = x;
+        }
+        lastF = F;
+    }
+    const int32_t score = H[m];
+    a.scores[pid] = score;
+    auto bits = [&](int i, int j) -> uint32_t { return T[(int64_t)j * CS + i]; };
+    walkCounts(n, m, h, v, A, score, E[m], lastF, bits, cnt);
 }
 
 template <int G, int R, bool DNA>
@@ -545,6 +812,44 @@ hipError_t launchFwd16(const Variant16 &v, const FwdArgs &args, int gridBlocks, 
     SQMI_FOR_EACH_GR(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+int traceSteps(int maxN) { return Fwd16<16, 16, true, kTrackEnd, true>::stepsFor(maxN); }
+
+hipError_t launchFwdTrace16(bool dna, const FwdArgs &args, int gridBlocks, hipStream_t stream)
+{
+    if (dna)
+        hipLaunchKernelGGL(kFwdTrace16<true>, dim3(gridBlocks), dim3(256), 0, stream, args);
+    else
+        hipLaunchKernelGGL(kFwdTrace16<false>, dim3(gridBlocks), dim3(256), 0, stream, args);
+    return hipGetLastError();
+}
+
+int occupancyTrace16(bool dna)
+{
+    int blocks = 0;
+    const void *k = dna ? (const void *)&kFwdTrace16<true> : (const void *)&kFwdTrace16<false>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 256, 0) != hipSuccess)
+        return 1;
+    return blocks > 0 ? blocks : 1;
+}
+
+hipError_t launchWalk16(const WalkArgs &args, hipStream_t stream)
+{
+    if (args.nPairs <= 0)
+        return hipSuccess;
+    const int S = args.alpha * args.alpha + 2;
+    const size_t lds = S <= kWalkLdsStride ? (size_t)256 * S * sizeof(int32_t) : 0;
+    hipLaunchKernelGGL(kWalk16, dim3((args.nPairs + 255) / 256), dim3(256), lds, stream, args);
+    return hipGetLastError();
+}
+
+hipError_t launchCounts32(const Counts32Args &args, hipStream_t stream)
+{
+    if (args.nIds <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kCounts32, dim3((args.nIds + 255) / 256), dim3(256), 0, stream, args);
+    return hipGetLastError();
 }
 
 hipError_t launchSimple32(const Simple32Args &args, hipStream_t stream)

@@ -48,6 +48,55 @@ struct FwdArgs {
     int32_t alpha;              // alphabet size; residue code `alpha` is padding
     const int16_t *tTable;      // (alpha+1)^2 entries: s(h,v) - gapOpen, pad row/col
     const uint32_t *dnaProfile; // 5 dwords: bytes v=0..3 of s(h,v)-gapOpen, h=0..3, pad
+    // Trace variant only:
+    uint4 *trace;               // 16-byte trace records (64 per step)
+    const int64_t *traceOff;    // per item: first record (units of 64 records)
+    int32_t *endEF;             // per pair id: E and F of cell (m, n)
+};
+
+// Where the trace records of one pair live: rows are bottom-aligned in the
+// item's blocks of 256 rows (row i is item row p + i), lane group lane0..+15,
+// int16 half `half` of every dword.
+struct TraceLoc {
+    int64_t rec0;               // first record of the item (units of 64 records)
+    int32_t steps;              // column steps per block
+    int32_t p;                  // item row of the pair's row 0
+    int32_t lane0;              // first lane of the pair's group
+    int32_t half;               // 0: low int16 half, 1: high
+};
+
+// Traceback walk: one thread per pair.
+struct WalkArgs {
+    const uint8_t *hRes;
+    const uint8_t *vRes;
+    const int64_t *hOff;
+    const int64_t *vOff;
+    const int32_t *ids;         // pair ids, one per thread
+    const TraceLoc *locs;       // per thread
+    int32_t nPairs;
+    const uint4 *trace;
+    const int32_t *scores;      // per pair id: H(m, n)
+    const int32_t *endEF;       // per pair id: E(m, n), F(m, n)
+    int32_t *counts;            // per pair id: alpha^2 + 2 counts
+    int32_t alpha;
+};
+
+// int32 traceback, one thread per pair, trace bytes in global scratch.
+struct Counts32Args {
+    const uint8_t *hRes;
+    const uint8_t *vRes;
+    const int64_t *hOff;
+    const int64_t *vOff;
+    const int32_t *ids;
+    const int64_t *scratchOff;  // per thread: byte offset of its scratch
+    int32_t nIds;
+    uint8_t *scratch;           // 8 (m+1) bytes of H/E columns + (n+1)(m+1) trace bytes per pair
+    int32_t *scores;
+    int32_t *counts;
+    int32_t gapOpen;
+    int32_t gapExtend;
+    int32_t alpha;
+    const int32_t *table;
 };
 
 // Simple int32 kernel: one thread per pair, H/E column in global scratch.
@@ -87,5 +136,14 @@ bool variantSupported(const Variant16 &v);
 hipError_t launchSimple32(const Simple32Args &args, hipStream_t stream);
 // scores[ids[k]] = value for k < n.
 hipError_t launchFillScores(int32_t *scores, const int32_t *ids, int n, int32_t value, hipStream_t stream);
+
+// Trace family.
+constexpr int kTraceRows = 256;       // rows per block of the trace kernel (G = 16, R = 16)
+constexpr int kTraceLanes = 16;
+int traceSteps(int maxN);             // column steps per block for an item of width maxN
+hipError_t launchFwdTrace16(bool dna, const FwdArgs &args, int gridBlocks, hipStream_t stream);
+int occupancyTrace16(bool dna);
+hipError_t launchWalk16(const WalkArgs &args, hipStream_t stream);
+hipError_t launchCounts32(const Counts32Args &args, hipStream_t stream);
 
 }  // namespace sqmi
