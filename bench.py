@@ -8,10 +8,14 @@ and 2% geometric indels, semi-global AlignConfig<true,false,false,true>,
 Score<int,Simple>(2,-3,-1,-5) -- scored by one call of the library with the
 residues already resident in HBM.  Prints one JSON line.
 
-Also measured in the same run: the plain one-pair-per-launch GPU form on a
-sample, and the reference SeqAn 2 CPU path (oracle/_ref, built from the
-reference sources) on a bounded sample using all host cores; the GPU scores of
-that sample are bit-checked against the reference's.
+Also measured in the same run, each reported beside the headline figure:
+  - the plain one-pair-per-launch GPU form on a sample of the same pairs;
+  - fwd+bwd: score plus traceback counts (sqmi_align_counts) of 65,536 DNA
+    pairs of 1 kbp, global, and the forward alone on the same pairs;
+  - protein local: 262,144 Smith-Waterman pairs, Blosum62(-1, -11);
+  - the reference SeqAn 2 CPU path (oracle/_ref, built from the reference
+    sources) on bounded samples of each workload, all host cores and one
+    core; the GPU results of each sample are bit-checked against it.
 """
 import argparse
 import json
@@ -30,9 +34,13 @@ METRIC = "GCUPS exact affine DP, batched vs plain on 1 GPU; fwd+bwd GCUPS; % VAL
 SEED = 0x5EED0002
 N_PAIRS = 1 << 20
 MODE = sa.FREE_TOP | sa.FREE_BOTTOM
+SEED_PROTEIN, N_PROTEIN = 0x5EED0003, 1 << 18
+SEED_DNA1K, N_DNA1K = 0x5EED0004, 1 << 16
 # Peak packed-int16 VALU element operations: 256 CU x 64 lanes x 2 halves x 2.4 GHz.
 PEAK_TOPS = 256 * 64 * 2 * 2.4e9 / 1e12
 OPS_PER_CELL = 8          # Hd+s, E+e, F+e, H+o, max E, max F, 2x max H
+OPS_PER_CELL_LOCAL = 9    # + max(H, 0)
+PROFILE = os.path.join(ROOT, "profiles", "fwd16_dna_lognormal_semiglobal.md")
 
 
 def log(*a):
@@ -46,25 +54,128 @@ def host_cores():
         return os.cpu_count() or 1
 
 
-def cpu_baseline(H, V, score, count, gpu_scores):
-    """Reference SeqAn 2 ExecutionPolicy<Parallel,Vectorial> on the first
-    `count` pairs (all host cores); returns (dict, parity_ok)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def head(S, count):
+    return sa.SeqSet(S.residues[:S.offsets[count]], S.offsets[:count + 1])
+
+
+def reference_run(op, H, V, count, mode, threads, repeats, gpu_scores, gpu_counts=None, **kw):
+    """The reference SeqAn 2 CPU path (ExecutionPolicy<Parallel,Vectorial>)
+    on the first `count` pairs; returns a baseline dict (GCUPS, parity with
+    the GPU results of the same pairs) or None when oracle/_ref is absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     if po.ref_binary() is None:
-        return None, None
-    hs = sa.SeqSet(H.residues[:H.offsets[count]], H.offsets[:count + 1])
-    vs = sa.SeqSet(V.residues[:V.offsets[count]], V.offsets[:count + 1])
-    cores = host_cores()
-    r = po.run_reference(po.OP_SCORE, hs.residues, hs.offsets, vs.residues, vs.offsets, MODE, alphabet="dna",
-                         match=2, mismatch=-3, gap_extend=-1, gap_open=-5, threads=cores, repeats=2)
+        return None
+    hs, vs = head(H, count), head(V, count)
+    r = po.run_reference(op, hs.residues, hs.offsets, vs.residues, vs.offsets, mode, threads=threads,
+                         repeats=repeats, **kw)
     cells = int((hs.lengths() * vs.lengths()).sum())
     ok = bool(np.array_equal(r["scores"], gpu_scores[:count]))
+    if gpu_counts is not None:
+        ok = ok and bool(np.array_equal(r["counts"], gpu_counts[:count]))
     return {"value": round(cells / r["best_seconds"] / 1e9, 3), "unit": "GCUPS", "cores": int(r["threads"]),
             "kind": "reference",
-            "sample": f"first {count} pairs ({cells:.3e} cells) of the same workload; "
-                      f"{r['binary']} ({r['simd_lanes']} int32 lanes), best of 2 timed calls",
-            "parity_vs_gpu": ok}, ok
+            "sample": f"first {count} pairs ({cells:.3e} cells) of the same workload; {r['binary']} "
+                      f"({r['simd_lanes']} int32 lanes), best of {repeats} timed call(s); {cpu_model()}",
+            "parity_vs_gpu": ok}
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None when absent)."""
+    try:
+        with open(PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def timed(fn, steps, warmup, label):
+    for _ in range(warmup):
+        fn()
+    sa.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        fn()
+    sa.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    log(f"{label}: {ms:.2f} ms/step")
+    return ms
+
+
+def backward_leg(ctx, steps, warmup, cpu):
+    """Score + traceback counts of the 1 kbp global workload, beside the
+    forward alone on the same pairs; inputs and outputs resident in HBM."""
+    H, V = sa.workload(sa.WL_DNA1K, SEED_DNA1K, 0, N_DNA1K)
+    cells = int((H.lengths() * V.lengths()).sum())
+    score = sa.simple_score(2, -3, -1, -5)
+    S = score.alphabet_size ** 2 + 2
+    dh, dv = sa.DeviceBuffer.from_array(H.residues), sa.DeviceBuffer.from_array(V.residues)
+    ds, dc = sa.DeviceBuffer(4 * N_DNA1K), sa.DeviceBuffer(4 * S * N_DNA1K)
+    dev = dict(h_dev=dh.address, v_dev=dv.address)
+    fwd_ms = timed(lambda: ctx.align_score(H, V, score, 0, scores_dev=ds.address, **dev), steps, warmup,
+                   "dna1k forward")
+    fwd_dev = ctx.stats()["device_ms"]
+    bwd_ms = timed(lambda: ctx.align_counts(H, V, score, 0, scores_dev=ds.address, counts_dev=dc.address, **dev),
+                   steps, warmup, "dna1k fwd+bwd")
+    st = ctx.stats()
+    scores = ds.to_array(np.int32, N_DNA1K)
+    counts = dc.to_array(np.int32, S * N_DNA1K).reshape(N_DNA1K, S)
+    par = np.concatenate([score.table.ravel(), [score.gap_open, score.gap_extend]]).astype(np.int64)
+    ident = bool(np.array_equal(counts.astype(np.int64) @ par, scores.astype(np.int64)))
+    res = {"value": round(cells / (bwd_ms / 1e3) / 1e9, 2), "unit": "GCUPS", "ms_per_step": round(bwd_ms, 3),
+           "forward_ms_per_step": round(fwd_ms, 3), "backward_over_forward": round(bwd_ms / fwd_ms, 3),
+           "device_ms": round(st["device_ms"], 3), "forward_device_ms": round(fwd_dev, 3),
+           "workload": "65,536 DNA pairs of 1 kbp, 10% subst, 2% indels, global, Score<int,Simple>(2,-3,-1,-5)",
+           "pairs": N_DNA1K, "cells": cells, "seed": SEED_DNA1K, "sum_count_param_eq_score": ident}
+    if cpu:
+        res["cpu_baseline"] = reference_run(1, H, V, 4096, 0, host_cores(), 1, scores, counts, alphabet="dna",
+                                            match=2, mismatch=-3, gap_extend=-1, gap_open=-5)
+        log(f"dna1k cpu baseline: {res['cpu_baseline']}")
+    for b in (dh, dv, ds, dc):
+        b.free()
+    return res
+
+
+def protein_leg(ctx, steps, warmup, cpu):
+    """Smith-Waterman scores of the protein workload (Blosum62(-1, -11))."""
+    H, V = sa.workload(sa.WL_PROTEIN, SEED_PROTEIN, 0, N_PROTEIN)
+    cells = int((H.lengths() * V.lengths()).sum())
+    score = sa.blosum62(-1, -11)
+    dh, dv = sa.DeviceBuffer.from_array(H.residues), sa.DeviceBuffer.from_array(V.residues)
+    ds = sa.DeviceBuffer(4 * N_PROTEIN)
+    ms = timed(lambda: ctx.align_score(H, V, score, sa.LOCAL, h_dev=dh.address, v_dev=dv.address,
+                                       scores_dev=ds.address), steps, warmup, "protein local")
+    st = ctx.stats()
+    scores = ds.to_array(np.int32, N_PROTEIN)
+    kern = st["device_ms"]
+    achieved = OPS_PER_CELL_LOCAL * cells / (kern / 1e3) / 1e12
+    res = {"value": round(cells / (ms / 1e3) / 1e9, 2), "unit": "GCUPS", "ms_per_step": round(ms, 3),
+           "device_ms": round(kern, 3), "roofline_frac": round(achieved / PEAK_TOPS, 4),
+           "workload": "262,144 protein pairs, query U[50,1000] aa, target U[50,2000] aa, 30% homologs, "
+                       "local, Blosum62(-1,-11)",
+           "pairs": N_PROTEIN, "cells": cells, "seed": SEED_PROTEIN, "pairs_int32": int(st["pairs_int32"])}
+    if cpu:
+        res["cpu_baseline"] = reference_run(0, H, V, 16384, sa.LOCAL, host_cores(), 1, scores, alphabet="aa",
+                                            matrix="blosum62", gap_extend=-1, gap_open=-11)
+        log(f"protein cpu baseline: {res['cpu_baseline']}")
+    for b in (dh, dv, ds):
+        b.free()
+    return res
 
 
 def summarize_rocpd(paths):
@@ -104,6 +215,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=N_PAIRS // 8)
     ap.add_argument("--plain-sample", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the fwd+bwd and protein legs")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also copy this many MiB device-to-device (counter calibration)")
     args = ap.parse_args()
@@ -151,9 +263,10 @@ def main():
     kern_ms = float(np.mean(dev_ms))
     achieved = OPS_PER_CELL * cells / (kern_ms / 1e3) / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_TOPS, 2), "unit": "TOP/s",
-                "frac": round(achieved / PEAK_TOPS, 4), "traffic": None,
+                "frac": round(achieved / PEAK_TOPS, 4), "traffic": profiled_traffic(),
                 "note": "packed-int16 VALU element ops, 8 per algorithmic cell; kernel time from HIP events "
-                        "on the library stream around the alignment kernels"}
+                        "on the library stream around the alignment kernels; traffic (HBM bytes per launch) "
+                        "from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/"}
 
     if args.copy_calib:
         nb = args.copy_calib << 20
@@ -191,8 +304,19 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu, ok = cpu_baseline(H, V, score, min(args.cpu_sample, args.pairs), scores)
+        count = min(args.cpu_sample, args.pairs)
+        cpu = reference_run(0, H, V, count, MODE, host_cores(), 2, scores, alphabet="dna", match=2, mismatch=-3,
+                            gap_extend=-1, gap_open=-5)
         log(f"cpu baseline: {cpu}")
+        if cpu is not None:
+            one = reference_run(0, H, V, count // 16, MODE, 1, 1, scores, alphabet="dna", match=2, mismatch=-3,
+                                gap_extend=-1, gap_open=-5)
+            cpu["one_core"] = {"value": one["value"], "sample": one["sample"], "parity_vs_gpu": one["parity_vs_gpu"]}
+            log(f"cpu baseline, one core: {one}")
+    extra = {}
+    if not args.no_extra:
+        extra["fwd_bwd"] = backward_leg(ctx, 3, 1, not args.no_cpu)
+        extra["protein_local"] = protein_leg(ctx, 3, 1, not args.no_cpu)
 
     res = {
         "metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
@@ -208,6 +332,7 @@ def main():
         "host_plan_ms": round(float(np.mean(plan_ms)), 2),
         "pairs_int32": int(st["pairs_int32"]),
     }
+    res.update(extra)
     print(json.dumps(res))
 
 
