@@ -357,6 +357,16 @@ int64_t traceBudget()
     return (int64_t)8 << 30;
 }
 
+// Work items with at least this many row blocks are swept by the four waves
+// of a workgroup together (8, or SQMI_COOP_MIN_BLOCKS; 0 disables).
+int coopMinBlocks()
+{
+    const char *env = std::getenv("SQMI_COOP_MIN_BLOCKS");
+    if (env && *env)
+        return std::max(0, std::atoi(env));
+    return 8;
+}
+
 // Device copies of what every entry point needs: residues (unless already on
 // the device), offsets, the output scores buffer (unless the caller's is on
 // the device) and the substitution tables: t(h, v) = s(h, v) - gapOpen with a
@@ -648,6 +658,16 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
         a.items = static_cast<const WorkItem *>(ctx->items.p);
         a.nItems = (int32_t)nIt;
         a.queue = static_cast<uint32_t *>(ctx->queue.p);
+        // Items are ordered by row blocks, most first: the leading ones with
+        // at least coopMinBlocks() blocks are shared by a workgroup's waves.
+        {
+            const int minBlocks = coopMinBlocks();
+            int32_t nLong = 0;
+            while (minBlocks > 0 && nLong < (int32_t)nIt && pl.items[nLong].nblocks >= minBlocks)
+                ++nLong;
+            a.nLong = nLong;
+            a.queueLong = static_cast<uint32_t *>(ctx->queue.p) + 1;
+        }
         a.rowbuf = static_cast<uint32_t *>(ctx->rowbuf.p);
         a.rowCols = rowCols;
         a.scores = dScores;

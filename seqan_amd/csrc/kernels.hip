@@ -107,6 +107,33 @@ __device__ __forceinline__ void rbStore(uint32_t *p, uint64_t v)
     __hip_atomic_store(reinterpret_cast<uint64_t *>(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Progress words of the hand-off between the waves of a workgroup: block b
+// has made columns 0..col of its bottom row available.  Tags of later blocks
+// compare greater, so one word serves a slot's successive blocks.
+__device__ __forceinline__ uint32_t progTag(int b, int col) { return ((uint32_t)b << 16) | (uint32_t)(col + 1); }
+
+// Spins (wave-uniformly: every lane reads the same word) until *w >= target.
+// Only waves of the same workgroup publish these words, so the producer is
+// resident and the wait ends.
+__device__ __forceinline__ void waitAtLeast(uint32_t *w, uint32_t target)
+{
+    while (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < target)
+        __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ __forceinline__ void publish(uint32_t *w, uint32_t v)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if ((threadIdx.x & 63) == 0)
+        __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Producer progress words of the cooperative hand-off, one per wave of the
+// workgroup (the wave's row buffer is the slot its blocks write).
+constexpr int kCoopWaves = 4;
+__shared__ uint32_t gProd[kCoopWaves];
+
 // Per-lane state of one pair couple (lo = pair A, hi = pair B).
 struct Couple {
     const uint8_t *h[2];
@@ -133,10 +160,26 @@ struct Fwd16 {
 
     __host__ __device__ __forceinline__ static int stepsFor(int Nw) { return ((Nw + G + QD - 1) / QD) * QD; }
 
+    // Long work items are shared by the NW waves of a workgroup: wave w
+    // sweeps blocks w, w + NW, ...  Block b writes its bottom row into the
+    // row buffer of its wave and publishes its progress in gProd[b % NW];
+    // block b + 1 reads that buffer once the progress covers the columns.
+    // Block b only ever waits for block b - 1, so the waits end (block 0
+    // waits for nothing).  Reusing a wave's buffer for block b + NW needs no
+    // wait: block b + NW reaches column j only after blocks b + NW - 1, ...,
+    // b + 1 have, and block b + 1 read column j of block b before publishing
+    // its own progress past it.
+    static constexpr int NW = kCoopWaves;
+    static constexpr int PUB = 4;   // publish progress every PUB chunks of QD steps
+
     // One block of rows: column 0 init plus the skewed sweep over columns.
-    template <bool HEAD, bool LAST>
+    // rbIn: row buffer holding block b - 1's bottom row; rbOut: where this
+    // block's bottom row goes (the same buffer unless COOP).  COOP: prodIn /
+    // prodOut are the progress words of blocks b - 1 and b.
+    template <bool HEAD, bool LAST, bool COOP>
     __device__ __forceinline__ static void block(const FwdArgs &a, const Couple &c, int b, int Nw, int l,
-                                                 uint32_t *rowbuf, const uint32_t *sProf, const int16_t *sT,
+                                                 const uint32_t *rbIn, uint32_t *rbOut, uint32_t *prodIn,
+                                                 uint32_t *prodOut, const uint32_t *sProf, const int16_t *sT,
                                                  s16x2 &best, uint4 *trItem, s16x2 &endE, s16x2 &endF)
     {
         const int o = a.gapOpen, e = a.gapExtend;
@@ -180,9 +223,23 @@ struct Fwd16 {
         }
 
         // Row-buffer entry for step 0 (read before this block's column-0 write).
-        uint64_t rbv = (b > 0) ? rbLoad(rowbuf) : ((uint64_t)asu(neg2) << 32 | asu(neg2));
+        uint64_t rbv = (uint64_t)asu(neg2) << 32 | asu(neg2);
+        const int steps = stepsFor(Nw);
+        int known = -1;   // COOP: columns of block b - 1 known to be available
+        if constexpr (COOP) {
+            if (b > 0) {
+                waitAtLeast(prodIn, progTag(b - 1, 0));
+                known = 0;
+            }
+        }
+        if (b > 0)
+            rbv = rbLoad(rbIn);
         if (!LAST && l == G - 1)
-            rbStore(rowbuf, ((uint64_t)asu(neg2) << 32) | asu(Gk[R - 1]));
+            rbStore(rbOut, ((uint64_t)asu(neg2) << 32) | asu(Gk[R - 1]));
+        if constexpr (COOP) {
+            if (!LAST)
+                publish(prodOut, progTag(b, 0));
+        }
 
         if constexpr (LAST && (TRK & kTrackBottom)) {
             if (l == G - 1)
@@ -192,7 +249,6 @@ struct Fwd16 {
         uint32_t botG = asu(Gk[R - 1]), botF = asu(neg2);
         s16x2 upGprev = neg2;
         const s16x2 remBase = pack(c.n[0] + l, c.n[1] + l);
-        const int steps = stepsFor(Nw);
         uint4 *trBlock = nullptr;
         if constexpr (TB)
             trBlock = trItem + (size_t)b * steps * 64;
@@ -205,7 +261,7 @@ struct Fwd16 {
             const s16x2 upF = as16(shiftIn<G>(botF, (uint32_t)(rbv >> 32)));
             // Prefetch the next row-buffer entry (column s+1 for the group head).
             if (b > 0 && s + 1 <= Nw)
-                rbv = rbLoad(rowbuf + 2 * (s + 1));
+                rbv = rbLoad(rbIn + 2 * (s + 1));
             const s16x2 diagTop = upGprev;
             upGprev = upG;
             if (j >= 1 && j <= Nw) {
@@ -286,7 +342,7 @@ struct Fwd16 {
                     }
                 }
                 if (!LAST && l == G - 1)
-                    rbStore(rowbuf + 2 * j, ((uint64_t)botF << 32) | botG);
+                    rbStore(rbOut + 2 * j, ((uint64_t)botF << 32) | botG);
                 if constexpr (TB)
                     trBlock[(size_t)s * 64] = make_uint4(accE, accF, accD, accH);
             }
@@ -314,6 +370,14 @@ struct Fwd16 {
             q[u] = (uint32_t)hcode(0, u - l) | ((uint32_t)hcode(1, u - l) << 8);
         uint32_t pA = prof(q[0] & 0xff), pB = prof(q[0] >> 8);
         for (int s0 = 0; s0 < steps; s0 += QD) {
+            if constexpr (COOP) {
+                // This chunk reads columns up to s0 + QD of block b - 1.
+                const int need = min(s0 + QD, Nw);
+                if (b > 0 && need > known) {
+                    waitAtLeast(prodIn, progTag(b - 1, need));
+                    known = (int)(__hip_atomic_load(prodIn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & 0xFFFFu) - 1;
+                }
+            }
 #pragma unroll
             for (int u = 0; u < QD; ++u) {
                 const int s = s0 + u;
@@ -324,6 +388,11 @@ struct Fwd16 {
                 pA = nA;
                 pB = nB;
             }
+            if constexpr (COOP) {
+                // Columns up to s0 + QD - G of this block are written.
+                if (!LAST && ((s0 / QD) % PUB == PUB - 1 || s0 + QD >= steps))
+                    publish(prodOut, progTag(b, max(0, min(s0 + QD - G, Nw))));
+            }
         }
         // Make this block's row-buffer writes visible to the next block.
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -331,7 +400,40 @@ struct Fwd16 {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
 
-    __device__ static void run(const FwdArgs &a)
+    __device__ __forceinline__ static bool wellFormed(const FwdArgs &a, const WorkItem &w)
+    {
+        // A malformed descriptor is skipped rather than allowed to run past
+        // the row buffer or loop without bound.
+        return !(w.count < 1 || w.count > 2 * GROUPS || w.maxN < 1 || w.maxN + G + 2 > a.rowCols ||
+                 w.nblocks < 1 || w.nblocks > (1 << 20) / BR + 1);
+    }
+
+    __device__ __forceinline__ static Couple couple(const FwdArgs &a, const WorkItem &w, int grp)
+    {
+        Couple c;
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            const int sidx = 2 * grp + x;
+            const int pid = sidx < w.count ? a.order[w.first + sidx] : -1;
+            int64_t ho = 0, vo = 0;
+            int n = 0, m = 0;
+            if (pid >= 0) {
+                ho = a.hOff[pid];
+                vo = a.vOff[pid];
+                n = (int)(a.hOff[pid + 1] - ho);
+                m = (int)(a.vOff[pid + 1] - vo);
+            }
+            c.h[x] = a.hRes + ho;
+            c.v[x] = a.vRes + vo;
+            c.n[x] = n;
+            c.m[x] = m;
+            c.pid[x] = pid;
+            c.p[x] = w.nblocks * BR - 1 - m;
+        }
+        return c;
+    }
+
+    __device__ __forceinline__ static void run(const FwdArgs &a)
     {
         __shared__ uint32_t sProf[8];
         __shared__ int16_t sT[33 * 33];
@@ -348,42 +450,88 @@ struct Fwd16 {
         const int lane = threadIdx.x & 63;
         const int l = lane % G;
         const int grp = lane / G;
-        const int slot = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        const int wave = threadIdx.x >> 6;
+        const int slot = blockIdx.x * (blockDim.x >> 6) + wave;
         uint32_t *rowbuf = a.rowbuf + ((size_t)slot * GROUPS + grp) * (size_t)a.rowCols * 2;
 
+        // Phase 1: items [0, nLong) (the ones with the most row blocks), one
+        // per workgroup, blocks spread over its NW waves.
+        if constexpr (!TB) {
+            __shared__ uint32_t sBest[NW][GROUPS];
+            __shared__ int sItem;
+            const int nLong = (blockDim.x == 64 * NW) ? a.nLong : 0;
+            // Row buffer of this lane's group in the wave that owns slot x.
+            auto rbOf = [&](int x) {
+                return a.rowbuf + ((size_t)(blockIdx.x * NW + x) * GROUPS + grp) * (size_t)a.rowCols * 2;
+            };
+            while (nLong > 0) {
+                if (threadIdx.x == 0)
+                    sItem = (int)atomicAdd(a.queueLong, 1u);
+                if (threadIdx.x < NW)
+                    gProd[threadIdx.x] = 0;
+                __syncthreads();
+                const int it = sItem;
+                if (it >= nLong)
+                    break;   // the same item index in every wave
+                const WorkItem w = a.items[it];
+                if (wellFormed(a, w)) {
+                    const Couple c = couple(a, w, grp);
+                    s16x2 best = (TRK & kTrackLocal) ? splat(a.gapOpen) : splat(kNeg16);
+                    s16x2 endE = splat(kNeg16), endF = splat(kNeg16);
+                    const int Nw = w.maxN;
+                    // Block b reads the buffer of wave (b - 1) % NW and writes
+                    // its own; one call site per block kind, in sequence.
+                    const int prev = (wave + NW - 1) % NW;
+                    uint32_t *rbPrev = rbOf(prev);
+                    if (w.nblocks == 1) {
+                        if (wave == 0)
+                            block<true, true, true>(a, c, 0, Nw, l, rbPrev, rowbuf, &gProd[prev], &gProd[wave], sProf,
+                                                    sT, best, nullptr, endE, endF);
+                    } else {
+                        int b = wave;
+                        if (b == 0) {
+                            block<true, false, true>(a, c, 0, Nw, l, rbPrev, rowbuf, &gProd[prev], &gProd[wave], sProf,
+                                                     sT, best, nullptr, endE, endF);
+                            b += NW;
+                        }
+                        for (; b < w.nblocks - 1; b += NW)
+                            block<false, false, true>(a, c, b, Nw, l, rbPrev, rowbuf, &gProd[prev], &gProd[wave], sProf,
+                                                      sT, best, nullptr, endE, endF);
+                        if (b == w.nblocks - 1)
+                            block<false, true, true>(a, c, b, Nw, l, rbPrev, rowbuf, &gProd[prev], &gProd[wave], sProf,
+                                                     sT, best, nullptr, endE, endF);
+                    }
+                    best = groupMax<G>(best);
+                    if (l == 0)
+                        sBest[wave][grp] = asu(best);
+                    __syncthreads();
+                    if (wave == 0 && l == 0) {
+                        s16x2 m = as16(sBest[0][grp]);
+#pragma unroll
+                        for (int v = 1; v < NW; ++v)
+                            m = vmax(m, as16(sBest[v][grp]));
+                        if (c.pid[0] >= 0)
+                            a.scores[c.pid[0]] = (int)m.x - a.gapOpen;
+                        if (c.pid[1] >= 0)
+                            a.scores[c.pid[1]] = (int)m.y - a.gapOpen;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+
+        // Phase 2: the remaining items, one per wave from the work queue.
         for (;;) {
             int it = 0;
             if (lane == 0)
                 it = (int)atomicAdd(a.queue, 1u);
-            it = __builtin_amdgcn_readfirstlane(it);
+            it = __builtin_amdgcn_readfirstlane(it) + a.nLong;
             if (it >= a.nItems)
                 break;
             const WorkItem w = a.items[it];
-            // A malformed descriptor is skipped rather than allowed to run
-            // past the row buffer or loop without bound.
-            if (w.count < 1 || w.count > 2 * GROUPS || w.maxN < 1 || w.maxN + G + 2 > a.rowCols ||
-                w.nblocks < 1 || w.nblocks > (1 << 20) / BR + 1)
+            if (!wellFormed(a, w))
                 continue;
-            Couple c;
-#pragma unroll
-            for (int x = 0; x < 2; ++x) {
-                const int sidx = 2 * grp + x;
-                const int pid = sidx < w.count ? a.order[w.first + sidx] : -1;
-                int64_t ho = 0, vo = 0;
-                int n = 0, m = 0;
-                if (pid >= 0) {
-                    ho = a.hOff[pid];
-                    vo = a.vOff[pid];
-                    n = (int)(a.hOff[pid + 1] - ho);
-                    m = (int)(a.vOff[pid + 1] - vo);
-                }
-                c.h[x] = a.hRes + ho;
-                c.v[x] = a.vRes + vo;
-                c.n[x] = n;
-                c.m[x] = m;
-                c.pid[x] = pid;
-                c.p[x] = w.nblocks * BR - 1 - m;
-            }
+            const Couple c = couple(a, w, grp);
             s16x2 best = (TRK & kTrackLocal) ? splat(a.gapOpen) : splat(kNeg16);
             s16x2 endE = splat(kNeg16), endF = splat(kNeg16);
             const int Nw = w.maxN;
@@ -391,12 +539,16 @@ struct Fwd16 {
             if constexpr (TB)
                 trItem = a.trace + (size_t)a.traceOff[it] * 64 + lane;
             if (w.nblocks == 1) {
-                block<true, true>(a, c, 0, Nw, l, rowbuf, sProf, sT, best, trItem, endE, endF);
+                block<true, true, false>(a, c, 0, Nw, l, rowbuf, rowbuf, nullptr, nullptr, sProf, sT, best, trItem,
+                                         endE, endF);
             } else {
-                block<true, false>(a, c, 0, Nw, l, rowbuf, sProf, sT, best, trItem, endE, endF);
+                block<true, false, false>(a, c, 0, Nw, l, rowbuf, rowbuf, nullptr, nullptr, sProf, sT, best, trItem,
+                                          endE, endF);
                 for (int b = 1; b < w.nblocks - 1; ++b)
-                    block<false, false>(a, c, b, Nw, l, rowbuf, sProf, sT, best, trItem, endE, endF);
-                block<false, true>(a, c, w.nblocks - 1, Nw, l, rowbuf, sProf, sT, best, trItem, endE, endF);
+                    block<false, false, false>(a, c, b, Nw, l, rowbuf, rowbuf, nullptr, nullptr, sProf, sT, best,
+                                               trItem, endE, endF);
+                block<false, true, false>(a, c, w.nblocks - 1, Nw, l, rowbuf, rowbuf, nullptr, nullptr, sProf, sT,
+                                          best, trItem, endE, endF);
             }
             if constexpr (TB) {
                 // E and F of the end cell (m, n), for the end-cell rule of
@@ -423,15 +575,17 @@ struct Fwd16 {
     }
 };
 
+// Three waves per SIMD: the register budget (168 VGPRs) the R = 16 shapes
+// need without spilling inside the column loop.
 template <int G, int R, bool DNA, uint32_t TRK>
-__global__ __launch_bounds__(256) void kFwd16(FwdArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kFwd16(FwdArgs a)
 {
     Fwd16<G, R, DNA, TRK>::run(a);
 }
 
 // Forward pass with trace records (global alignment, G = 16, R = 16).
 template <bool DNA>
-__global__ __launch_bounds__(256) void kFwdTrace16(FwdArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void kFwdTrace16(FwdArgs a)
 {
     Fwd16<16, 16, DNA, kTrackEnd, true>::run(a);
 }

@@ -38,7 +38,9 @@ struct FwdArgs {
     const int32_t *order;       // pair ids grouped by work item
     const WorkItem *items;
     int32_t nItems;
-    uint32_t *queue;            // work-queue counter (zeroed before launch)
+    uint32_t *queue;            // work-queue counter of the per-wave items (zeroed before launch)
+    int32_t nLong;              // items [0, nLong) are shared by the 4 waves of a workgroup
+    uint32_t *queueLong;        // work-queue counter of those (zeroed before launch)
     uint32_t *rowbuf;           // per wave slot: GROUPS * rowCols * 2 dwords
     int32_t rowCols;            // columns per group row buffer
     int32_t *scores;            // indexed by pair id

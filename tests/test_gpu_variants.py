@@ -65,3 +65,24 @@ def test_plain_per_pair_matches_batched(ctx):
     got = ctx.align_score(H, V, dna, 9, flags=sa.PLAIN_PER_PAIR)
     assert np.array_equal(got, _oracle(H, V, dna, 9))
     assert np.array_equal(got, ctx.align_score(H, V, dna, 9))
+
+
+@pytest.mark.parametrize("min_blocks", ["1", "2", "3", "0"])
+def test_cooperative_items_match_oracle(ctx, monkeypatch, min_blocks):
+    # Work items with at least SQMI_COOP_MIN_BLOCKS row blocks are swept by
+    # the four waves of a workgroup together (0: never); every setting gives
+    # the oracle's scores, including items with fewer blocks than waves.
+    monkeypatch.setenv("SQMI_COOP_MIN_BLOCKS", min_blocks)
+    rng = np.random.default_rng(int(min_blocks) + 40)
+    dna = sa.simple_score(2, -3, -1, -5)
+    H, V = _random_set(rng, 200, 1, 1500, 4)
+    for mode in (0, 9, 15, 4, 2):
+        print("coop", min_blocks, "dna mode", mode, flush=True)
+        got = ctx.align_score(H, V, dna, mode)
+        assert np.array_equal(got, _oracle(H, V, dna, mode)), mode
+    P, Q = _random_set(rng, 60, 1, 1200, 24)
+    b62 = sa.blosum62(-1, -11)
+    for mode in (16, 0):
+        print("coop", min_blocks, "protein mode", mode, flush=True)
+        got = ctx.align_score(P, Q, b62, mode)
+        assert np.array_equal(got, _oracle(P, Q, b62, mode)), mode
