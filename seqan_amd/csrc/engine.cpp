@@ -13,6 +13,9 @@
 #include "seqan_amd.h"
 
 #include <algorithm>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <chrono>
 #include <climits>
 #include <cstdio>
@@ -40,6 +43,7 @@ struct sqmi_context
     hipStream_t stream = nullptr;
     hipStream_t ownStream = nullptr;
     int numCUs = 0;
+    int64_t totalMem = 0;   // device memory, bytes
     std::string lastError;
     sqmi_stats stats{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -185,6 +189,62 @@ struct Planned
     int64_t cells = 0;
 };
 
+// Stable LSD radix sort of 26-bit keys in two passes of 13 bits, each pass
+// split into contiguous per-thread ranges so that equal digits keep their
+// input order.
+void radixSort(std::vector<Key> &keys)
+{
+    const size_t n = keys.size();
+    if (n < 2)
+        return;
+    std::vector<Key> tmp(n);
+    int T = 1;
+#ifdef _OPENMP
+    T = std::max(1, std::min(omp_get_max_threads(), (int)(n / 16384) + 1));
+#endif
+    constexpr int B = 13, NB = 1 << B;
+    std::vector<uint32_t> hist((size_t)T * NB);
+    for (int shift = 0; shift < 2 * B; shift += B) {
+        const uint32_t mask = (uint32_t)NB - 1;
+        std::fill(hist.begin(), hist.end(), 0u);
+#pragma omp parallel num_threads(T)
+        {
+            int t = 0;
+#ifdef _OPENMP
+            t = omp_get_thread_num();
+#endif
+            const size_t lo = n * t / T, hi = n * (t + 1) / T;
+            uint32_t *h = hist.data() + (size_t)t * NB;
+            for (size_t k = lo; k < hi; ++k)
+                ++h[(keys[k].key >> shift) & mask];
+#pragma omp barrier
+#pragma omp single
+            {
+                uint32_t run = 0;
+                for (int d = 0; d < NB; ++d)
+                    for (int u = 0; u < T; ++u) {
+                        uint32_t &c = hist[(size_t)u * NB + d];
+                        const uint32_t v = c;
+                        c = run;
+                        run += v;
+                    }
+            }
+            for (size_t k = lo; k < hi; ++k)
+                tmp[h[(keys[k].key >> shift) & mask]++] = keys[k];
+        }
+        keys.swap(tmp);
+    }
+}
+
+// Bucket keys (26 bits): class (2 bits: 0 / 1 int16 shape index, 2 int32,
+// 3 empty) | 255 - row blocks (8 bits) | 32767 - |H| (15 bits).  Ascending
+// order puts int16 pairs first, grouped by shape and row blocks (most first),
+// widest first, then the int32 and the empty pairs in input order.
+constexpr int kKeyNBits = 15, kKeyBlockBits = 8, kKeyClassShift = kKeyNBits + kKeyBlockBits;
+inline uint32_t bucketOf(uint32_t key) { return key >> kKeyNBits; }
+inline int32_t widthOf(uint32_t key) { return (int32_t)((1u << kKeyNBits) - 1 - (key & ((1u << kKeyNBits) - 1))); }
+inline int32_t blocksOf(uint32_t key) { return (int32_t)(255u - ((key >> kKeyNBits) & 255u)); }
+
 // Routing (empty / int16 / int32) and length bucketing into work items.
 // Pure host code: needs no device.
 void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &scoring, uint32_t mode,
@@ -219,83 +279,135 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
             track = kTrackEnd;
     }
     P.track = track;
-    // Route every pair and give each int16 pair a 32-bit bucket key:
-    // kernel shape (2 bits) | row blocks, most first (14 bits) | |H|, longest
-    // first (16 bits).  Sorting by it groups pairs of equal block count and
-    // similar width, and orders work items longest first for the queue.
-    std::vector<Key> &keys16 = P.keys16;
-    keys16.reserve(N);
-    std::vector<Variant16> shapes;   // distinct shapes, indexed by the key's top bits
+    // Route every pair and give it a bucket key (see bucketOf); shape index
+    // 0 is the forced shape, the trace shape or R = 16, index 1 is R = 8.
+    std::vector<Variant16> shapes;
+    if (traceShape)
+        shapes.push_back(Variant16{kTraceLanes, kTraceRows / kTraceLanes, si.dna, track});
+    else if (((flags >> 8) & 0xFF) && ((flags >> 16) & 0xFF))
+        shapes.push_back(chooseVariant(si.dna, track, 1, 1, flags));
+    else {
+        shapes.push_back(Variant16{16, 16, si.dna, track});
+        shapes.push_back(Variant16{16, 8, si.dna, track});
+    }
+    std::vector<Key> keys((size_t)N);
+    int64_t cells = 0;
+#pragma omp parallel for schedule(static) reduction(+ : cells)
     for (int64_t i = 0; i < N; ++i) {
         const int64_t n = H.offsets[i + 1] - H.offsets[i];
         const int64_t m = V.offsets[i + 1] - V.offsets[i];
-        if (n == 0 || m == 0) {
-            P.emptyIds.push_back((int32_t)i);
-            continue;
-        }
-        P.cells += n * m;
-        if (costsOk16 && fitsInt16(si, n, m, local)) {
-            const Variant16 v = traceShape ? Variant16{kTraceLanes, kTraceRows / kTraceLanes, si.dna, track}
-                                           : chooseVariant(si.dna, track, n, m, flags);
-            uint32_t vidx = 0;
-            while (vidx < shapes.size() && !(shapes[vidx].G == v.G && shapes[vidx].R == v.R))
-                ++vidx;
-            if (vidx == shapes.size())
-                shapes.push_back(v);
-            const int64_t BR = (int64_t)v.G * v.R;
-            const uint32_t nblocks = (uint32_t)((m + 1 + BR - 1) / BR);
-            const uint32_t key = (vidx << 30) | ((0x3FFFu - nblocks) << 16) | (0xFFFFu - (uint32_t)n);
-            keys16.push_back({key, (int32_t)i});
-        } else {
-            P.ids32.push_back((int32_t)i);
-            P.scratchOff32.push_back(P.scratch32);
-            // score: H and E columns (int32 pairs); traceback: the columns
-            // plus one trace byte per cell, 16-byte aligned.
-            P.scratch32 += traceShape ? ((8 * (m + 1) + (n + 1) * (m + 1) + 15) & ~(int64_t)15) : m + 1;
-        }
-    }
-    // LSD radix sort, two stable 16-bit passes (ties keep input order).
-    {
-        std::vector<Key> tmp(keys16.size());
-        std::vector<uint32_t> hist(1u << 16);
-        for (int pass = 0; pass < 2; ++pass) {
-            const int shift = 16 * pass;
-            std::fill(hist.begin(), hist.end(), 0u);
-            for (const Key &k : keys16)
-                ++hist[(k.key >> shift) & 0xFFFFu];
-            uint32_t run = 0;
-            for (uint32_t &h : hist) {
-                const uint32_t c = h;
-                h = run;
-                run += c;
+        uint32_t key = 3u << kKeyClassShift;
+        if (n != 0 && m != 0) {
+            cells += n * m;
+            key = 2u << kKeyClassShift;
+            if (costsOk16 && fitsInt16(si, n, m, local)) {
+                const uint32_t vidx = shapes.size() == 2 && m + 1 <= 128 ? 1u : 0u;
+                const int64_t BR = (int64_t)shapes[vidx].G * shapes[vidx].R;
+                const int64_t nblocks = (m + 1 + BR - 1) / BR;
+                if (nblocks <= 255 && n < (1 << kKeyNBits))
+                    key = (vidx << kKeyClassShift) | ((255u - (uint32_t)nblocks) << kKeyNBits) |
+                          ((1u << kKeyNBits) - 1 - (uint32_t)n);
             }
-            for (const Key &k : keys16)
-                tmp[hist[(k.key >> shift) & 0xFFFFu]++] = k;
-            keys16.swap(tmp);
         }
+        keys[i] = Key{key, (int32_t)i};
+    }
+    P.cells = cells;
+    radixSort(keys);
+    // Segments of the sorted order: int16 (by shape), int32, empty.
+    size_t n16 = 0;
+    while (n16 < keys.size() && (keys[n16].key >> kKeyClassShift) < 2) {
+        // Jump by bucket boundaries found with a binary search.
+        const uint32_t cls = keys[n16].key >> kKeyClassShift;
+        n16 = (size_t)(std::upper_bound(keys.begin() + n16, keys.end(), ((cls + 1) << kKeyClassShift) - 1,
+                                        [](uint32_t v, const Key &k) { return v < k.key; }) -
+                       keys.begin());
+    }
+    size_t e32 = n16;
+    while (e32 < keys.size() && (keys[e32].key >> kKeyClassShift) == 2)
+        ++e32;
+    for (size_t k = n16; k < e32; ++k) {
+        const int32_t i = keys[k].id;
+        const int64_t n = H.offsets[i + 1] - H.offsets[i];
+        const int64_t m = V.offsets[i + 1] - V.offsets[i];
+        P.ids32.push_back(i);
+        P.scratchOff32.push_back(P.scratch32);
+        // score: H and E columns (int32 pairs); traceback: the columns plus
+        // one trace byte per cell, 16-byte aligned.
+        P.scratch32 += traceShape ? ((8 * (m + 1) + (n + 1) * (m + 1) + 15) & ~(int64_t)15) : m + 1;
+    }
+    for (size_t k = e32; k < keys.size(); ++k)
+        P.emptyIds.push_back(keys[k].id);
+    keys.resize(n16);
+    std::vector<Key> &keys16 = P.keys16;
+    keys16.swap(keys);
+
+    // Work items: each bucket (shape, row blocks) is cut into runs of up to
+    // pairs-per-wave pairs, in sorted order.  Bucket starts are found in
+    // parallel, then every bucket writes its own items.
+    std::vector<size_t> starts;
+    {
+        int T = 1;
+#ifdef _OPENMP
+        T = std::max(1, std::min(omp_get_max_threads(), (int)(n16 / 16384) + 1));
+#endif
+        std::vector<std::vector<size_t>> part((size_t)T);
+#pragma omp parallel num_threads(T)
+        {
+            int t = 0;
+#ifdef _OPENMP
+            t = omp_get_thread_num();
+#endif
+            const size_t lo = n16 * t / T, hi = n16 * (t + 1) / T;
+            for (size_t k = lo; k < hi; ++k)
+                if (k == 0 || bucketOf(keys16[k].key) != bucketOf(keys16[k - 1].key))
+                    part[t].push_back(k);
+        }
+        for (auto &v : part)
+            starts.insert(starts.end(), v.begin(), v.end());
+        starts.push_back(n16);
+    }
+    const size_t nb = starts.size() - 1;
+    std::vector<size_t> itemOff(nb + 1, 0);
+    for (size_t b = 0; b < nb; ++b) {
+        const int PP = pairsPerWave(shapes[keys16[starts[b]].key >> kKeyClassShift]);
+        itemOff[b + 1] = itemOff[b] + (starts[b + 1] - starts[b] + PP - 1) / PP;
     }
     std::vector<Plan16> &plans = P.plans;
-    for (size_t k = 0; k < keys16.size();) {
-        const uint32_t vidx = keys16[k].key >> 30;
-        const Variant16 v = shapes[vidx];
-        if (plans.empty() || !(plans.back().var.G == v.G && plans.back().var.R == v.R)) {
-            plans.push_back(Plan16{});
-            plans.back().var = v;
-        }
+    for (size_t v = 0; v < shapes.size(); ++v) {
+        // The shape's contiguous range of buckets and of sorted pairs.
+        size_t b0 = 0;
+        while (b0 < nb && (keys16[starts[b0]].key >> kKeyClassShift) < v)
+            ++b0;
+        size_t b1 = b0;
+        while (b1 < nb && (keys16[starts[b1]].key >> kKeyClassShift) == v)
+            ++b1;
+        if (b0 == b1)
+            continue;
+        plans.push_back(Plan16{});
         Plan16 &pl = plans.back();
-        const int PP = pairsPerWave(v);
-        const uint32_t bucket = keys16[k].key >> 16;   // shape and row blocks
-        WorkItem w{(int32_t)pl.order.size(), 0, 0, (int32_t)(0x3FFFu - (bucket & 0x3FFFu))};
-        size_t e = k;
-        while (e < keys16.size() && (int)(e - k) < PP && (keys16[e].key >> 16) == bucket) {
-            pl.order.push_back(keys16[e].id);
-            w.maxN = std::max<int32_t>(w.maxN, (int32_t)(0xFFFFu - (keys16[e].key & 0xFFFFu)));
-            ++e;
+        pl.var = shapes[v];
+        const size_t p0 = starts[b0], p1 = starts[b1];
+        const size_t i0 = itemOff[b0];
+        const int PP = pairsPerWave(pl.var);
+        pl.order.resize(p1 - p0);
+        pl.items.resize(itemOff[b1] - i0);
+        int32_t maxN = 0;
+#pragma omp parallel for schedule(static) reduction(max : maxN)
+        for (int64_t k = (int64_t)p0; k < (int64_t)p1; ++k) {
+            pl.order[k - p0] = keys16[k].id;
+            maxN = std::max(maxN, widthOf(keys16[k].key));
         }
-        w.count = (int32_t)(e - k);
-        pl.maxN = std::max(pl.maxN, w.maxN);
-        pl.items.push_back(w);
-        k = e;
+        pl.maxN = maxN;
+#pragma omp parallel for schedule(dynamic, 8)
+        for (int64_t b = (int64_t)b0; b < (int64_t)b1; ++b) {
+            size_t it = itemOff[b] - i0;
+            for (size_t k = starts[b]; k < starts[b + 1]; k += PP, ++it) {
+                const size_t e = std::min(starts[b + 1], k + PP);
+                // Widest first inside a bucket: the first pair has the item's width.
+                pl.items[it] = WorkItem{(int32_t)(k - p0), (int32_t)(e - k), widthOf(keys16[k].key),
+                                        blocksOf(keys16[k].key)};
+            }
+        }
     }
 }
 
@@ -345,8 +457,10 @@ sqmi_status checkForced(sqmi_context *ctx, uint32_t flags)
 
 
 // Bytes the traceback may hold in trace records or int32 trace scratch at
-// once: 8 GiB, or SQMI_TRACE_BUDGET_MB from the environment.
-int64_t traceBudget()
+// once: a quarter of the device's memory, at least 8 GiB and at most 64 GiB
+// (72 GB -> 64 GiB on a 288 GB MI355X), or SQMI_TRACE_BUDGET_MB from the
+// environment.
+int64_t traceBudget(const sqmi_context *ctx)
 {
     const char *env = std::getenv("SQMI_TRACE_BUDGET_MB");
     if (env && *env) {
@@ -354,7 +468,8 @@ int64_t traceBudget()
         if (mb > 0)
             return (int64_t)mb << 20;
     }
-    return (int64_t)8 << 30;
+    const int64_t quarter = ctx ? ctx->totalMem / 4 : 0;
+    return std::min<int64_t>(std::max<int64_t>(quarter, (int64_t)8 << 30), (int64_t)64 << 30);
 }
 
 // Work items with at least this many row blocks are swept by the four waves
@@ -476,6 +591,7 @@ sqmi_status sqmi_create(int device, sqmi_context **out)
         return SQMI_ERR_OUT_OF_MEMORY;
     ctx->device = device;
     ctx->numCUs = prop.multiProcessorCount;
+    ctx->totalMem = (int64_t)prop.totalGlobalMem;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->ownStream) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
         delete ctx;
@@ -798,7 +914,7 @@ sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *setH, con
     fillStats(P, ctx->stats);
     const ScoringInfo &si = P.si;
     const int A = si.alpha, S = A * A + 2;
-    const int64_t budget = traceBudget();
+    const int64_t budget = traceBudget(ctx);
 
     // Host-side chunking of the int16 items and of the int32 pairs.
     const Plan16 *pl = P.plans.empty() ? nullptr : &P.plans[0];

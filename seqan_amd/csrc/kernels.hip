@@ -55,13 +55,13 @@ __device__ __forceinline__ uint32_t bfi(uint32_t mask, uint32_t a, uint32_t b) {
 // Saturating a - b per half: its sign bit is exactly (a < b), whatever the
 // magnitudes (v_pk_sub_i16 with clamp).
 __device__ __forceinline__ s16x2 subSat(s16x2 a, s16x2 b) { return __builtin_elementwise_sub_sat(a, b); }
-// Shifts the accumulator right by one bit per half and inserts the sign bits
-// of t at bits 15 and 31: after 16 rows, row k's bit sits at bit k (pair A)
-// and bit 16 + k (pair B).
-__device__ __forceinline__ uint32_t pushSign(uint32_t acc, s16x2 t)
+// Four sign bytes from two packed differences: v_perm_b32 selectors 8..11
+// replicate bits 15, 31 (of the second operand) and 47, 63 (of the first)
+// into whole bytes, giving bytes {sign(t0.x), sign(t0.y), sign(t1.x),
+// sign(t1.y)} as 0x00 / 0xFF.
+__device__ __forceinline__ uint32_t signBytes(s16x2 t0, s16x2 t1)
 {
-    const u16x2 sh = __builtin_bit_cast(u16x2, acc) >> (unsigned short)1;
-    return __builtin_bit_cast(uint32_t, sh) | (asu(t) & 0x80008000u);
+    return __builtin_amdgcn_perm(asu(t1), asu(t0), 0x0B0A0908u);
 }
 
 // Lane l of every group of G lanes receives v from lane l-1; lane 0 of the
@@ -141,16 +141,20 @@ struct Couple {
     int n[2], m[2], p[2], pid[2];
 };
 
-// TB: also write the trace bits of every cell (global alignment only):
-// per lane and column step one 16-byte record of four dwords, each holding
-// one bit per row (bit k: pair A, bit 16 + k: pair B):
-//   x  E(i,j) opened a gap:    E(i,j-1) + e <  G(i,j-1)   (TraceBitMap_ HORIZONTAL_OPEN)
-//   y  F(i,j) opened a gap:    F(i-1,j) + e <  G(i-1,j)   (VERTICAL_OPEN)
-//   z  H(i,j) is not diagonal: D < max(E, F)              (not DIAGONAL)
-//   w  E beats F:              F < E                      (MAX_FROM_HORIZONTAL_MATRIX)
-// which is what dp_formula_affine.h:62-126 and _maxScore (dp_formula.h:165-201)
-// record, ties resolved the same way.  Records of block b, step s of a work
-// item sit at trace[(traceOff[item] + b * steps + s) * 64 + lane].
+// TB: also write the trace bits of every cell (global alignment only).  Per
+// cell four decisions, which is what dp_formula_affine.h:62-126 and
+// _maxScore (dp_formula.h:165-201) record, ties resolved the same way:
+//   E  E(i,j) opened a gap:    E(i,j-1) + e <  G(i,j-1)   (TraceBitMap_ HORIZONTAL_OPEN)
+//   F  F(i,j) opened a gap:    F(i-1,j) + e <  G(i-1,j)   (VERTICAL_OPEN)
+//   D  H(i,j) is not diagonal: D < max(E, F)              (not DIAGONAL)
+//   X  E beats F:              F < E                      (MAX_FROM_HORIZONTAL_MATRIX)
+// Per lane and column step one 16-byte record; bit kk of each byte is row
+// kk (x, z: rows 0-7; y, w: rows 8-15) and the bytes are
+//   x, y: E of pair A, E of pair B, F of pair A, F of pair B
+//   z, w: D of pair A, D of pair B, X of pair A, X of pair B.
+// Records of block b, step s of a work item sit at
+// trace[(traceOff[item] + b * steps + s) * 64 + lane]: one coalesced 1 KB
+// store per wave and step.
 template <int G, int R, bool DNA, uint32_t TRK, bool TB = false>
 struct Fwd16 {
     static constexpr int BR = G * R;
@@ -173,14 +177,20 @@ struct Fwd16 {
     static constexpr int PUB = 4;   // publish progress every PUB chunks of QD steps
 
     // One block of rows: column 0 init plus the skewed sweep over columns.
-    // rbIn: row buffer holding block b - 1's bottom row; rbOut: where this
-    // block's bottom row goes (the same buffer unless COOP).  COOP: prodIn /
-    // prodOut are the progress words of blocks b - 1 and b.
-    template <bool HEAD, bool LAST, bool COOP>
+    // HEAD: block 0 (rows above the pairs' row 0 are floored).  last: the
+    // block holding row m (result tracking, no row-buffer output).  rbIn: row
+    // buffer holding block b - 1's bottom row; rbOut: where this block's
+    // bottom row goes (the same buffer unless coop).  coop: prodIn / prodOut
+    // are the progress words of blocks b - 1 and b.  b, last and coop are
+    // wave-uniform; keeping them runtime values keeps the kernel to two
+    // copies of the column loop (HEAD or not), which fit the instruction
+    // cache.
+    template <bool HEAD>
     __device__ __forceinline__ static void block(const FwdArgs &a, const Couple &c, int b, int Nw, int l,
-                                                 const uint32_t *rbIn, uint32_t *rbOut, uint32_t *prodIn,
-                                                 uint32_t *prodOut, const uint32_t *sProf, const int16_t *sT,
-                                                 s16x2 &best, uint4 *trItem, s16x2 &endE, s16x2 &endF)
+                                                 const bool last, const bool coop, const uint32_t *rbIn,
+                                                 uint32_t *rbOut, uint32_t *prodIn, uint32_t *prodOut,
+                                                 const uint32_t *sProf, const int16_t *sT, s16x2 &best,
+                                                 uint4 *trItem, s16x2 &endE, s16x2 &endF)
     {
         const int o = a.gapOpen, e = a.gapExtend;
         const s16x2 o2 = splat(o), e2 = splat(e), neg2 = splat(kNeg16);
@@ -225,24 +235,20 @@ struct Fwd16 {
         // Row-buffer entry for step 0 (read before this block's column-0 write).
         uint64_t rbv = (uint64_t)asu(neg2) << 32 | asu(neg2);
         const int steps = stepsFor(Nw);
-        int known = -1;   // COOP: columns of block b - 1 known to be available
-        if constexpr (COOP) {
-            if (b > 0) {
-                waitAtLeast(prodIn, progTag(b - 1, 0));
-                known = 0;
-            }
+        int known = -1;   // coop: columns of block b - 1 known to be available
+        if (coop && b > 0) {
+            waitAtLeast(prodIn, progTag(b - 1, 0));
+            known = 0;
         }
         if (b > 0)
             rbv = rbLoad(rbIn);
-        if (!LAST && l == G - 1)
+        if (!last && l == G - 1)
             rbStore(rbOut, ((uint64_t)asu(neg2) << 32) | asu(Gk[R - 1]));
-        if constexpr (COOP) {
-            if (!LAST)
-                publish(prodOut, progTag(b, 0));
-        }
+        if (coop && !last)
+            publish(prodOut, progTag(b, 0));
 
-        if constexpr (LAST && (TRK & kTrackBottom)) {
-            if (l == G - 1)
+        if constexpr (TRK & kTrackBottom) {
+            if (last && l == G - 1)
                 best = vmax(best, Gk[R - 1]);   // H(m, 0) is a tracked cell
         }
 
@@ -266,7 +272,7 @@ struct Fwd16 {
             upGprev = upG;
             if (j >= 1 && j <= Nw) {
                 s16x2 diag = diagTop, uG = upG, uF = upF;
-                uint32_t accE = 0, accF = 0, accD = 0, accH = 0;
+                uint32_t accEF0 = 0, accEF1 = 0, accDX0 = 0, accDX1 = 0;
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
                     uint32_t t;
@@ -281,14 +287,24 @@ struct Fwd16 {
                     s16x2 F, H;
                     if constexpr (TB) {
                         const s16x2 Eext = Ek[k] + e2, Fext = uF + e2;
-                        accE = pushSign(accE, subSat(Eext, Gk[k]));
-                        accF = pushSign(accF, subSat(Fext, uG));
+                        const uint32_t ef = signBytes(subSat(Eext, Gk[k]), subSat(Fext, uG));
                         Ek[k] = vmax(Eext, Gk[k]);
                         F = vmax(Fext, uG);
                         const s16x2 fe = vmax(Ek[k], F);
-                        accD = pushSign(accD, subSat(D, fe));
-                        accH = pushSign(accH, subSat(F, Ek[k]));
+                        const uint32_t dx = signBytes(subSat(D, fe), subSat(F, Ek[k]));
                         H = vmax(D, fe);
+                        // Insert bit kk of every byte.  Written as v_bfi_b32 statements
+                        // so that each row's bits are taken while its values are live
+                        // (left to itself the compiler defers all 16 rows to one OR tree
+                        // at the end of the column and needs ~250 VGPRs).
+                        const uint32_t bit = 0x01010101u << (k & 7);
+                        if (k < 8) {
+                            asm volatile("v_bfi_b32 %0, %1, %2, %0" : "+v"(accEF0) : "s"(bit), "v"(ef));
+                            asm volatile("v_bfi_b32 %0, %1, %2, %0" : "+v"(accDX0) : "s"(bit), "v"(dx));
+                        } else {
+                            asm volatile("v_bfi_b32 %0, %1, %2, %0" : "+v"(accEF1) : "s"(bit), "v"(ef));
+                            asm volatile("v_bfi_b32 %0, %1, %2, %0" : "+v"(accDX1) : "s"(bit), "v"(dx));
+                        }
                     } else {
                         Ek[k] = vmax(Ek[k] + e2, Gk[k]);
                         F = vmax(uF + e2, uG);
@@ -311,18 +327,20 @@ struct Fwd16 {
                 botF = asu(uF);
                 if constexpr (TRK & (kTrackEnd | kTrackBottom | kTrackRight)) {
                     const s16x2 rem = remBase - splat(s);   // n - j per half
-                    if constexpr (LAST && (TRK & kTrackBottom)) {
-                        const uint32_t after = asu(rem >> splat(15));   // 0xffff where j > n
-                        const s16x2 cand = as16(bfi(after, asu(neg2), asu(uG)));
-                        if (l == G - 1)
-                            best = vmax(best, cand);
+                    if constexpr (TRK & kTrackBottom) {
+                        if (last) {
+                            const uint32_t after = asu(rem >> splat(15));   // 0xffff where j > n
+                            const s16x2 cand = as16(bfi(after, asu(neg2), asu(uG)));
+                            if (l == G - 1)
+                                best = vmax(best, cand);
+                        }
                     }
-                    if constexpr ((LAST && (TRK & kTrackEnd)) || (TRK & kTrackRight)) {
+                    if constexpr (TRK & (kTrackEnd | kTrackRight)) {
                         // eq = 0xffff in halves where rem == 0.
                         const uint16_t r0 = (uint16_t)rem.x, r1 = (uint16_t)rem.y;
                         const uint32_t eq = (r0 == 0 ? 0xffffu : 0u) | (r1 == 0 ? 0xffff0000u : 0u);
-                        if constexpr (LAST && (TRK & kTrackEnd)) {
-                            if (l == G - 1) {
+                        if constexpr (TRK & kTrackEnd) {
+                            if (last && l == G - 1) {
                                 best = as16(bfi(eq, asu(uG), asu(best)));
                                 if constexpr (TB) {
                                     endE = as16(bfi(eq, asu(Ek[R - 1]), asu(endE)));
@@ -341,10 +359,10 @@ struct Fwd16 {
                         }
                     }
                 }
-                if (!LAST && l == G - 1)
+                if (!last && l == G - 1)
                     rbStore(rbOut + 2 * j, ((uint64_t)botF << 32) | botG);
                 if constexpr (TB)
-                    trBlock[(size_t)s * 64] = make_uint4(accE, accF, accD, accH);
+                    trBlock[(size_t)s * 64] = make_uint4(accEF0, accEF1, accDX0, accDX1);
             }
         };
 
@@ -370,7 +388,7 @@ struct Fwd16 {
             q[u] = (uint32_t)hcode(0, u - l) | ((uint32_t)hcode(1, u - l) << 8);
         uint32_t pA = prof(q[0] & 0xff), pB = prof(q[0] >> 8);
         for (int s0 = 0; s0 < steps; s0 += QD) {
-            if constexpr (COOP) {
+            if (coop) {
                 // This chunk reads columns up to s0 + QD of block b - 1.
                 const int need = min(s0 + QD, Nw);
                 if (b > 0 && need > known) {
@@ -388,9 +406,9 @@ struct Fwd16 {
                 pA = nA;
                 pB = nB;
             }
-            if constexpr (COOP) {
+            if (coop) {
                 // Columns up to s0 + QD - G of this block are written.
-                if (!LAST && ((s0 / QD) % PUB == PUB - 1 || s0 + QD >= steps))
+                if (!last && ((s0 / QD) % PUB == PUB - 1 || s0 + QD >= steps))
                     publish(prodOut, progTag(b, max(0, min(s0 + QD - G, Nw))));
             }
         }
@@ -437,6 +455,8 @@ struct Fwd16 {
     {
         __shared__ uint32_t sProf[8];
         __shared__ int16_t sT[33 * 33];
+        __shared__ uint32_t sBest[NW][GROUPS];
+        __shared__ int sItem;
         if constexpr (DNA) {
             if (threadIdx.x < 5)
                 sProf[threadIdx.x] = a.dnaProfile[threadIdx.x];
@@ -450,87 +470,46 @@ struct Fwd16 {
         const int lane = threadIdx.x & 63;
         const int l = lane % G;
         const int grp = lane / G;
-        const int wave = threadIdx.x >> 6;
-        const int slot = blockIdx.x * (blockDim.x >> 6) + wave;
-        uint32_t *rowbuf = a.rowbuf + ((size_t)slot * GROUPS + grp) * (size_t)a.rowCols * 2;
+        const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+        uint32_t *rowbuf = a.rowbuf + ((size_t)(blockIdx.x * (blockDim.x >> 6) + wave) * GROUPS + grp) *
+                                          (size_t)a.rowCols * 2;
 
-        // Phase 1: items [0, nLong) (the ones with the most row blocks), one
-        // per workgroup, blocks spread over its NW waves.
-        if constexpr (!TB) {
-            __shared__ uint32_t sBest[NW][GROUPS];
-            __shared__ int sItem;
-            const int nLong = (blockDim.x == 64 * NW) ? a.nLong : 0;
-            // Row buffer of this lane's group in the wave that owns slot x.
-            auto rbOf = [&](int x) {
-                return a.rowbuf + ((size_t)(blockIdx.x * NW + x) * GROUPS + grp) * (size_t)a.rowCols * 2;
-            };
-            while (nLong > 0) {
+        // Items [0, nLong) (the ones with the most row blocks) go one per
+        // workgroup, their blocks spread over its NW waves; the rest go one
+        // per wave.  The long phase runs first and ends for all waves of the
+        // workgroup at once (they read the same item index).
+        const int nLong = (!TB && blockDim.x == 64 * NW) ? a.nLong : 0;
+        const uint32_t *rbPrev = nLong > 0 ? a.rowbuf + ((size_t)(blockIdx.x * NW + (wave + NW - 1) % NW) * GROUPS +
+                                                         grp) * (size_t)a.rowCols * 2
+                                           : rowbuf;
+        bool longPhase = nLong > 0;
+        for (;;) {
+            int it;
+            bool coop;
+            if (longPhase) {
                 if (threadIdx.x == 0)
                     sItem = (int)atomicAdd(a.queueLong, 1u);
                 if (threadIdx.x < NW)
                     gProd[threadIdx.x] = 0;
                 __syncthreads();
-                const int it = sItem;
-                if (it >= nLong)
-                    break;   // the same item index in every wave
-                const WorkItem w = a.items[it];
-                if (wellFormed(a, w)) {
-                    const Couple c = couple(a, w, grp);
-                    s16x2 best = (TRK & kTrackLocal) ? splat(a.gapOpen) : splat(kNeg16);
-                    s16x2 endE = splat(kNeg16), endF = splat(kNeg16);
-                    const int Nw = w.maxN;
-                    // Block b reads the buffer of wave (b - 1) % NW and writes
-                    // its own; one call site per block kind, in sequence.
-                    const int prev = (wave + NW - 1) % NW;
-                    uint32_t *rbPrev = rbOf(prev);
-                    if (w.nblocks == 1) {
-                        if (wave == 0)
-                            block<true, true, true>(a, c, 0, Nw, l, rbPrev, rowbuf, &gProd[prev], &gProd[wave], sProf,
-                                                    sT, best, nullptr, endE, endF);
-                    } else {
-                        int b = wave;
-                        if (b == 0) {
-                            block<true, false, true>(a, c, 0, Nw, l, rbPrev, rowbuf, &gProd[prev], &gProd[wave], sProf,
-                                                     sT, best, nullptr, endE, endF);
-                            b += NW;
-                        }
-                        for (; b < w.nblocks - 1; b += NW)
-                            block<false, false, true>(a, c, b, Nw, l, rbPrev, rowbuf, &gProd[prev], &gProd[wave], sProf,
-                                                      sT, best, nullptr, endE, endF);
-                        if (b == w.nblocks - 1)
-                            block<false, true, true>(a, c, b, Nw, l, rbPrev, rowbuf, &gProd[prev], &gProd[wave], sProf,
-                                                     sT, best, nullptr, endE, endF);
-                    }
-                    best = groupMax<G>(best);
-                    if (l == 0)
-                        sBest[wave][grp] = asu(best);
-                    __syncthreads();
-                    if (wave == 0 && l == 0) {
-                        s16x2 m = as16(sBest[0][grp]);
-#pragma unroll
-                        for (int v = 1; v < NW; ++v)
-                            m = vmax(m, as16(sBest[v][grp]));
-                        if (c.pid[0] >= 0)
-                            a.scores[c.pid[0]] = (int)m.x - a.gapOpen;
-                        if (c.pid[1] >= 0)
-                            a.scores[c.pid[1]] = (int)m.y - a.gapOpen;
-                    }
+                it = sItem;
+                __syncthreads();   // every wave has read sItem before thread 0 writes it again
+                if (it >= nLong) {
+                    longPhase = false;
+                    continue;
                 }
-                __syncthreads();
+                coop = true;
+            } else {
+                it = 0;
+                if (lane == 0)
+                    it = (int)atomicAdd(a.queue, 1u);
+                it = __builtin_amdgcn_readfirstlane(it) + a.nLong;
+                if (it >= a.nItems)
+                    break;
+                coop = false;
             }
-        }
-
-        // Phase 2: the remaining items, one per wave from the work queue.
-        for (;;) {
-            int it = 0;
-            if (lane == 0)
-                it = (int)atomicAdd(a.queue, 1u);
-            it = __builtin_amdgcn_readfirstlane(it) + a.nLong;
-            if (it >= a.nItems)
-                break;
             const WorkItem w = a.items[it];
-            if (!wellFormed(a, w))
-                continue;
+            const bool ok = wellFormed(a, w);
             const Couple c = couple(a, w, grp);
             s16x2 best = (TRK & kTrackLocal) ? splat(a.gapOpen) : splat(kNeg16);
             s16x2 endE = splat(kNeg16), endF = splat(kNeg16);
@@ -538,22 +517,26 @@ struct Fwd16 {
             uint4 *trItem = nullptr;
             if constexpr (TB)
                 trItem = a.trace + (size_t)a.traceOff[it] * 64 + lane;
-            if (w.nblocks == 1) {
-                block<true, true, false>(a, c, 0, Nw, l, rowbuf, rowbuf, nullptr, nullptr, sProf, sT, best, trItem,
-                                         endE, endF);
-            } else {
-                block<true, false, false>(a, c, 0, Nw, l, rowbuf, rowbuf, nullptr, nullptr, sProf, sT, best, trItem,
-                                          endE, endF);
-                for (int b = 1; b < w.nblocks - 1; ++b)
-                    block<false, false, false>(a, c, b, Nw, l, rowbuf, rowbuf, nullptr, nullptr, sProf, sT, best,
-                                               trItem, endE, endF);
-                block<false, true, false>(a, c, w.nblocks - 1, Nw, l, rowbuf, rowbuf, nullptr, nullptr, sProf, sT,
-                                          best, trItem, endE, endF);
+            // Blocks of this wave: all of them (one item per wave) or wave,
+            // wave + NW, ... (coop: block b reads the buffer of wave
+            // (b - 1) % NW, its progress in gProd[(b - 1) % NW], and publishes
+            // in gProd[b % NW]).
+            const int b0 = coop ? wave : 0, bs = coop ? NW : 1;
+            const uint32_t *rbIn = coop ? rbPrev : rowbuf;
+            uint32_t *prodIn = &gProd[(wave + NW - 1) % NW], *prodOut = &gProd[wave];
+            int b = ok ? b0 : w.nblocks;
+            if (b == 0) {
+                block<true>(a, c, 0, Nw, l, w.nblocks == 1, coop, rbIn, rowbuf, prodIn, prodOut, sProf, sT, best,
+                            trItem, endE, endF);
+                b += bs;
             }
+            for (; b < w.nblocks; b += bs)
+                block<false>(a, c, b, Nw, l, b == w.nblocks - 1, coop, rbIn, rowbuf, prodIn, prodOut, sProf, sT, best,
+                             trItem, endE, endF);
             if constexpr (TB) {
                 // E and F of the end cell (m, n), for the end-cell rule of
                 // the traceback (dp_algorithm_impl.h:1190-1207).
-                if (l == G - 1) {
+                if (ok && l == G - 1) {
                     if (c.pid[0] >= 0) {
                         a.endEF[2 * c.pid[0]] = (int)endE.x;
                         a.endEF[2 * c.pid[0] + 1] = (int)endF.x;
@@ -565,7 +548,19 @@ struct Fwd16 {
                 }
             }
             best = groupMax<G>(best);
-            if (l == 0) {
+            if (coop) {
+                // Combine the waves' results of the item.
+                if (l == 0)
+                    sBest[wave][grp] = asu(best);
+                __syncthreads();
+                if (wave == 0) {
+#pragma unroll
+                    for (int v = 1; v < NW; ++v)
+                        best = vmax(best, as16(sBest[v][grp]));
+                }
+                __syncthreads();
+            }
+            if (ok && l == 0 && (!coop || wave == 0)) {
                 if (c.pid[0] >= 0)
                     a.scores[c.pid[0]] = (int)best.x - a.gapOpen;
                 if (c.pid[1] >= 0)
@@ -585,7 +580,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 
 // Forward pass with trace records (global alignment, G = 16, R = 16).
 template <bool DNA>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void kFwdTrace16(FwdArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kFwdTrace16(FwdArgs a)
 {
     Fwd16<16, 16, DNA, kTrackEnd, true>::run(a);
 }
@@ -700,8 +695,8 @@ __device__ __forceinline__ int dirOf(uint32_t bits)   // 0 diagonal, 1 vertical,
 }
 
 template <class Bits>
-__device__ void walkCounts(int n, int m, const uint8_t *h, const uint8_t *v, int A, int score, int endE,
-                           int endF, Bits &bits, int32_t *cnt)
+__device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const uint8_t *v, int A, int score, int endE,
+                                           int endF, Bits &bits, int32_t *cnt)
 {
     int i = m, j = n;
     int prev = 0;   // last emitted operation: 0 none / aligned pair, 1 gap in H (vertical), 2 gap in V
@@ -758,8 +753,11 @@ __device__ void walkCounts(int n, int m, const uint8_t *h, const uint8_t *v, int
     cnt[A * A + 1] += exts;
 }
 
-// Trace bits of cell (i, j) of one pair from the packed records, with the
-// last record read kept (vertical moves inside a lane reuse it).
+// Trace bits of cell (i, j) of one pair from the packed records of
+// kFwdTrace16, with the last record read kept (moves up inside a lane reuse
+// it).  The walk is latency-bound (each move depends on the last record
+// read); plain per-move reads measured faster than prefetching a lane's next
+// records, batching the residue loads or a branch-free move loop.
 struct Bits16 {
     const uint4 *trace;
     TraceLoc loc;
@@ -774,15 +772,21 @@ struct Bits16 {
             w = trace[idx];
             key = idx;
         }
-        const int sh = k + 16 * loc.half;
-        return ((w.x >> sh) & 1u) | (((w.y >> sh) & 1u) << 1) | (((w.z >> sh) & 1u) << 2) | (((w.w >> sh) & 1u) << 3);
+        const uint32_t ef = k < 8 ? w.x : w.y, dx = k < 8 ? w.z : w.w;
+        const int sh = (k & 7) + 8 * loc.half;
+        return ((ef >> sh) & 1u) | (((ef >> (sh + 16)) & 1u) << 1) | (((dx >> sh) & 1u) << 2) |
+               (((dx >> (sh + 16)) & 1u) << 3);
     }
 };
 
+constexpr int kWalkThreads = 64;
 // Counts live in LDS while walking when alpha^2 + 2 <= kWalkLdsStride.
 constexpr int kWalkLdsStride = 51;
 
-__global__ __launch_bounds__(256) void kWalk16(WalkArgs a)
+// One thread per pair.  LDS: counts accumulate in LDS, else in the output
+// row itself (two kernels rather than one pointer of unknown address space).
+template <bool LDS>
+__global__ __launch_bounds__(kWalkThreads) void kWalk16(WalkArgs a)
 {
     extern __shared__ int32_t sCnt[];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -793,15 +797,18 @@ __global__ __launch_bounds__(256) void kWalk16(WalkArgs a)
     const int64_t ho = a.hOff[pid], vo = a.vOff[pid];
     const int n = (int)(a.hOff[pid + 1] - ho), m = (int)(a.vOff[pid + 1] - vo);
     int32_t *out = a.counts + (int64_t)pid * S;
-    const bool lds = S <= kWalkLdsStride;
-    int32_t *cnt = lds ? sCnt + threadIdx.x * S : out;
-    for (int k = 0; k < S; ++k)
-        cnt[k] = 0;
+    int32_t *cnt;
+    if constexpr (LDS)
+        cnt = sCnt + threadIdx.x * S;
+    else
+        cnt = out;
+    for (int q = 0; q < S; ++q)
+        cnt[q] = 0;
     Bits16 bits{a.trace, a.locs[t]};
     walkCounts(n, m, a.hRes + ho, a.vRes + vo, A, a.scores[pid], a.endEF[2 * pid], a.endEF[2 * pid + 1], bits, cnt);
-    if (lds)
-        for (int k = 0; k < S; ++k)
-            out[k] = cnt[k];
+    if constexpr (LDS)
+        for (int q = 0; q < S; ++q)
+            out[q] = cnt[q];
 }
 
 // int32 forward with trace bytes (same encoding), column-major with column
@@ -993,8 +1000,11 @@ hipError_t launchWalk16(const WalkArgs &args, hipStream_t stream)
     if (args.nPairs <= 0)
         return hipSuccess;
     const int S = args.alpha * args.alpha + 2;
-    const size_t lds = S <= kWalkLdsStride ? (size_t)256 * S * sizeof(int32_t) : 0;
-    hipLaunchKernelGGL(kWalk16, dim3((args.nPairs + 255) / 256), dim3(256), lds, stream, args);
+    const dim3 grid((args.nPairs + kWalkThreads - 1) / kWalkThreads), block(kWalkThreads);
+    if (S <= kWalkLdsStride)
+        hipLaunchKernelGGL(kWalk16<true>, grid, block, (size_t)kWalkThreads * S * sizeof(int32_t), stream, args);
+    else
+        hipLaunchKernelGGL(kWalk16<false>, grid, block, 0, stream, args);
     return hipGetLastError();
 }
 

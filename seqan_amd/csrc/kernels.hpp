@@ -58,7 +58,7 @@ struct FwdArgs {
 
 // Where the trace records of one pair live: rows are bottom-aligned in the
 // item's blocks of 256 rows (row i is item row p + i), lane group lane0..+15,
-// int16 half `half` of every dword.
+// pair A or B (`half`) of the record's byte layout.
 struct TraceLoc {
     int64_t rec0;               // first record of the item (units of 64 records)
     int32_t steps;              // column steps per block
