@@ -473,13 +473,14 @@ int64_t traceBudget(const sqmi_context *ctx)
 }
 
 // Work items with at least this many row blocks are swept by the four waves
-// of a workgroup together (8, or SQMI_COOP_MIN_BLOCKS; 0 disables).
+// of a workgroup together (10, i.e. more than 2304 rows, or
+// SQMI_COOP_MIN_BLOCKS; 0 disables).  Shorter items gain nothing from it.
 int coopMinBlocks()
 {
     const char *env = std::getenv("SQMI_COOP_MIN_BLOCKS");
     if (env && *env)
         return std::max(0, std::atoi(env));
-    return 8;
+    return 10;
 }
 
 // Device copies of what every entry point needs: residues (unless already on
