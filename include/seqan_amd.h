@@ -103,6 +103,7 @@ typedef struct sqmi_stats {
     double host_plan_ms;        /* host planning (bucketing) time               */
     double device_ms;           /* device time of the kernels (HIP events)      */
     double total_ms;            /* wall time of the call                        */
+    int64_t pairs_long;         /* of pairs_int32: long pairs, tiled kernels    */
 } sqmi_stats;
 
 typedef struct sqmi_context sqmi_context;

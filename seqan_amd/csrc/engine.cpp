@@ -51,6 +51,8 @@ struct sqmi_context
         scratch32, table32, emptyIds;
     // Traceback (sqmi_align_counts).
     DevBuf trace, traceOff, locs, endEF, counts;
+    // Long pairs (tiled int32 kernels).
+    DevBuf longCk, longPairs, longWalk, longTrace;
 };
 
 namespace {
@@ -112,6 +114,16 @@ bool fitsInt16(const ScoringInfo &si, int64_t n, int64_t m, bool local)
     const int64_t lowH = local ? 0 : 2LL * si.o + (n + m) * (int64_t)std::min(std::max(si.o, si.e), 0);
     const int64_t low = lowH + std::min({si.o, si.sMin, 0}) + std::min(si.e, 0);
     return up <= 32767 - kGuard16 && low >= (int64_t)kNeg16 + kGuard16 && n <= 30000 && m <= 30000;
+}
+
+// The same bound for the tiled int32 kernels of long pairs, whose minus
+// infinity is -2^29: every real value stays within +-2^28.
+bool fitsLong32(const ScoringInfo &si, int64_t n, int64_t m)
+{
+    const int64_t up = (int64_t)std::max(0, si.sMax) * std::min(n, m);
+    const int64_t lowH = 2LL * si.o + (n + m) * (int64_t)std::min(std::max(si.o, si.e), 0);
+    const int64_t low = lowH + std::min({si.o, si.sMin, 0}) + std::min(si.e, 0);
+    return up < (1LL << 28) && low > -(1LL << 28) && n < (1LL << 30) && m < (1LL << 30);
 }
 
 struct Plan16
@@ -182,12 +194,27 @@ struct Planned
     bool local = false;
     uint32_t track = 0;
     std::vector<int32_t> emptyIds, ids32;
+    std::vector<int32_t> idsLong;   // int32 pairs for the tiled long-pair kernels
     std::vector<int64_t> scratchOff32;
     int64_t scratch32 = 0;
     std::vector<Key> keys16;        // int16 pairs, sorted by bucket key
     std::vector<Plan16> plans;      // one per kernel variant
     int64_t cells = 0;
 };
+
+// int32 pairs whose sides are both at least this long (and that are DNA,
+// global) run the tiled long-pair kernels instead of one thread per pair
+// (2048, or SQMI_LONG_MIN_SIDE).
+int64_t longMinSide()
+{
+    const char *env = std::getenv("SQMI_LONG_MIN_SIDE");
+    if (env && *env && std::atoll(env) > 0)
+        return std::atoll(env);
+    return 2048;
+}
+
+// Columns per tile of the long-pair kernels.
+constexpr int kLongTileC = 512;
 
 // Stable LSD radix sort of 26-bit keys in two passes of 13 bits, each pass
 // split into contiguous per-thread ranges so that equal digits keep their
@@ -325,10 +352,16 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
     size_t e32 = n16;
     while (e32 < keys.size() && (keys[e32].key >> kKeyClassShift) == 2)
         ++e32;
+    const int64_t longMin = longMinSide();
+    const bool longOk = si.dna && mode == 0;
     for (size_t k = n16; k < e32; ++k) {
         const int32_t i = keys[k].id;
         const int64_t n = H.offsets[i + 1] - H.offsets[i];
         const int64_t m = V.offsets[i + 1] - V.offsets[i];
+        if (longOk && n >= longMin && m >= longMin && fitsLong32(si, n, m)) {
+            P.idsLong.push_back(i);
+            continue;
+        }
         P.ids32.push_back(i);
         P.scratchOff32.push_back(P.scratch32);
         // score: H and E columns (int32 pairs); traceback: the columns plus
@@ -413,9 +446,10 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
 
 void fillStats(const Planned &P, sqmi_stats &st)
 {
-    st.pairs = (int64_t)(P.emptyIds.size() + P.ids32.size() + P.keys16.size());
+    st.pairs = (int64_t)(P.emptyIds.size() + P.ids32.size() + P.idsLong.size() + P.keys16.size());
     st.cells = P.cells;
-    st.pairs_int32 = (int64_t)P.ids32.size();
+    st.pairs_int32 = (int64_t)(P.ids32.size() + P.idsLong.size());
+    st.pairs_long = (int64_t)P.idsLong.size();
     st.pairs_int16 = (int64_t)P.keys16.size();
     int64_t items = 0, padded = 0;
     for (const Plan16 &pl : P.plans) {
@@ -552,6 +586,103 @@ sqmi_status uploadInputs(sqmi_context *ctx, const sqmi_sequences *setH, const sq
     return SQMI_OK;
 }
 
+// Long pairs: the tiled int32 forward (one launch per tile anti-diagonal),
+// then, with `dCounts`, the checkpointed traceback: per round every
+// unfinished pair's current tile is recomputed with trace records and
+// walked.  Pairs run in chunks whose checkpoints fit traceBudget().  `pairs`
+// holds the host copy of the descriptors until the caller synchronises.
+sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV, const Planned &P,
+                    const Inputs &in, int32_t *dCounts, std::vector<LongPair> &pairs)
+{
+    if (P.idsLong.empty())
+        return SQMI_OK;
+    hipStream_t stream = ctx->stream;
+    const int C = kLongTileC;
+    const int64_t budget = traceBudget(ctx);
+    const int64_t N = setH->count;
+    // Descriptors with checkpoint offsets relative to the pair's chunk.
+    std::vector<size_t> chunkEnd;
+    int64_t run = 0, maxRun = 0;
+    pairs.clear();
+    for (int32_t id : P.idsLong) {
+        LongPair lp{};
+        lp.pid = id;
+        lp.n = (int32_t)(setH->offsets[id + 1] - setH->offsets[id]);
+        lp.m = (int32_t)(setV->offsets[id + 1] - setV->offsets[id]);
+        lp.nblocks = (int32_t)(((int64_t)lp.m + 1 + kTileRows - 1) / kTileRows);
+        lp.ntiles = (lp.n + C - 1) / C;
+        lp.p = lp.nblocks * kTileRows - 1 - lp.m;
+        const int64_t need = (int64_t)lp.nblocks * (lp.n + 1) * 2 + (int64_t)lp.nblocks * lp.ntiles * kTileRows * 2;
+        if (run > 0 && (run + need) * 4 > budget) {
+            chunkEnd.push_back(pairs.size());
+            run = 0;
+        }
+        lp.rowck = run;
+        lp.colck = run + (int64_t)lp.nblocks * (lp.n + 1) * 2;
+        run += need;
+        maxRun = std::max(maxRun, run);
+        pairs.push_back(lp);
+    }
+    chunkEnd.push_back(pairs.size());
+    const size_t np = pairs.size();
+    hipError_t err;
+    if ((err = ensure(ctx->longCk, (size_t)maxRun * 4)) != hipSuccess ||
+        (err = ensure(ctx->longPairs, sizeof(LongPair) * np)) != hipSuccess ||
+        (err = ensure(ctx->endEF, (size_t)8 * N)) != hipSuccess)
+        return hipFail(ctx, err, "allocating long-pair buffers");
+    if (dCounts && ((err = ensure(ctx->longWalk, (size_t)4 * kWalkStateInts * np)) != hipSuccess ||
+                    (err = ensure(ctx->longTrace, sizeof(uint2) * (size_t)(C + kTileSlack) * 16 * np)) != hipSuccess))
+        return hipFail(ctx, err, "allocating long-pair traceback buffers");
+    if ((err = hipMemcpyAsync(ctx->longPairs.p, pairs.data(), sizeof(LongPair) * np, hipMemcpyHostToDevice,
+                              stream)) != hipSuccess)
+        return hipFail(ctx, err, "uploading long pairs");
+    size_t b = 0;
+    for (size_t e : chunkEnd) {
+        TileArgs a{};
+        a.hRes = in.dH;
+        a.vRes = in.dV;
+        a.hOff = static_cast<const int64_t *>(ctx->hOff.p);
+        a.vOff = static_cast<const int64_t *>(ctx->vOff.p);
+        a.pairs = static_cast<const LongPair *>(ctx->longPairs.p) + b;
+        a.nPairs = (int32_t)(e - b);
+        a.tileC = C;
+        a.ck = static_cast<int32_t *>(ctx->longCk.p);
+        a.gapOpen = P.si.o;
+        a.gapExtend = P.si.e;
+        a.dnaProfile = static_cast<const uint32_t *>(ctx->dnaProf.p);
+        a.scores = in.dScores;
+        a.endEF = static_cast<int32_t *>(ctx->endEF.p);
+        a.alpha = P.si.alpha;
+        int maxB = 0, maxT = 0, maxBT = 0;
+        for (size_t k = b; k < e; ++k) {
+            maxB = std::max(maxB, pairs[k].nblocks);
+            maxT = std::max(maxT, pairs[k].ntiles);
+            maxBT = std::max(maxBT, pairs[k].nblocks + pairs[k].ntiles);
+        }
+        a.wavesPerPair = (maxB + 3) / 4;
+        for (int d = 0; d < maxB + maxT - 1; ++d) {
+            a.diag = d;
+            if ((err = launchTileFwd32(a, stream)) != hipSuccess)
+                return hipFail(ctx, err, "launching long-pair forward");
+        }
+        if (dCounts) {
+            a.trace = static_cast<uint2 *>(ctx->longTrace.p);
+            a.walk = static_cast<int32_t *>(ctx->longWalk.p);
+            a.counts = dCounts;
+            if ((err = launchTileWalkInit(a, stream)) != hipSuccess)
+                return hipFail(ctx, err, "launching long-pair walk init");
+            // A walk crosses at most nblocks + ntiles tiles.
+            for (int r = 0; r < maxBT; ++r) {
+                if ((err = launchTileTrace32(a, stream)) != hipSuccess ||
+                    (err = launchTileWalk32(a, stream)) != hipSuccess)
+                    return hipFail(ctx, err, "launching long-pair traceback");
+            }
+        }
+        b = e;
+    }
+    return SQMI_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -612,7 +743,7 @@ sqmi_status sqmi_destroy(sqmi_context *ctx)
     for (DevBuf *b : {&ctx->hRes, &ctx->vRes, &ctx->hOff, &ctx->vOff, &ctx->order, &ctx->items, &ctx->queue,
                       &ctx->rowbuf, &ctx->scores, &ctx->tTable, &ctx->dnaProf, &ctx->ids32, &ctx->scratchOff32,
                       &ctx->scratch32, &ctx->table32, &ctx->emptyIds, &ctx->trace, &ctx->traceOff, &ctx->locs,
-                      &ctx->endEF, &ctx->counts})
+                      &ctx->endEF, &ctx->counts, &ctx->longCk, &ctx->longPairs, &ctx->longWalk, &ctx->longTrace})
         release(*b);
     if (ctx->ev0)
         (void)hipEventDestroy(ctx->ev0);
@@ -835,6 +966,9 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
         if ((err = launchSimple32(a, stream)) != hipSuccess)
             return hipFail(ctx, err, "launching int32 kernel");
     }
+    std::vector<LongPair> longPairs;
+    if ((st = runLong(ctx, setH, setV, P, in, nullptr, longPairs)) != SQMI_OK)
+        return st;
     if (!emptyIds.empty()) {
         if ((err = ensure(ctx->emptyIds, 4 * emptyIds.size())) != hipSuccess)
             return hipFail(ctx, err, "allocating empty list");
@@ -1094,6 +1228,9 @@ sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *setH, con
             b = e;
         }
     }
+    std::vector<LongPair> longPairs;
+    if ((st = runLong(ctx, setH, setV, P, in, dCounts, longPairs)) != SQMI_OK)
+        return st;
     if (!P.emptyIds.empty()) {
         if ((err = ensure(ctx->emptyIds, 4 * P.emptyIds.size())) != hipSuccess)
             return hipFail(ctx, err, "allocating empty list");

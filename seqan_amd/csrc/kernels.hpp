@@ -139,6 +139,51 @@ hipError_t launchSimple32(const Simple32Args &args, hipStream_t stream);
 // scores[ids[k]] = value for k < n.
 hipError_t launchFillScores(int32_t *scores, const int32_t *ids, int n, int32_t value, hipStream_t stream);
 
+// Long pairs (int32, tiled).  A pair with m rows and n columns is cut into
+// row blocks of kTileRows (bottom-aligned like the packed kernels: item row
+// p + i holds pair row i, p = nblocks * kTileRows - 1 - m) and column tiles of
+// tileC columns (tile t holds columns t * tileC + 1 ... (t + 1) * tileC).
+// Tile (b, t) needs the bottom row of tile (b - 1, t) and the right column of
+// tile (b, t - 1): the forward runs one launch per tile anti-diagonal b + t,
+// and keeps every tile border as a checkpoint for the traceback.
+constexpr int kTileRows = 256;
+struct LongPair {
+    int32_t pid;
+    int32_t n, m;
+    int32_t nblocks, ntiles;
+    int32_t p;
+    int64_t rowck;      // int32 offset of the row checkpoints: nblocks x (n + 1) x (G, F)
+    int64_t colck;      // int32 offset of the column checkpoints: nblocks x ntiles x kTileRows x (G, E)
+};
+struct TileArgs {
+    const uint8_t *hRes;
+    const uint8_t *vRes;
+    const int64_t *hOff;
+    const int64_t *vOff;
+    const LongPair *pairs;
+    int32_t nPairs;
+    int32_t wavesPerPair;       // forward: waves per pair in the grid (4 blocks per wave)
+    int32_t diag;               // forward: tile anti-diagonal b + t
+    int32_t tileC;
+    int32_t *ck;                // checkpoint storage
+    int32_t gapOpen;
+    int32_t gapExtend;
+    const uint32_t *dnaProfile;
+    int32_t *scores;            // per pair id
+    int32_t *endEF;             // per pair id: E, F of cell (m, n)
+    // traceback
+    uint2 *trace;               // per long pair: (tileC + kTileSlack) steps x 16 lanes records
+    int32_t *walk;              // per long pair: walk state (kWalkStateInts ints)
+    int32_t *counts;            // per pair id: alpha^2 + 2 counts
+    int32_t alpha;
+};
+constexpr int kTileSlack = 24;        // steps per tile beyond its width (skew + prefetch)
+constexpr int kWalkStateInts = 8;
+hipError_t launchTileFwd32(const TileArgs &args, hipStream_t stream);
+hipError_t launchTileTrace32(const TileArgs &args, hipStream_t stream);
+hipError_t launchTileWalkInit(const TileArgs &args, hipStream_t stream);
+hipError_t launchTileWalk32(const TileArgs &args, hipStream_t stream);
+
 // Trace family.
 constexpr int kTraceRows = 256;       // rows per block of the trace kernel (G = 16, R = 16)
 constexpr int kTraceLanes = 16;

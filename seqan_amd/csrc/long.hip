@@ -1,0 +1,404 @@
+// long.hip -- int32 kernels for long pairs (global alignment, DNA profile):
+// the tiled forward that keeps every tile border as a checkpoint, and the
+// checkpointed traceback that recomputes one tile per pair at a time.
+//
+// The recurrence, borders and traceback decisions are those of the packed
+// kernels (kernels.hip), in int32: SeqAn 2's Gotoh formula
+// (include/seqan/align/dp_formula_affine.h:62-126), the global borders
+// (dp_formula_affine.h:244-330), the trace decisions with the tie rules of
+// _maxScore (dp_formula.h:165-201), and the walk of dp_traceback_impl.h:
+// 115-546 counted like computeAlignmentStats (evaluate_alignment.h:215-300).
+//
+// A tile is kTileRows rows (one row block of the pair, bottom-aligned) by
+// tileC columns.  One group of 16 lanes sweeps a tile: lane l owns rows
+// 16 l .. 16 l + 15 of the block and handles column c0 - 1 + s - l at step s
+// (a skewed wavefront; the cell above the lane's first row comes from lane
+// l - 1 by DPP row_shr:1, lane 0 reads the block above from the row
+// checkpoint).  Inputs: the left column (G = H + gapOpen and E of the
+// block's rows at column c0 - 1) and the top row (G and F of the block
+// above at columns c0 - 1 .. c1).  Outputs: the bottom row and the right
+// column (forward), or 8-byte trace records per lane and step (traceback).
+#include "kernels.hpp"
+#include "seqan_amd.h"
+
+namespace sqmi {
+namespace {
+
+constexpr int kNegL = -(1 << 29);   // minus infinity; real values stay far above
+constexpr int TG = 16;              // lanes per group
+constexpr int TR = kTileRows / TG;  // rows per lane
+constexpr int TQD = 4;              // residue prefetch depth (steps)
+
+__device__ __forceinline__ int shiftIn16(int v, int first)
+{
+    return __builtin_amdgcn_update_dpp(first, v, 0x111, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ int subSat32(int a, int b) { return __builtin_elementwise_sub_sat(a, b); }
+
+// Bit k of 32-bit mask pairs: (1 << k) | (1 << (16 + k)).
+__device__ __forceinline__ uint32_t rowBits(int k) { return 0x00010001u << k; }
+
+struct TileGeom {
+    int b, t;            // row block and column tile
+    int c0, c1;          // columns c0 .. c1 of the tile (1-based)
+    bool active;         // group has a tile to sweep
+};
+
+// One tile sweep for the lane's group.  TRACE: write trace records instead
+// of the forward outputs.  HEAD: the tile may hold rows above the pair's row
+// 0 (block 0), which are floored at -inf.
+template <bool HEAD, bool TRACE>
+__device__ __forceinline__ void tileSweep(const TileArgs &a, const LongPair &P, const TileGeom &g, int l,
+                                          const uint32_t *sProf, uint2 *rec)
+{
+    const int o = a.gapOpen, e = a.gapExtend, eo = max(o, e);
+    const int C = a.tileC;
+    const uint8_t *h = a.hRes + a.hOff[P.pid];
+    const uint8_t *v = a.vRes + a.vOff[P.pid];
+    const int n = P.n, m = P.m;
+    const int rowbase = g.b * kTileRows + l * TR;
+    int32_t *ck = a.ck;
+    const int64_t rowIn = P.rowck + ((int64_t)(g.b - 1) * (n + 1)) * 2;   // block b - 1's bottom row
+    const int64_t rowOut = P.rowck + ((int64_t)g.b * (n + 1)) * 2;
+    const int64_t colIn = P.colck + (((int64_t)g.b * P.ntiles + (g.t - 1)) * kTileRows + l * TR) * 2;
+    const int64_t colOut = P.colck + (((int64_t)g.b * P.ntiles + g.t) * kTileRows + l * TR) * 2;
+
+    uint32_t sel[TR];
+    int Gk[TR], Ek[TR];
+#pragma unroll
+    for (int k = 0; k < TR; ++k) {
+        const int i = rowbase + k - P.p;
+        int code = 4;
+        if (i >= 1 && i <= m)
+            code = min((int)v[i - 1], 4);
+        sel[k] = code < 4 ? (0x0c0c0c00u | (uint32_t)code) : 0x0d0d0d0cu;   // pad: t = -256
+        if (g.t == 0) {
+            // Column 0 of a global alignment: -inf above row 0, H(0,0) = 0,
+            // then the gap chain growing by max(o, e) per row.
+            Gk[k] = (i < 0) ? kNegL : (i == 0) ? o : (o + (i - 1) * eo) + o;
+            Ek[k] = kNegL;
+        } else if (g.active) {
+            Gk[k] = ck[colIn + 2 * k];
+            Ek[k] = ck[colIn + 2 * k + 1];
+        } else {
+            Gk[k] = kNegL;
+            Ek[k] = kNegL;
+        }
+    }
+    // Top input of the group head at column c0 - 1 + s, prefetched a step ahead.
+    auto topAt = [&](int col) -> int2 {
+        if (g.b == 0 || !g.active || col > g.c1)
+            return make_int2(kNegL, kNegL);
+        return make_int2(ck[rowIn + 2 * col], ck[rowIn + 2 * col + 1]);
+    };
+    int2 top = topAt(g.c0 - 1);
+    if (!TRACE && g.active && g.t == 0 && l == TG - 1)
+        *reinterpret_cast<int2 *>(ck + rowOut) = make_int2(Gk[TR - 1], kNegL);   // column 0 of the bottom row
+
+    int botG = Gk[TR - 1], botF = kNegL;
+    int upGprev = kNegL;
+    const int width = g.c1 - g.c0 + 1;
+    const int steps = ((width + TG + TQD - 1) / TQD) * TQD;
+    auto hcode = [&](int jj) -> int {
+        const int idx = min(max(jj, 1), n) - 1;
+        const int c = min((int)h[idx], 4);
+        return (jj >= 1 && jj <= n) ? c : 4;
+    };
+    uint32_t q[TQD];
+#pragma unroll
+    for (int u = 0; u < TQD; ++u)
+        q[u] = (uint32_t)hcode(g.c0 - 1 + u - l);
+    uint32_t prof = sProf[q[0]];
+
+    auto step = [&](int s, uint32_t pr) {
+        const int jj = g.c0 - 1 + s - l;
+        const int upG = shiftIn16(botG, top.x);
+        const int upF = shiftIn16(botF, top.y);
+        top = topAt(g.c0 + s);
+        const int diagTop = upGprev;
+        upGprev = upG;
+        if (jj >= g.c0 && jj <= g.c1) {
+            int diag = diagTop, uG = upG, uF = upF;
+            uint32_t accEF = 0, accDX = 0;
+#pragma unroll
+            for (int k = 0; k < TR; ++k) {
+                const int tv = (int)__builtin_amdgcn_perm(0u, pr, sel[k]);
+                const int D = diag + tv;
+                const int Eext = Ek[k] + e, Fext = uF + e;
+                int F, H;
+                if constexpr (TRACE) {
+                    // Sign bytes of the decisions: E opened, F opened (bits k,
+                    // 16 + k of accEF); not diagonal, E beats F (accDX).
+                    const uint32_t ef = __builtin_amdgcn_perm((uint32_t)subSat32(Fext, uG),
+                                                              (uint32_t)subSat32(Eext, Gk[k]), 0x0b0b0909u);
+                    Ek[k] = max(Eext, Gk[k]);
+                    F = max(Fext, uG);
+                    const int fe = max(Ek[k], F);
+                    const uint32_t dx = __builtin_amdgcn_perm((uint32_t)subSat32(F, Ek[k]),
+                                                              (uint32_t)subSat32(D, fe), 0x0b0b0909u);
+                    H = max(D, fe);
+                    const uint32_t bit = rowBits(k);
+                    asm volatile("v_bfi_b32 %0, %1, %2, %0" : "+v"(accEF) : "s"(bit), "v"(ef));
+                    asm volatile("v_bfi_b32 %0, %1, %2, %0" : "+v"(accDX) : "s"(bit), "v"(dx));
+                } else {
+                    Ek[k] = max(Eext, Gk[k]);
+                    F = max(Fext, uG);
+                    H = max(D, max(Ek[k], F));
+                }
+                if constexpr (HEAD)
+                    H = max(H, kNegL);
+                diag = Gk[k];
+                const int Gn = H + o;
+                Gk[k] = Gn;
+                uG = Gn;
+                uF = F;
+            }
+            botG = uG;
+            botF = uF;
+            if (g.active) {
+                if constexpr (TRACE) {
+                    rec[s * TG + l] = make_uint2(accEF, accDX);
+                } else {
+                    if (l == TG - 1)
+                        *reinterpret_cast<int2 *>(ck + rowOut + 2 * jj) = make_int2(botG, botF);
+                    if (jj == g.c1 && g.t + 1 < P.ntiles) {
+#pragma unroll
+                        for (int k = 0; k < TR; ++k)
+                            *reinterpret_cast<int2 *>(ck + colOut + 2 * k) = make_int2(Gk[k], Ek[k]);
+                    }
+                    // Cell (m, n): the last row of the last block.
+                    if (l == TG - 1 && jj == n && g.b == P.nblocks - 1) {
+                        a.scores[P.pid] = botG - o;
+                        a.endEF[2 * P.pid] = Ek[TR - 1];
+                        a.endEF[2 * P.pid + 1] = botF;
+                    }
+                }
+            }
+        }
+    };
+    for (int s0 = 0; s0 < steps; s0 += TQD) {
+#pragma unroll
+        for (int u = 0; u < TQD; ++u) {
+            const int s = s0 + u;
+            const uint32_t nxt = sProf[q[(u + 1) % TQD]];
+            q[u] = (uint32_t)hcode(g.c0 - 1 + s + TQD - l);
+            step(s, prof);
+            prof = nxt;
+        }
+    }
+}
+
+__device__ __forceinline__ void loadProfile(const TileArgs &a, uint32_t *sProf)
+{
+    if (threadIdx.x < 5)
+        sProf[threadIdx.x] = a.dnaProfile[threadIdx.x];
+    __syncthreads();
+}
+
+// Forward: the tiles of anti-diagonal a.diag.  Each wave takes four
+// consecutive row blocks of one pair (one per 16-lane group).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kTileFwd32(TileArgs a)
+{
+    __shared__ uint32_t sProf[8];
+    loadProfile(a, sProf);
+    const int lane = threadIdx.x & 63, l = lane % TG, grp = lane / TG;
+    const int w = blockIdx.x * (blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+    const int pi = w / a.wavesPerPair;
+    if (pi >= a.nPairs)
+        return;
+    const LongPair P = a.pairs[pi];
+    const int b0 = (w % a.wavesPerPair) * 4;
+    TileGeom g;
+    g.b = b0 + grp;
+    g.t = a.diag - g.b;
+    g.active = g.b < P.nblocks && g.t >= 0 && g.t < P.ntiles;
+    if (!__any(g.active))
+        return;   // the whole wave is idle on this diagonal
+    if (!g.active) {
+        g.b = min(g.b, P.nblocks - 1);
+        g.t = min(max(g.t, 0), P.ntiles - 1);
+    }
+    g.c0 = g.t * a.tileC + 1;
+    g.c1 = min((g.t + 1) * a.tileC, P.n);
+    if (b0 == 0)
+        tileSweep<true, false>(a, P, g, l, sProf, nullptr);
+    else
+        tileSweep<false, false>(a, P, g, l, sProf, nullptr);
+}
+
+// Walk state of one long pair: the cell (i, j) reached, the walk mode
+// (0: choose from the cell's H decision, 1 / 2: inside a vertical /
+// horizontal gap), the last operation, gap opens / extensions, done flag.
+enum { WS_I = 0, WS_J, WS_MODE, WS_PREV, WS_OPENS, WS_EXTS, WS_DONE };
+
+__device__ __forceinline__ TileGeom walkTile(const TileArgs &a, const LongPair &P, const int32_t *ws)
+{
+    TileGeom g;
+    const int i = ws[WS_I], j = ws[WS_J];
+    g.active = ws[WS_DONE] == 0 && i > 0 && j > 0;
+    const int ii = g.active ? i : P.m, jj = g.active ? j : P.n;
+    g.b = (P.p + ii) / kTileRows;
+    g.t = (jj - 1) / a.tileC;
+    g.c0 = g.t * a.tileC + 1;
+    g.c1 = min((g.t + 1) * a.tileC, P.n);
+    return g;
+}
+
+// Traceback, step 1: recompute each unfinished pair's current tile (the one
+// holding its walk position) with trace records.  One pair per group.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kTileTrace32(TileArgs a)
+{
+    __shared__ uint32_t sProf[8];
+    loadProfile(a, sProf);
+    const int lane = threadIdx.x & 63, l = lane % TG;
+    const int pi = (blockIdx.x * (blockDim.x >> 6) + (int)(threadIdx.x >> 6)) * 4 + lane / TG;
+    const bool exists = pi < a.nPairs;
+    const int pc = exists ? pi : a.nPairs - 1;
+    const LongPair P = a.pairs[pc];
+    TileGeom g = walkTile(a, P, a.walk + (int64_t)pc * kWalkStateInts);
+    g.active = g.active && exists;
+    if (!__any(g.active))
+        return;
+    uint2 *rec = a.trace + (int64_t)pc * (a.tileC + kTileSlack) * TG;
+    tileSweep<true, true>(a, P, g, l, sProf, rec);
+}
+
+__global__ void kTileWalkInit(TileArgs a)
+{
+    const int pi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= a.nPairs)
+        return;
+    const LongPair P = a.pairs[pi];
+    int32_t *ws = a.walk + (int64_t)pi * kWalkStateInts;
+    const int score = a.scores[P.pid], endE = a.endEF[2 * P.pid], endF = a.endEF[2 * P.pid + 1];
+    ws[WS_I] = P.m;
+    ws[WS_J] = P.n;
+    // End cell: F, then E, win ties (dp_algorithm_impl.h:1190-1207,
+    // PreferGapsAtEnd_ dp_traceback_impl.h:115-122).
+    ws[WS_MODE] = (endF == score) ? 1 : (endE == score) ? 2 : 0;
+    ws[WS_PREV] = 0;
+    ws[WS_OPENS] = 0;
+    ws[WS_EXTS] = 0;
+    ws[WS_DONE] = 0;
+    const int S = a.alpha * a.alpha + 2;
+    for (int k = 0; k < S; ++k)
+        a.counts[(int64_t)P.pid * S + k] = 0;
+}
+
+// Traceback, step 2: walk each unfinished pair through its current tile.
+__global__ __launch_bounds__(64) void kTileWalk32(TileArgs a)
+{
+    const int pi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= a.nPairs)
+        return;
+    const LongPair P = a.pairs[pi];
+    int32_t *ws = a.walk + (int64_t)pi * kWalkStateInts;
+    if (ws[WS_DONE])
+        return;
+    const TileGeom g = walkTile(a, P, ws);
+    const uint2 *rec = a.trace + (int64_t)pi * (a.tileC + kTileSlack) * TG;
+    const uint8_t *h = a.hRes + a.hOff[P.pid];
+    const uint8_t *v = a.vRes + a.vOff[P.pid];
+    const int A = a.alpha, S = A * A + 2;
+    int32_t *cnt = a.counts + (int64_t)P.pid * S;
+    // Substitution counts of this stretch in LDS (DNA: alpha <= 4), added to
+    // the pair's counts at the end.
+    __shared__ int32_t sC[64 * 16];
+    int32_t *lc = sC + threadIdx.x * 16;
+    for (int k = 0; k < 16; ++k)
+        lc[k] = 0;
+    int i = ws[WS_I], j = ws[WS_J], mode = ws[WS_MODE], prev = ws[WS_PREV];
+    int opens = ws[WS_OPENS], exts = ws[WS_EXTS];
+    const int iLo = g.b * kTileRows - P.p;   // first pair row of the tile
+    while (i > 0 && j > 0 && i >= iLo && j >= g.c0) {
+        const int r = P.p + i - g.b * kTileRows;
+        const int l = r / TR, k = r % TR;
+        const uint2 w = rec[(j - (g.c0 - 1) + l) * TG + l];
+        const uint32_t sh = (uint32_t)k;
+        const uint32_t bits = ((w.x >> sh) & 1u) | (((w.x >> (sh + 16)) & 1u) << 1) | (((w.y >> sh) & 1u) << 2) |
+                              (((w.y >> (sh + 16)) & 1u) << 3);
+        const int d = mode == 0 ? ((bits & 4u) ? ((bits & 8u) ? 2 : 1) : 0) : mode;
+        if (d == 0) {
+            const int hc = min((int)h[j - 1], A - 1), vc = min((int)v[i - 1], A - 1);
+            lc[hc * A + vc] += 1;
+            prev = 0;
+            --i;
+            --j;
+            mode = 0;
+        } else {
+            // One gap character; the gap goes on while the cell's gap
+            // extends, down to row / column 1 (dp_traceback_impl.h:220-341).
+            const bool vert = d == 1;
+            const bool open = (bits & (vert ? 2u : 1u)) != 0;
+            if (prev == d) ++exts; else ++opens;
+            prev = d;
+            const bool stop = open || (vert ? i == 1 : j == 1);
+            if (vert)
+                --i;
+            else
+                --j;
+            mode = stop ? 0 : d;
+        }
+    }
+    for (int k = 0; k < A * A; ++k)
+        if (lc[k])
+            cnt[k] += lc[k];
+    if (i == 0 || j == 0) {
+        // Leading gaps down to the origin (dp_traceback_impl.h:522-545).
+        if (i > 0) {
+            if (prev == 1) exts += i; else { ++opens; exts += i - 1; }
+            prev = 1;
+        }
+        if (j > 0) {
+            if (prev == 2) exts += j; else { ++opens; exts += j - 1; }
+        }
+        cnt[A * A] += opens;
+        cnt[A * A + 1] += exts;
+        ws[WS_DONE] = 1;
+    }
+    ws[WS_I] = i;
+    ws[WS_J] = j;
+    ws[WS_MODE] = mode;
+    ws[WS_PREV] = prev;
+    ws[WS_OPENS] = opens;
+    ws[WS_EXTS] = exts;
+}
+
+}  // namespace
+
+hipError_t launchTileFwd32(const TileArgs &args, hipStream_t stream)
+{
+    const int waves = args.nPairs * args.wavesPerPair;
+    if (waves <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kTileFwd32, dim3((waves + 3) / 4), dim3(256), 0, stream, args);
+    return hipGetLastError();
+}
+
+hipError_t launchTileTrace32(const TileArgs &args, hipStream_t stream)
+{
+    const int waves = (args.nPairs + 3) / 4;
+    if (waves <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kTileTrace32, dim3((waves + 3) / 4), dim3(256), 0, stream, args);
+    return hipGetLastError();
+}
+
+hipError_t launchTileWalkInit(const TileArgs &args, hipStream_t stream)
+{
+    if (args.nPairs <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kTileWalkInit, dim3((args.nPairs + 63) / 64), dim3(64), 0, stream, args);
+    return hipGetLastError();
+}
+
+hipError_t launchTileWalk32(const TileArgs &args, hipStream_t stream)
+{
+    if (args.nPairs <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kTileWalk32, dim3((args.nPairs + 63) / 64), dim3(64), 0, stream, args);
+    return hipGetLastError();
+}
+
+}  // namespace sqmi

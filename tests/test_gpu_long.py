@@ -1,0 +1,188 @@
+"""Long pairs: the tiled int32 forward that keeps every tile border (256 rows
+x 512 columns) as a checkpoint, and the checkpointed traceback that walks
+back one recomputed tile at a time.
+
+Global DNA pairs outside the int16 window with both sides >=
+SQMI_LONG_MIN_SIDE (2048 by default) take this route.  The parity tests force
+int32 (FORCE_INT32) and lower the threshold to 256 so that pairs of a few
+thousand bases, which the oracle finishes in seconds, cross many row blocks
+and column tiles; the full 64 x 100 kbp workload is checked through
+properties every alignment satisfies (sum of count x parameter == score,
+aligned pairs + gap characters == residues) and against the packed kernels
+on its first pairs cut short."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+import seqan_amd as sa
+
+pytestmark = pytest.mark.gpu
+
+F32 = sa.FORCE_INT32
+SCORES = [sa.simple_score(2, -3, -1, -5), sa.simple_score(1, -1, -1, -2), sa.simple_score(5, -4, -2, -11),
+          sa.simple_score(3, -2, -3, -2)]
+
+
+def _params(score):
+    return np.concatenate([score.table.ravel(), [score.gap_open, score.gap_extend]]).astype(np.int64)
+
+
+def _check_invariants(H, V, score, scores, counts):
+    # sum(count x parameter) == score needs gapExtend >= gapOpen: otherwise
+    # the DP may score two adjacent gap characters as two opens, which the
+    # run-length counts (evaluate_alignment.h:215-300) report as open + extend.
+    A = score.alphabet_size
+    if score.gap_extend >= score.gap_open:
+        assert np.array_equal(counts.astype(np.int64) @ _params(score), scores.astype(np.int64))
+    matched = counts[:, :A * A].sum(axis=1).astype(np.int64)
+    gaps = counts[:, A * A].astype(np.int64) + counts[:, A * A + 1]
+    assert np.array_equal(gaps, H.lengths() + V.lengths() - 2 * matched)
+    assert (counts >= 0).all()
+
+
+def _mutated_pairs(rng, count, lo, hi, sub=0.08, indel=0.02, unrelated=0.2):
+    hs, vs = [], []
+    for _ in range(count):
+        n = int(rng.integers(lo, hi + 1))
+        h = rng.integers(0, 4, n).astype(np.uint8)
+        if rng.random() < unrelated:
+            v = rng.integers(0, 4, int(rng.integers(lo, hi + 1))).astype(np.uint8)
+        else:
+            keep = rng.random(n) >= indel / 2
+            v = h[keep].copy()
+            flip = rng.random(len(v)) < sub
+            v[flip] = rng.integers(0, 4, int(flip.sum()))
+            ins = np.nonzero(rng.random(len(v)) < indel / 2)[0]
+            v = np.insert(v, ins, rng.integers(0, 4, len(ins)).astype(np.uint8))
+        hs.append(h)
+        vs.append(v.astype(np.uint8))
+    return sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+
+
+def _oracle_scores(H, V, score):
+    return po.score_batch(H.residues, H.offsets, V.residues, V.offsets,
+                          po.Scoring(score.table, score.gap_open, score.gap_extend), 0)
+
+
+def _oracle_counts(H, V, score):
+    return po.traceback_batch(H.residues, H.offsets, V.residues, V.offsets,
+                              po.Scoring(score.table, score.gap_open, score.gap_extend), 0)
+
+
+@pytest.fixture
+def long_256(monkeypatch):
+    monkeypatch.setenv("SQMI_LONG_MIN_SIDE", "256")
+
+
+@pytest.mark.parametrize("lo,hi", [(256, 700), (900, 2600)])
+def test_long_scores_match_oracle(ctx, long_256, lo, hi):
+    rng = np.random.default_rng(lo + hi)
+    H, V = _mutated_pairs(rng, 40, lo, hi)
+    for score in SCORES:
+        print("long scores", lo, hi, score.table[0, :2], score.gap_extend, score.gap_open, flush=True)
+        s = ctx.align_score(H, V, score, 0, flags=F32)
+        assert ctx.stats()["pairs_long"] == len(H)
+        assert np.array_equal(s, _oracle_scores(H, V, score))
+
+
+@pytest.mark.parametrize("lo,hi", [(256, 700), (900, 2600)])
+def test_long_counts_match_oracle(ctx, long_256, lo, hi):
+    rng = np.random.default_rng(3 * lo + hi)
+    H, V = _mutated_pairs(rng, 40, lo, hi)
+    for score in SCORES:
+        print("long counts", lo, hi, score.table[0, :2], score.gap_extend, score.gap_open, flush=True)
+        s, c = ctx.align_counts(H, V, score, flags=F32)
+        assert ctx.stats()["pairs_long"] == len(H)
+        ws, wc = _oracle_counts(H, V, score)
+        assert np.array_equal(s, ws)
+        bad = np.nonzero((c != wc).any(axis=1))[0]
+        assert len(bad) == 0, (bad[:5], c[bad[:1]], wc[bad[:1]])
+        _check_invariants(H, V, score, s, c)
+
+
+def test_long_edges(ctx, long_256):
+    # Sides at and around the block (256 rows) and tile (512 columns) edges,
+    # very unequal sides, and identical / unrelated sequences.
+    rng = np.random.default_rng(11)
+    shapes = [(256, 256), (511, 257), (512, 255 + 256), (513, 767), (1024, 1023), (3000, 260), (260, 3000),
+              (2049, 2048), (700, 1800)]
+    hs, vs = [], []
+    for n, m in shapes:
+        hs.append(rng.integers(0, 4, n).astype(np.uint8))
+        vs.append(rng.integers(0, 4, m).astype(np.uint8))
+    same = rng.integers(0, 4, 1500).astype(np.uint8)
+    hs.append(same)
+    vs.append(same.copy())
+    H, V = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+    for score in SCORES[:2]:
+        print("long edges", score.table[0, :2], flush=True)
+        s, c = ctx.align_counts(H, V, score, flags=F32)
+        assert ctx.stats()["pairs_long"] == len(H)
+        ws, wc = _oracle_counts(H, V, score)
+        assert np.array_equal(s, ws)
+        assert np.array_equal(c, wc)
+        assert np.array_equal(ctx.align_score(H, V, score, 0, flags=F32), ws)
+
+
+def test_long_mixed_with_packed_routes(ctx, long_256):
+    # Long pairs beside short ones, an empty pair and a local-mode call
+    # (local never takes the long route).
+    rng = np.random.default_rng(12)
+    Hl, Vl = _mutated_pairs(rng, 6, 600, 1200)
+    hs = [Hl[i] for i in range(6)] + [rng.integers(0, 4, 100).astype(np.uint8), np.zeros(0, np.uint8)]
+    vs = [Vl[i] for i in range(6)] + [rng.integers(0, 4, 90).astype(np.uint8), rng.integers(0, 4, 5).astype(np.uint8)]
+    H, V = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+    score = SCORES[0]
+    s, c = ctx.align_counts(H, V, score, flags=F32)
+    assert ctx.stats()["pairs_long"] == 6
+    ws, wc = _oracle_counts(H, V, score)
+    assert np.array_equal(s, ws) and np.array_equal(c, wc)
+    for mode in (0, 9, 15):
+        print("mixed mode", mode, flush=True)
+        want = po.score_batch(H.residues, H.offsets, V.residues, V.offsets,
+                              po.Scoring(score.table, score.gap_open, score.gap_extend), mode)
+        assert np.array_equal(ctx.align_score(H, V, score, mode, flags=F32), want)
+        assert ctx.stats()["pairs_long"] == (6 if mode == 0 else 0)
+
+
+def test_long_chunked_checkpoints(ctx, long_256, monkeypatch):
+    # A 1 MiB budget puts every pair in a chunk of its own.
+    rng = np.random.default_rng(13)
+    H, V = _mutated_pairs(rng, 10, 800, 1600)
+    score = SCORES[0]
+    whole = ctx.align_counts(H, V, score, flags=F32)
+    assert ctx.stats()["pairs_long"] == len(H)
+    monkeypatch.setenv("SQMI_TRACE_BUDGET_MB", "1")
+    print("long chunked", flush=True)
+    chunked = ctx.align_counts(H, V, score, flags=F32)
+    assert np.array_equal(whole[0], chunked[0]) and np.array_equal(whole[1], chunked[1])
+    assert np.array_equal(ctx.align_score(H, V, score, 0, flags=F32), whole[0])
+
+
+def test_long_workload_prefix_matches_packed(ctx):
+    # The 100 kbp workload's first 16 pairs cut to 4 kbp: long route (int32
+    # forced) against the packed int16 kernels.
+    H, V = sa.workload(sa.WL_DNA100K, 0x5EED0005, 0, 16, 4000)
+    score = SCORES[0]
+    s_long, c_long = ctx.align_counts(H, V, score, flags=F32)
+    assert ctx.stats()["pairs_long"] == 16
+    s_pk, c_pk = ctx.align_counts(H, V, score)
+    assert ctx.stats()["pairs_long"] == 0 and ctx.stats()["pairs_int16"] == 16
+    assert np.array_equal(s_long, s_pk) and np.array_equal(c_long, c_pk)
+
+
+def test_long_full_100kbp_workload(ctx):
+    # 64 pairs of 100 kbp: scores and counts by the long route, checked by
+    # the alignment invariants and score == forward score.
+    H, V = sa.workload(sa.WL_DNA100K, 0x5EED0005, 0, 64)
+    score = SCORES[0]
+    print("100 kbp: 64 pairs, forward", flush=True)
+    s = ctx.align_score(H, V, score, 0)
+    st = ctx.stats()
+    print("100 kbp forward stats", st, flush=True)
+    assert st["pairs_long"] == 64
+    print("100 kbp: backward", flush=True)
+    s2, c = ctx.align_counts(H, V, score)
+    print("100 kbp backward stats", ctx.stats(), flush=True)
+    assert np.array_equal(s, s2)
+    _check_invariants(H, V, score, s2, c)
