@@ -213,8 +213,6 @@ int64_t longMinSide()
     return 2048;
 }
 
-// Columns per tile of the long-pair kernels.
-constexpr int kLongTileC = 512;
 
 // Stable LSD radix sort of 26-bit keys in two passes of 13 bits, each pass
 // split into contiguous per-thread ranges so that equal digits keep their
@@ -631,7 +629,8 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
         (err = ensure(ctx->endEF, (size_t)8 * N)) != hipSuccess)
         return hipFail(ctx, err, "allocating long-pair buffers");
     if (dCounts && ((err = ensure(ctx->longWalk, (size_t)4 * kWalkStateInts * np)) != hipSuccess ||
-                    (err = ensure(ctx->longTrace, sizeof(uint2) * (size_t)(C + kTileSlack) * 16 * np)) != hipSuccess))
+                    (err = ensure(ctx->longTrace, sizeof(uint2) * (size_t)kTileRecSteps * 16 * kRoundSlots * np)) !=
+                        hipSuccess))
         return hipFail(ctx, err, "allocating long-pair traceback buffers");
     if ((err = hipMemcpyAsync(ctx->longPairs.p, pairs.data(), sizeof(LongPair) * np, hipMemcpyHostToDevice,
                               stream)) != hipSuccess)
@@ -653,11 +652,10 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
         a.scores = in.dScores;
         a.endEF = static_cast<int32_t *>(ctx->endEF.p);
         a.alpha = P.si.alpha;
-        int maxB = 0, maxT = 0, maxBT = 0;
+        int maxB = 0, maxT = 0;
         for (size_t k = b; k < e; ++k) {
             maxB = std::max(maxB, pairs[k].nblocks);
             maxT = std::max(maxT, pairs[k].ntiles);
-            maxBT = std::max(maxBT, pairs[k].nblocks + pairs[k].ntiles);
         }
         a.wavesPerPair = (maxB + 3) / 4;
         for (int d = 0; d < maxB + maxT - 1; ++d) {
@@ -671,8 +669,10 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
             a.counts = dCounts;
             if ((err = launchTileWalkInit(a, stream)) != hipSuccess)
                 return hipFail(ctx, err, "launching long-pair walk init");
-            // A walk crosses at most nblocks + ntiles tiles.
-            for (int r = 0; r < maxBT; ++r) {
+            // Each round moves the walk up kRoundBlocks row blocks or left
+            // kRoundTiles tiles (or ends it); extra rounds do nothing.
+            const int rounds = (maxB + kRoundBlocks - 1) / kRoundBlocks + (maxT + kRoundTiles - 1) / kRoundTiles + 1;
+            for (int r = 0; r < rounds; ++r) {
                 if ((err = launchTileTrace32(a, stream)) != hipSuccess ||
                     (err = launchTileWalk32(a, stream)) != hipSuccess)
                     return hipFail(ctx, err, "launching long-pair traceback");

@@ -172,13 +172,18 @@ struct TileArgs {
     int32_t *scores;            // per pair id
     int32_t *endEF;             // per pair id: E, F of cell (m, n)
     // traceback
-    uint2 *trace;               // per long pair: (tileC + kTileSlack) steps x 16 lanes records
+    uint2 *trace;               // per long pair: kRoundSlots tiles x kTileRecSteps steps x 16 lanes records
     int32_t *walk;              // per long pair: walk state (kWalkStateInts ints)
     int32_t *counts;            // per pair id: alpha^2 + 2 counts
     int32_t alpha;
 };
 constexpr int kTileSlack = 24;        // steps per tile beyond its width (skew + prefetch)
 constexpr int kWalkStateInts = 8;
+// Traceback window of one round: kRoundBlocks row blocks x kRoundTiles column
+// tiles recomputed with trace records (tileC = kLongTileC).
+constexpr int kLongTileC = 512;
+constexpr int kTileRecSteps = kLongTileC + kTileSlack;
+constexpr int kRoundBlocks = 8, kRoundTiles = 4, kRoundSlots = kRoundBlocks * kRoundTiles;
 hipError_t launchTileFwd32(const TileArgs &args, hipStream_t stream);
 hipError_t launchTileTrace32(const TileArgs &args, hipStream_t stream);
 hipError_t launchTileWalkInit(const TileArgs &args, hipStream_t stream);

@@ -86,13 +86,13 @@ __device__ __forceinline__ void tileSweep(const TileArgs &a, const LongPair &P, 
             Ek[k] = kNegL;
         }
     }
-    // Top input of the group head at column c0 - 1 + s, prefetched a step ahead.
-    auto topAt = [&](int col) -> int2 {
-        if (g.b == 0 || !g.active || col > g.c1)
-            return make_int2(kNegL, kNegL);
-        return make_int2(ck[rowIn + 2 * col], ck[rowIn + 2 * col + 1]);
-    };
-    int2 top = topAt(g.c0 - 1);
+    // Top input of the group head at column c0 - 1 + s (the bottom row of
+    // block b - 1), from a ring of TQD loads in flight.  The loads are
+    // unconditional: columns past c1 read column c1, and block 0 (no block
+    // above) reads its own output row; those values are replaced by -inf.
+    const int32_t *topRow = ck + (g.b > 0 ? rowIn : rowOut);
+    const bool topReal = g.b > 0 && g.active;
+    auto topLoad = [&](int col) -> int2 { return *reinterpret_cast<const int2 *>(topRow + 2 * min(col, g.c1)); };
     if (!TRACE && g.active && g.t == 0 && l == TG - 1)
         *reinterpret_cast<int2 *>(ck + rowOut) = make_int2(Gk[TR - 1], kNegL);   // column 0 of the bottom row
 
@@ -100,58 +100,63 @@ __device__ __forceinline__ void tileSweep(const TileArgs &a, const LongPair &P, 
     int upGprev = kNegL;
     const int width = g.c1 - g.c0 + 1;
     const int steps = ((width + TG + TQD - 1) / TQD) * TQD;
-    auto hcode = [&](int jj) -> int {
-        const int idx = min(max(jj, 1), n) - 1;
-        const int c = min((int)h[idx], 4);
-        return (jj >= 1 && jj <= n) ? c : 4;
-    };
+    // Residues of H: raw bytes in a ring of TQD loads in flight (the index is
+    // clamped into the sequence; columns outside it are never computed).
+    auto hLoad = [&](int jj) -> uint32_t { return (uint32_t)h[min(max(jj, 1), n) - 1]; };
     uint32_t q[TQD];
+    int2 tq[TQD];
 #pragma unroll
-    for (int u = 0; u < TQD; ++u)
-        q[u] = (uint32_t)hcode(g.c0 - 1 + u - l);
-    uint32_t prof = sProf[q[0]];
+    for (int u = 0; u < TQD; ++u) {
+        q[u] = hLoad(g.c0 - 1 + u - l);
+        tq[u] = topLoad(g.c0 - 1 + u);
+    }
+    uint32_t prof = sProf[min(q[0], 4u)];
 
-    auto step = [&](int s, uint32_t pr) {
+    // `top` is the ring slot of this step: consumed by the shifts, then
+    // reloaded for step s + TQD (one register pair per slot, no copies).
+    auto step = [&](int s, uint32_t pr, int2 &top) {
         const int jj = g.c0 - 1 + s - l;
-        const int upG = shiftIn16(botG, top.x);
-        const int upF = shiftIn16(botF, top.y);
-        top = topAt(g.c0 + s);
+        const bool topOk = topReal && g.c0 - 1 + s <= g.c1;
+        const int upG = shiftIn16(botG, topOk ? top.x : kNegL);
+        const int upF = shiftIn16(botF, topOk ? top.y : kNegL);
+        top = topLoad(g.c0 - 1 + s + TQD);
         const int diagTop = upGprev;
         upGprev = upG;
         if (jj >= g.c0 && jj <= g.c1) {
-            int diag = diagTop, uG = upG, uF = upF;
+            int uG = upG, uF = upF;
             uint32_t accEF = 0, accDX = 0;
+            // D: the diagonal term of row k, H(i-1, j-1) + s(h, v), taken
+            // from the old G of row k - 1 before it is overwritten.
+            int D = diagTop + (int)__builtin_amdgcn_perm(0u, pr, sel[0]);
 #pragma unroll
             for (int k = 0; k < TR; ++k) {
-                const int tv = (int)__builtin_amdgcn_perm(0u, pr, sel[k]);
-                const int D = diag + tv;
                 const int Eext = Ek[k] + e, Fext = uF + e;
-                int F, H;
+                const int Gleft = Gk[k];
+                const int E = max(Eext, Gleft);
+                const int F = max(Fext, uG);
+                int H;
                 if constexpr (TRACE) {
                     // Sign bytes of the decisions: E opened, F opened (bits k,
                     // 16 + k of accEF); not diagonal, E beats F (accDX).
                     const uint32_t ef = __builtin_amdgcn_perm((uint32_t)subSat32(Fext, uG),
-                                                              (uint32_t)subSat32(Eext, Gk[k]), 0x0b0b0909u);
-                    Ek[k] = max(Eext, Gk[k]);
-                    F = max(Fext, uG);
-                    const int fe = max(Ek[k], F);
-                    const uint32_t dx = __builtin_amdgcn_perm((uint32_t)subSat32(F, Ek[k]),
+                                                              (uint32_t)subSat32(Eext, Gleft), 0x0b0b0909u);
+                    const int fe = max(E, F);
+                    const uint32_t dx = __builtin_amdgcn_perm((uint32_t)subSat32(F, E),
                                                               (uint32_t)subSat32(D, fe), 0x0b0b0909u);
                     H = max(D, fe);
                     const uint32_t bit = rowBits(k);
                     asm volatile("v_bfi_b32 %0, %1, %2, %0" : "+v"(accEF) : "s"(bit), "v"(ef));
                     asm volatile("v_bfi_b32 %0, %1, %2, %0" : "+v"(accDX) : "s"(bit), "v"(dx));
                 } else {
-                    Ek[k] = max(Eext, Gk[k]);
-                    F = max(Fext, uG);
-                    H = max(D, max(Ek[k], F));
+                    H = max(D, max(E, F));
                 }
                 if constexpr (HEAD)
                     H = max(H, kNegL);
-                diag = Gk[k];
-                const int Gn = H + o;
-                Gk[k] = Gn;
-                uG = Gn;
+                if (k + 1 < TR)
+                    D = Gleft + (int)__builtin_amdgcn_perm(0u, pr, sel[k + 1]);
+                Ek[k] = E;
+                Gk[k] = H + o;
+                uG = Gk[k];
                 uF = F;
             }
             botG = uG;
@@ -181,9 +186,9 @@ __device__ __forceinline__ void tileSweep(const TileArgs &a, const LongPair &P, 
 #pragma unroll
         for (int u = 0; u < TQD; ++u) {
             const int s = s0 + u;
-            const uint32_t nxt = sProf[q[(u + 1) % TQD]];
-            q[u] = (uint32_t)hcode(g.c0 - 1 + s + TQD - l);
-            step(s, prof);
+            const uint32_t nxt = sProf[min(q[(u + 1) % TQD], 4u)];
+            q[u] = hLoad(g.c0 - 1 + s + TQD - l);
+            step(s, prof, tq[u]);
             prof = nxt;
         }
     }
@@ -198,7 +203,7 @@ __device__ __forceinline__ void loadProfile(const TileArgs &a, uint32_t *sProf)
 
 // Forward: the tiles of anti-diagonal a.diag.  Each wave takes four
 // consecutive row blocks of one pair (one per 16-lane group).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kTileFwd32(TileArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kTileFwd32(TileArgs a)
 {
     __shared__ uint32_t sProf[8];
     loadProfile(a, sProf);
@@ -245,22 +250,46 @@ __device__ __forceinline__ TileGeom walkTile(const TileArgs &a, const LongPair &
     return g;
 }
 
-// Traceback, step 1: recompute each unfinished pair's current tile (the one
-// holding its walk position) with trace records.  One pair per group.
+// Traceback rounds.  A round recomputes, with trace records, the window of
+// kRoundBlocks x kRoundTiles tiles whose bottom-right tile holds the walk's
+// position (every tile of it only needs checkpoints, so all of them run at
+// once), then walks the pair through the window.  The walk leaves a window
+// through its top or left side, so each round moves it up kRoundBlocks row
+// blocks or left kRoundTiles tiles.
+
+// Window slot of tile (b, t) for a window whose bottom-right tile is (b0, t0).
+__device__ __forceinline__ int windowSlot(int b0, int t0, int b, int t) { return (b0 - b) * kRoundTiles + (t0 - t); }
+
+// Step 1: one 16-lane group per (pair, window slot).  Workgroup w serves
+// pairs with the same w mod 8 (the walk of pair p runs in workgroup p, and
+// workgroups are spread over the 8 XCDs by id mod 8, so both kernels of a
+// pair share one L2).
+constexpr int kTraceGroupsPerWG = 16;
+constexpr int kTraceWGsPerPair = kRoundSlots / kTraceGroupsPerWG;
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kTileTrace32(TileArgs a)
 {
     __shared__ uint32_t sProf[8];
     loadProfile(a, sProf);
     const int lane = threadIdx.x & 63, l = lane % TG;
-    const int pi = (blockIdx.x * (blockDim.x >> 6) + (int)(threadIdx.x >> 6)) * 4 + lane / TG;
+    const int w = blockIdx.x;
+    const int pi = 8 * (w / (8 * kTraceWGsPerPair)) + (w % 8);
+    const int slot = ((w / 8) % kTraceWGsPerPair) * kTraceGroupsPerWG + (int)(threadIdx.x / TG);
     const bool exists = pi < a.nPairs;
     const int pc = exists ? pi : a.nPairs - 1;
     const LongPair P = a.pairs[pc];
     TileGeom g = walkTile(a, P, a.walk + (int64_t)pc * kWalkStateInts);
-    g.active = g.active && exists;
+    g.b -= slot / kRoundTiles;
+    g.t -= slot % kRoundTiles;
+    g.active = g.active && exists && g.b >= 0 && g.t >= 0;
     if (!__any(g.active))
         return;
-    uint2 *rec = a.trace + (int64_t)pc * (a.tileC + kTileSlack) * TG;
+    if (!g.active) {
+        g.b = max(g.b, 0);
+        g.t = max(g.t, 0);
+    }
+    g.c0 = g.t * a.tileC + 1;
+    g.c1 = min((g.t + 1) * a.tileC, P.n);
+    uint2 *rec = a.trace + ((int64_t)pc * kRoundSlots + slot) * kTileRecSteps * TG;
     tileSweep<true, true>(a, P, g, l, sProf, rec);
 }
 
@@ -286,83 +315,132 @@ __global__ void kTileWalkInit(TileArgs a)
         a.counts[(int64_t)P.pid * S + k] = 0;
 }
 
-// Traceback, step 2: walk each unfinished pair through its current tile.
-__global__ __launch_bounds__(64) void kTileWalk32(TileArgs a)
+// Step 2: one workgroup per pair walks it through the window.  The records
+// of half a tile (128 rows: lanes 0-7 or 8-15 of the sweep) are staged in
+// LDS by the whole workgroup; thread 0 walks them, and the next half tile is
+// staged when the walk leaves this one.
+constexpr int kWalkWGThreads = 256;
+constexpr int kHalfLanes = TG / 2;
+__global__ __launch_bounds__(kWalkWGThreads) void kTileWalk32(TileArgs a)
 {
-    const int pi = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint2 sRec[kTileRecSteps * kHalfLanes];
+    __shared__ uint8_t sH[kLongTileC], sV[kTileRows / 2];   // residues of the half tile's columns and rows
+    __shared__ int32_t sCnt[16];
+    __shared__ int32_t sUnit[4];   // b, t, half, stop
+    const int pi = blockIdx.x;
     if (pi >= a.nPairs)
         return;
     const LongPair P = a.pairs[pi];
     int32_t *ws = a.walk + (int64_t)pi * kWalkStateInts;
-    if (ws[WS_DONE])
-        return;
-    const TileGeom g = walkTile(a, P, ws);
-    const uint2 *rec = a.trace + (int64_t)pi * (a.tileC + kTileSlack) * TG;
+    const TileGeom g0 = walkTile(a, P, ws);
+    if (!g0.active)
+        return;   // finished (the same for every thread of the workgroup)
+    const int A = a.alpha, S = A * A + 2;
     const uint8_t *h = a.hRes + a.hOff[P.pid];
     const uint8_t *v = a.vRes + a.vOff[P.pid];
-    const int A = a.alpha, S = A * A + 2;
-    int32_t *cnt = a.counts + (int64_t)P.pid * S;
-    // Substitution counts of this stretch in LDS (DNA: alpha <= 4), added to
-    // the pair's counts at the end.
-    __shared__ int32_t sC[64 * 16];
-    int32_t *lc = sC + threadIdx.x * 16;
-    for (int k = 0; k < 16; ++k)
-        lc[k] = 0;
-    int i = ws[WS_I], j = ws[WS_J], mode = ws[WS_MODE], prev = ws[WS_PREV];
-    int opens = ws[WS_OPENS], exts = ws[WS_EXTS];
-    const int iLo = g.b * kTileRows - P.p;   // first pair row of the tile
-    while (i > 0 && j > 0 && i >= iLo && j >= g.c0) {
-        const int r = P.p + i - g.b * kTileRows;
-        const int l = r / TR, k = r % TR;
-        const uint2 w = rec[(j - (g.c0 - 1) + l) * TG + l];
-        const uint32_t sh = (uint32_t)k;
-        const uint32_t bits = ((w.x >> sh) & 1u) | (((w.x >> (sh + 16)) & 1u) << 1) | (((w.y >> sh) & 1u) << 2) |
-                              (((w.y >> (sh + 16)) & 1u) << 3);
-        const int d = mode == 0 ? ((bits & 4u) ? ((bits & 8u) ? 2 : 1) : 0) : mode;
-        if (d == 0) {
-            const int hc = min((int)h[j - 1], A - 1), vc = min((int)v[i - 1], A - 1);
-            lc[hc * A + vc] += 1;
-            prev = 0;
-            --i;
-            --j;
-            mode = 0;
-        } else {
-            // One gap character; the gap goes on while the cell's gap
-            // extends, down to row / column 1 (dp_traceback_impl.h:220-341).
-            const bool vert = d == 1;
-            const bool open = (bits & (vert ? 2u : 1u)) != 0;
-            if (prev == d) ++exts; else ++opens;
-            prev = d;
-            const bool stop = open || (vert ? i == 1 : j == 1);
-            if (vert)
-                --i;
-            else
-                --j;
-            mode = stop ? 0 : d;
-        }
+    const int tid = threadIdx.x;
+    if (tid < 16)
+        sCnt[tid] = 0;
+    int i = 0, j = 0, mode = 0, prev = 0, opens = 0, exts = 0;
+    if (tid == 0) {
+        i = ws[WS_I];
+        j = ws[WS_J];
+        mode = ws[WS_MODE];
+        prev = ws[WS_PREV];
+        opens = ws[WS_OPENS];
+        exts = ws[WS_EXTS];
     }
-    for (int k = 0; k < A * A; ++k)
-        if (lc[k])
-            cnt[k] += lc[k];
-    if (i == 0 || j == 0) {
-        // Leading gaps down to the origin (dp_traceback_impl.h:522-545).
-        if (i > 0) {
-            if (prev == 1) exts += i; else { ++opens; exts += i - 1; }
-            prev = 1;
+    for (;;) {
+        if (tid == 0) {
+            // The unit (tile, half) holding (i, j), or stop: finished or
+            // outside the window.
+            const int b = (P.p + i) / kTileRows, t = (j - 1) / a.tileC;
+            const int half = ((P.p + i) % kTileRows) / (kTileRows / 2);
+            const bool inside = i > 0 && j > 0 && b <= g0.b && b > g0.b - kRoundBlocks && t <= g0.t &&
+                                t > g0.t - kRoundTiles;
+            sUnit[0] = b;
+            sUnit[1] = t;
+            sUnit[2] = half;
+            sUnit[3] = inside ? 0 : 1;
         }
-        if (j > 0) {
-            if (prev == 2) exts += j; else { ++opens; exts += j - 1; }
+        __syncthreads();
+        const int ub = sUnit[0], ut = sUnit[1], uh = sUnit[2];
+        if (sUnit[3])
+            break;
+        // Stage the half tile: records rec[s * 16 + 8 * half + 0..7] for all steps.
+        const uint2 *rec = a.trace + ((int64_t)pi * kRoundSlots + windowSlot(g0.b, g0.t, ub, ut)) * kTileRecSteps * TG;
+        const uint4 *src = reinterpret_cast<const uint4 *>(rec);
+        uint4 *dst = reinterpret_cast<uint4 *>(sRec);
+        for (int x = tid; x < kTileRecSteps * kHalfLanes / 2; x += kWalkWGThreads) {
+            const int st = x / (kHalfLanes / 2), c = x % (kHalfLanes / 2);
+            dst[x] = src[st * (TG / 2) + uh * (kHalfLanes / 2) + c];
         }
-        cnt[A * A] += opens;
-        cnt[A * A + 1] += exts;
-        ws[WS_DONE] = 1;
+        const int c0 = ut * a.tileC + 1;
+        const int rowTop = ub * kTileRows + uh * (kTileRows / 2) - P.p;   // first pair row of the half
+        for (int x = tid; x < a.tileC; x += kWalkWGThreads)
+            sH[x] = h[min(c0 + x, P.n) - 1];
+        if (tid < kTileRows / 2)
+            sV[tid] = v[min(max(rowTop + tid, 1), P.m) - 1];
+        __syncthreads();
+        if (tid == 0) {
+            while (i > 0 && j > 0 && i >= rowTop && j >= c0) {
+                const int r = P.p + i - ub * kTileRows;
+                const int l = r / TR, k = r % TR;
+                const uint2 w = sRec[(j - (c0 - 1) + l) * kHalfLanes + (l - uh * kHalfLanes)];
+                const uint32_t sh = (uint32_t)k;
+                const uint32_t bits = ((w.x >> sh) & 1u) | (((w.x >> (sh + 16)) & 1u) << 1) |
+                                      (((w.y >> sh) & 1u) << 2) | (((w.y >> (sh + 16)) & 1u) << 3);
+                const int d = mode == 0 ? ((bits & 4u) ? ((bits & 8u) ? 2 : 1) : 0) : mode;
+                if (d == 0) {
+                    const int hc = min((int)sH[j - c0], A - 1), vc = min((int)sV[i - rowTop], A - 1);
+                    sCnt[hc * A + vc] += 1;
+                    prev = 0;
+                    --i;
+                    --j;
+                    mode = 0;
+                } else {
+                    // One gap character; the gap goes on while the cell's gap
+                    // extends, down to row / column 1 (dp_traceback_impl.h:220-341).
+                    const bool vert = d == 1;
+                    const bool open = (bits & (vert ? 2u : 1u)) != 0;
+                    if (prev == d) ++exts; else ++opens;
+                    prev = d;
+                    const bool stop = open || (vert ? i == 1 : j == 1);
+                    if (vert)
+                        --i;
+                    else
+                        --j;
+                    mode = stop ? 0 : d;
+                }
+            }
+        }
+        __syncthreads();   // sRec is restaged next
     }
-    ws[WS_I] = i;
-    ws[WS_J] = j;
-    ws[WS_MODE] = mode;
-    ws[WS_PREV] = prev;
-    ws[WS_OPENS] = opens;
-    ws[WS_EXTS] = exts;
+    if (tid == 0) {
+        int32_t *cnt = a.counts + (int64_t)P.pid * S;
+        for (int k = 0; k < A * A; ++k)
+            if (sCnt[k])
+                cnt[k] += sCnt[k];
+        if (i == 0 || j == 0) {
+            // Leading gaps down to the origin (dp_traceback_impl.h:522-545).
+            if (i > 0) {
+                if (prev == 1) exts += i; else { ++opens; exts += i - 1; }
+                prev = 1;
+            }
+            if (j > 0) {
+                if (prev == 2) exts += j; else { ++opens; exts += j - 1; }
+            }
+            cnt[A * A] += opens;
+            cnt[A * A + 1] += exts;
+            ws[WS_DONE] = 1;
+        }
+        ws[WS_I] = i;
+        ws[WS_J] = j;
+        ws[WS_MODE] = mode;
+        ws[WS_PREV] = prev;
+        ws[WS_OPENS] = opens;
+        ws[WS_EXTS] = exts;
+    }
 }
 
 }  // namespace
@@ -378,10 +456,10 @@ hipError_t launchTileFwd32(const TileArgs &args, hipStream_t stream)
 
 hipError_t launchTileTrace32(const TileArgs &args, hipStream_t stream)
 {
-    const int waves = (args.nPairs + 3) / 4;
-    if (waves <= 0)
+    if (args.nPairs <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(kTileTrace32, dim3((waves + 3) / 4), dim3(256), 0, stream, args);
+    const int wgs = ((args.nPairs + 7) / 8) * 8 * kTraceWGsPerPair;
+    hipLaunchKernelGGL(kTileTrace32, dim3(wgs), dim3(256), 0, stream, args);
     return hipGetLastError();
 }
 
@@ -397,7 +475,7 @@ hipError_t launchTileWalk32(const TileArgs &args, hipStream_t stream)
 {
     if (args.nPairs <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(kTileWalk32, dim3((args.nPairs + 63) / 64), dim3(64), 0, stream, args);
+    hipLaunchKernelGGL(kTileWalk32, dim3(args.nPairs), dim3(kWalkWGThreads), 0, stream, args);
     return hipGetLastError();
 }
 

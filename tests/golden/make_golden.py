@@ -2,6 +2,7 @@
 holds the reference tree and the oracle/_ref build):
 
     make -C oracle && python tests/golden/make_golden.py
+    python tests/golden/make_golden.py --long   # the 100 kbp fixture (~25 GB of RAM)
 
 * ``ref_*.npz``   -- inputs (residue ordinals + offsets) and the outputs of
   the reference library itself (oracle/_ref: SeqAn 2 globalAlignmentScore /
@@ -141,5 +142,18 @@ def main():
          gap_extend=-1, gap_open=-2)
 
 
+def long_pairs():
+    """Config-5 fixture: the first two pairs of the 64 x 100 kbp workload at
+    full length, counted by the reference's scalar globalAlignment (a 10 GB
+    trace matrix per pair, so this runs on its own: --long)."""
+    H, V = sa.workload(sa.WL_DNA100K, 0x5EED0005, 0, 2)
+    rc = po.run_reference(po.OP_COUNTS_SCALAR, H.residues, H.offsets, V.residues, V.offsets, 0, alphabet="dna",
+                          threads=2, **SIMPLE)
+    save("ref_dna100k_counts.npz", H, V, scores=rc["scores"], counts=rc["counts"], seed=0x5EED0005, mode=0)
+
+
 if __name__ == "__main__":
-    main()
+    if "--long" in sys.argv:
+        long_pairs()
+    else:
+        main()
