@@ -181,9 +181,9 @@ constexpr int kTileSlack = 24;        // steps per tile beyond its width (skew +
 constexpr int kWalkStateInts = 8;
 // Traceback window of one round: kRoundBlocks row blocks x kRoundTiles column
 // tiles recomputed with trace records (tileC = kLongTileC).
-constexpr int kLongTileC = 512;
+constexpr int kLongTileC = 256;
 constexpr int kTileRecSteps = kLongTileC + kTileSlack;
-constexpr int kRoundBlocks = 8, kRoundTiles = 4, kRoundSlots = kRoundBlocks * kRoundTiles;
+constexpr int kRoundBlocks = 8, kRoundTiles = 8, kRoundSlots = kRoundBlocks * kRoundTiles;
 hipError_t launchTileFwd32(const TileArgs &args, hipStream_t stream);
 hipError_t launchTileTrace32(const TileArgs &args, hipStream_t stream);
 hipError_t launchTileWalkInit(const TileArgs &args, hipStream_t stream);

@@ -27,7 +27,20 @@ namespace {
 constexpr int kNegL = -(1 << 29);   // minus infinity; real values stay far above
 constexpr int TG = 16;              // lanes per group
 constexpr int TR = kTileRows / TG;  // rows per lane
-constexpr int TQD = 4;              // residue prefetch depth (steps)
+constexpr int TQD = 4;              // steps per unrolled loop body
+constexpr int TK = 32;              // steps per refill of a group's LDS rings
+constexpr int TRING = 64;           // ring entries (two refills)
+
+// Per-group LDS rings of the tile sweep.  top[s % TRING]: the block above's
+// bottom row (G, F) at column c0 - 1 + s (lane 0's top input at step s);
+// pr[x % TRING]: the substitution profile word of column c0 - 1 + x - 15
+// (lane l reads entry s - l + 15 at step s).  Both are refilled every TK
+// steps from loads issued TK steps earlier, so the sweep itself issues no
+// global loads.
+struct GroupRings {
+    int2 top[TRING];
+    uint32_t pr[TRING];
+};
 
 __device__ __forceinline__ int shiftIn16(int v, int first)
 {
@@ -50,7 +63,7 @@ struct TileGeom {
 // 0 (block 0), which are floored at -inf.
 template <bool HEAD, bool TRACE>
 __device__ __forceinline__ void tileSweep(const TileArgs &a, const LongPair &P, const TileGeom &g, int l,
-                                          const uint32_t *sProf, uint2 *rec)
+                                          const uint32_t *sProf, GroupRings *rg, uint2 *rec)
 {
     const int o = a.gapOpen, e = a.gapExtend, eo = max(o, e);
     const int C = a.tileC;
@@ -87,9 +100,9 @@ __device__ __forceinline__ void tileSweep(const TileArgs &a, const LongPair &P, 
         }
     }
     // Top input of the group head at column c0 - 1 + s (the bottom row of
-    // block b - 1), from a ring of TQD loads in flight.  The loads are
-    // unconditional: columns past c1 read column c1, and block 0 (no block
-    // above) reads its own output row; those values are replaced by -inf.
+    // block b - 1).  The loads are unconditional: columns past c1 read column
+    // c1, and block 0 (no block above) reads its own output row; those values
+    // are replaced by -inf.
     const int32_t *topRow = ck + (g.b > 0 ? rowIn : rowOut);
     const bool topReal = g.b > 0 && g.active;
     auto topLoad = [&](int col) -> int2 { return *reinterpret_cast<const int2 *>(topRow + 2 * min(col, g.c1)); };
@@ -100,26 +113,38 @@ __device__ __forceinline__ void tileSweep(const TileArgs &a, const LongPair &P, 
     int upGprev = kNegL;
     const int width = g.c1 - g.c0 + 1;
     const int steps = ((width + TG + TQD - 1) / TQD) * TQD;
-    // Residues of H: raw bytes in a ring of TQD loads in flight (the index is
-    // clamped into the sequence; columns outside it are never computed).
-    auto hLoad = [&](int jj) -> uint32_t { return (uint32_t)h[min(max(jj, 1), n) - 1]; };
-    uint32_t q[TQD];
-    int2 tq[TQD];
-#pragma unroll
-    for (int u = 0; u < TQD; ++u) {
-        q[u] = hLoad(g.c0 - 1 + u - l);
-        tq[u] = topLoad(g.c0 - 1 + u);
-    }
-    uint32_t prof = sProf[min(q[0], 4u)];
+    // Ring refills: the refill for steps [s, s + TK) covers columns
+    // c0 - 1 + s + 2 l and + 1 of lane l (top entries s + 2 l, profile
+    // entries s + 15 + 2 l); residue indexes are clamped into H (columns
+    // outside it are never computed).
+    auto hLoad = [&](int col) -> uint32_t { return (uint32_t)h[min(max(col, 1), n) - 1]; };
+    int2 t0, t1;
+    uint32_t h0, h1;
+    auto refillLoad = [&](int s) {
+        const int cb = g.c0 - 1 + s + 2 * l;
+        t0 = topLoad(cb);
+        t1 = topLoad(cb + 1);
+        h0 = hLoad(cb);
+        h1 = hLoad(cb + 1);
+    };
+    auto refillStore = [&](int s) {
+        rg->top[(s + 2 * l) & (TRING - 1)] = t0;
+        rg->top[(s + 2 * l + 1) & (TRING - 1)] = t1;
+        rg->pr[(s + TG - 1 + 2 * l) & (TRING - 1)] = sProf[min(h0, 4u)];
+        rg->pr[(s + TG + 2 * l) & (TRING - 1)] = sProf[min(h1, 4u)];
+        __builtin_amdgcn_wave_barrier();
+    };
+    refillLoad(0);
+    refillStore(0);
+    refillLoad(TK);
+    uint32_t prof = rg->pr[(TG - 1 - l) & (TRING - 1)];
+    int2 topv = rg->top[0];
 
-    // `top` is the ring slot of this step: consumed by the shifts, then
-    // reloaded for step s + TQD (one register pair per slot, no copies).
-    auto step = [&](int s, uint32_t pr, int2 &top) {
+    auto step = [&](int s, uint32_t pr, int2 top) {
         const int jj = g.c0 - 1 + s - l;
         const bool topOk = topReal && g.c0 - 1 + s <= g.c1;
         const int upG = shiftIn16(botG, topOk ? top.x : kNegL);
         const int upF = shiftIn16(botF, topOk ? top.y : kNegL);
-        top = topLoad(g.c0 - 1 + s + TQD);
         const int diagTop = upGprev;
         upGprev = upG;
         if (jj >= g.c0 && jj <= g.c1) {
@@ -183,13 +208,20 @@ __device__ __forceinline__ void tileSweep(const TileArgs &a, const LongPair &P, 
         }
     };
     for (int s0 = 0; s0 < steps; s0 += TQD) {
+        if ((s0 & (TK - 1)) == TK - TQD) {
+            // Rings for the next TK steps (read from the next step on), and
+            // the loads for the refill after it.
+            refillStore(s0 + TQD);
+            refillLoad(s0 + TQD + TK);
+        }
 #pragma unroll
         for (int u = 0; u < TQD; ++u) {
             const int s = s0 + u;
-            const uint32_t nxt = sProf[min(q[(u + 1) % TQD], 4u)];
-            q[u] = hLoad(g.c0 - 1 + s + TQD - l);
-            step(s, prof, tq[u]);
-            prof = nxt;
+            const uint32_t prN = rg->pr[(s + 1 + TG - 1 - l) & (TRING - 1)];
+            const int2 topN = rg->top[(s + 1) & (TRING - 1)];
+            step(s, prof, topv);
+            prof = prN;
+            topv = topN;
         }
     }
 }
@@ -206,6 +238,7 @@ __device__ __forceinline__ void loadProfile(const TileArgs &a, uint32_t *sProf)
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kTileFwd32(TileArgs a)
 {
     __shared__ uint32_t sProf[8];
+    __shared__ GroupRings sRings[256 / TG];
     loadProfile(a, sProf);
     const int lane = threadIdx.x & 63, l = lane % TG, grp = lane / TG;
     const int w = blockIdx.x * (blockDim.x >> 6) + (int)(threadIdx.x >> 6);
@@ -227,9 +260,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     g.c0 = g.t * a.tileC + 1;
     g.c1 = min((g.t + 1) * a.tileC, P.n);
     if (b0 == 0)
-        tileSweep<true, false>(a, P, g, l, sProf, nullptr);
+        tileSweep<true, false>(a, P, g, l, sProf, &sRings[threadIdx.x / TG], nullptr);
     else
-        tileSweep<false, false>(a, P, g, l, sProf, nullptr);
+        tileSweep<false, false>(a, P, g, l, sProf, &sRings[threadIdx.x / TG], nullptr);
 }
 
 // Walk state of one long pair: the cell (i, j) reached, the walk mode
@@ -269,6 +302,7 @@ constexpr int kTraceWGsPerPair = kRoundSlots / kTraceGroupsPerWG;
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kTileTrace32(TileArgs a)
 {
     __shared__ uint32_t sProf[8];
+    __shared__ GroupRings sRings[256 / TG];
     loadProfile(a, sProf);
     const int lane = threadIdx.x & 63, l = lane % TG;
     const int w = blockIdx.x;
@@ -290,7 +324,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     g.c0 = g.t * a.tileC + 1;
     g.c1 = min((g.t + 1) * a.tileC, P.n);
     uint2 *rec = a.trace + ((int64_t)pc * kRoundSlots + slot) * kTileRecSteps * TG;
-    tileSweep<true, true>(a, P, g, l, sProf, rec);
+    tileSweep<true, true>(a, P, g, l, sProf, &sRings[threadIdx.x / TG], rec);
 }
 
 __global__ void kTileWalkInit(TileArgs a)

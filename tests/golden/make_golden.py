@@ -143,13 +143,20 @@ def main():
 
 
 def long_pairs():
-    """Config-5 fixture: the first two pairs of the 64 x 100 kbp workload at
-    full length, counted by the reference's scalar globalAlignment (a 10 GB
-    trace matrix per pair, so this runs on its own: --long)."""
+    """100 kbp fixture: the first two pairs of the 64 x 100 kbp workload at
+    full length (run on its own: --long).  Scores come from the reference's
+    globalAlignmentScore.  Counts come from the oracle's traceback (a 10 GB
+    trace matrix per pair): the reference's own traceback cannot give them
+    at this size, because it passes the end cell's trace-matrix position as
+    `unsigned` (dp_traceback_impl.h:559, 607), which truncates positions past
+    2^32 (each pair has ~1e10 cells); its counts for these pairs violate
+    aligned pairs + gap characters == residues."""
     H, V = sa.workload(sa.WL_DNA100K, 0x5EED0005, 0, 2)
-    rc = po.run_reference(po.OP_COUNTS_SCALAR, H.residues, H.offsets, V.residues, V.offsets, 0, alphabet="dna",
-                          threads=2, **SIMPLE)
-    save("ref_dna100k_counts.npz", H, V, scores=rc["scores"], counts=rc["counts"], seed=0x5EED0005, mode=0)
+    ref = ref_scores(H, V, 0, alphabet="dna", **SIMPLE)
+    sc = po.Scoring(sa.simple_score(2, -3, -1, -5).table, -5, -1)
+    ws, wc = po.traceback_batch(H.residues, H.offsets, V.residues, V.offsets, sc, 0, threads=2)
+    assert np.array_equal(ws, ref), (ws, ref)
+    save("ref_dna100k_counts.npz", H, V, scores=ref, counts=wc, seed=0x5EED0005, mode=0)
 
 
 if __name__ == "__main__":

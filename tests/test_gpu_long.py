@@ -1,6 +1,6 @@
 """Long pairs: the tiled int32 forward that keeps every tile border (256 rows
-x 512 columns) as a checkpoint, and the checkpointed traceback that walks
-back one recomputed tile at a time.
+x 256 columns) as a checkpoint, and the checkpointed traceback that walks
+back through windows of recomputed tiles.
 
 Global DNA pairs outside the int16 window with both sides >=
 SQMI_LONG_MIN_SIDE (2048 by default) take this route.  The parity tests force
@@ -8,8 +8,10 @@ int32 (FORCE_INT32) and lower the threshold to 256 so that pairs of a few
 thousand bases, which the oracle finishes in seconds, cross many row blocks
 and column tiles; the full 64 x 100 kbp workload is checked through
 properties every alignment satisfies (sum of count x parameter == score,
-aligned pairs + gap characters == residues) and against the packed kernels
-on its first pairs cut short."""
+aligned pairs + gap characters == residues), against the packed kernels on
+its first pairs cut short, and against a fixture of its first two pairs."""
+import os
+
 import numpy as np
 import pytest
 
@@ -17,6 +19,7 @@ import pyoracle as po
 import seqan_amd as sa
 
 pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 F32 = sa.FORCE_INT32
 SCORES = [sa.simple_score(2, -3, -1, -5), sa.simple_score(1, -1, -1, -2), sa.simple_score(5, -4, -2, -11),
@@ -101,7 +104,7 @@ def test_long_counts_match_oracle(ctx, long_256, lo, hi):
 
 
 def test_long_edges(ctx, long_256):
-    # Sides at and around the block (256 rows) and tile (512 columns) edges,
+    # Sides at and around the block (256 rows) and tile (256 columns) edges,
     # very unequal sides, and identical / unrelated sequences.
     rng = np.random.default_rng(11)
     shapes = [(256, 256), (511, 257), (512, 255 + 256), (513, 767), (1024, 1023), (3000, 260), (260, 3000),
@@ -169,6 +172,19 @@ def test_long_workload_prefix_matches_packed(ctx):
     s_pk, c_pk = ctx.align_counts(H, V, score)
     assert ctx.stats()["pairs_long"] == 0 and ctx.stats()["pairs_int16"] == 16
     assert np.array_equal(s_long, s_pk) and np.array_equal(c_long, c_pk)
+
+
+def test_long_100kbp_fixture(ctx):
+    # The 100 kbp workload's first two pairs at full length: scores from the
+    # reference's globalAlignmentScore, counts from the oracle's traceback.
+    z = np.load(os.path.join(GOLDEN, "ref_dna100k_counts.npz"), allow_pickle=False)
+    H, V = sa.SeqSet(z["h_res"], z["h_off"]), sa.SeqSet(z["v_res"], z["v_off"])
+    score = SCORES[0]
+    print("100 kbp fixture: 2 pairs", flush=True)
+    assert np.array_equal(ctx.align_score(H, V, score, 0), z["scores"])
+    assert ctx.stats()["pairs_long"] == 2
+    s, c = ctx.align_counts(H, V, score)
+    assert np.array_equal(s, z["scores"]) and np.array_equal(c, z["counts"])
 
 
 def test_long_full_100kbp_workload(ctx):

@@ -200,6 +200,19 @@ def test_oracle_traceback_counts_match_reference(fixture):
     assert np.array_equal(c.astype(np.int64) @ par, s.astype(np.int64))
 
 
+def test_100kbp_fixture_is_a_valid_alignment():
+    # Two 100 kbp pairs: reference scores, oracle counts (the reference's own
+    # traceback truncates trace-matrix positions past 2^32 at this size).
+    # The counts form an alignment of both sequences whose value is the score.
+    z = _load("ref_dna100k_counts.npz")
+    c = z["counts"].astype(np.int64)
+    par = np.array([2 if a == b else -3 for a in range(4) for b in range(4)] + [-5, -1])
+    assert np.array_equal(c @ par, z["scores"].astype(np.int64))
+    n, m = np.diff(z["h_off"]), np.diff(z["v_off"])
+    assert np.array_equal(2 * c[:, :16].sum(axis=1) + c[:, 16] + c[:, 17], n + m)
+    assert (n >= 99000).all() and (m >= 99000).all()
+
+
 @pytest.mark.parametrize("mode", list(range(16)) + [16])
 def test_oracle_vs_brute_force(mode):
     rng = np.random.default_rng(mode)
