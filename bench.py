@@ -13,6 +13,8 @@ Also measured in the same run, each reported beside the headline figure:
   - fwd+bwd: score plus traceback counts (sqmi_align_counts) of 65,536 DNA
     pairs of 1 kbp, global, and the forward alone on the same pairs;
   - protein local: 262,144 Smith-Waterman pairs, Blosum62(-1, -11);
+  - long fwd+bwd: score plus traceback counts of 64 DNA pairs of 100 kbp
+    (tiled int32 kernels with checkpoints), and the forward alone;
   - the reference SeqAn 2 CPU path (oracle/_ref, built from the reference
     sources) on bounded samples of each workload, all host cores and one
     core; the GPU results of each sample are bit-checked against it.
@@ -36,6 +38,9 @@ N_PAIRS = 1 << 20
 MODE = sa.FREE_TOP | sa.FREE_BOTTOM
 SEED_PROTEIN, N_PROTEIN = 0x5EED0003, 1 << 18
 SEED_DNA1K, N_DNA1K = 0x5EED0004, 1 << 16
+SEED_DNA100K, N_DNA100K = 0x5EED0005, 64
+# Peak int32 VALU operations (one element per lane): 256 CU x 64 lanes x 2.4 GHz.
+PEAK_TOPS_INT32 = 256 * 64 * 2.4e9 / 1e12
 # Peak packed-int16 VALU element operations: 256 CU x 64 lanes x 2 halves x 2.4 GHz.
 PEAK_TOPS = 256 * 64 * 2 * 2.4e9 / 1e12
 OPS_PER_CELL = 8          # Hd+s, E+e, F+e, H+o, max E, max F, 2x max H
@@ -151,6 +156,60 @@ def backward_leg(ctx, steps, warmup, cpu):
     return res
 
 
+def long_leg(ctx, steps, warmup, cpu):
+    """Score, and score + traceback counts, of 64 DNA pairs of 100 kbp
+    (global): the tiled int32 long-pair kernels, inputs and outputs resident
+    in HBM."""
+    H, V = sa.workload(sa.WL_DNA100K, SEED_DNA100K, 0, N_DNA100K)
+    cells = int((H.lengths() * V.lengths()).sum())
+    score = sa.simple_score(2, -3, -1, -5)
+    S = score.alphabet_size ** 2 + 2
+    dh, dv = sa.DeviceBuffer.from_array(H.residues), sa.DeviceBuffer.from_array(V.residues)
+    ds, dc = sa.DeviceBuffer(4 * N_DNA100K), sa.DeviceBuffer(4 * S * N_DNA100K)
+    dev = dict(h_dev=dh.address, v_dev=dv.address)
+    fwd_ms = timed(lambda: ctx.align_score(H, V, score, 0, scores_dev=ds.address, **dev), steps, warmup,
+                   "dna100k forward")
+    st_f = ctx.stats()
+    bwd_ms = timed(lambda: ctx.align_counts(H, V, score, 0, scores_dev=ds.address, counts_dev=dc.address, **dev),
+                   steps, warmup, "dna100k fwd+bwd")
+    st = ctx.stats()
+    scores = ds.to_array(np.int32, N_DNA100K)
+    counts = dc.to_array(np.int32, S * N_DNA100K).reshape(N_DNA100K, S).astype(np.int64)
+    par = np.concatenate([score.table.ravel(), [score.gap_open, score.gap_extend]]).astype(np.int64)
+    ident = bool(np.array_equal(counts @ par, scores.astype(np.int64)) and np.array_equal(
+        2 * counts[:, :S - 2].sum(axis=1) + counts[:, S - 2] + counts[:, S - 1], H.lengths() + V.lengths()))
+    achieved = OPS_PER_CELL * cells / (st_f["device_ms"] / 1e3) / 1e12
+    res = {"value": round(cells / (bwd_ms / 1e3) / 1e9, 2), "unit": "GCUPS", "ms_per_step": round(bwd_ms, 3),
+           "forward_ms_per_step": round(fwd_ms, 3), "forward_gcups": round(cells / (fwd_ms / 1e3) / 1e9, 2),
+           "backward_over_forward": round(bwd_ms / fwd_ms, 3), "device_ms": round(st["device_ms"], 3),
+           "forward_device_ms": round(st_f["device_ms"], 3),
+           "forward_roofline": {"bound": "valu", "achieved": round(achieved, 3), "unit": "TOP/s",
+                                "frac_int32": round(achieved / PEAK_TOPS_INT32, 4),
+                                "frac_packed_int16": round(achieved / PEAK_TOPS, 4),
+                                "note": "8 int32 ops per cell; the values exceed int16, so the kernels run "
+                                        "one cell per lane (int32 peak = half the packed-int16 peak)"},
+           "workload": "64 DNA pairs of 100 kbp, 12% divergence, 3% geometric indels, global, "
+                       "Score<int,Simple>(2,-3,-1,-5)",
+           "pairs": N_DNA100K, "pairs_long": int(st["pairs_long"]), "cells": cells, "seed": SEED_DNA100K,
+           "counts_form_alignments": ident}
+    if cpu:
+        # The reference runs 16 pairs per SIMD batch on one thread; full-length
+        # pairs would take minutes, so the sample is the first 16 pairs of the
+        # same generator cut to 10 kbp.
+        Hs, Vs = sa.workload(sa.WL_DNA100K, SEED_DNA100K, 0, 16, 10000)
+        gs = ctx.align_score(Hs, Vs, score, 0)
+        res["cpu_baseline"] = reference_run(0, Hs, Vs, 16, 0, host_cores(), 1, gs, alphabet="dna", match=2,
+                                            mismatch=-3, gap_extend=-1, gap_open=-5)
+        if res["cpu_baseline"] is not None:
+            cs = int((Hs.lengths() * Vs.lengths()).sum())
+            res["cpu_baseline"]["sample"] = (f"first 16 pairs of the 100 kbp generator (same seed) cut to 10 kbp "
+                                             f"({cs:.3e} cells), one timed call; {cpu_model()}")
+        log(f"dna100k cpu baseline: {res['cpu_baseline']}")
+    for b in (dh, dv, ds, dc):
+        b.free()
+    return res
+
+
 def protein_leg(ctx, steps, warmup, cpu):
     """Smith-Waterman scores of the protein workload (Blosum62(-1, -11))."""
     H, V = sa.workload(sa.WL_PROTEIN, SEED_PROTEIN, 0, N_PROTEIN)
@@ -216,12 +275,19 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the fwd+bwd and protein legs")
+    ap.add_argument("--leg", choices=["fwd_bwd", "protein_local", "long_fwd_bwd"], default=None,
+                    help="run only this extra leg and print its JSON (for profiling)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also copy this many MiB device-to-device (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
+
+    if args.leg:
+        legs = {"fwd_bwd": backward_leg, "protein_local": protein_leg, "long_fwd_bwd": long_leg}
+        print(json.dumps({args.leg: legs[args.leg](sa.Context(0), args.steps, args.warmup, not args.no_cpu)}))
+        return
 
     t0 = time.time()
     H, V = sa.workload(sa.WL_DNA_LOGNORMAL, SEED, 0, args.pairs)
@@ -316,6 +382,7 @@ def main():
     if not args.no_extra:
         extra["fwd_bwd"] = backward_leg(ctx, 3, 1, not args.no_cpu)
         extra["protein_local"] = protein_leg(ctx, 3, 1, not args.no_cpu)
+        extra["long_fwd_bwd"] = long_leg(ctx, 3, 1, not args.no_cpu)
 
     res = {
         "metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
