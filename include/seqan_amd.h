@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define SQMI_ABI_VERSION 1
+#define SQMI_ABI_VERSION 2   /* 2: sqmi_stats.pairs_long */
 
 typedef enum sqmi_status {
     SQMI_OK = 0,

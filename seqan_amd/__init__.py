@@ -70,6 +70,9 @@ class Stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+# SQMI_ABI_VERSION of include/seqan_amd.h that the structures below mirror.
+ABI_VERSION = 2
+
 # Every symbol the C ABI declares (include/seqan_amd.h).
 ABI_SYMBOLS = (
     "sqmi_create", "sqmi_destroy", "sqmi_set_stream", "sqmi_align_score", "sqmi_align_counts",
@@ -115,6 +118,9 @@ def lib():
             f = getattr(L, fn)
             if fn not in ("sqmi_blosum62", "sqmi_status_string", "sqmi_last_error"):
                 f.restype = ctypes.c_int
+        if L.sqmi_abi_version() != ABI_VERSION:
+            raise ImportError(f"{LIB_PATH} has ABI version {L.sqmi_abi_version()}, this package expects "
+                              f"{ABI_VERSION}: rebuild it with `python -m seqan_amd.build`")
         _lib = L
     return _lib
 
