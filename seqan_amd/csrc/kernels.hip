@@ -271,19 +271,24 @@ struct Fwd16 {
             const s16x2 diagTop = upGprev;
             upGprev = upG;
             if (j >= 1 && j <= Nw) {
-                s16x2 diag = diagTop, uG = upG, uF = upF;
+                s16x2 uG = upG, uF = upF;
                 uint32_t accEF0 = 0, accEF1 = 0, accDX0 = 0, accDX1 = 0;
-#pragma unroll
-                for (int k = 0; k < R; ++k) {
-                    uint32_t t;
+                // t(h, v) - gapOpen of row k for both pairs.
+                auto tOf = [&](int k) -> uint32_t {
                     if constexpr (DNA) {
-                        t = __builtin_amdgcn_perm(pB, pA, sel[k]);
+                        return __builtin_amdgcn_perm(pB, pA, sel[k]);
                     } else {
                         const uint32_t lo = (uint16_t)sT[(int)pA + (int)(sel[k] & 0xffffu)];
                         const uint32_t hi = (uint16_t)sT[(int)pB + (int)(sel[k] >> 16)];
-                        t = lo | (hi << 16);
+                        return lo | (hi << 16);
                     }
-                    const s16x2 D = diag + as16(t);
+                };
+                // D: the diagonal term of row k, G(i-1, j-1) + t, taken from
+                // the old G of row k - 1 before it is overwritten (no copies
+                // of the column's G values).
+                s16x2 D = diagTop + as16(tOf(0));
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
                     s16x2 F, H;
                     if constexpr (TB) {
                         const s16x2 Eext = Ek[k] + e2, Fext = uF + e2;
@@ -315,7 +320,8 @@ struct Fwd16 {
                     } else if constexpr (HEAD) {
                         H = vmax(H, floorK[k]);
                     }
-                    diag = Gk[k];
+                    if (k + 1 < R)
+                        D = Gk[k] + as16(tOf(k + 1));
                     const s16x2 Gn = H + o2;
                     Gk[k] = Gn;
                     if constexpr (TRK & kTrackLocal)
