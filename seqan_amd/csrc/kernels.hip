@@ -134,6 +134,13 @@ __device__ __forceinline__ void publish(uint32_t *w, uint32_t v)
 constexpr int kCoopWaves = 4;
 __shared__ uint32_t gProd[kCoopWaves];
 
+// Per-group LDS ring of the profile words of both pairs of a couple, by
+// column: entry x holds column x - 15 (lane l reads entry s - l + 15 at
+// step s).  Refilled every kRefill steps from residues loaded one refill
+// earlier, so the column sweep issues no residue loads.
+constexpr int kRing = 64, kRefill = 32;
+__shared__ uint2 gRing[256 / 16][kRing];
+
 // Per-lane state of one pair couple (lo = pair A, hi = pair B).
 struct Couple {
     const uint8_t *h[2];
@@ -372,12 +379,13 @@ struct Fwd16 {
             }
         };
 
-        // Residue of pair x at column jj (padding outside 1..n); the address
-        // is clamped so the load is always in bounds.
-        auto hcode = [&](int x, int jj) -> int {
-            const int idx = min(max(jj, 1), max(c.n[x], 1)) - 1;
-            const int cc = min((int)c.h[x][idx], PAD);
-            return (jj >= 1 && jj <= c.n[x]) ? cc : PAD;
+        // Residue byte of pair x at column jj (the address is clamped so the
+        // load is always in bounds), and its code (padding outside 1..n).
+        auto hraw = [&](int x, int jj) -> uint32_t {
+            return c.h[x][min(max(jj, 1), max(c.n[x], 1)) - 1];
+        };
+        auto hcode = [&](int x, int jj, uint32_t raw) -> int {
+            return (jj >= 1 && jj <= c.n[x]) ? min((int)raw, PAD) : PAD;
         };
         auto prof = [&](int code) -> uint32_t {
             if constexpr (DNA)
@@ -385,14 +393,30 @@ struct Fwd16 {
             else
                 return (uint32_t)(code * TW);
         };
-        // Residues are loaded QD steps ahead of their use and the profile of
-        // the next step is read one step ahead, so neither latency sits on
-        // the critical path of a step.
-        uint32_t q[QD];   // residue codes of pair A (bits 0-7) and B (bits 8-15)
-#pragma unroll
-        for (int u = 0; u < QD; ++u)
-            q[u] = (uint32_t)hcode(0, u - l) | ((uint32_t)hcode(1, u - l) << 8);
-        uint32_t pA = prof(q[0] & 0xff), pB = prof(q[0] >> 8);
+        // The couple's profile words come from the group's LDS ring.  A
+        // refill for steps [s, s + kRefill) writes columns s + 2 l and
+        // s + 2 l + 1 of lane l; their residues were loaded a refill earlier.
+        static_assert(G == 16 && kRefill == 2 * G && kRefill % QD == 0, "ring refill covers 2 columns per lane");
+        uint2 *ring = gRing[threadIdx.x / G];
+        uint32_t hr[4];
+        auto refillLoad = [&](int s) {
+            const int j0 = s + 2 * l;
+            hr[0] = hraw(0, j0);
+            hr[1] = hraw(0, j0 + 1);
+            hr[2] = hraw(1, j0);
+            hr[3] = hraw(1, j0 + 1);
+        };
+        auto refillStore = [&](int s) {
+            const int j0 = s + 2 * l;
+            ring[(s + G - 1 + 2 * l) & (kRing - 1)] = make_uint2(prof(hcode(0, j0, hr[0])), prof(hcode(1, j0, hr[2])));
+            ring[(s + G + 2 * l) & (kRing - 1)] =
+                make_uint2(prof(hcode(0, j0 + 1, hr[1])), prof(hcode(1, j0 + 1, hr[3])));
+            __builtin_amdgcn_wave_barrier();
+        };
+        refillLoad(0);
+        refillStore(0);
+        refillLoad(kRefill);
+        uint2 pp = ring[(G - 1 - l) & (kRing - 1)];
         for (int s0 = 0; s0 < steps; s0 += QD) {
             if (coop) {
                 // This chunk reads columns up to s0 + QD of block b - 1.
@@ -402,15 +426,18 @@ struct Fwd16 {
                     known = (int)(__hip_atomic_load(prodIn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & 0xFFFFu) - 1;
                 }
             }
+            if ((s0 & (kRefill - 1)) == kRefill - QD) {
+                // The ring for the next kRefill steps (read from the next step
+                // on), and the residues of the refill after it.
+                refillStore(s0 + QD);
+                refillLoad(s0 + QD + kRefill);
+            }
 #pragma unroll
             for (int u = 0; u < QD; ++u) {
                 const int s = s0 + u;
-                const uint32_t qn = q[(u + 1) % QD];
-                const uint32_t nA = prof(qn & 0xff), nB = prof(qn >> 8);
-                q[u] = (uint32_t)hcode(0, s + QD - l) | ((uint32_t)hcode(1, s + QD - l) << 8);
-                step(s, pA, pB);
-                pA = nA;
-                pB = nB;
+                const uint2 ppN = ring[(s + 1 + G - 1 - l) & (kRing - 1)];
+                step(s, pp.x, pp.y);
+                pp = ppN;
             }
             if (coop) {
                 // Columns up to s0 + QD - G of this block are written.
