@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define SQMI_ABI_VERSION 2   /* 2: sqmi_stats.pairs_long */
+#define SQMI_ABI_VERSION 2   /* 2: sqmi_stats.pairs_long, pairs_long_packed */
 
 typedef enum sqmi_status {
     SQMI_OK = 0,
@@ -104,6 +104,7 @@ typedef struct sqmi_stats {
     double device_ms;           /* device time of the kernels (HIP events)      */
     double total_ms;            /* wall time of the call                        */
     int64_t pairs_long;         /* of pairs_int32: long pairs, tiled kernels    */
+    int64_t pairs_long_packed;  /* of pairs_long: forward in packed int16       */
 } sqmi_stats;
 
 typedef struct sqmi_context sqmi_context;

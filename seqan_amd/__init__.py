@@ -64,7 +64,7 @@ class Stats(ctypes.Structure):
                 ("pairs_int32", ctypes.c_int64), ("work_items", ctypes.c_int64),
                 ("padded_cells", ctypes.c_int64), ("host_plan_ms", ctypes.c_double),
                 ("device_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("pairs_long", ctypes.c_int64)]
+                ("pairs_long", ctypes.c_int64), ("pairs_long_packed", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -191,6 +191,7 @@ struct Key
 struct Planned
 {
     ScoringInfo si;
+    uint32_t flags = 0;
     bool local = false;
     uint32_t track = 0;
     std::vector<int32_t> emptyIds, ids32;
@@ -205,6 +206,23 @@ struct Planned
 // int32 pairs whose sides are both at least this long (and that are DNA,
 // global) run the tiled long-pair kernels instead of one thread per pair
 // (2048, or SQMI_LONG_MIN_SIDE).
+// The packed long-pair forward keeps tile values relative to a base: it
+// needs kLong16Span steps of at most D = max|s| + |o| + |e| inside the int16
+// window.
+bool longPacked(const ScoringInfo &si, uint32_t flags)
+{
+    const int64_t D = std::max(std::abs(si.sMax), std::abs(si.sMin)) + std::abs(si.o) + std::abs(si.e);
+    return !(flags & SQMI_FORCE_INT32) && (int64_t)kLong16Span * D <= std::min(-kNeg16 - kGuard16, 32767 - kGuard16);
+}
+
+// Test and tuning switch: SQMI_LONG_ALL=1 sends every pair the long route
+// can take there, also those the packed kernels could score.
+bool longAll()
+{
+    const char *env = std::getenv("SQMI_LONG_ALL");
+    return env && *env == '1';
+}
+
 int64_t longMinSide()
 {
     const char *env = std::getenv("SQMI_LONG_MIN_SIDE");
@@ -275,6 +293,7 @@ inline int32_t blocksOf(uint32_t key) { return (int32_t)(255u - ((key >> kKeyNBi
 void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &scoring, uint32_t mode,
           uint32_t flags, Planned &P, bool traceShape = false)
 {
+    P.flags = flags;
     const int64_t N = H.count;
     ScoringInfo &si = P.si;
     si.alpha = scoring.alphabet_size;
@@ -317,6 +336,8 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
     }
     std::vector<Key> keys((size_t)N);
     int64_t cells = 0;
+    const bool allLong = longAll() && si.dna && mode == 0;
+    const int64_t longMin0 = longMinSide();
 #pragma omp parallel for schedule(static) reduction(+ : cells)
     for (int64_t i = 0; i < N; ++i) {
         const int64_t n = H.offsets[i + 1] - H.offsets[i];
@@ -325,7 +346,9 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
         if (n != 0 && m != 0) {
             cells += n * m;
             key = 2u << kKeyClassShift;
-            if (costsOk16 && fitsInt16(si, n, m, local)) {
+            if (allLong && n >= longMin0 && m >= longMin0) {
+                // long route (see longAll)
+            } else if (costsOk16 && fitsInt16(si, n, m, local)) {
                 const uint32_t vidx = shapes.size() == 2 && m + 1 <= 128 ? 1u : 0u;
                 const int64_t BR = (int64_t)shapes[vidx].G * shapes[vidx].R;
                 const int64_t nblocks = (m + 1 + BR - 1) / BR;
@@ -448,6 +471,7 @@ void fillStats(const Planned &P, sqmi_stats &st)
     st.cells = P.cells;
     st.pairs_int32 = (int64_t)(P.ids32.size() + P.idsLong.size());
     st.pairs_long = (int64_t)P.idsLong.size();
+    st.pairs_long_packed = longPacked(P.si, P.flags) ? st.pairs_long : 0;
     st.pairs_int16 = (int64_t)P.keys16.size();
     int64_t items = 0, padded = 0;
     for (const Plan16 &pl : P.plans) {
@@ -590,7 +614,7 @@ sqmi_status uploadInputs(sqmi_context *ctx, const sqmi_sequences *setH, const sq
 // walked.  Pairs run in chunks whose checkpoints fit traceBudget().  `pairs`
 // holds the host copy of the descriptors until the caller synchronises.
 sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV, const Planned &P,
-                    const Inputs &in, int32_t *dCounts, std::vector<LongPair> &pairs)
+                    const Inputs &in, uint32_t flags, int32_t *dCounts, std::vector<LongPair> &pairs)
 {
     if (P.idsLong.empty())
         return SQMI_OK;
@@ -598,11 +622,19 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
     const int C = kLongTileC;
     const int64_t budget = traceBudget(ctx);
     const int64_t N = setH->count;
+    const bool packed = longPacked(P.si, flags);
+    // Longest first, so that the couples of the packed forward pair similar shapes.
+    std::vector<int32_t> ids(P.idsLong);
+    std::stable_sort(ids.begin(), ids.end(), [&](int32_t x, int32_t y) {
+        const int64_t mx = setV->offsets[x + 1] - setV->offsets[x], my = setV->offsets[y + 1] - setV->offsets[y];
+        const int64_t nx = setH->offsets[x + 1] - setH->offsets[x], ny = setH->offsets[y + 1] - setH->offsets[y];
+        return mx != my ? mx > my : nx > ny;
+    });
     // Descriptors with checkpoint offsets relative to the pair's chunk.
     std::vector<size_t> chunkEnd;
     int64_t run = 0, maxRun = 0;
     pairs.clear();
-    for (int32_t id : P.idsLong) {
+    for (int32_t id : ids) {
         LongPair lp{};
         lp.pid = id;
         lp.n = (int32_t)(setH->offsets[id + 1] - setH->offsets[id]);
@@ -660,7 +692,7 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
         a.wavesPerPair = (maxB + 3) / 4;
         for (int d = 0; d < maxB + maxT - 1; ++d) {
             a.diag = d;
-            if ((err = launchTileFwd32(a, stream)) != hipSuccess)
+            if ((err = packed ? launchTileFwd16(a, stream) : launchTileFwd32(a, stream)) != hipSuccess)
                 return hipFail(ctx, err, "launching long-pair forward");
         }
         if (dCounts) {
@@ -967,7 +999,7 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
             return hipFail(ctx, err, "launching int32 kernel");
     }
     std::vector<LongPair> longPairs;
-    if ((st = runLong(ctx, setH, setV, P, in, nullptr, longPairs)) != SQMI_OK)
+    if ((st = runLong(ctx, setH, setV, P, in, flags, nullptr, longPairs)) != SQMI_OK)
         return st;
     if (!emptyIds.empty()) {
         if ((err = ensure(ctx->emptyIds, 4 * emptyIds.size())) != hipSuccess)
@@ -1229,7 +1261,7 @@ sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *setH, con
         }
     }
     std::vector<LongPair> longPairs;
-    if ((st = runLong(ctx, setH, setV, P, in, dCounts, longPairs)) != SQMI_OK)
+    if ((st = runLong(ctx, setH, setV, P, in, flags, dCounts, longPairs)) != SQMI_OK)
         return st;
     if (!P.emptyIds.empty()) {
         if ((err = ensure(ctx->emptyIds, 4 * P.emptyIds.size())) != hipSuccess)

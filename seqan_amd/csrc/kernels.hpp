@@ -185,6 +185,10 @@ constexpr int kLongTileC = 256;
 constexpr int kTileRecSteps = kLongTileC + kTileSlack;
 constexpr int kRoundBlocks = 8, kRoundTiles = 8, kRoundSlots = kRoundBlocks * kRoundTiles;
 hipError_t launchTileFwd32(const TileArgs &args, hipStream_t stream);
+// Packed variant: pair couples (2 c, 2 c + 1), values relative to a per-tile
+// base; valid when kLong16Span x (max|s| + |o| + |e|) fits the int16 window.
+constexpr int kLong16Span = 800;
+hipError_t launchTileFwd16(const TileArgs &args, hipStream_t stream);
 hipError_t launchTileTrace32(const TileArgs &args, hipStream_t stream);
 hipError_t launchTileWalkInit(const TileArgs &args, hipStream_t stream);
 hipError_t launchTileWalk32(const TileArgs &args, hipStream_t stream);

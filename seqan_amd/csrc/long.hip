@@ -265,6 +265,281 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         tileSweep<false, false>(a, P, g, l, sProf, &sRings[threadIdx.x / TG], nullptr);
 }
 
+// ---------------------------------------------------------------------------
+// Packed forward: two pairs per lane (low / high int16 half), values held
+// relative to a per-tile base.  Within one tile every value lies within
+// (256 rows + 2 x 256 columns + skew) x D of the base, D = max|s| + |o| + |e|
+// (one row or column moves H by at most D; the columns past a shorter
+// pair's end lose at most |o| each), so the host takes this kernel when
+// 800 x D stays inside the int16 window; -inf is kNeg16 as in the packed
+// kernels.  The checkpoints stay int32 (converted at the ring refills and
+// at the tile's right edge), so the traceback recomputes tiles exactly as
+// after the int32 forward.
+
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ s16x2 as16(uint32_t x) { return __builtin_bit_cast(s16x2, x); }
+__device__ __forceinline__ uint32_t asu(s16x2 x) { return __builtin_bit_cast(uint32_t, x); }
+__device__ __forceinline__ s16x2 vmax(s16x2 a, s16x2 b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ s16x2 splat(int x)
+{
+    s16x2 r = {(short)x, (short)x};
+    return r;
+}
+__device__ __forceinline__ uint32_t packu(int lo, int hi) { return (uint32_t)(uint16_t)lo | ((uint32_t)hi << 16); }
+// Saturating packed add (v_pk_add_i16 with clamp): the rows above a pair's
+// row 0 sink towards -32768 instead of wrapping, so no block needs a floor.
+__device__ __forceinline__ s16x2 adds(s16x2 a, s16x2 b) { return __builtin_elementwise_add_sat(a, b); }
+
+// int32 value -> int16 relative to base (-inf stays -inf), and back.
+__device__ __forceinline__ int toRel(int v, int base)
+{
+    return v <= kNegL / 2 ? kNeg16 : min(max(v - base, kNeg16), 32767);
+}
+__device__ __forceinline__ int fromRel(int r, int base) { return r <= kNeg16 + kGuard16 ? kNegL : r + base; }
+
+// Per-group LDS rings of the packed sweep: top[s]: the block above's bottom
+// row (G, F, both halves packed, relative) at column c0 - 1 + s; pr[x]:
+// the profile words (pair A, pair B) of column c0 - 1 + x - 15; out[s]: the
+// group's bottom row (G, F packed) of step s, converted to int32 and
+// written to the row checkpoints at the next refill.
+struct PairRings {
+    uint2 top[TRING];
+    uint2 pr[TRING];
+    uint2 out[TRING];
+};
+
+__device__ __forceinline__ void tileSweep16(const TileArgs &a, const LongPair (&P)[2], const bool (&act)[2], int b, int t,
+                                            int l, const uint32_t *sProf, PairRings *rg)
+{
+    const int o = a.gapOpen, e = a.gapExtend, eo = max(o, e);
+    const s16x2 o2 = splat(o), e2 = splat(e), neg2 = splat(kNeg16);
+    const int C = a.tileC;
+    const int c0 = t * C + 1;
+    int32_t *ck = a.ck;
+    int c1[2], base[2];
+    int64_t rowIn[2], rowOut[2], colIn[2], colOut[2];
+    const uint8_t *h[2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+        c1[x] = min((t + 1) * C, P[x].n);
+        h[x] = a.hRes + a.hOff[P[x].pid];
+        // Inactive halves point at block 0's own checkpoints (in bounds; values unused).
+        const int bb = act[x] ? b : 0, tt = act[x] ? t : 0;
+        rowIn[x] = P[x].rowck + (bb > 0 ? (int64_t)(bb - 1) * (P[x].n + 1) * 2 : 0);
+        rowOut[x] = P[x].rowck + (int64_t)bb * (P[x].n + 1) * 2;
+        colIn[x] = P[x].colck + (((int64_t)bb * P[x].ntiles + max(tt - 1, 0)) * kTileRows + l * TR) * 2;
+        colOut[x] = P[x].colck + (((int64_t)bb * P[x].ntiles + tt) * kTileRows + l * TR) * 2;
+        // Base: G at the block's last row (always a real row), column c0 - 1.
+        base[x] = 0;
+        if (act[x]) {
+            const int iLast = b * kTileRows + kTileRows - 1 - P[x].p;
+            base[x] = t == 0 ? ((iLast == 0) ? o : (o + (iLast - 1) * eo) + o)
+                             : ck[P[x].colck + (((int64_t)b * P[x].ntiles + (t - 1)) * kTileRows + kTileRows - 1) * 2];
+        }
+    }
+    const int c1max = max(act[0] ? c1[0] : 0, act[1] ? c1[1] : 0);
+
+    uint32_t sel[TR];
+    s16x2 Gk[TR], Ek[TR];
+    const int rowbase = b * kTileRows + l * TR;
+#pragma unroll
+    for (int k = 0; k < TR; ++k) {
+        int code[2], g[2], ev[2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            const int i = rowbase + k - P[x].p;
+            const uint8_t *v = a.vRes + a.vOff[P[x].pid];
+            code[x] = (i >= 1 && i <= P[x].m) ? min((int)v[i - 1], 4) : 4;
+            if (!act[x]) {
+                g[x] = kNeg16;
+                ev[x] = kNeg16;
+            } else if (t == 0) {
+                // Column 0: -inf above row 0, H(0,0) = 0, then the gap chain.
+                g[x] = (i < 0) ? kNeg16 : toRel((i == 0) ? o : (o + (i - 1) * eo) + o, base[x]);
+                ev[x] = kNeg16;
+            } else {
+                g[x] = toRel(ck[colIn[x] + 2 * k], base[x]);
+                ev[x] = toRel(ck[colIn[x] + 2 * k + 1], base[x]);
+            }
+        }
+        const uint32_t sa = code[0] < 4 ? (uint32_t)(code[0] | 0x0c00) : 0x0d0cu;
+        const uint32_t sb = code[1] < 4 ? (uint32_t)((code[1] + 4) | 0x0c00) : 0x0d0cu;
+        sel[k] = sa | (sb << 16);
+        Gk[k] = as16(packu(g[0], g[1]));
+        Ek[k] = as16(packu(ev[0], ev[1]));
+    }
+    if (t == 0 && l == TG - 1) {
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+            if (act[x])   // column 0 of the bottom row
+                *reinterpret_cast<int2 *>(ck + rowOut[x]) =
+                    make_int2(fromRel(x ? Gk[TR - 1].y : Gk[TR - 1].x, base[x]), kNegL);
+    }
+
+    // Ring refills (the same scheme as tileSweep): lane l covers columns
+    // c0 - 1 + s + 2 l and + 1 of both halves.
+    int2 tv[2][2];
+    uint32_t hv[2][2];
+    auto refillLoad = [&](int s) {
+        const int cb = c0 - 1 + s + 2 * l;
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                const int col = min(cb + y, max(c1[x], 0));
+                tv[x][y] = *reinterpret_cast<const int2 *>(ck + rowIn[x] + 2 * col);
+                hv[x][y] = h[x][min(max(cb + y, 1), P[x].n) - 1];
+            }
+    };
+    auto refillStore = [&](int s) {
+        const int cb = c0 - 1 + s + 2 * l;
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            int tg[2], tf[2];
+            uint32_t pw[2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                const bool real = act[x] && b > 0 && cb + y <= c1[x];
+                tg[x] = real ? toRel(tv[x][y].x, base[x]) : kNeg16;
+                tf[x] = real ? toRel(tv[x][y].y, base[x]) : kNeg16;
+                const int jj = cb + y;
+                pw[x] = sProf[(jj >= 1 && jj <= P[x].n) ? min(hv[x][y], 4u) : 4u];
+            }
+            rg->top[(s + 2 * l + y) & (TRING - 1)] = make_uint2(packu(tg[0], tg[1]), packu(tf[0], tf[1]));
+            rg->pr[(s + TG - 1 + 2 * l + y) & (TRING - 1)] = make_uint2(pw[0], pw[1]);
+        }
+        __builtin_amdgcn_wave_barrier();
+    };
+    // Bottom-row outputs of steps [f, s) to the row checkpoints.
+    auto flushOut = [&](int f, int s) {
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const int st = f + 2 * l + y;
+            const int jj = c0 - 1 + st - (TG - 1);
+            if (st < s && jj >= c0) {
+                const uint2 w = rg->out[st & (TRING - 1)];
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+                    if (act[x] && jj <= c1[x]) {
+                        const int gr = x ? as16(w.x).y : as16(w.x).x, fr = x ? as16(w.y).y : as16(w.y).x;
+                        *reinterpret_cast<int2 *>(ck + rowOut[x] + 2 * jj) =
+                            make_int2(fromRel(gr, base[x]), fromRel(fr, base[x]));
+                    }
+            }
+        }
+    };
+
+    int endCol[2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+        endCol[x] = (act[x] && b == P[x].nblocks - 1 && c1[x] == P[x].n) ? P[x].n : -1;
+    uint32_t botG = asu(Gk[TR - 1]), botF = asu(neg2);
+    s16x2 upGprev = neg2;
+    const int width = max(c1max - c0 + 1, 0);
+    const int steps = ((width + TG + TQD - 1) / TQD) * TQD;
+    refillLoad(0);
+    refillStore(0);
+    refillLoad(TK);
+    uint2 pp = rg->pr[(TG - 1 - l) & (TRING - 1)];
+    uint2 topv = rg->top[0];
+    int flushed = 0;
+
+    auto step = [&](int s, uint2 pr, uint2 top) {
+        const int jj = c0 - 1 + s - l;
+        const s16x2 upG = as16((uint32_t)shiftIn16((int)botG, (int)top.x));
+        const s16x2 upF = as16((uint32_t)shiftIn16((int)botF, (int)top.y));
+        const s16x2 diagTop = upGprev;
+        upGprev = upG;
+        if (jj >= c0 && jj <= c1max) {
+            s16x2 uG = upG, uF = upF;
+            // D of row k from the old G of row k - 1 (no register copies).
+            s16x2 D = adds(diagTop, as16(__builtin_amdgcn_perm(pr.y, pr.x, sel[0])));
+#pragma unroll
+            for (int k = 0; k < TR; ++k) {
+                const s16x2 E = vmax(adds(Ek[k], e2), Gk[k]);
+                const s16x2 F = vmax(adds(uF, e2), uG);
+                const s16x2 H = vmax(vmax(D, E), F);
+                if (k + 1 < TR)
+                    D = adds(Gk[k], as16(__builtin_amdgcn_perm(pr.y, pr.x, sel[k + 1])));
+                Ek[k] = E;
+                Gk[k] = adds(H, o2);
+                uG = Gk[k];
+                uF = F;
+            }
+            botG = asu(uG);
+            botF = asu(uF);
+            if (l == TG - 1)
+                rg->out[s & (TRING - 1)] = make_uint2(botG, botF);
+            // Cell (m, n): the last row of the last block.
+            if (l == TG - 1 && (jj == endCol[0] || jj == endCol[1])) {
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+                    if (jj == endCol[y]) {
+                        a.scores[P[y].pid] = fromRel(y ? uG.y : uG.x, base[y]) - o;
+                        a.endEF[2 * P[y].pid] = fromRel(y ? Ek[TR - 1].y : Ek[TR - 1].x, base[y]);
+                        a.endEF[2 * P[y].pid + 1] = fromRel(y ? uF.y : uF.x, base[y]);
+                    }
+            }
+        }
+    };
+    for (int s0 = 0; s0 < steps; s0 += TQD) {
+        if ((s0 & (TK - 1)) == TK - TQD) {
+            refillStore(s0 + TQD);
+            refillLoad(s0 + TQD + TK);
+            flushOut(flushed, s0);
+            flushed = s0;
+        }
+#pragma unroll
+        for (int u = 0; u < TQD; ++u) {
+            const int s = s0 + u;
+            const uint2 prN = rg->pr[(s + 1 + TG - 1 - l) & (TRING - 1)];
+            const uint2 topN = rg->top[(s + 1) & (TRING - 1)];
+            step(s, pp, topv);
+            pp = prN;
+            topv = topN;
+        }
+    }
+    flushOut(flushed, steps);
+    // Right column of a full-width tile (the column the sweep ended on).
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+        if (act[x] && t + 1 < P[x].ntiles && c1[x] == c1max) {
+#pragma unroll
+            for (int k = 0; k < TR; ++k)
+                *reinterpret_cast<int2 *>(ck + colOut[x] + 2 * k) =
+                    make_int2(fromRel(x ? Gk[k].y : Gk[k].x, base[x]), fromRel(x ? Ek[k].y : Ek[k].x, base[x]));
+        }
+}
+
+// Packed forward of the tiles of anti-diagonal a.diag: wave w takes four
+// consecutive row blocks (one per 16-lane group) of the pair couple
+// (2 c, 2 c + 1) of the chunk.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kTileFwd16(TileArgs a)
+{
+    __shared__ uint32_t sProf[8];
+    __shared__ PairRings sRings[256 / TG];
+    loadProfile(a, sProf);
+    const int lane = threadIdx.x & 63, l = lane % TG, grp = lane / TG;
+    const int w = blockIdx.x * (blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+    const int cp = w / a.wavesPerPair;
+    if (2 * cp >= a.nPairs)
+        return;
+    const bool hasB = 2 * cp + 1 < a.nPairs;
+    const LongPair P[2] = {a.pairs[2 * cp], a.pairs[hasB ? 2 * cp + 1 : 2 * cp]};
+    const int b0 = (w % a.wavesPerPair) * 4;
+    const int b = b0 + grp, t = a.diag - b;
+    bool act[2];
+    act[0] = b < P[0].nblocks && t >= 0 && t < P[0].ntiles;
+    act[1] = hasB && b < P[1].nblocks && t >= 0 && t < P[1].ntiles;
+    if (!__any(act[0] || act[1]))
+        return;   // the whole wave is idle on this diagonal
+    // Idle groups sweep a clamped tile of pair A with both halves inactive.
+    const int bq = (act[0] || act[1]) ? b : min(b, P[0].nblocks - 1);
+    const int tq = (act[0] || act[1]) ? t : min(max(t, 0), P[0].ntiles - 1);
+    tileSweep16(a, P, act, bq, tq, l, sProf, &sRings[threadIdx.x / TG]);
+}
+
 // Walk state of one long pair: the cell (i, j) reached, the walk mode
 // (0: choose from the cell's H decision, 1 / 2: inside a vertical /
 // horizontal gap), the last operation, gap opens / extensions, done flag.
@@ -485,6 +760,15 @@ hipError_t launchTileFwd32(const TileArgs &args, hipStream_t stream)
     if (waves <= 0)
         return hipSuccess;
     hipLaunchKernelGGL(kTileFwd32, dim3((waves + 3) / 4), dim3(256), 0, stream, args);
+    return hipGetLastError();
+}
+
+hipError_t launchTileFwd16(const TileArgs &args, hipStream_t stream)
+{
+    const int waves = (args.nPairs + 1) / 2 * args.wavesPerPair;
+    if (waves <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kTileFwd16, dim3((waves + 3) / 4), dim3(256), 0, stream, args);
     return hipGetLastError();
 }
 

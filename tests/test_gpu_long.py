@@ -3,10 +3,13 @@ x 256 columns) as a checkpoint, and the checkpointed traceback that walks
 back through windows of recomputed tiles.
 
 Global DNA pairs outside the int16 window with both sides >=
-SQMI_LONG_MIN_SIDE (2048 by default) take this route.  The parity tests force
-int32 (FORCE_INT32) and lower the threshold to 256 so that pairs of a few
-thousand bases, which the oracle finishes in seconds, cross many row blocks
-and column tiles; the full 64 x 100 kbp workload is checked through
+SQMI_LONG_MIN_SIDE (2048 by default) take this route; its forward runs in
+packed int16 relative to a per-tile base when the scoring allows, else in
+int32.  The parity tests lower the threshold to 256 and run every case on
+both forwards: int32 (FORCE_INT32) and packed (SQMI_LONG_ALL=1, which sends
+pairs the packed kernels could score down the long route too), so that
+pairs of a few thousand bases, which the oracle finishes in seconds, cross
+many row blocks and column tiles; the full 64 x 100 kbp workload is checked through
 properties every alignment satisfies (sum of count x parameter == score,
 aligned pairs + gap characters == residues), against the packed kernels on
 its first pairs cut short, and against a fixture of its first two pairs."""
@@ -72,30 +75,41 @@ def _oracle_counts(H, V, score):
                               po.Scoring(score.table, score.gap_open, score.gap_extend), 0)
 
 
-@pytest.fixture
-def long_256(monkeypatch):
+@pytest.fixture(params=["int32", "packed"])
+def route(request, monkeypatch):
+    """Flags for the long route with threshold 256 and the given forward."""
     monkeypatch.setenv("SQMI_LONG_MIN_SIDE", "256")
+    if request.param == "packed":
+        monkeypatch.setenv("SQMI_LONG_ALL", "1")
+        return 0
+    return F32
+
+
+def _check_route(ctx, count, flags):
+    st = ctx.stats()
+    assert st["pairs_long"] == count
+    assert st["pairs_long_packed"] == (0 if flags & F32 else count)
 
 
 @pytest.mark.parametrize("lo,hi", [(256, 700), (900, 2600)])
-def test_long_scores_match_oracle(ctx, long_256, lo, hi):
+def test_long_scores_match_oracle(ctx, route, lo, hi):
     rng = np.random.default_rng(lo + hi)
     H, V = _mutated_pairs(rng, 40, lo, hi)
     for score in SCORES:
         print("long scores", lo, hi, score.table[0, :2], score.gap_extend, score.gap_open, flush=True)
-        s = ctx.align_score(H, V, score, 0, flags=F32)
-        assert ctx.stats()["pairs_long"] == len(H)
+        s = ctx.align_score(H, V, score, 0, flags=route)
+        _check_route(ctx, len(H), route)
         assert np.array_equal(s, _oracle_scores(H, V, score))
 
 
 @pytest.mark.parametrize("lo,hi", [(256, 700), (900, 2600)])
-def test_long_counts_match_oracle(ctx, long_256, lo, hi):
+def test_long_counts_match_oracle(ctx, route, lo, hi):
     rng = np.random.default_rng(3 * lo + hi)
     H, V = _mutated_pairs(rng, 40, lo, hi)
     for score in SCORES:
         print("long counts", lo, hi, score.table[0, :2], score.gap_extend, score.gap_open, flush=True)
-        s, c = ctx.align_counts(H, V, score, flags=F32)
-        assert ctx.stats()["pairs_long"] == len(H)
+        s, c = ctx.align_counts(H, V, score, flags=route)
+        _check_route(ctx, len(H), route)
         ws, wc = _oracle_counts(H, V, score)
         assert np.array_equal(s, ws)
         bad = np.nonzero((c != wc).any(axis=1))[0]
@@ -103,7 +117,7 @@ def test_long_counts_match_oracle(ctx, long_256, lo, hi):
         _check_invariants(H, V, score, s, c)
 
 
-def test_long_edges(ctx, long_256):
+def test_long_edges(ctx, route):
     # Sides at and around the block (256 rows) and tile (256 columns) edges,
     # very unequal sides, and identical / unrelated sequences.
     rng = np.random.default_rng(11)
@@ -119,15 +133,15 @@ def test_long_edges(ctx, long_256):
     H, V = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
     for score in SCORES[:2]:
         print("long edges", score.table[0, :2], flush=True)
-        s, c = ctx.align_counts(H, V, score, flags=F32)
-        assert ctx.stats()["pairs_long"] == len(H)
+        s, c = ctx.align_counts(H, V, score, flags=route)
+        _check_route(ctx, len(H), route)
         ws, wc = _oracle_counts(H, V, score)
         assert np.array_equal(s, ws)
         assert np.array_equal(c, wc)
-        assert np.array_equal(ctx.align_score(H, V, score, 0, flags=F32), ws)
+        assert np.array_equal(ctx.align_score(H, V, score, 0, flags=route), ws)
 
 
-def test_long_mixed_with_packed_routes(ctx, long_256):
+def test_long_mixed_with_packed_routes(ctx, route):
     # Long pairs beside short ones, an empty pair and a local-mode call
     # (local never takes the long route).
     rng = np.random.default_rng(12)
@@ -136,33 +150,36 @@ def test_long_mixed_with_packed_routes(ctx, long_256):
     vs = [Vl[i] for i in range(6)] + [rng.integers(0, 4, 90).astype(np.uint8), rng.integers(0, 4, 5).astype(np.uint8)]
     H, V = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
     score = SCORES[0]
-    s, c = ctx.align_counts(H, V, score, flags=F32)
-    assert ctx.stats()["pairs_long"] == 6
+    s, c = ctx.align_counts(H, V, score, flags=route)
+    _check_route(ctx, 6, route)
     ws, wc = _oracle_counts(H, V, score)
     assert np.array_equal(s, ws) and np.array_equal(c, wc)
     for mode in (0, 9, 15):
         print("mixed mode", mode, flush=True)
         want = po.score_batch(H.residues, H.offsets, V.residues, V.offsets,
                               po.Scoring(score.table, score.gap_open, score.gap_extend), mode)
-        assert np.array_equal(ctx.align_score(H, V, score, mode, flags=F32), want)
+        assert np.array_equal(ctx.align_score(H, V, score, mode, flags=route), want)
         assert ctx.stats()["pairs_long"] == (6 if mode == 0 else 0)
 
 
-def test_long_chunked_checkpoints(ctx, long_256, monkeypatch):
-    # A 1 MiB budget puts every pair in a chunk of its own.
+def test_long_chunked_checkpoints(ctx, route, monkeypatch):
+    # A 1 MiB budget puts every pair in a chunk of its own (and an odd
+    # pair count leaves the last couple of the packed forward half empty).
     rng = np.random.default_rng(13)
-    H, V = _mutated_pairs(rng, 10, 800, 1600)
+    H, V = _mutated_pairs(rng, 11, 800, 1600)
     score = SCORES[0]
-    whole = ctx.align_counts(H, V, score, flags=F32)
-    assert ctx.stats()["pairs_long"] == len(H)
+    whole = ctx.align_counts(H, V, score, flags=route)
+    _check_route(ctx, len(H), route)
     monkeypatch.setenv("SQMI_TRACE_BUDGET_MB", "1")
     print("long chunked", flush=True)
-    chunked = ctx.align_counts(H, V, score, flags=F32)
+    chunked = ctx.align_counts(H, V, score, flags=route)
     assert np.array_equal(whole[0], chunked[0]) and np.array_equal(whole[1], chunked[1])
-    assert np.array_equal(ctx.align_score(H, V, score, 0, flags=F32), whole[0])
+    assert np.array_equal(ctx.align_score(H, V, score, 0, flags=route), whole[0])
+    ws, wc = _oracle_counts(H, V, score)
+    assert np.array_equal(whole[0], ws) and np.array_equal(whole[1], wc)
 
 
-def test_long_workload_prefix_matches_packed(ctx):
+def test_long_workload_prefix_matches_packed(ctx, monkeypatch):
     # The 100 kbp workload's first 16 pairs cut to 4 kbp: long route (int32
     # forced) against the packed int16 kernels.
     H, V = sa.workload(sa.WL_DNA100K, 0x5EED0005, 0, 16, 4000)
@@ -172,18 +189,23 @@ def test_long_workload_prefix_matches_packed(ctx):
     s_pk, c_pk = ctx.align_counts(H, V, score)
     assert ctx.stats()["pairs_long"] == 0 and ctx.stats()["pairs_int16"] == 16
     assert np.array_equal(s_long, s_pk) and np.array_equal(c_long, c_pk)
+    monkeypatch.setenv("SQMI_LONG_ALL", "1")
+    s_l16, c_l16 = ctx.align_counts(H, V, score)
+    assert ctx.stats()["pairs_long_packed"] == 16
+    assert np.array_equal(s_l16, s_pk) and np.array_equal(c_l16, c_pk)
 
 
-def test_long_100kbp_fixture(ctx):
+@pytest.mark.parametrize("flags", [0, F32], ids=["packed", "int32"])
+def test_long_100kbp_fixture(ctx, flags):
     # The 100 kbp workload's first two pairs at full length: scores from the
     # reference's globalAlignmentScore, counts from the oracle's traceback.
     z = np.load(os.path.join(GOLDEN, "ref_dna100k_counts.npz"), allow_pickle=False)
     H, V = sa.SeqSet(z["h_res"], z["h_off"]), sa.SeqSet(z["v_res"], z["v_off"])
     score = SCORES[0]
-    print("100 kbp fixture: 2 pairs", flush=True)
-    assert np.array_equal(ctx.align_score(H, V, score, 0), z["scores"])
-    assert ctx.stats()["pairs_long"] == 2
-    s, c = ctx.align_counts(H, V, score)
+    print("100 kbp fixture: 2 pairs, flags", flags, flush=True)
+    assert np.array_equal(ctx.align_score(H, V, score, 0, flags=flags), z["scores"])
+    _check_route(ctx, 2, flags)
+    s, c = ctx.align_counts(H, V, score, flags=flags)
     assert np.array_equal(s, z["scores"]) and np.array_equal(c, z["counts"])
 
 
@@ -196,7 +218,9 @@ def test_long_full_100kbp_workload(ctx):
     s = ctx.align_score(H, V, score, 0)
     st = ctx.stats()
     print("100 kbp forward stats", st, flush=True)
-    assert st["pairs_long"] == 64
+    _check_route(ctx, 64, 0)
+    assert np.array_equal(ctx.align_score(H, V, score, 0, flags=F32), s)
+    _check_route(ctx, 64, F32)
     print("100 kbp: backward", flush=True)
     s2, c = ctx.align_counts(H, V, score)
     print("100 kbp backward stats", ctx.stats(), flush=True)
