@@ -17,6 +17,10 @@
  *                           include/seqan/align_parallel/parallel_align_interface.h:288-299
  *                         followed by computeAlignmentStats on the rows
  *                           include/seqan/align/evaluate_alignment.h:215-300
+ *   sqmi_align_rows    <- the same globalAlignment call, returning the rows
+ *                           its traceback writes into the Gaps objects
+ *                           (dp_traceback_impl.h:115-546, gaps set through
+ *                           include/seqan/align/dp_traceback_adaptor.h)
  *
  * Semantics follow the reference exactly: affine gaps where a gap of k
  * characters costs gap_open + (k-1)*gap_extend (Score<int,Simple> constructor
@@ -33,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQMI_ABI_VERSION 2   /* 2: sqmi_stats.pairs_long, pairs_long_packed */
+#define SQMI_ABI_VERSION 2   /* 2: sqmi_stats.pairs_long, pairs_long_packed; sqmi_align_rows */
 
 typedef enum sqmi_status {
     SQMI_OK = 0,
@@ -131,6 +135,22 @@ sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *set_h,
                               const sqmi_sequences *set_v, const sqmi_scoring *scoring,
                               uint32_t mode, int32_t *scores, int32_t *counts,
                               uint32_t flags);
+
+/* sqmi_align_counts plus the alignment rows: one byte per alignment column,
+ * 'M' (a residue of H over a residue of V), 'V' (a gap in the H row: a
+ * residue of V alone) or 'H' (a gap in the V row: a residue of H alone), in
+ * left-to-right order.  Pair i may use ops[op_offsets[i] .. op_offsets[i+1]),
+ * which must hold at least |H_i| + |V_i| bytes; its rows end at
+ * op_offsets[i+1] and start at op_offsets[i+1] - op_lengths[i] (the bytes
+ * before them are unspecified).  op_offsets: count + 1 int64 in host memory;
+ * ops and op_lengths (count int64) are host memory, or device memory with
+ * SQMI_SCORES_ON_DEVICE.  An empty pair gets INT32_MIN, zero counts and no
+ * rows.  Global alignment (mode 0) only. */
+sqmi_status sqmi_align_rows(sqmi_context *ctx, const sqmi_sequences *set_h,
+                            const sqmi_sequences *set_v, const sqmi_scoring *scoring,
+                            uint32_t mode, int32_t *scores, int32_t *counts,
+                            uint8_t *ops, const int64_t *op_offsets, int64_t *op_lengths,
+                            uint32_t flags);
 
 sqmi_status sqmi_get_stats(const sqmi_context *ctx, sqmi_stats *out);
 

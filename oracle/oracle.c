@@ -323,6 +323,24 @@ void orc_traceback_batch(const uint8_t *h_res, const int64_t *h_off,
     (void)threads;
 }
 
+void orc_rows_batch(const uint8_t *h_res, const int64_t *h_off,
+                    const uint8_t *v_res, const int64_t *v_off, int64_t count,
+                    const orc_scoring *sc, uint32_t mode, int32_t *scores,
+                    int32_t *counts, char *ops, const int64_t *op_off, int64_t *op_len,
+                    int threads)
+{
+    const int64_t stride = (int64_t)sc->alpha * sc->alpha + 2;
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads_wrapper();
+#pragma omp parallel for schedule(dynamic, 4) num_threads(threads)
+#endif
+    for (int64_t k = 0; k < count; ++k)
+        scores[k] = orc_traceback(h_res + h_off[k], h_off[k + 1] - h_off[k],
+                                  v_res + v_off[k], v_off[k + 1] - v_off[k], sc, mode,
+                                  counts ? counts + k * stride : NULL, ops + op_off[k], op_len + k);
+    (void)threads;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Brute force: enumerate every alignment as an op string and score it with  */
 /* per-row gap runs (open for the first gap character, extend for the rest). */

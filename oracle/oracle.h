@@ -64,6 +64,14 @@ void orc_traceback_batch(const uint8_t *h_res, const int64_t *h_off,
                          const orc_scoring *sc, uint32_t mode, int32_t *scores,
                          int32_t *counts, int threads);
 
+/* Batch form returning the rows too: pair k's ops (as orc_traceback) at
+ * ops + op_off[k] (capacity >= n + m + 2), their number in op_len[k]. */
+void orc_rows_batch(const uint8_t *h_res, const int64_t *h_off,
+                    const uint8_t *v_res, const int64_t *v_off, int64_t count,
+                    const orc_scoring *sc, uint32_t mode, int32_t *scores,
+                    int32_t *counts, char *ops, const int64_t *op_off, int64_t *op_len,
+                    int threads);
+
 /* Exhaustive maximum over all alignments (tiny inputs only, n, m <= 8). */
 int32_t orc_brute_force(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
                         const orc_scoring *sc, uint32_t mode);

@@ -58,6 +58,9 @@ def lib():
         L.orc_traceback_batch.restype = None
         L.orc_traceback_batch.argtypes = [u8p, i64p, u8p, i64p, ctypes.c_int64, sp,
                                           ctypes.c_uint32, i32p, i32p, ctypes.c_int]
+        L.orc_rows_batch.restype = None
+        L.orc_rows_batch.argtypes = [u8p, i64p, u8p, i64p, ctypes.c_int64, sp, ctypes.c_uint32, i32p, i32p,
+                                     u8p, i64p, i64p, ctypes.c_int]
         L.orc_brute_force.restype = ctypes.c_int32
         L.orc_brute_force.argtypes = [u8p, ctypes.c_int64, u8p, ctypes.c_int64, sp, ctypes.c_uint32]
         _lib = L
@@ -126,6 +129,31 @@ def score_batch(h_res, h_off, v_res, v_off, sc: Scoring, mode: int, threads: int
                           _ptr(v_res, ctypes.c_uint8), _ptr(v_off, ctypes.c_int64), n,
                           ctypes.byref(sc._c), mode, _ptr(out, ctypes.c_int32), threads)
     return out
+
+
+def rows_batch(h_res, h_off, v_res, v_off, sc: Scoring, mode: int = 0, threads: int = 0):
+    """Scores, counts and rows of every pair: returns (scores, counts,
+    op_off, ops) with pair k's ops ('M', 'V', 'H' bytes, left to right) in
+    ops[op_off[k]:op_off[k + 1]]."""
+    h_res, v_res = _u8(h_res), _u8(v_res)
+    h_off = np.ascontiguousarray(h_off, dtype=np.int64)
+    v_off = np.ascontiguousarray(v_off, dtype=np.int64)
+    n = len(h_off) - 1
+    cap = np.zeros(n + 1, dtype=np.int64)
+    cap[1:] = np.cumsum(np.diff(h_off) + np.diff(v_off) + 2)
+    scores = np.zeros(n, dtype=np.int32)
+    counts = np.zeros((n, sc.alpha * sc.alpha + 2), dtype=np.int32)
+    buf = np.zeros(int(cap[-1]), dtype=np.uint8)
+    lens = np.zeros(n, dtype=np.int64)
+    lib().orc_rows_batch(_ptr(h_res, ctypes.c_uint8), _ptr(h_off, ctypes.c_int64),
+                         _ptr(v_res, ctypes.c_uint8), _ptr(v_off, ctypes.c_int64), n,
+                         ctypes.byref(sc._c), mode, _ptr(scores, ctypes.c_int32),
+                         _ptr(counts, ctypes.c_int32), _ptr(buf, ctypes.c_uint8), _ptr(cap, ctypes.c_int64),
+                         _ptr(lens, ctypes.c_int64), threads)
+    op_off = np.zeros(n + 1, dtype=np.int64)
+    op_off[1:] = np.cumsum(lens)
+    ops = np.concatenate([buf[cap[k]:cap[k] + lens[k]] for k in range(n)]) if n else np.zeros(0, np.uint8)
+    return scores, counts, op_off, ops
 
 
 def traceback_batch(h_res, h_off, v_res, v_off, sc: Scoring, mode: int = 0, threads: int = 0):
@@ -202,7 +230,12 @@ def run_reference(op, h_res, h_off, v_res, v_off, mode, *, alphabet="dna", matri
            "binary": os.path.basename(exe)}
     if op in (OP_COUNTS_SIMD, OP_COUNTS_SCALAR):
         alpha = 4 if alphabet == "dna" else 27
-        out["counts"] = np.frombuffer(raw, dtype=np.int32, offset=off).reshape(cnt, alpha * alpha + 2).copy()
+        S = alpha * alpha + 2
+        out["counts"] = np.frombuffer(raw, dtype=np.int32, count=cnt * S, offset=off).reshape(cnt, S).copy()
+        off += 4 * cnt * S
+        out["op_off"] = np.frombuffer(raw, dtype=np.int64, count=cnt + 1, offset=off).copy()
+        off += 8 * (cnt + 1)
+        out["ops"] = np.frombuffer(raw, dtype=np.uint8, count=int(out["op_off"][-1]), offset=off).copy()
     return out
 
 

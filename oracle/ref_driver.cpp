@@ -26,7 +26,10 @@
 //   i64 h_off[count+1], i64 v_off[count+1], u8 h_res[h_off[count]],
 //   u8 v_res[v_off[count]]   (residue ordinals).
 // Output file: i64 count, f64 best_seconds, i32 simd_lanes, i32 threads,
-//   i32 scores[count], then for ops 1/2: i32 counts[count][A*A+2].
+//   i32 scores[count], then for ops 1/2: i32 counts[count][A*A+2],
+//   i64 op_off[count+1], u8 ops[op_off[count]]: the rows of each alignment
+//   as one byte per column, 'M' (both rows hold a residue), 'V' (a gap in
+//   the H row) or 'H' (a gap in the V row).
 #include <seqan/align.h>
 #include <seqan/align_parallel.h>
 #include <seqan/score.h>
@@ -185,6 +188,8 @@ int run(RefInput const & in, TScore const & sc, int alpha, std::FILE * out)
 
     std::vector<int32_t> scores(h.count, 0);
     std::vector<int32_t> counts;
+    std::vector<int64_t> opOff;
+    std::vector<uint8_t> ops;
     double best = 1e300;
     if (h.op == 0)
     {
@@ -227,8 +232,16 @@ int run(RefInput const & in, TScore const & sc, int alpha, std::FILE * out)
         }
         auto t1 = std::chrono::steady_clock::now();
         best = std::chrono::duration<double>(t1 - t0).count();
+        opOff.assign(1, 0);
         for (int64_t k = 0; k < h.count; ++k)
+        {
             rowCounts(gH[k], gV[k], sc, alpha, counts.data() + k * (alpha * alpha + 2));
+            auto it0 = begin(gH[k]);
+            auto it1 = begin(gV[k]);
+            for (; it0 != end(gH[k]); ++it0, ++it1)
+                ops.push_back(isGap(it0) ? 'V' : isGap(it1) ? 'H' : 'M');
+            opOff.push_back(static_cast<int64_t>(ops.size()));
+        }
     }
     else
     {
@@ -240,7 +253,11 @@ int run(RefInput const & in, TScore const & sc, int alpha, std::FILE * out)
     std::fwrite(&threads, sizeof(int32_t), 1, out);
     std::fwrite(scores.data(), sizeof(int32_t), scores.size(), out);
     if (!counts.empty())
+    {
         std::fwrite(counts.data(), sizeof(int32_t), counts.size(), out);
+        std::fwrite(opOff.data(), sizeof(int64_t), opOff.size(), out);
+        std::fwrite(ops.data(), 1, ops.size(), out);
+    }
     return 0;
 }
 

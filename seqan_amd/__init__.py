@@ -75,7 +75,7 @@ ABI_VERSION = 2
 
 # Every symbol the C ABI declares (include/seqan_amd.h).
 ABI_SYMBOLS = (
-    "sqmi_create", "sqmi_destroy", "sqmi_set_stream", "sqmi_align_score", "sqmi_align_counts",
+    "sqmi_create", "sqmi_destroy", "sqmi_set_stream", "sqmi_align_score", "sqmi_align_counts", "sqmi_align_rows",
     "sqmi_get_stats", "sqmi_plan_stats", "sqmi_blosum62", "sqmi_status_string", "sqmi_last_error", "sqmi_abi_version",
     "sqmi_workload_create", "sqmi_workload_sizes", "sqmi_workload_copy", "sqmi_workload_destroy",
 )
@@ -100,6 +100,8 @@ def lib():
                                        ctypes.POINTER(_Scoring), ctypes.c_uint32, vp, ctypes.c_uint32]
         L.sqmi_align_counts.argtypes = [vp, ctypes.POINTER(_Sequences), ctypes.POINTER(_Sequences),
                                         ctypes.POINTER(_Scoring), ctypes.c_uint32, vp, vp, ctypes.c_uint32]
+        L.sqmi_align_rows.argtypes = [vp, ctypes.POINTER(_Sequences), ctypes.POINTER(_Sequences),
+                                      ctypes.POINTER(_Scoring), ctypes.c_uint32, vp, vp, vp, vp, vp, ctypes.c_uint32]
         L.sqmi_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.sqmi_plan_stats.argtypes = [ctypes.POINTER(_Sequences), ctypes.POINTER(_Sequences),
                                       ctypes.POINTER(_Scoring), ctypes.c_uint32, ctypes.c_uint32,
@@ -327,6 +329,31 @@ class Context:
                                        mode, scores.ctypes.data, counts.ctypes.data, flags)
         self._check(st, "sqmi_align_counts")
         return scores, counts
+
+    def align_rows(self, set_h: SeqSet, set_v: SeqSet, score: Score, mode=0, flags=0):
+        """Scores, counts and alignment rows of all pairs (global alignment).
+        Returns (scores, counts, op_off, ops): pair i's rows are
+        ops[op_off[i]:op_off[i + 1]], one byte per column: b'M' (residues of
+        H and V), b'V' (a gap in the H row), b'H' (a gap in the V row)."""
+        ch, keep_h = _seqs(set_h)
+        cv, keep_v = _seqs(set_v)
+        sc = score._c()
+        n = len(set_h.offsets) - 1
+        A = score.alphabet_size
+        scores = np.empty(n, np.int32)
+        counts = np.empty((n, A * A + 2), np.int32)
+        seg = np.zeros(n + 1, np.int64)
+        seg[1:] = np.cumsum(set_h.lengths() + set_v.lengths())
+        buf = np.empty(max(int(seg[-1]), 1), np.uint8)
+        lens = np.zeros(n, np.int64)
+        st = self._L.sqmi_align_rows(self._h, ctypes.byref(ch), ctypes.byref(cv), ctypes.byref(sc), mode,
+                                     scores.ctypes.data, counts.ctypes.data, buf.ctypes.data, seg.ctypes.data,
+                                     lens.ctypes.data, flags)
+        self._check(st, "sqmi_align_rows")
+        op_off = np.zeros(n + 1, np.int64)
+        op_off[1:] = np.cumsum(lens)
+        ops = np.concatenate([buf[seg[i + 1] - lens[i]:seg[i + 1]] for i in range(n)]) if n else buf[:0]
+        return scores, counts, op_off, ops
 
 
 # ---------------------------------------------------------------------------

@@ -53,6 +53,8 @@ struct sqmi_context
     DevBuf trace, traceOff, locs, endEF, counts;
     // Long pairs (tiled int32 kernels).
     DevBuf longCk, longPairs, longWalk, longTrace;
+    // Alignment rows (sqmi_align_rows).
+    DevBuf ops, opOff, opLen;
 };
 
 namespace {
@@ -613,8 +615,16 @@ sqmi_status uploadInputs(sqmi_context *ctx, const sqmi_sequences *setH, const sq
 // unfinished pair's current tile is recomputed with trace records and
 // walked.  Pairs run in chunks whose checkpoints fit traceBudget().  `pairs`
 // holds the host copy of the descriptors until the caller synchronises.
+// Device addresses of the alignment rows output (all null: no rows).
+struct RowsDev {
+    uint8_t *ops = nullptr;
+    const int64_t *opOff = nullptr;
+    int64_t *opLen = nullptr;
+};
+
 sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV, const Planned &P,
-                    const Inputs &in, uint32_t flags, int32_t *dCounts, std::vector<LongPair> &pairs)
+                    const Inputs &in, uint32_t flags, int32_t *dCounts, const RowsDev &rows,
+                    std::vector<LongPair> &pairs)
 {
     if (P.idsLong.empty())
         return SQMI_OK;
@@ -699,6 +709,9 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
             a.trace = static_cast<uint2 *>(ctx->longTrace.p);
             a.walk = static_cast<int32_t *>(ctx->longWalk.p);
             a.counts = dCounts;
+            a.ops = rows.ops;
+            a.opOff = rows.opOff;
+            a.opLen = rows.opLen;
             if ((err = launchTileWalkInit(a, stream)) != hipSuccess)
                 return hipFail(ctx, err, "launching long-pair walk init");
             // Each round moves the walk up kRoundBlocks row blocks or left
@@ -775,7 +788,8 @@ sqmi_status sqmi_destroy(sqmi_context *ctx)
     for (DevBuf *b : {&ctx->hRes, &ctx->vRes, &ctx->hOff, &ctx->vOff, &ctx->order, &ctx->items, &ctx->queue,
                       &ctx->rowbuf, &ctx->scores, &ctx->tTable, &ctx->dnaProf, &ctx->ids32, &ctx->scratchOff32,
                       &ctx->scratch32, &ctx->table32, &ctx->emptyIds, &ctx->trace, &ctx->traceOff, &ctx->locs,
-                      &ctx->endEF, &ctx->counts, &ctx->longCk, &ctx->longPairs, &ctx->longWalk, &ctx->longTrace})
+                      &ctx->endEF, &ctx->counts, &ctx->longCk, &ctx->longPairs, &ctx->longWalk, &ctx->longTrace,
+                      &ctx->ops, &ctx->opOff, &ctx->opLen})
         release(*b);
     if (ctx->ev0)
         (void)hipEventDestroy(ctx->ev0);
@@ -999,7 +1013,7 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
             return hipFail(ctx, err, "launching int32 kernel");
     }
     std::vector<LongPair> longPairs;
-    if ((st = runLong(ctx, setH, setV, P, in, flags, nullptr, longPairs)) != SQMI_OK)
+    if ((st = runLong(ctx, setH, setV, P, in, flags, nullptr, RowsDev{}, longPairs)) != SQMI_OK)
         return st;
     if (!emptyIds.empty()) {
         if ((err = ensure(ctx->emptyIds, 4 * emptyIds.size())) != hipSuccess)
@@ -1049,11 +1063,12 @@ sqmi_status sqmi_plan_stats(const sqmi_sequences *setH, const sqmi_sequences *se
 // after each chunk one thread per pair walks its path.  Pairs outside the
 // int16 window run the int32 kernel with one trace byte per cell, chunked by
 // the same budget.
-sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
-                              const sqmi_scoring *scoring, uint32_t mode, int32_t *scores, int32_t *counts,
-                              uint32_t flags)
+static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
+                                 const sqmi_scoring *scoring, uint32_t mode, int32_t *scores, int32_t *counts,
+                                 uint8_t *ops, const int64_t *opOffsets, int64_t *opLengths, uint32_t flags)
 {
     const auto tStart = std::chrono::steady_clock::now();
+    const bool wantRows = ops != nullptr;
     if (!ctx)
         return SQMI_ERR_INVALID_ARGUMENT;
     ctx->lastError.clear();
@@ -1062,7 +1077,7 @@ sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *setH, con
     if ((st = validate(ctx, setH, setV, scoring, mode)) != SQMI_OK)
         return st;
     if (mode != 0)
-        return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_counts: global alignment (mode 0) only");
+        return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_counts / sqmi_align_rows: global alignment (mode 0) only");
     if (flags & (SQMI_PLAIN_PER_PAIR | 0x00FFFF00u))
         return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_counts: no plain form or forced shapes");
     const int64_t N = setH->count;
@@ -1070,6 +1085,17 @@ sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *setH, con
         return SQMI_OK;
     if (!scores || !counts)
         return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "scores or counts is null");
+    if (wantRows) {
+        if (!opOffsets || !opLengths)
+            return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "op_offsets or op_lengths is null");
+        if (opOffsets[0] < 0)
+            return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "op_offsets[0] is negative");
+        for (int64_t i = 0; i < N; ++i) {
+            const int64_t need = (setH->offsets[i + 1] - setH->offsets[i]) + (setV->offsets[i + 1] - setV->offsets[i]);
+            if (opOffsets[i + 1] - opOffsets[i] < need)
+                return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "op segment " + std::to_string(i) + " holds fewer than |H| + |V| bytes");
+        }
+    }
     if (hipSetDevice(ctx->device) != hipSuccess)
         return fail(ctx, SQMI_ERR_DEVICE, "hipSetDevice failed");
     hipStream_t stream = ctx->stream;
@@ -1148,6 +1174,29 @@ sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *setH, con
     }
     if ((err = hipMemsetAsync(dCounts, 0, (size_t)4 * S * N, stream)) != hipSuccess)
         return hipFail(ctx, err, "clearing counts");
+    RowsDev rows;
+    if (wantRows) {
+        // Offsets always come from the host; ops and lengths live where the
+        // scores do.
+        const int64_t opBytes = opOffsets[N];
+        if ((err = ensure(ctx->opOff, 8 * (N + 1))) != hipSuccess)
+            return hipFail(ctx, err, "allocating op offsets");
+        if ((err = hipMemcpyAsync(ctx->opOff.p, opOffsets, 8 * (N + 1), hipMemcpyHostToDevice, stream)) != hipSuccess)
+            return hipFail(ctx, err, "uploading op offsets");
+        rows.opOff = static_cast<const int64_t *>(ctx->opOff.p);
+        if (flags & SQMI_SCORES_ON_DEVICE) {
+            rows.ops = ops;
+            rows.opLen = opLengths;
+        } else {
+            if ((err = ensure(ctx->ops, (size_t)std::max<int64_t>(opBytes, 1))) != hipSuccess ||
+                (err = ensure(ctx->opLen, 8 * N)) != hipSuccess)
+                return hipFail(ctx, err, "allocating rows");
+            rows.ops = static_cast<uint8_t *>(ctx->ops.p);
+            rows.opLen = static_cast<int64_t *>(ctx->opLen.p);
+        }
+        if ((err = hipMemsetAsync(rows.opLen, 0, 8 * N, stream)) != hipSuccess)
+            return hipFail(ctx, err, "clearing op lengths");
+    }
     if ((err = hipEventRecord(ctx->ev0, stream)) != hipSuccess)
         return hipFail(ctx, err, "event");
 
@@ -1224,6 +1273,9 @@ sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *setH, con
             wa.endEF = a.endEF;
             wa.counts = dCounts;
             wa.alpha = A;
+            wa.ops = rows.ops;
+            wa.opOff = rows.opOff;
+            wa.opLen = rows.opLen;
             if ((err = launchWalk16(wa, stream)) != hipSuccess)
                 return hipFail(ctx, err, "launching walk kernel");
             b = e;
@@ -1255,13 +1307,16 @@ sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *setH, con
             a.gapExtend = si.e;
             a.alpha = A;
             a.table = static_cast<const int32_t *>(ctx->table32.p);
+            a.ops = rows.ops;
+            a.opOff = rows.opOff;
+            a.opLen = rows.opLen;
             if ((err = launchCounts32(a, stream)) != hipSuccess)
                 return hipFail(ctx, err, "launching int32 traceback kernel");
             b = e;
         }
     }
     std::vector<LongPair> longPairs;
-    if ((st = runLong(ctx, setH, setV, P, in, flags, dCounts, longPairs)) != SQMI_OK)
+    if ((st = runLong(ctx, setH, setV, P, in, flags, dCounts, rows, longPairs)) != SQMI_OK)
         return st;
     if (!P.emptyIds.empty()) {
         if ((err = ensure(ctx->emptyIds, 4 * P.emptyIds.size())) != hipSuccess)
@@ -1279,6 +1334,11 @@ sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *setH, con
         if ((err = hipMemcpyAsync(scores, in.dScores, 4 * N, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
             (err = hipMemcpyAsync(counts, dCounts, (size_t)4 * S * N, hipMemcpyDeviceToHost, stream)) != hipSuccess)
             return hipFail(ctx, err, "copying results");
+        if (wantRows &&
+            ((err = hipMemcpyAsync(ops + opOffsets[0], rows.ops + opOffsets[0], (size_t)(opOffsets[N] - opOffsets[0]),
+                                   hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+             (err = hipMemcpyAsync(opLengths, rows.opLen, 8 * N, hipMemcpyDeviceToHost, stream)) != hipSuccess))
+            return hipFail(ctx, err, "copying rows");
     }
     if ((err = hipStreamSynchronize(stream)) != hipSuccess)
         return hipFail(ctx, err, "traceback kernels");
@@ -1287,6 +1347,22 @@ sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *setH, con
         ctx->stats.device_ms = ms;
     ctx->stats.total_ms = msSince(tStart);
     return SQMI_OK;
+}
+
+sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
+                              const sqmi_scoring *scoring, uint32_t mode, int32_t *scores, int32_t *counts,
+                              uint32_t flags)
+{
+    return alignBackward(ctx, setH, setV, scoring, mode, scores, counts, nullptr, nullptr, nullptr, flags);
+}
+
+sqmi_status sqmi_align_rows(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
+                            const sqmi_scoring *scoring, uint32_t mode, int32_t *scores, int32_t *counts,
+                            uint8_t *ops, const int64_t *op_offsets, int64_t *op_lengths, uint32_t flags)
+{
+    if (!ops)
+        return ctx ? fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "ops is null") : SQMI_ERR_INVALID_ARGUMENT;
+    return alignBackward(ctx, setH, setV, scoring, mode, scores, counts, ops, op_offsets, op_lengths, flags);
 }
 
 }  // extern "C"

@@ -729,11 +729,20 @@ __device__ __forceinline__ int dirOf(uint32_t bits)   // 0 diagonal, 1 vertical,
 
 template <class Bits>
 __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const uint8_t *v, int A, int score, int endE,
-                                           int endF, Bits &bits, int32_t *cnt)
+                                           int endF, Bits &bits, int32_t *cnt, uint8_t *opEnd = nullptr,
+                                           int64_t *opLen = nullptr)
 {
     int i = m, j = n;
     int prev = 0;   // last emitted operation: 0 none / aligned pair, 1 gap in H (vertical), 2 gap in V
     int opens = 0, exts = 0;
+    // Rows: one byte per alignment column, written backwards from opEnd
+    // ('M' both residues, 'V' gap in the H row, 'H' gap in the V row).
+    int64_t nOps = 0;
+    auto emit = [&](uint8_t op) {
+        if (opEnd)
+            opEnd[-1 - nOps] = op;
+        ++nOps;
+    };
     // End cell: F, then E, win ties against the recorded direction
     // (_correctTraceValue, dp_algorithm_impl.h:1190-1207), and a gap start
     // there follows the gap matrix (PreferGapsAtEnd_, dp_traceback_impl.h:115-122).
@@ -746,6 +755,7 @@ __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const
         if (dir == 0) {
             const int hc = min((int)h[j - 1], A - 1), vc = min((int)v[i - 1], A - 1);
             cnt[hc * A + vc] += 1;
+            emit('M');
             prev = 0;
             --i;
             --j;
@@ -754,6 +764,7 @@ __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const
                 const bool open = (bits(i, j) & kBitFOpen) != 0;
                 if (prev == 1) ++exts; else ++opens;
                 prev = 1;
+                emit('V');
                 const bool stop = open || i == 1;
                 --i;
                 if (stop)
@@ -764,6 +775,7 @@ __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const
                 const bool open = (bits(i, j) & kBitEOpen) != 0;
                 if (prev == 2) ++exts; else ++opens;
                 prev = 2;
+                emit('H');
                 const bool stop = open || j == 1;
                 --j;
                 if (stop)
@@ -778,12 +790,18 @@ __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const
     if (i > 0) {
         if (prev == 1) exts += i; else { ++opens; exts += i - 1; }
         prev = 1;
+        for (int k = 0; k < i; ++k)
+            emit('V');
     }
     if (j > 0) {
         if (prev == 2) exts += j; else { ++opens; exts += j - 1; }
+        for (int k = 0; k < j; ++k)
+            emit('H');
     }
     cnt[A * A] += opens;
     cnt[A * A + 1] += exts;
+    if (opLen)
+        *opLen = nOps;
 }
 
 // Trace bits of cell (i, j) of one pair from the packed records of
@@ -838,7 +856,8 @@ __global__ __launch_bounds__(kWalkThreads) void kWalk16(WalkArgs a)
     for (int q = 0; q < S; ++q)
         cnt[q] = 0;
     Bits16 bits{a.trace, a.locs[t]};
-    walkCounts(n, m, a.hRes + ho, a.vRes + vo, A, a.scores[pid], a.endEF[2 * pid], a.endEF[2 * pid + 1], bits, cnt);
+    walkCounts(n, m, a.hRes + ho, a.vRes + vo, A, a.scores[pid], a.endEF[2 * pid], a.endEF[2 * pid + 1], bits, cnt,
+               a.ops ? a.ops + a.opOff[pid + 1] : nullptr, a.ops ? a.opLen + pid : nullptr);
     if constexpr (LDS)
         for (int q = 0; q < S; ++q)
             out[q] = cnt[q];
@@ -911,7 +930,8 @@ __global__ __launch_bounds__(256) void kCounts32(Counts32Args a)
     const int32_t score = H[m];
     a.scores[pid] = score;
     auto bits = [&](int i, int j) -> uint32_t { return T[(int64_t)j * CS + i]; };
-    walkCounts(n, m, h, v, A, score, E[m], lastF, bits, cnt);
+    walkCounts(n, m, h, v, A, score, E[m], lastF, bits, cnt, a.ops ? a.ops + a.opOff[pid + 1] : nullptr,
+               a.ops ? a.opLen + pid : nullptr);
 }
 
 template <int G, int R, bool DNA>

@@ -81,6 +81,11 @@ struct WalkArgs {
     const int32_t *endEF;       // per pair id: E(m, n), F(m, n)
     int32_t *counts;            // per pair id: alpha^2 + 2 counts
     int32_t alpha;
+    // Rows (optional): pair id p's ops end at ops[opOff[p + 1]] (written
+    // backwards), their number goes to opLen[p].
+    uint8_t *ops;
+    const int64_t *opOff;
+    int64_t *opLen;
 };
 
 // int32 traceback, one thread per pair, trace bytes in global scratch.
@@ -99,6 +104,9 @@ struct Counts32Args {
     int32_t gapExtend;
     int32_t alpha;
     const int32_t *table;
+    uint8_t *ops;               // rows, as in WalkArgs (optional)
+    const int64_t *opOff;
+    int64_t *opLen;
 };
 
 // Simple int32 kernel: one thread per pair, H/E column in global scratch.
@@ -176,6 +184,9 @@ struct TileArgs {
     int32_t *walk;              // per long pair: walk state (kWalkStateInts ints)
     int32_t *counts;            // per pair id: alpha^2 + 2 counts
     int32_t alpha;
+    uint8_t *ops;               // rows, as in WalkArgs (optional)
+    const int64_t *opOff;
+    int64_t *opLen;
 };
 constexpr int kTileSlack = 24;        // steps per tile beyond its width (skew + prefetch)
 constexpr int kWalkStateInts = 8;

@@ -543,7 +543,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // Walk state of one long pair: the cell (i, j) reached, the walk mode
 // (0: choose from the cell's H decision, 1 / 2: inside a vertical /
 // horizontal gap), the last operation, gap opens / extensions, done flag.
-enum { WS_I = 0, WS_J, WS_MODE, WS_PREV, WS_OPENS, WS_EXTS, WS_DONE };
+enum { WS_I = 0, WS_J, WS_MODE, WS_PREV, WS_OPENS, WS_EXTS, WS_DONE, WS_NOPS };
 
 __device__ __forceinline__ TileGeom walkTile(const TileArgs &a, const LongPair &P, const int32_t *ws)
 {
@@ -619,6 +619,7 @@ __global__ void kTileWalkInit(TileArgs a)
     ws[WS_OPENS] = 0;
     ws[WS_EXTS] = 0;
     ws[WS_DONE] = 0;
+    ws[WS_NOPS] = 0;
     const int S = a.alpha * a.alpha + 2;
     for (int k = 0; k < S; ++k)
         a.counts[(int64_t)P.pid * S + k] = 0;
@@ -650,7 +651,7 @@ __global__ __launch_bounds__(kWalkWGThreads) void kTileWalk32(TileArgs a)
     const int tid = threadIdx.x;
     if (tid < 16)
         sCnt[tid] = 0;
-    int i = 0, j = 0, mode = 0, prev = 0, opens = 0, exts = 0;
+    int i = 0, j = 0, mode = 0, prev = 0, opens = 0, exts = 0, nOps = 0;
     if (tid == 0) {
         i = ws[WS_I];
         j = ws[WS_J];
@@ -658,7 +659,15 @@ __global__ __launch_bounds__(kWalkWGThreads) void kTileWalk32(TileArgs a)
         prev = ws[WS_PREV];
         opens = ws[WS_OPENS];
         exts = ws[WS_EXTS];
+        nOps = ws[WS_NOPS];
     }
+    // Rows: written backwards from the end of the pair's op segment.
+    uint8_t *opEnd = a.ops ? a.ops + a.opOff[P.pid + 1] : nullptr;
+    auto emit = [&](uint8_t op) {
+        if (opEnd)
+            opEnd[-1 - (int64_t)nOps] = op;
+        ++nOps;
+    };
     for (;;) {
         if (tid == 0) {
             // The unit (tile, half) holding (i, j), or stop: finished or
@@ -703,6 +712,7 @@ __global__ __launch_bounds__(kWalkWGThreads) void kTileWalk32(TileArgs a)
                 if (d == 0) {
                     const int hc = min((int)sH[j - c0], A - 1), vc = min((int)sV[i - rowTop], A - 1);
                     sCnt[hc * A + vc] += 1;
+                    emit('M');
                     prev = 0;
                     --i;
                     --j;
@@ -714,6 +724,7 @@ __global__ __launch_bounds__(kWalkWGThreads) void kTileWalk32(TileArgs a)
                     const bool open = (bits & (vert ? 2u : 1u)) != 0;
                     if (prev == d) ++exts; else ++opens;
                     prev = d;
+                    emit(vert ? 'V' : 'H');
                     const bool stop = open || (vert ? i == 1 : j == 1);
                     if (vert)
                         --i;
@@ -735,12 +746,18 @@ __global__ __launch_bounds__(kWalkWGThreads) void kTileWalk32(TileArgs a)
             if (i > 0) {
                 if (prev == 1) exts += i; else { ++opens; exts += i - 1; }
                 prev = 1;
+                for (int k = 0; k < i; ++k)
+                    emit('V');
             }
             if (j > 0) {
                 if (prev == 2) exts += j; else { ++opens; exts += j - 1; }
+                for (int k = 0; k < j; ++k)
+                    emit('H');
             }
             cnt[A * A] += opens;
             cnt[A * A + 1] += exts;
+            if (a.ops)
+                a.opLen[P.pid] = nOps;
             ws[WS_DONE] = 1;
         }
         ws[WS_I] = i;
@@ -749,6 +766,7 @@ __global__ __launch_bounds__(kWalkWGThreads) void kTileWalk32(TileArgs a)
         ws[WS_PREV] = prev;
         ws[WS_OPENS] = opens;
         ws[WS_EXTS] = exts;
+        ws[WS_NOPS] = nOps;
     }
 }
 

@@ -7,7 +7,8 @@ holds the reference tree and the oracle/_ref build):
 * ``ref_*.npz``   -- inputs (residue ordinals + offsets) and the outputs of
   the reference library itself (oracle/_ref: SeqAn 2 globalAlignmentScore /
   localAlignmentScore / globalAlignment under ExecutionPolicy<Parallel,
-  Vectorial>, plus computeAlignmentStats on the rows).
+  Vectorial>, plus computeAlignmentStats on the rows, and the rows as one
+  byte per column: M, V (gap in the H row), H (gap in the V row)).
 * ``pair_align.json`` -- sequences and expected scores / rows copied from the
   reference's app tests (apps/pair_align/tests/*.fa, *.stdout, *_out.fa).
 * ``BLOSUM62.txt`` -- the reference's tests/score/BLOSUM62 data file.
@@ -132,14 +133,23 @@ def main():
     rc = po.run_reference(po.OP_COUNTS_SCALAR, H.residues, H.offsets, V.residues, V.offsets, 0, alphabet="dna",
                           **SIMPLE)
     assert np.array_equal(rs["counts"], rc["counts"]) and np.array_equal(rs["scores"], rc["scores"])
-    save("ref_dna1k_counts.npz", H, V, scores=rs["scores"], counts=rs["counts"], seed=0x5EED0004, mode=0)
+    assert np.array_equal(rs["ops"], rc["ops"]) and np.array_equal(rs["op_off"], rc["op_off"])
+    save("ref_dna1k_counts.npz", H, V, scores=rs["scores"], counts=rs["counts"], op_off=rs["op_off"], ops=rs["ops"],
+         seed=0x5EED0004, mode=0)
 
     # --- short random DNA traceback counts with many ties ----------------
     H, V = random_pairs(rng, 300, 1, 60, 4)
     rs = po.run_reference(po.OP_COUNTS_SIMD, H.residues, H.offsets, V.residues, V.offsets, 0, alphabet="dna",
                           match=1, mismatch=-1, gap_extend=-1, gap_open=-2)
-    save("ref_dna_short_counts.npz", H, V, scores=rs["scores"], counts=rs["counts"], match=1, mismatch=-1,
-         gap_extend=-1, gap_open=-2)
+    save("ref_dna_short_counts.npz", H, V, scores=rs["scores"], counts=rs["counts"], op_off=rs["op_off"],
+         ops=rs["ops"], match=1, mismatch=-1, gap_extend=-1, gap_open=-2)
+
+    # --- protein traceback counts and rows, Blosum62(-1, -11) --------------
+    H, V = random_pairs(rng, 200, 20, 400, 24)
+    rs = po.run_reference(po.OP_COUNTS_SIMD, H.residues, H.offsets, V.residues, V.offsets, 0, alphabet="aa",
+                          matrix="blosum62", gap_extend=-1, gap_open=-11)
+    save("ref_protein_counts.npz", H, V, scores=rs["scores"], counts=rs["counts"], op_off=rs["op_off"],
+         ops=rs["ops"], gap_extend=-1, gap_open=-11)
 
 
 def long_pairs():

@@ -1,0 +1,158 @@
+"""Alignment rows (sqmi_align_rows): the rows the reference's globalAlignment
+writes into its Gaps objects (traceback dp_traceback_impl.h:115-546), one
+byte per alignment column: M (residues of H and V), V (a gap in the H row),
+H (a gap in the V row).
+
+Checked against the reference's own rows (fixtures written by the compiled
+reference), the oracle's rows (pinned to those fixtures by the CPU tests),
+and what any alignment's rows must satisfy: they hold every residue of both
+sequences in order, and the counts read from them are the counts
+sqmi_align_counts returns."""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import seqan_amd as sa
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+M, V, H = ord("M"), ord("V"), ord("H")
+
+
+def _load(name):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+def _score_of(z, fixture):
+    if "match" in z:
+        return sa.simple_score(int(z["match"]), int(z["mismatch"]), int(z["gap_extend"]), int(z["gap_open"]))
+    if "protein" in fixture:
+        return sa.blosum62(int(z["gap_extend"]), int(z["gap_open"]))
+    return sa.simple_score(2, -3, -1, -5)
+
+
+def _counts_from_rows(Hs, Vs, op_off, ops, A):
+    """Counts read from the rows the way computeAlignmentStats reads them
+    (evaluate_alignment.h:215-300), plus a check that the rows hold both
+    sequences."""
+    out = np.zeros((len(Hs), A * A + 2), np.int64)
+    for k in range(len(Hs)):
+        row = ops[op_off[k]:op_off[k + 1]]
+        h, v = Hs[k], Vs[k]
+        assert (row == M).sum() + (row == H).sum() == len(h)
+        assert (row == M).sum() + (row == V).sum() == len(v)
+        hi = vi = 0
+        prev = 0
+        for op in row:
+            if op == M:
+                out[k, int(h[hi]) * A + int(v[vi])] += 1
+                hi += 1
+                vi += 1
+            else:
+                out[k, A * A + (1 if prev == op else 0)] += 1
+                if op == V:
+                    vi += 1
+                else:
+                    hi += 1
+            prev = op if op != M else 0
+    return out
+
+
+def _oracle_rows(Hs, Vs, score):
+    return po.rows_batch(Hs.residues, Hs.offsets, Vs.residues, Vs.offsets,
+                         po.Scoring(score.table, score.gap_open, score.gap_extend), 0)
+
+
+@pytest.mark.parametrize("fixture", ["ref_dna1k_counts.npz", "ref_dna_short_counts.npz", "ref_protein_counts.npz"])
+@pytest.mark.parametrize("flags", [0, sa.FORCE_INT32], ids=["packed", "int32"])
+def test_rows_match_reference_fixtures(ctx, fixture, flags):
+    z = _load(fixture)
+    score = _score_of(z, fixture)
+    Hs, Vs = sa.SeqSet(z["h_res"], z["h_off"]), sa.SeqSet(z["v_res"], z["v_off"])
+    print("rows fixture", fixture, "flags", flags, flush=True)
+    s, c, op_off, ops = ctx.align_rows(Hs, Vs, score, flags=flags)
+    assert np.array_equal(s, z["scores"])
+    assert np.array_equal(c, z["counts"])
+    assert np.array_equal(op_off, z["op_off"])
+    assert np.array_equal(ops, z["ops"])
+
+
+def test_rows_match_oracle_and_counts(ctx):
+    rng = np.random.default_rng(21)
+    hs, vs = [], []
+    for _ in range(300):
+        n = int(rng.integers(1, 900))
+        h = rng.integers(0, 4, n).astype(np.uint8)
+        v = h.copy()
+        flip = rng.random(n) < 0.15
+        v[flip] = rng.integers(0, 4, int(flip.sum()))
+        cut = np.nonzero(rng.random(n) < 0.03)[0]
+        v = np.delete(v, cut)
+        if rng.random() < 0.2:
+            v = rng.integers(0, 4, int(rng.integers(1, 900))).astype(np.uint8)
+        hs.append(h)
+        vs.append(v)
+    Hs, Vs = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+    for score in (sa.simple_score(2, -3, -1, -5), sa.simple_score(1, -1, -1, -2), sa.simple_score(3, -2, -3, -2)):
+        print("rows vs oracle", score.table[0, :2], score.gap_extend, score.gap_open, flush=True)
+        s, c, op_off, ops = ctx.align_rows(Hs, Vs, score)
+        ws, wc, woff, wops = _oracle_rows(Hs, Vs, score)
+        assert np.array_equal(s, ws) and np.array_equal(c, wc)
+        assert np.array_equal(op_off, woff) and np.array_equal(ops, wops)
+        s2, c2 = ctx.align_counts(Hs, Vs, score)
+        assert np.array_equal(s2, s) and np.array_equal(c2, c)
+        assert np.array_equal(_counts_from_rows(hs, vs, op_off, ops, 4), c)
+
+
+@pytest.mark.parametrize("route", ["int32", "packed"])
+def test_rows_long_pairs(ctx, monkeypatch, route):
+    # The long-pair route (threshold lowered to 256) walks windows of
+    # recomputed tiles; its rows are written across traceback rounds.
+    monkeypatch.setenv("SQMI_LONG_MIN_SIDE", "256")
+    flags = sa.FORCE_INT32
+    if route == "packed":
+        monkeypatch.setenv("SQMI_LONG_ALL", "1")
+        flags = 0
+    rng = np.random.default_rng(22)
+    hs, vs = [], []
+    for _ in range(20):
+        n = int(rng.integers(256, 2500))
+        h = rng.integers(0, 4, n).astype(np.uint8)
+        v = np.delete(h, np.nonzero(rng.random(n) < 0.02)[0])
+        flip = rng.random(len(v)) < 0.1
+        v[flip] = rng.integers(0, 4, int(flip.sum()))
+        hs.append(h)
+        vs.append(v if len(v) >= 256 else h.copy())
+    Hs, Vs = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+    score = sa.simple_score(2, -3, -1, -5)
+    s, c, op_off, ops = ctx.align_rows(Hs, Vs, score, flags=flags)
+    assert ctx.stats()["pairs_long"] == len(hs)
+    ws, wc, woff, wops = _oracle_rows(Hs, Vs, score)
+    assert np.array_equal(s, ws) and np.array_equal(c, wc)
+    assert np.array_equal(op_off, woff) and np.array_equal(ops, wops)
+
+
+def test_rows_100kbp_fixture(ctx):
+    # Two 100 kbp pairs: the rows give the fixture's counts (reference scores,
+    # oracle counts) and hold both sequences.
+    z = _load("ref_dna100k_counts.npz")
+    Hs, Vs = sa.SeqSet(z["h_res"], z["h_off"]), sa.SeqSet(z["v_res"], z["v_off"])
+    score = sa.simple_score(2, -3, -1, -5)
+    print("rows 100 kbp fixture", flush=True)
+    s, c, op_off, ops = ctx.align_rows(Hs, Vs, score)
+    assert np.array_equal(s, z["scores"]) and np.array_equal(c, z["counts"])
+    assert np.array_equal(_counts_from_rows([Hs[0], Hs[1]], [Vs[0], Vs[1]], op_off, ops, 4), z["counts"])
+
+
+def test_rows_empty_pairs_and_errors(ctx):
+    Hs = sa.SeqSet.from_list([[0, 1, 2], [], [3, 3], [1]])
+    Vs = sa.SeqSet.from_list([[0, 2], [1, 1], [], [1]])
+    score = sa.simple_score(2, -3, -1, -5)
+    s, c, op_off, ops = ctx.align_rows(Hs, Vs, score)
+    assert s[1] == np.iinfo(np.int32).min and s[2] == np.iinfo(np.int32).min
+    assert list(np.diff(op_off)) == [3, 0, 0, 1]
+    assert bytes(ops[op_off[3]:op_off[4]]) == b"M"
+    with pytest.raises(sa.SqmiError):
+        ctx.align_rows(Hs, Vs, score, mode=sa.FREE_TOP)

@@ -200,6 +200,24 @@ def test_oracle_traceback_counts_match_reference(fixture):
     assert np.array_equal(c.astype(np.int64) @ par, s.astype(np.int64))
 
 
+@pytest.mark.parametrize("fixture", ["ref_dna1k_counts.npz", "ref_dna_short_counts.npz", "ref_protein_counts.npz"])
+def test_oracle_rows_match_reference(fixture):
+    # The rows the reference's globalAlignment writes into its Gaps objects,
+    # one byte per column (M, V: gap in the H row, H: gap in the V row).
+    z = _load(fixture)
+    if "match" in z:
+        sc = po.Scoring.simple(int(z["match"]), int(z["mismatch"]), int(z["gap_extend"]), int(z["gap_open"]))
+    elif "ref_protein" in fixture:
+        sc = po.Scoring(_blosum62_fixture(), int(z["gap_open"]), int(z["gap_extend"]))
+    else:
+        sc = po.Scoring.simple(2, -3, -1, -5)
+    s, c, op_off, ops = po.rows_batch(z["h_res"], z["h_off"], z["v_res"], z["v_off"], sc, 0)
+    assert np.array_equal(s, z["scores"])
+    assert np.array_equal(c, z["counts"])
+    assert np.array_equal(op_off, z["op_off"])
+    assert np.array_equal(ops, z["ops"])
+
+
 def test_100kbp_fixture_is_a_valid_alignment():
     # Two 100 kbp pairs: reference scores, oracle counts (the reference's own
     # traceback truncates trace-matrix positions past 2^32 at this size).
