@@ -5,10 +5,12 @@
  *
  *   globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial>, setH, setV, score [, AlignConfig<T,L,R,B>])
  *   localAlignmentScore(ExecutionPolicy<Parallel, Vectorial>, setH, setV, score)
+ *   globalAlignment(ExecutionPolicy<Parallel, Vectorial>, gapsSetH, gapsSetV, score)
  *
  * to the MI355X library (libseqan_amd.so, C ABI in seqan_amd.h) whenever
  *   - setH and setV are StringSets over one of Dna, Dna5, Rna, Rna5, AminoAcid,
- *     both over the same alphabet, and
+ *     both over the same alphabet (for globalAlignment: StringSets of Gaps
+ *     over such sequences), and
  *   - score is Score<int, Simple> or Score<int, ScoreMatrix<A, Spec>> with A
  *     the sets' alphabet (Blosum62 and the other built-in matrices included).
  * Every other combination (another policy, another alphabet, a non-int score,
@@ -20,6 +22,11 @@
  *       include/seqan/align_parallel/parallel_align_interface.h:332-343
  *   localAlignmentScore(ExecutionPolicy<TPar,TVec> const &, TArgs && ...)
  *       include/seqan/align_parallel/parallel_align_interface.h:354-365
+ *   globalAlignment(ExecutionPolicy<TPar,TVec> const &, TArgs && ...)
+ *       include/seqan/align_parallel/parallel_align_interface.h:288-299
+ * globalAlignment writes the rows of the reference's traceback into the Gaps
+ * objects (the way the reference does: clear gaps and clipping, then insert
+ * the gap runs through Gaps iterators) and returns the scores.
  * Both take forwarding references; so do the overloads below, which makes the
  * conversion sequences of the two candidates identical for const, non-const
  * and rvalue arguments alike.  Partial ordering then prefers these overloads
@@ -133,6 +140,23 @@ struct IsAlignConfig<seqan2::AlignConfig<TOP, LEFT, RIGHT, BOTTOM, TSpec>> : std
                                      (RIGHT ? SQMI_FREE_RIGHT : 0u) | (BOTTOM ? SQMI_FREE_BOTTOM : 0u);
 };
 
+/* Alphabet of a StringSet<Gaps<TSequence, ...>, ...>; void for anything else. */
+template <typename T> struct GapsSetAlphabet { using Type = void; };
+template <typename TSequence, typename TGapsSpec, typename TSetSpec>
+struct GapsSetAlphabet<seqan2::StringSet<seqan2::Gaps<TSequence, TGapsSpec>, TSetSpec>>
+{
+    using Type = typename seqan2::Value<TSequence>::Type;
+};
+
+template <typename TGapsSetH, typename TGapsSetV, typename TScore>
+struct IsGpuRowsCall
+{
+    using AlphaH = typename GapsSetAlphabet<std::decay_t<TGapsSetH>>::Type;
+    using AlphaV = typename GapsSetAlphabet<std::decay_t<TGapsSetV>>::Type;
+    static constexpr bool value = std::is_same<AlphaH, AlphaV>::value && IsGpuAlphabet<AlphaH>::value &&
+                                  IsGpuScore<std::decay_t<TScore>, AlphaH>::value;
+};
+
 template <typename TSetH, typename TSetV, typename TScore>
 struct IsGpuCall
 {
@@ -142,23 +166,23 @@ struct IsGpuCall
                                   IsGpuScore<std::decay_t<TScore>, AlphaH>::value;
 };
 
-/* Concatenated residue ordinals and count+1 offsets of a StringSet. */
-template <typename TSet>
-void pack(TSet const & set, std::vector<uint8_t> & residues, std::vector<int64_t> & offsets)
+/* Concatenated residue ordinals and count+1 offsets of the sequences
+ * get(0) ... get(count - 1) (get may return a reference or, for sets that
+ * hand out infixes, a value). */
+template <typename TAlphabet, typename TGet>
+void packWith(int64_t count, TGet && get, std::vector<uint8_t> & residues, std::vector<int64_t> & offsets)
 {
-    using TAlphabet = typename SetAlphabet<TSet>::Type;
-    int64_t const count = static_cast<int64_t>(seqan2::length(set));
     offsets.resize(count + 1);
     offsets[0] = 0;
     for (int64_t i = 0; i < count; ++i)
-        offsets[i + 1] = offsets[i] + static_cast<int64_t>(seqan2::length(set[i]));
+        offsets[i + 1] = offsets[i] + static_cast<int64_t>(seqan2::length(get(i)));
     residues.resize(static_cast<size_t>(offsets[count]));
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static)
 #endif
     for (int64_t i = 0; i < count; ++i)
     {
-        auto const & seq = set[i];
+        auto const & seq = get(i);
         uint8_t * out = residues.data() + offsets[i];
         int64_t const len = offsets[i + 1] - offsets[i];
         using TIter = typename seqan2::Iterator<std::decay_t<decltype(seq)> const, seqan2::Standard>::Type;
@@ -172,6 +196,52 @@ void pack(TSet const & set, std::vector<uint8_t> & residues, std::vector<int64_t
             for (int64_t k = 0; k < len; ++k)
                 out[k] = static_cast<uint8_t>(seqan2::ordValue(static_cast<TAlphabet>(seq[k])));
         }
+    }
+}
+
+/* A StringSet's sequences. */
+template <typename TSet>
+void pack(TSet const & set, std::vector<uint8_t> & residues, std::vector<int64_t> & offsets)
+{
+    packWith<typename SetAlphabet<TSet>::Type>(
+        static_cast<int64_t>(seqan2::length(set)), [&](int64_t i) -> decltype(auto) { return set[i]; }, residues,
+        offsets);
+}
+
+/* The source sequences of a StringSet of Gaps. */
+template <typename TGapsSet>
+void packSources(TGapsSet & set, std::vector<uint8_t> & residues, std::vector<int64_t> & offsets)
+{
+    packWith<typename GapsSetAlphabet<TGapsSet>::Type>(
+        static_cast<int64_t>(seqan2::length(set)), [&](int64_t i) -> decltype(auto) { return seqan2::source(set[i]); },
+        residues, offsets);
+}
+
+/* Rows of one alignment into its two Gaps objects: clear gaps and clipping,
+ * then walk both rows with Gaps iterators, inserting each gap run ('V': a
+ * run in the H row, 'H': in the V row). */
+template <typename TGapsH, typename TGapsV>
+void write_rows(TGapsH & gapsH, TGapsV & gapsV, uint8_t const * row, int64_t len)
+{
+    seqan2::clearGaps(gapsH);
+    seqan2::clearClipping(gapsH);
+    seqan2::clearGaps(gapsV);
+    seqan2::clearClipping(gapsV);
+    auto itH = seqan2::begin(gapsH, seqan2::Standard());
+    auto itV = seqan2::begin(gapsV, seqan2::Standard());
+    for (int64_t k = 0; k < len;)
+    {
+        uint8_t const op = row[k];
+        int64_t run = 1;
+        while (k + run < len && row[k + run] == op)
+            ++run;
+        if (op == 'V')
+            seqan2::insertGaps(itH, run);
+        else if (op == 'H')
+            seqan2::insertGaps(itV, run);
+        seqan2::goFurther(itH, run);
+        seqan2::goFurther(itV, run);
+        k += run;
     }
 }
 
@@ -216,6 +286,42 @@ seqan2::String<int> align_scores(TSetH const & setH, TSetV const & setV, TScore 
     return result;
 }
 
+template <typename TGapsSetH, typename TGapsSetV, typename TScore>
+seqan2::String<int> align_rows(TGapsSetH & gapsH, TGapsSetV & gapsV, TScore const & scheme)
+{
+    using TAlphabet = typename GapsSetAlphabet<TGapsSetH>::Type;
+    if (seqan2::length(gapsH) != seqan2::length(gapsV))
+        throw Error(SQMI_ERR_LENGTH_MISMATCH, "seqan_amd: gapsSetH and gapsSetV differ in length");
+    seqan2::String<int> result;
+    int64_t const count = static_cast<int64_t>(seqan2::length(gapsH));
+    if (count == 0)
+        return result;
+    std::vector<uint8_t> resH, resV;
+    std::vector<int64_t> offH, offV;
+    packSources(gapsH, resH, offH);
+    packSources(gapsV, resV, offV);
+    std::vector<int32_t> table;
+    sqmi_scoring const sc = make_scoring<TAlphabet>(scheme, table);
+    sqmi_sequences const h{resH.data(), offH.data(), count};
+    sqmi_sequences const v{resV.data(), offV.data(), count};
+    std::vector<int64_t> opOff(count + 1, 0), opLen(count, 0);
+    for (int64_t i = 0; i < count; ++i)
+        opOff[i + 1] = opOff[i] + (offH[i + 1] - offH[i]) + (offV[i + 1] - offV[i]);
+    std::vector<uint8_t> ops(static_cast<size_t>(opOff[count]) + 1);
+    std::vector<int32_t> counts(static_cast<size_t>(count) * (sc.alphabet_size * sc.alphabet_size + 2));
+    seqan2::resize(result, count);
+    sqmi_context * ctx = thread_context();
+    check(sqmi_align_rows(ctx, &h, &v, &sc, 0u, &result[0], counts.data(), ops.data(), opOff.data(), opLen.data(),
+                          0u),
+          ctx);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (int64_t i = 0; i < count; ++i)
+        write_rows(gapsH[i], gapsV[i], ops.data() + opOff[i + 1] - opLen[i], opLen[i]);
+    return result;
+}
+
 }  // namespace detail
 }  // namespace seqan_amd
 
@@ -249,6 +355,13 @@ inline String<int>
 localAlignmentScore(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
                     TSetH && setH, TSetV && setV, TScore && scoringScheme)
 SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores(setH, setV, scoringScheme, static_cast<uint32_t>(SQMI_LOCAL)))
+
+template <typename TGapsSetH, typename TGapsSetV, typename TScore,
+          std::enable_if_t<seqan_amd::detail::IsGpuRowsCall<TGapsSetH, TGapsSetV, TScore>::value, int> = 0>
+inline String<int>
+globalAlignment(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
+                TGapsSetH && gapsSetH, TGapsSetV && gapsSetV, TScore && scoringScheme)
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_rows(gapsSetH, gapsSetV, scoringScheme))
 
 }  // namespace seqan2
 #undef SEQAN_AMD_OVERLOAD_BODY

@@ -25,7 +25,12 @@ template <typename, typename... A> struct LocalCallable : std::false_type {};
 template <typename... A>
 struct LocalCallable<void_t<decltype(localAlignmentScore(std::declval<A>()...))>, A...> : std::true_type {};
 
+template <typename, typename... A> struct RowsCallable : std::false_type {};
+template <typename... A>
+struct RowsCallable<void_t<decltype(globalAlignment(std::declval<A>()...))>, A...> : std::true_type {};
+
 template <typename... A> constexpr bool gpuGlobal() { return !GlobalCallable<void, A...>::value; }
+template <typename... A> constexpr bool gpuRows() { return !RowsCallable<void, A...>::value; }
 template <typename... A> constexpr bool gpuLocal() { return !LocalCallable<void, A...>::value; }
 
 using ExPV = ExecutionPolicy<Parallel, Vectorial>;
@@ -38,6 +43,8 @@ using AaSet = StringSet<String<AminoAcid>>;
 using DnaConcat = StringSet<String<Dna>, Owner<ConcatDirect<>>>;
 using CharSet = StringSet<String<char>>;
 using Simple = Score<int, seqan2::Simple>;
+using DnaGaps = StringSet<Gaps<String<Dna>>>;
+using AaGaps = StringSet<Gaps<String<AminoAcid>>>;
 using Semi = AlignConfig<true, false, false, true>;
 
 // Non-const lvalues, const lvalues, rvalues and mixtures all reach the library.
@@ -61,7 +68,14 @@ static_assert(gpuLocal<ExPV const &, AaSet const &, AaSet const &, Blosum62 cons
 static_assert(gpuLocal<ExPV, AaSet, AaSet, Blosum62>(), "local, rvalues");
 static_assert(gpuLocal<ExPV, DnaSet &, DnaSet &, Simple &>(), "local DNA");
 
+// globalAlignment on StringSets of Gaps (parallel_align_interface.h:288-299).
+static_assert(gpuRows<ExPV const &, DnaGaps &, DnaGaps &, Simple &>(), "rows, DNA");
+static_assert(gpuRows<ExPV, DnaGaps &, DnaGaps &, Simple const &>(), "rows, const score");
+static_assert(gpuRows<ExPV const &, AaGaps &, AaGaps &, Blosum62 &>(), "rows, protein");
+
 // Everything else stays with the reference templates.
+static_assert(!gpuRows<ExPS const &, DnaGaps &, DnaGaps &, Simple &>(), "rows, Parallel, Serial");
+static_assert(!gpuRows<ExPV const &, DnaGaps &, DnaGaps &, Simple &, Semi &>(), "rows with an AlignConfig");
 static_assert(!gpuGlobal<ExPS const &, DnaSet &, DnaSet &, Simple &>(), "Parallel, Serial");
 static_assert(!gpuGlobal<ExSV const &, DnaSet &, DnaSet &, Simple &>(), "Serial, Vectorial");
 static_assert(!gpuGlobal<ExPV const &, CharSet &, CharSet &, Simple &>(), "char alphabet");

@@ -79,6 +79,51 @@ void runCase(char const * name, TFn && gpu, TRef && ref)
     }
 }
 
+// One row of an alignment as text: '-' for a gap, 'x' for a residue.
+template <typename TGaps>
+std::string rowText(TGaps & gaps)
+{
+    std::string out;
+    for (auto it = begin(gaps); it != end(gaps); ++it)
+        out += isGap(it) ? '-' : 'x';
+    return out;
+}
+
+// globalAlignment on StringSets of Gaps: scores and both rows of every pair
+// against the reference's own globalAlignment (non-empty pairs only).
+template <typename TAlphabet, typename TScore, typename TGpu, typename TCpu>
+void runRowsCase(char const * name, StringSet<String<TAlphabet>> const & setH,
+                 StringSet<String<TAlphabet>> const & setV, TScore const & sc, TGpu const & ex, TCpu const & cpu)
+{
+    using TGaps = Gaps<String<TAlphabet> const>;
+    StringSet<TGaps> gH, gV, rH, rV;
+    for (size_t i = 0; i < length(setH); ++i)
+    {
+        if (length(setH[i]) == 0 || length(setV[i]) == 0)
+            continue;
+        appendValue(gH, TGaps(setH[i]));
+        appendValue(gV, TGaps(setV[i]));
+        appendValue(rH, TGaps(setH[i]));
+        appendValue(rV, TGaps(setV[i]));
+    }
+    String<int> expect = globalAlignment(cpu, rH, rV, sc);
+    try
+    {
+        String<int> got = globalAlignment(ex, gH, gV, sc);
+        for (size_t i = 0; i < length(expect); ++i)
+            if (got[i] != expect[i] || rowText(gH[i]) != rowText(rH[i]) || rowText(gV[i]) != rowText(rV[i]))
+            {
+                std::printf("case %s mismatch pair %zu: %d vs %d\n", name, i, got[i], expect[i]);
+                return;
+            }
+        std::printf("case %s match %zu\n", name, (size_t)length(got));
+    }
+    catch (seqan_amd::Error const & e)
+    {
+        std::printf("case %s %s (%s)\n", name, e.status() == SQMI_ERR_NO_DEVICE ? "nodevice" : "error", e.what());
+    }
+}
+
 }  // namespace
 
 int main()
@@ -130,6 +175,9 @@ int main()
             [&] { return globalAlignmentScore(cpu, ph, pv, b62); });
     runCase("protein_semiglobal_blosum62", [&] { return globalAlignmentScore(ex, ph, pv, b62, semi); },
             [&] { return globalAlignmentScore(cpu, ph, pv, b62, semi); });
+
+    runRowsCase("dna_rows", dh, dv, dna, ex, cpu);
+    runRowsCase("protein_rows_blosum62", ph, pv, b62, ex, cpu);
 
     // |setH| != |setV| raises instead of asserting.
     StringSet<String<Dna>> shorter = dv;

@@ -1,4 +1,5 @@
-"""The SeqAn 2 drop-in header include/seqan_amd_seqan2.hpp.
+"""The SeqAn 2 drop-in header include/seqan_amd_seqan2.hpp (scores and, for
+globalAlignment on StringSets of Gaps, the rows).
 
 CPU: its overloads win overload resolution against the reference templates
 (parallel_align_interface.h:332-365) for const, non-const and rvalue
@@ -20,7 +21,7 @@ RUNNER = os.path.join(SHIM, "_build", "shim_run")
 REF_HEADERS = "/root/reference/include/seqan/align.h"
 CASES = ["dna_global_nonconst", "dna_semiglobal_const", "dna_overlap_rvalue", "dna_local", "dna_concat_direct",
          "dna5_linear", "protein_local_blosum62", "protein_global_blosum62", "protein_semiglobal_blosum62",
-         "length_mismatch"]
+         "dna_rows", "protein_rows_blosum62", "length_mismatch"]
 
 
 def _run_cases():
