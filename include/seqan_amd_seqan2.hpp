@@ -6,6 +6,8 @@
  *   globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial>, setH, setV, score [, AlignConfig<T,L,R,B>])
  *   localAlignmentScore(ExecutionPolicy<Parallel, Vectorial>, setH, setV, score)
  *   globalAlignment(ExecutionPolicy<Parallel, Vectorial>, gapsSetH, gapsSetV, score)
+ *   globalAlignmentScore(ExecutionPolicy<WavefrontAlignment<S>, V>, setH, setV, score [, AlignConfig])
+ *   localAlignmentScore(ExecutionPolicy<WavefrontAlignment<S>, V>, setH, setV, score)
  *
  * to the MI355X library (libseqan_amd.so, C ABI in seqan_amd.h) whenever
  *   - setH and setV are StringSets over one of Dna, Dna5, Rna, Rna5, AminoAcid,
@@ -27,6 +29,10 @@
  * globalAlignment writes the rows of the reference's traceback into the Gaps
  * objects (the way the reference does: clear gaps and clipping, then insert
  * the gap runs through Gaps iterators) and returns the scores.
+ * The wavefront policy (intra-pair parallelism for long sequences,
+ * parallel_align_interface.h:367-420) returns std::vector<int> like the
+ * reference; its block size and parallel-alignment settings are not used:
+ * long pairs take the library's tiled long-pair kernels by themselves.
  * Both take forwarding references; so do the overloads below, which makes the
  * conversion sequences of the two candidates identical for const, non-const
  * and rvalue arguments alike.  Partial ordering then prefers these overloads
@@ -286,6 +292,13 @@ seqan2::String<int> align_scores(TSetH const & setH, TSetV const & setV, TScore 
     return result;
 }
 
+template <typename TSetH, typename TSetV, typename TScore>
+std::vector<int> align_scores_vector(TSetH const & setH, TSetV const & setV, TScore const & scheme, uint32_t mode)
+{
+    seqan2::String<int> s = align_scores(setH, setV, scheme, mode);
+    return std::vector<int>(seqan2::begin(s, seqan2::Standard()), seqan2::end(s, seqan2::Standard()));
+}
+
 template <typename TGapsSetH, typename TGapsSetV, typename TScore>
 seqan2::String<int> align_rows(TGapsSetH & gapsH, TGapsSetV & gapsV, TScore const & scheme)
 {
@@ -355,6 +368,41 @@ inline String<int>
 localAlignmentScore(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
                     TSetH && setH, TSetV && setV, TScore && scoringScheme)
 SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores(setH, setV, scoringScheme, static_cast<uint32_t>(SQMI_LOCAL)))
+
+// The wavefront policy.  The reference's global overloads take the sets and
+// the score by const reference; these take StringSets, which partial
+// ordering prefers.
+template <typename TWaveSpec, typename TVec, typename TStrH, typename TSpecH, typename TStrV, typename TSpecV,
+          typename TScore,
+          std::enable_if_t<seqan_amd::detail::IsGpuCall<StringSet<TStrH, TSpecH>, StringSet<TStrV, TSpecV>,
+                                                        TScore>::value, int> = 0>
+inline std::vector<int>
+globalAlignmentScore(ExecutionPolicy<WavefrontAlignment<TWaveSpec>, TVec> const & /*execPolicy*/,
+                     StringSet<TStrH, TSpecH> const & setH, StringSet<TStrV, TSpecV> const & setV,
+                     TScore const & scoringScheme)
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores_vector(setH, setV, scoringScheme, 0u))
+
+template <typename TWaveSpec, typename TVec, typename TStrH, typename TSpecH, typename TStrV, typename TSpecV,
+          typename TScore, typename TConfig,
+          std::enable_if_t<seqan_amd::detail::IsGpuCall<StringSet<TStrH, TSpecH>, StringSet<TStrV, TSpecV>,
+                                                        TScore>::value &&
+                           seqan_amd::detail::IsAlignConfig<TConfig>::value, int> = 0>
+inline std::vector<int>
+globalAlignmentScore(ExecutionPolicy<WavefrontAlignment<TWaveSpec>, TVec> const & /*execPolicy*/,
+                     StringSet<TStrH, TSpecH> const & setH, StringSet<TStrV, TSpecV> const & setV,
+                     TScore const & scoringScheme, TConfig const & /*alignConfig*/)
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores_vector(setH, setV, scoringScheme,
+                                                               seqan_amd::detail::IsAlignConfig<TConfig>::mode))
+
+// The reference's local wavefront overload takes forwarding references, so
+// this one does too (and wins by its first parameter being non-variadic).
+template <typename TWaveSpec, typename TVec, typename TSetH, typename TSetV, typename TScore,
+          std::enable_if_t<seqan_amd::detail::IsGpuCall<TSetH, TSetV, TScore>::value, int> = 0>
+inline std::vector<int>
+localAlignmentScore(ExecutionPolicy<WavefrontAlignment<TWaveSpec>, TVec> const & /*execPolicy*/,
+                    TSetH && setH, TSetV && setV, TScore && scoringScheme)
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores_vector(setH, setV, scoringScheme,
+                                                               static_cast<uint32_t>(SQMI_LOCAL)))
 
 template <typename TGapsSetH, typename TGapsSetV, typename TScore,
           std::enable_if_t<seqan_amd::detail::IsGpuRowsCall<TGapsSetH, TGapsSetV, TScore>::value, int> = 0>

@@ -68,6 +68,22 @@ static_assert(gpuLocal<ExPV const &, AaSet const &, AaSet const &, Blosum62 cons
 static_assert(gpuLocal<ExPV, AaSet, AaSet, Blosum62>(), "local, rvalues");
 static_assert(gpuLocal<ExPV, DnaSet &, DnaSet &, Simple &>(), "local DNA");
 
+// The wavefront policy (parallel_align_interface.h:367-420).
+using ExWave = ExecutionPolicy<WavefrontAlignment<BlockOffsetOptimization>, Vectorial>;
+using ExWaveS = ExecutionPolicy<WavefrontAlignment<>, Serial>;
+static_assert(gpuGlobal<ExWave const &, DnaSet const &, DnaSet const &, Simple const &>(), "wavefront, const");
+static_assert(gpuGlobal<ExWave &, DnaSet &, DnaSet &, Simple &>(), "wavefront, non-const");
+static_assert(gpuGlobal<ExWave const &, DnaSet &, DnaSet &, Simple &, Semi &>(), "wavefront + config");
+static_assert(gpuGlobal<ExWaveS const &, AaSet &, AaSet &, Blosum62 &>(), "wavefront, serial vectorisation");
+static_assert(gpuLocal<ExWave const &, AaSet &, AaSet &, Blosum62 &>(), "wavefront local");
+static_assert(std::is_same<decltype(globalAlignmentScore(std::declval<ExPS const &>(), std::declval<DnaSet &>(),
+                                                         std::declval<DnaSet &>(), std::declval<Simple &>())),
+                           String<int>>::value, "reference return type (Parallel)");
+
+// Runtime-selected and other built-in matrices.
+static_assert(gpuGlobal<ExPV const &, AaSet &, AaSet &, Pam120 &>(), "PAM120");
+static_assert(gpuLocal<ExPV const &, AaSet &, AaSet &, SelectableAminoAcidMatrix &>(), "selectable matrix");
+
 // globalAlignment on StringSets of Gaps (parallel_align_interface.h:288-299).
 static_assert(gpuRows<ExPV const &, DnaGaps &, DnaGaps &, Simple &>(), "rows, DNA");
 static_assert(gpuRows<ExPV, DnaGaps &, DnaGaps &, Simple const &>(), "rows, const score");
@@ -82,6 +98,7 @@ static_assert(!gpuGlobal<ExPV const &, CharSet &, CharSet &, Simple &>(), "char 
 static_assert(!gpuGlobal<ExPV const &, DnaSet &, DnaSet &, Simple &, Semi &, Gotoh &>(), "algorithm tag");
 static_assert(!gpuLocal<ExPS const &, AaSet &, AaSet &, Blosum62 &>(), "local, Parallel, Serial");
 static_assert(!gpuLocal<ExPV const &, CharSet &, CharSet &, Simple &>(), "local, char alphabet");
+static_assert(!gpuGlobal<ExWave const &, CharSet &, CharSet &, Simple &>(), "wavefront, char alphabet");
 
 // The traits the overloads are constrained on.
 static_assert(!seqan_amd::detail::IsGpuCall<DnaSet, Dna5Set, Simple>::value, "alphabets must agree");

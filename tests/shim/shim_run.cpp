@@ -176,6 +176,28 @@ int main()
     runCase("protein_semiglobal_blosum62", [&] { return globalAlignmentScore(ex, ph, pv, b62, semi); },
             [&] { return globalAlignmentScore(cpu, ph, pv, b62, semi); });
 
+    // Other matrices: PAM120 and one selected at run time.
+    Pam120 pam(-2, -10);
+    runCase("protein_global_pam120", [&] { return globalAlignmentScore(ex, ph, pv, pam); },
+            [&] { return globalAlignmentScore(cpu, ph, pv, pam); });
+    SelectableAminoAcidMatrix sel(-1, -8);
+    setScoreMatrixById(sel, AminoAcidScoreMatrixID::BLOSUM80);
+    runCase("protein_local_selectable_blosum80", [&] { return localAlignmentScore(ex, ph, pv, sel); },
+            [&] { return localAlignmentScore(cpu, ph, pv, sel); });
+
+    // The wavefront policy returns std::vector<int>.
+    ExecutionPolicy<WavefrontAlignment<BlockOffsetOptimization>, Vectorial> wave;
+    auto asString = [](std::vector<int> const & v) {
+        String<int> s;
+        for (int x : v)
+            appendValue(s, x);
+        return s;
+    };
+    runCase("wavefront_global_semi", [&] { return asString(globalAlignmentScore(wave, dh, dv, dna, semi)); },
+            [&] { return globalAlignmentScore(cpu, dh, dv, dna, semi); });
+    runCase("wavefront_local_blosum62", [&] { return asString(localAlignmentScore(wave, ph, pv, b62)); },
+            [&] { return localAlignmentScore(cpu, ph, pv, b62); });
+
     runRowsCase("dna_rows", dh, dv, dna, ex, cpu);
     runRowsCase("protein_rows_blosum62", ph, pv, b62, ex, cpu);
 
