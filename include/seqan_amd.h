@@ -130,7 +130,10 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *set_h,
 /* Score plus the exact subgradient of the score with respect to the scoring
  * parameters: substitution counts, gap opens and gap extensions along the
  * optimal path chosen by the reference traceback (single trace, gaps left,
- * gaps preferred at the end cell).  Global alignment (mode 0) only. */
+ * gaps preferred at the end cell), counted on the rows like
+ * computeAlignmentStats (free end gaps included).  Global alignment with any
+ * free-end bits; SQMI_LOCAL is not supported.  With free end gaps the pairs
+ * run on the int32 one-thread-per-pair kernel. */
 sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *set_h,
                               const sqmi_sequences *set_v, const sqmi_scoring *scoring,
                               uint32_t mode, int32_t *scores, int32_t *counts,
@@ -145,7 +148,7 @@ sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *set_h,
  * before them are unspecified).  op_offsets: count + 1 int64 in host memory;
  * ops and op_lengths (count int64) are host memory, or device memory with
  * SQMI_SCORES_ON_DEVICE.  An empty pair gets INT32_MIN, zero counts and no
- * rows.  Global alignment (mode 0) only. */
+ * rows.  Modes as sqmi_align_counts. */
 sqmi_status sqmi_align_rows(sqmi_context *ctx, const sqmi_sequences *set_h,
                             const sqmi_sequences *set_v, const sqmi_scoring *scoring,
                             uint32_t mode, int32_t *scores, int32_t *counts,

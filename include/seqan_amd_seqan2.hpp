@@ -5,7 +5,7 @@
  *
  *   globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial>, setH, setV, score [, AlignConfig<T,L,R,B>])
  *   localAlignmentScore(ExecutionPolicy<Parallel, Vectorial>, setH, setV, score)
- *   globalAlignment(ExecutionPolicy<Parallel, Vectorial>, gapsSetH, gapsSetV, score)
+ *   globalAlignment(ExecutionPolicy<Parallel, Vectorial>, gapsSetH, gapsSetV, score [, AlignConfig<T,L,R,B>])
  *   globalAlignmentScore(ExecutionPolicy<WavefrontAlignment<S>, V>, setH, setV, score [, AlignConfig])
  *   localAlignmentScore(ExecutionPolicy<WavefrontAlignment<S>, V>, setH, setV, score)
  *
@@ -300,7 +300,7 @@ std::vector<int> align_scores_vector(TSetH const & setH, TSetV const & setV, TSc
 }
 
 template <typename TGapsSetH, typename TGapsSetV, typename TScore>
-seqan2::String<int> align_rows(TGapsSetH & gapsH, TGapsSetV & gapsV, TScore const & scheme)
+seqan2::String<int> align_rows(TGapsSetH & gapsH, TGapsSetV & gapsV, TScore const & scheme, uint32_t mode)
 {
     using TAlphabet = typename GapsSetAlphabet<TGapsSetH>::Type;
     if (seqan2::length(gapsH) != seqan2::length(gapsV))
@@ -324,8 +324,8 @@ seqan2::String<int> align_rows(TGapsSetH & gapsH, TGapsSetV & gapsV, TScore cons
     std::vector<int32_t> counts(static_cast<size_t>(count) * (sc.alphabet_size * sc.alphabet_size + 2));
     seqan2::resize(result, count);
     sqmi_context * ctx = thread_context();
-    check(sqmi_align_rows(ctx, &h, &v, &sc, 0u, &result[0], counts.data(), ops.data(), opOff.data(), opLen.data(),
-                          0u),
+    check(sqmi_align_rows(ctx, &h, &v, &sc, mode, &result[0], counts.data(), ops.data(), opOff.data(),
+                          opLen.data(), 0u),
           ctx);
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 16)
@@ -409,7 +409,16 @@ template <typename TGapsSetH, typename TGapsSetV, typename TScore,
 inline String<int>
 globalAlignment(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
                 TGapsSetH && gapsSetH, TGapsSetV && gapsSetV, TScore && scoringScheme)
-SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_rows(gapsSetH, gapsSetV, scoringScheme))
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_rows(gapsSetH, gapsSetV, scoringScheme, 0u))
+
+template <typename TGapsSetH, typename TGapsSetV, typename TScore, typename TConfig,
+          std::enable_if_t<seqan_amd::detail::IsGpuRowsCall<TGapsSetH, TGapsSetV, TScore>::value &&
+                           seqan_amd::detail::IsAlignConfig<std::decay_t<TConfig>>::value, int> = 0>
+inline String<int>
+globalAlignment(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
+                TGapsSetH && gapsSetH, TGapsSetV && gapsSetV, TScore && scoringScheme, TConfig && /*alignConfig*/)
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_rows(gapsSetH, gapsSetV, scoringScheme,
+                                                      seqan_amd::detail::IsAlignConfig<std::decay_t<TConfig>>::mode))
 
 }  // namespace seqan2
 #undef SEQAN_AMD_OVERLOAD_BODY

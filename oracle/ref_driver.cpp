@@ -157,6 +157,32 @@ String<int> runScore(TExec const & ex, TSetH const & setH, TSetV const & setV, T
     }
 }
 
+// The AlignConfig of the mode bits (1 TOP, 2 LEFT, 4 RIGHT, 8 BOTTOM), passed
+// to fn as a value.
+template <typename TFn>
+auto withConfig(uint32_t mode, TFn && fn)
+{
+    switch (mode & 15u)
+    {
+        case 0: return fn(AlignConfig<false, false, false, false>());
+        case 1: return fn(AlignConfig<true, false, false, false>());
+        case 2: return fn(AlignConfig<false, true, false, false>());
+        case 3: return fn(AlignConfig<true, true, false, false>());
+        case 4: return fn(AlignConfig<false, false, true, false>());
+        case 5: return fn(AlignConfig<true, false, true, false>());
+        case 6: return fn(AlignConfig<false, true, true, false>());
+        case 7: return fn(AlignConfig<true, true, true, false>());
+        case 8: return fn(AlignConfig<false, false, false, true>());
+        case 9: return fn(AlignConfig<true, false, false, true>());
+        case 10: return fn(AlignConfig<false, true, false, true>());
+        case 11: return fn(AlignConfig<true, true, false, true>());
+        case 12: return fn(AlignConfig<false, false, true, true>());
+        case 13: return fn(AlignConfig<true, false, true, true>());
+        case 14: return fn(AlignConfig<false, true, true, true>());
+        default: return fn(AlignConfig<true, true, true, true>());
+    }
+}
+
 // Substitution / gap counts of one alignment, read from its two rows.
 template <typename TGaps, typename TScore>
 void rowCounts(TGaps const & row0, TGaps const & row1, TScore const & sc, int alpha, int32_t * out)
@@ -206,8 +232,8 @@ int run(RefInput const & in, TScore const & sc, int alpha, std::FILE * out)
     }
     else if (h.op == 1 || h.op == 2)
     {
-        if (h.mode != 0)
-            die("counts are produced for plain global alignment only");
+        if (h.mode & 16u)
+            die("counts are produced for global alignment (AlignConfig) only");
         counts.assign(h.count * (alpha * alpha + 2), 0);
         using TGaps = Gaps<String<TAlphabet>>;
         StringSet<TGaps> gH, gV;
@@ -221,14 +247,14 @@ int run(RefInput const & in, TScore const & sc, int alpha, std::FILE * out)
         auto t0 = std::chrono::steady_clock::now();
         if (h.op == 1)
         {
-            String<int> res = globalAlignment(ex, gH, gV, sc);
+            String<int> res = withConfig(h.mode, [&](auto cfg) { return globalAlignment(ex, gH, gV, sc, cfg); });
             for (int64_t k = 0; k < h.count; ++k)
                 scores[k] = res[k];
         }
         else
         {
             for (int64_t k = 0; k < h.count; ++k)
-                scores[k] = globalAlignment(gH[k], gV[k], sc);
+                scores[k] = withConfig(h.mode, [&](auto cfg) { return globalAlignment(gH[k], gV[k], sc, cfg); });
         }
         auto t1 = std::chrono::steady_clock::now();
         best = std::chrono::duration<double>(t1 - t0).count();

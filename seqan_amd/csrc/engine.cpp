@@ -310,7 +310,8 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
     const bool local = (mode & SQMI_LOCAL) != 0;
     P.local = local;
     const bool costsOk16 = si.o <= 0 && si.e <= 0 && si.o >= -1024 && si.e >= -1024 && si.sMax <= 1024 &&
-                           si.sMin >= -1024 && !(flags & SQMI_FORCE_INT32);
+                           si.sMin >= -1024 && !(flags & SQMI_FORCE_INT32) &&
+                           !(traceShape && mode != 0);   // free-end tracebacks run in int32
     si.dna = si.alpha <= 4 && !local && si.sMin - si.o >= 0 && si.sMax - si.o <= 255;
     uint32_t track;
     if (local)
@@ -1076,8 +1077,8 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
     sqmi_status st;
     if ((st = validate(ctx, setH, setV, scoring, mode)) != SQMI_OK)
         return st;
-    if (mode != 0)
-        return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_counts / sqmi_align_rows: global alignment (mode 0) only");
+    if (mode & SQMI_LOCAL)
+        return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_counts / sqmi_align_rows: global alignment only (no SQMI_LOCAL)");
     if (flags & (SQMI_PLAIN_PER_PAIR | 0x00FFFF00u))
         return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_counts: no plain form or forced shapes");
     const int64_t N = setH->count;
@@ -1305,6 +1306,7 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             a.counts = dCounts;
             a.gapOpen = si.o;
             a.gapExtend = si.e;
+            a.mode = mode;
             a.alpha = A;
             a.table = static_cast<const int32_t *>(ctx->table32.p);
             a.ops = rows.ops;

@@ -730,9 +730,11 @@ __device__ __forceinline__ int dirOf(uint32_t bits)   // 0 diagonal, 1 vertical,
 template <class Bits>
 __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const uint8_t *v, int A, int score, int endE,
                                            int endF, Bits &bits, int32_t *cnt, uint8_t *opEnd = nullptr,
-                                           int64_t *opLen = nullptr)
+                                           int64_t *opLen = nullptr, int i0 = -1, int j0 = -1)
 {
-    int i = m, j = n;
+    // The walk starts at the end cell (i0, j0), (m, n) unless free end gaps
+    // moved it; the cells after it are trailing gaps of the rows.
+    int i = i0 < 0 ? m : i0, j = j0 < 0 ? n : j0;
     int prev = 0;   // last emitted operation: 0 none / aligned pair, 1 gap in H (vertical), 2 gap in V
     int opens = 0, exts = 0;
     // Rows: one byte per alignment column, written backwards from opEnd
@@ -743,6 +745,17 @@ __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const
             opEnd[-1 - nOps] = op;
         ++nOps;
     };
+    // Trailing gaps after the end cell: the rest of V, then of H.
+    for (int k = i; k < m; ++k) {
+        if (prev == 1) ++exts; else ++opens;
+        prev = 1;
+        emit('V');
+    }
+    for (int k = j; k < n; ++k) {
+        if (prev == 2) ++exts; else ++opens;
+        prev = 2;
+        emit('H');
+    }
     // End cell: F, then E, win ties against the recorded direction
     // (_correctTraceValue, dp_algorithm_impl.h:1190-1207), and a gap start
     // there follows the gap matrix (PreferGapsAtEnd_, dp_traceback_impl.h:115-122).
@@ -885,24 +898,51 @@ __global__ __launch_bounds__(256) void kCounts32(Counts32Args a)
     int32_t *cnt = a.counts + (int64_t)pid * S;
     for (int k = 0; k < S; ++k)
         cnt[k] = 0;
+    // Free end gaps (AlignConfig, dp_setup.h:57-182): a free first row /
+    // column is 0; the result is the first strict maximum over the tracked
+    // cells in column order: (m, n), the last row if it is free, the last
+    // column if it is free (dp_scout.h:169-210, dp_meta_info.h:178-221).
+    const bool freeTop = (a.mode & SQMI_FREE_TOP) != 0, freeLeft = (a.mode & SQMI_FREE_LEFT) != 0;
+    const bool freeRight = (a.mode & SQMI_FREE_RIGHT) != 0, freeBottom = (a.mode & SQMI_FREE_BOTTOM) != 0;
+    int32_t best = kNeg32, bestE = kNeg32, bestF = kNeg32;
+    int bi = m, bj = n;
+    auto track = [&](bool lastCol, int i, int j, int32_t hv, int32_t ev, int32_t fv) {
+        const bool tracked = (i == m) ? (lastCol || freeBottom) : (lastCol && freeRight);
+        if (tracked && hv > best) {
+            best = hv;
+            bestE = ev;
+            bestF = fv;
+            bi = i;
+            bj = j;
+        }
+    };
     {
         int32_t F = kNeg32;
         H[0] = 0;
         E[0] = kNeg32;
         for (int i = 1; i <= m; ++i) {
-            F = max(F + e, H[i - 1] + o);
-            H[i] = F;
+            if (freeLeft) {
+                H[i] = 0;
+            } else {
+                F = max(F + e, H[i - 1] + o);
+                H[i] = F;
+            }
             E[i] = kNeg32;
+            track(false, i, 0, H[i], E[i], freeLeft ? kNeg32 : F);
         }
     }
-    int32_t lastF = kNeg32;
     for (int j = 1; j <= n; ++j) {
         const int hj = min((int)h[j - 1], A - 1);
         const int32_t *row = a.table + hj * A;
         int32_t diag = H[0];
         int32_t F = kNeg32;
-        E[0] = max(E[0] + e, H[0] + o);
-        H[0] = E[0];
+        if (freeTop) {
+            H[0] = 0;
+        } else {
+            E[0] = max(E[0] + e, H[0] + o);
+            H[0] = E[0];
+        }
+        track(j == n, 0, j, H[0], E[0], kNeg32);
         int32_t up = H[0];
         uint8_t *Tc = T + (int64_t)j * CS;
         for (int i = 1; i <= m; ++i) {
@@ -924,14 +964,13 @@ __global__ __launch_bounds__(256) void kCounts32(Counts32Args a)
             Tc[i] = (uint8_t)tb;
             H[i] = x;
             up = x;
+            track(j == n, i, j, x, ei, F);
         }
-        lastF = F;
     }
-    const int32_t score = H[m];
-    a.scores[pid] = score;
+    a.scores[pid] = best;
     auto bits = [&](int i, int j) -> uint32_t { return T[(int64_t)j * CS + i]; };
-    walkCounts(n, m, h, v, A, score, E[m], lastF, bits, cnt, a.ops ? a.ops + a.opOff[pid + 1] : nullptr,
-               a.ops ? a.opLen + pid : nullptr);
+    walkCounts(n, m, h, v, A, best, bestE, bestF, bits, cnt, a.ops ? a.ops + a.opOff[pid + 1] : nullptr,
+               a.ops ? a.opLen + pid : nullptr, bi, bj);
 }
 
 template <int G, int R, bool DNA>

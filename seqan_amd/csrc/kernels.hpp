@@ -102,6 +102,7 @@ struct Counts32Args {
     int32_t *counts;
     int32_t gapOpen;
     int32_t gapExtend;
+    uint32_t mode;              // free end gaps (SQMI_FREE_*); no local
     int32_t alpha;
     const int32_t *table;
     uint8_t *ops;               // rows, as in WalkArgs (optional)

@@ -3,6 +3,7 @@ holds the reference tree and the oracle/_ref build):
 
     make -C oracle && python tests/golden/make_golden.py
     python tests/golden/make_golden.py --long   # the 100 kbp fixture (~25 GB of RAM)
+    python tests/golden/make_golden.py --modes  # only the AlignConfig rows fixtures
 
 * ``ref_*.npz``   -- inputs (residue ordinals + offsets) and the outputs of
   the reference library itself (oracle/_ref: SeqAn 2 globalAlignmentScore /
@@ -152,6 +153,43 @@ def main():
          ops=rs["ops"], gap_extend=-1, gap_open=-11)
 
 
+def trimmed_pairs(rng, count, lo, hi):
+    """Related DNA pairs whose V often lacks a prefix or a suffix of H, so
+    that free end gaps matter."""
+    hs, vs = [], []
+    for _ in range(count):
+        h = rng.integers(0, 4, int(rng.integers(lo, hi + 1))).astype(np.uint8)
+        v = h.copy()
+        f = rng.random(len(v)) < 0.2
+        v[f] = rng.integers(0, 4, int(f.sum()))
+        if rng.random() < 0.5:
+            v = v[int(rng.integers(0, len(v) // 3 + 1)):]
+        if rng.random() < 0.5:
+            v = v[:max(1, len(v) - int(rng.integers(0, len(v) // 3 + 1)))]
+        if rng.random() < 0.3:
+            h, v = v, h
+        hs.append(h)
+        vs.append(v)
+    return sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+
+
+def mode_rows():
+    """Counts and rows of global alignment under each of the 16 AlignConfigs
+    (the reference's globalAlignment(ExecutionPolicy<Parallel, Vectorial>,
+    gapsH, gapsV, score, AlignConfig<T, L, R, B>))."""
+    rng = np.random.default_rng(0x5EED0100)
+    for name, count, lo, hi, par in (("ref_dna_modes_rows.npz", 128, 5, 300, (2, -3, -1, -5)),
+                                    ("ref_dna_modes_short_rows.npz", 200, 1, 40, (1, -1, -1, -2))):
+        H, V = trimmed_pairs(rng, count, lo, hi)
+        kw = dict(match=par[0], mismatch=par[1], gap_extend=par[2], gap_open=par[3])
+        res = [po.run_reference(po.OP_COUNTS_SIMD, H.residues, H.offsets, V.residues, V.offsets, mode,
+                                alphabet="dna", **kw) for mode in range(16)]
+        mode_off = np.cumsum([0] + [len(r["ops"]) for r in res]).astype(np.int64)
+        save(name, H, V, scores=np.stack([r["scores"] for r in res]), counts=np.stack([r["counts"] for r in res]),
+             op_off=np.stack([r["op_off"] for r in res]), ops=np.concatenate([r["ops"] for r in res]),
+             mode_off=mode_off, **kw)
+
+
 def long_pairs():
     """100 kbp fixture: the first two pairs of the 64 x 100 kbp workload at
     full length (run on its own: --long).  Scores come from the reference's
@@ -172,5 +210,8 @@ def long_pairs():
 if __name__ == "__main__":
     if "--long" in sys.argv:
         long_pairs()
+    elif "--modes" in sys.argv:
+        mode_rows()
     else:
         main()
+        mode_rows()

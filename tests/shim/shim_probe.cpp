@@ -88,10 +88,11 @@ static_assert(gpuLocal<ExPV const &, AaSet &, AaSet &, SelectableAminoAcidMatrix
 static_assert(gpuRows<ExPV const &, DnaGaps &, DnaGaps &, Simple &>(), "rows, DNA");
 static_assert(gpuRows<ExPV, DnaGaps &, DnaGaps &, Simple const &>(), "rows, const score");
 static_assert(gpuRows<ExPV const &, AaGaps &, AaGaps &, Blosum62 &>(), "rows, protein");
+static_assert(gpuRows<ExPV const &, DnaGaps &, DnaGaps &, Simple &, Semi &>(), "rows, AlignConfig");
 
 // Everything else stays with the reference templates.
 static_assert(!gpuRows<ExPS const &, DnaGaps &, DnaGaps &, Simple &>(), "rows, Parallel, Serial");
-static_assert(!gpuRows<ExPV const &, DnaGaps &, DnaGaps &, Simple &, Semi &>(), "rows with an AlignConfig");
+static_assert(!gpuRows<ExPV const &, DnaGaps &, DnaGaps &, Simple &, Semi &, Gotoh &>(), "rows, algorithm tag");
 static_assert(!gpuGlobal<ExPS const &, DnaSet &, DnaSet &, Simple &>(), "Parallel, Serial");
 static_assert(!gpuGlobal<ExSV const &, DnaSet &, DnaSet &, Simple &>(), "Serial, Vectorial");
 static_assert(!gpuGlobal<ExPV const &, CharSet &, CharSet &, Simple &>(), "char alphabet");

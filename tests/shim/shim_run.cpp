@@ -91,9 +91,10 @@ std::string rowText(TGaps & gaps)
 
 // globalAlignment on StringSets of Gaps: scores and both rows of every pair
 // against the reference's own globalAlignment (non-empty pairs only).
-template <typename TAlphabet, typename TScore, typename TGpu, typename TCpu>
+template <typename TAlphabet, typename TScore, typename TGpu, typename TCpu, typename... TConfig>
 void runRowsCase(char const * name, StringSet<String<TAlphabet>> const & setH,
-                 StringSet<String<TAlphabet>> const & setV, TScore const & sc, TGpu const & ex, TCpu const & cpu)
+                 StringSet<String<TAlphabet>> const & setV, TScore const & sc, TGpu const & ex, TCpu const & cpu,
+                 TConfig const &... cfg)
 {
     using TGaps = Gaps<String<TAlphabet> const>;
     StringSet<TGaps> gH, gV, rH, rV;
@@ -106,10 +107,10 @@ void runRowsCase(char const * name, StringSet<String<TAlphabet>> const & setH,
         appendValue(rH, TGaps(setH[i]));
         appendValue(rV, TGaps(setV[i]));
     }
-    String<int> expect = globalAlignment(cpu, rH, rV, sc);
+    String<int> expect = globalAlignment(cpu, rH, rV, sc, cfg...);
     try
     {
-        String<int> got = globalAlignment(ex, gH, gV, sc);
+        String<int> got = globalAlignment(ex, gH, gV, sc, cfg...);
         for (size_t i = 0; i < length(expect); ++i)
             if (got[i] != expect[i] || rowText(gH[i]) != rowText(rH[i]) || rowText(gV[i]) != rowText(rV[i]))
             {
@@ -200,6 +201,8 @@ int main()
 
     runRowsCase("dna_rows", dh, dv, dna, ex, cpu);
     runRowsCase("protein_rows_blosum62", ph, pv, b62, ex, cpu);
+    runRowsCase("dna_rows_semiglobal", dh, dv, dna, ex, cpu, semi);
+    runRowsCase("dna_rows_overlap", dh, dv, dna, ex, cpu, overlap);
 
     // |setH| != |setV| raises instead of asserting.
     StringSet<String<Dna>> shorter = dv;

@@ -155,4 +155,49 @@ def test_rows_empty_pairs_and_errors(ctx):
     assert list(np.diff(op_off)) == [3, 0, 0, 1]
     assert bytes(ops[op_off[3]:op_off[4]]) == b"M"
     with pytest.raises(sa.SqmiError):
-        ctx.align_rows(Hs, Vs, score, mode=sa.FREE_TOP)
+        ctx.align_rows(Hs, Vs, score, mode=sa.LOCAL)
+
+
+@pytest.mark.parametrize("fixture", ["ref_dna_modes_rows.npz", "ref_dna_modes_short_rows.npz"])
+def test_rows_every_align_config_match_reference(ctx, fixture):
+    # Free end gaps: the end cell moves to the best tracked cell and the rows
+    # carry the free leading / trailing gaps (counted like any gap).
+    z = _load(fixture)
+    score = _score_of(z, fixture)
+    Hs, Vs = sa.SeqSet(z["h_res"], z["h_off"]), sa.SeqSet(z["v_res"], z["v_off"])
+    for mode in range(16):
+        print("rows, AlignConfig mode", mode, flush=True)
+        s, c, op_off, ops = ctx.align_rows(Hs, Vs, score, mode=mode)
+        assert np.array_equal(s, z["scores"][mode]), mode
+        assert np.array_equal(c, z["counts"][mode]), mode
+        assert np.array_equal(op_off, z["op_off"][mode]), mode
+        assert np.array_equal(ops, z["ops"][z["mode_off"][mode]:z["mode_off"][mode + 1]]), mode
+        s2, c2 = ctx.align_counts(Hs, Vs, score, mode=mode)
+        assert np.array_equal(s2, s) and np.array_equal(c2, c), mode
+        assert np.array_equal(ctx.align_score(Hs, Vs, score, mode), s), mode
+
+
+def test_rows_align_configs_match_oracle(ctx):
+    rng = np.random.default_rng(23)
+    hs, vs = [], []
+    for _ in range(150):
+        h = rng.integers(0, 4, int(rng.integers(1, 600))).astype(np.uint8)
+        v = h[int(rng.integers(0, len(h) // 2 + 1)):].copy()
+        flip = rng.random(len(v)) < 0.12
+        v[flip] = rng.integers(0, 4, int(flip.sum()))
+        if len(v) == 0 or rng.random() < 0.2:
+            v = rng.integers(0, 4, int(rng.integers(1, 600))).astype(np.uint8)
+        if rng.random() < 0.5:
+            h, v = v, h
+        hs.append(h)
+        vs.append(v)
+    Hs, Vs = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+    score = sa.simple_score(2, -3, -1, -5)
+    for mode in (1, 2, 4, 8, 9, 6, 15):
+        print("rows vs oracle, mode", mode, flush=True)
+        s, c, op_off, ops = ctx.align_rows(Hs, Vs, score, mode=mode)
+        ws, wc, woff, wops = po.rows_batch(Hs.residues, Hs.offsets, Vs.residues, Vs.offsets,
+                                           po.Scoring(score.table, score.gap_open, score.gap_extend), mode)
+        assert np.array_equal(s, ws) and np.array_equal(c, wc)
+        assert np.array_equal(op_off, woff) and np.array_equal(ops, wops)
+        assert np.array_equal(_counts_from_rows(hs, vs, op_off, ops, 4), c)

@@ -218,6 +218,20 @@ def test_oracle_rows_match_reference(fixture):
     assert np.array_equal(ops, z["ops"])
 
 
+@pytest.mark.parametrize("fixture", ["ref_dna_modes_rows.npz", "ref_dna_modes_short_rows.npz"])
+def test_oracle_rows_match_reference_every_align_config(fixture):
+    # globalAlignment(..., AlignConfig<T, L, R, B>) for all 16 configs: free
+    # end gaps move the end cell and add leading / trailing gaps to the rows.
+    z = _load(fixture)
+    sc = po.Scoring.simple(int(z["match"]), int(z["mismatch"]), int(z["gap_extend"]), int(z["gap_open"]))
+    for mode in range(16):
+        s, c, op_off, ops = po.rows_batch(z["h_res"], z["h_off"], z["v_res"], z["v_off"], sc, mode)
+        assert np.array_equal(s, z["scores"][mode]), mode
+        assert np.array_equal(c, z["counts"][mode]), mode
+        assert np.array_equal(op_off, z["op_off"][mode]), mode
+        assert np.array_equal(ops, z["ops"][z["mode_off"][mode]:z["mode_off"][mode + 1]]), mode
+
+
 def test_100kbp_fixture_is_a_valid_alignment():
     # Two 100 kbp pairs: reference scores, oracle counts (the reference's own
     # traceback truncates trace-matrix positions past 2^32 at this size).

@@ -22,7 +22,8 @@ REF_HEADERS = "/root/reference/include/seqan/align.h"
 CASES = ["dna_global_nonconst", "dna_semiglobal_const", "dna_overlap_rvalue", "dna_local", "dna_concat_direct",
          "dna5_linear", "protein_local_blosum62", "protein_global_blosum62", "protein_semiglobal_blosum62",
          "protein_global_pam120", "protein_local_selectable_blosum80", "wavefront_global_semi",
-         "wavefront_local_blosum62", "dna_rows", "protein_rows_blosum62", "length_mismatch"]
+         "wavefront_local_blosum62", "dna_rows", "protein_rows_blosum62", "dna_rows_semiglobal",
+         "dna_rows_overlap", "length_mismatch"]
 
 
 def _run_cases():
