@@ -114,6 +114,88 @@ int32_t orc_score(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
     return best;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Banded score (global with free-end bits): only cells with                  */
+/* lower <= j - i <= upper exist; the others are -inf.  Band settings the     */
+/* reference rejects give INT32_MIN: a band missing the matrix, or missing    */
+/* (0, 0) / (m, n) where those ends are not free                             */
+/* (_checkBandProperties, dp_algorithm_impl.h:192-232).                       */
+/* ------------------------------------------------------------------------ */
+int32_t orc_score_banded(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
+                         const orc_scoring *sc, uint32_t mode, int64_t lower, int64_t upper)
+{
+    if (n <= 0 || m <= 0 || (mode & ORC_LOCAL)) return INT32_MIN;
+    if (upper < -m || lower > n) return INT32_MIN;
+    if (upper < 0 && !(mode & ORC_FREE_LEFT)) return INT32_MIN;
+    if (lower > 0 && !(mode & ORC_FREE_TOP)) return INT32_MIN;
+    if (upper + m < n && !(mode & ORC_FREE_BOTTOM)) return INT32_MIN;
+    if (lower + m > n && !(mode & ORC_FREE_RIGHT)) return INT32_MIN;
+    const int32_t o = sc->gap_open, e = sc->gap_extend;
+    int32_t *H = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m + 1));
+    int32_t *E = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m + 1));
+    int32_t best = NEG_INF;
+#define INBAND(I, J) ((J) - (I) >= lower && (J) - (I) <= upper)
+    {
+        int32_t F = NEG_INF;
+        H[0] = 0;
+        E[0] = NEG_INF;
+        if (!INBAND(0, 0)) H[0] = NEG_INF;
+        if (tracked(mode, 0, 0) && H[0] > best) best = H[0];
+        for (int64_t i = 1; i <= m; ++i) {
+            if (!INBAND(i, 0)) {
+                H[i] = NEG_INF;
+                F = NEG_INF;
+            } else if (mode & ORC_FREE_LEFT) {
+                H[i] = 0;
+            } else {
+                F = max2(F + e, H[i - 1] + o);
+                H[i] = F;
+            }
+            E[i] = NEG_INF;
+            if (tracked(mode, 0, i == m ? 2 : 1) && H[i] > best) best = H[i];
+        }
+    }
+    for (int64_t j = 1; j <= n; ++j) {
+        const int col = (j == n) ? 2 : 1;
+        const uint8_t hj = h[j - 1];
+        int32_t diag = H[0];
+        int32_t F = NEG_INF;
+        if (!INBAND(0, j)) {
+            H[0] = NEG_INF;
+            E[0] = NEG_INF;
+        } else if (mode & ORC_FREE_TOP) {
+            H[0] = 0;
+        } else {
+            E[0] = max2(E[0] + e, H[0] + o);
+            H[0] = E[0];
+        }
+        if (tracked(mode, col, 0) && H[0] > best) best = H[0];
+        int32_t up = H[0];
+        for (int64_t i = 1; i <= m; ++i) {
+            if (!INBAND(i, j)) {
+                diag = H[i];
+                H[i] = NEG_INF;
+                E[i] = NEG_INF;
+                F = NEG_INF;
+                up = NEG_INF;
+                continue;
+            }
+            const int32_t d = diag + subst(sc, hj, v[i - 1]);
+            diag = H[i];
+            E[i] = max2(E[i] + e, H[i] + o);
+            F = max2(F + e, up + o);
+            const int32_t x = max2(d, max2(F, E[i]));
+            H[i] = x;
+            up = x;
+            if (tracked(mode, col, i == m ? 2 : 1) && x > best) best = x;
+        }
+    }
+#undef INBAND
+    free(H);
+    free(E);
+    return best;
+}
+
 void orc_score_batch(const uint8_t *h_res, const int64_t *h_off,
                      const uint8_t *v_res, const int64_t *v_off, int64_t count,
                      const orc_scoring *sc, uint32_t mode, int32_t *scores,

@@ -58,6 +58,11 @@ int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
                       const orc_scoring *sc, uint32_t mode, int32_t *counts,
                       char *ops, int64_t *n_ops);
 
+/* Banded global score: cells with lower <= j - i <= upper only; band
+ * settings the reference rejects give INT32_MIN. */
+int32_t orc_score_banded(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
+                         const orc_scoring *sc, uint32_t mode, int64_t lower, int64_t upper);
+
 /* Batch form of orc_traceback (counts stride alpha*alpha + 2). */
 void orc_traceback_batch(const uint8_t *h_res, const int64_t *h_off,
                          const uint8_t *v_res, const int64_t *v_off, int64_t count,

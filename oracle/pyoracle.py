@@ -61,6 +61,9 @@ def lib():
         L.orc_rows_batch.restype = None
         L.orc_rows_batch.argtypes = [u8p, i64p, u8p, i64p, ctypes.c_int64, sp, ctypes.c_uint32, i32p, i32p,
                                      u8p, i64p, i64p, ctypes.c_int]
+        L.orc_score_banded.restype = ctypes.c_int32
+        L.orc_score_banded.argtypes = [u8p, ctypes.c_int64, u8p, ctypes.c_int64, sp, ctypes.c_uint32,
+                                       ctypes.c_int64, ctypes.c_int64]
         L.orc_brute_force.restype = ctypes.c_int32
         L.orc_brute_force.argtypes = [u8p, ctypes.c_int64, u8p, ctypes.c_int64, sp, ctypes.c_uint32]
         _lib = L
@@ -119,6 +122,12 @@ def traceback(h, v, sc: Scoring, mode: int = 0):
     return int(s), counts, buf.raw[:nops.value].decode()
 
 
+def score_banded(h, v, sc: Scoring, mode: int, lower: int, upper: int) -> int:
+    h, v = _u8(h), _u8(v)
+    return int(lib().orc_score_banded(_ptr(h, ctypes.c_uint8), len(h), _ptr(v, ctypes.c_uint8), len(v),
+                                      ctypes.byref(sc._c), mode, lower, upper))
+
+
 def score_batch(h_res, h_off, v_res, v_off, sc: Scoring, mode: int, threads: int = 0):
     h_res, v_res = _u8(h_res), _u8(v_res)
     h_off = np.ascontiguousarray(h_off, dtype=np.int64)
@@ -174,7 +183,7 @@ def traceback_batch(h_res, h_off, v_res, v_off, sc: Scoring, mode: int = 0, thre
 # The compiled reference (oracle/_ref).
 # --------------------------------------------------------------------------
 REF_DIR = os.path.join(HERE, "_ref")
-OP_SCORE, OP_COUNTS_SIMD, OP_COUNTS_SCALAR = 0, 1, 2
+OP_SCORE, OP_COUNTS_SIMD, OP_COUNTS_SCALAR, OP_SCORE_BANDED = 0, 1, 2, 4
 
 
 def _cpu_has_avx512():
@@ -198,7 +207,7 @@ def ref_binary():
 
 def run_reference(op, h_res, h_off, v_res, v_off, mode, *, alphabet="dna", matrix="simple",
                   match=0, mismatch=0, gap_extend=0, gap_open=0, threads=0, repeats=1,
-                  timeout=3600):
+                  timeout=3600, lower_diag=0, upper_diag=0):
     """Runs the reference library on a batch.  Returns a dict with scores,
     counts (ops 1/2), best_seconds, simd_lanes, threads."""
     exe = ref_binary()
@@ -216,6 +225,8 @@ def run_reference(op, h_res, h_off, v_res, v_off, mode, *, alphabet="dna", matri
             f.write(b"SQRI")
             f.write(struct.pack("<IIIiiiiIiiq", op, mode, alpha_id, match, mismatch, gap_extend,
                                 gap_open, mat_id, threads, repeats, n))
+            if op == OP_SCORE_BANDED:
+                f.write(struct.pack("<qq", lower_diag, upper_diag))
             f.write(h_off.tobytes())
             f.write(v_off.tobytes())
             f.write(h_res.tobytes())

@@ -18,7 +18,8 @@
 //       include/seqan/align/evaluate_alignment.h:215-300
 //
 // Input file (little endian):
-//   char[4] "SQRI", u32 op (0 score, 1 counts-simd, 2 counts-scalar, 3 table),
+//   char[4] "SQRI", u32 op (0 score, 1 counts-simd, 2 counts-scalar, 3 table,
+//   4 banded score: i64 lower_diag, i64 upper_diag follow the count),
 //   u32 mode (1 TOP, 2 LEFT, 4 RIGHT, 8 BOTTOM, 16 LOCAL),
 //   u32 alphabet (0 Dna, 1 AminoAcid), i32 match, i32 mismatch,
 //   i32 gap_extend, i32 gap_open, u32 matrix (0 Simple, 1 Blosum62),
@@ -54,6 +55,7 @@ struct RefHeader
     uint32_t matrix;
     int32_t threads, repeats;
     int64_t count;
+    int64_t lowerDiag = 0, upperDiag = 0;   // op 4 only
 };
 
 struct RefInput
@@ -98,6 +100,11 @@ RefInput readInput(char const * path)
     readOrDie(f, &h.threads, 1);
     readOrDie(f, &h.repeats, 1);
     readOrDie(f, &h.count, 1);
+    if (h.op == 4)
+    {
+        readOrDie(f, &h.lowerDiag, 1);
+        readOrDie(f, &h.upperDiag, 1);
+    }
     in.hOff.resize(h.count + 1);
     in.vOff.resize(h.count + 1);
     readOrDie(f, in.hOff.data(), in.hOff.size());
@@ -217,7 +224,36 @@ int run(RefInput const & in, TScore const & sc, int alpha, std::FILE * out)
     std::vector<int64_t> opOff;
     std::vector<uint8_t> ops;
     double best = 1e300;
-    if (h.op == 0)
+    if (h.op == 4)
+    {
+        // Banded globalAlignmentScore(setH, setV, score, AlignConfig, lowerDiag,
+        // upperDiag) (global_alignment_banded.h:555-583), SIMD over the sets.
+        if (h.mode & 16u)
+            die("banded scores are produced for global alignment (AlignConfig) only");
+        auto t0 = std::chrono::steady_clock::now();
+        if (h.repeats < 0)
+        {
+            // repeats == -1: the scalar banded function, pair by pair
+            // (global_alignment_banded.h:394-470).
+            for (int64_t k = 0; k < h.count; ++k)
+                scores[k] = withConfig(h.mode, [&](auto cfg) {
+                    return globalAlignmentScore(setH[k], setV[k], sc, cfg, static_cast<int>(h.lowerDiag),
+                                                static_cast<int>(h.upperDiag));
+                });
+        }
+        else
+        {
+            String<int> res = withConfig(h.mode, [&](auto cfg) {
+                return globalAlignmentScore(setH, setV, sc, cfg, static_cast<int>(h.lowerDiag),
+                                            static_cast<int>(h.upperDiag));
+            });
+            for (int64_t k = 0; k < h.count; ++k)
+                scores[k] = res[k];
+        }
+        auto t1 = std::chrono::steady_clock::now();
+        best = std::chrono::duration<double>(t1 - t0).count();
+    }
+    else if (h.op == 0)
     {
         int reps = h.repeats > 0 ? h.repeats : 1;
         for (int r = 0; r < reps; ++r)
