@@ -21,6 +21,12 @@
  *                           its traceback writes into the Gaps objects
  *                           (dp_traceback_impl.h:115-546, gaps set through
  *                           include/seqan/align/dp_traceback_adaptor.h)
+ *   sqmi_align_local_rows <- localAlignment(ExecutionPolicy<Parallel,Vectorial>,
+ *                           gapsSetH, gapsSetV, score)
+ *                           include/seqan/align_parallel/parallel_align_interface.h:312-321,
+ *                           set overload local_alignment_unbanded.h:367-396:
+ *                           the rows of the local window and its clipping
+ *                           (dp_traceback_adaptor.h:60-117)
  *
  * Semantics follow the reference exactly: affine gaps where a gap of k
  * characters costs gap_open + (k-1)*gap_extend (Score<int,Simple> constructor
@@ -37,7 +43,8 @@
 extern "C" {
 #endif
 
-#define SQMI_ABI_VERSION 2   /* 2: sqmi_stats.pairs_long, pairs_long_packed; sqmi_align_rows */
+#define SQMI_ABI_VERSION 3   /* 2: sqmi_stats.pairs_long, pairs_long_packed; sqmi_align_rows
+                                3: sqmi_align_local_rows; SQMI_LOCAL in sqmi_align_counts */
 
 typedef enum sqmi_status {
     SQMI_OK = 0,
@@ -131,9 +138,11 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *set_h,
  * parameters: substitution counts, gap opens and gap extensions along the
  * optimal path chosen by the reference traceback (single trace, gaps left,
  * gaps preferred at the end cell), counted on the rows like
- * computeAlignmentStats (free end gaps included).  Global alignment with any
- * free-end bits; SQMI_LOCAL is not supported.  With free end gaps the pairs
- * run on the int32 one-thread-per-pair kernel. */
+ * computeAlignmentStats (free end gaps included; for SQMI_LOCAL the rows of
+ * the local window).  Global alignment with any free-end bits, or SQMI_LOCAL
+ * (free-end bits beside it are ignored, as in sqmi_align_score).  With free
+ * end gaps or SQMI_LOCAL the pairs run on the int32 one-thread-per-pair
+ * kernel. */
 sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *set_h,
                               const sqmi_sequences *set_v, const sqmi_scoring *scoring,
                               uint32_t mode, int32_t *scores, int32_t *counts,
@@ -148,12 +157,27 @@ sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *set_h,
  * before them are unspecified).  op_offsets: count + 1 int64 in host memory;
  * ops and op_lengths (count int64) are host memory, or device memory with
  * SQMI_SCORES_ON_DEVICE.  An empty pair gets INT32_MIN, zero counts and no
- * rows.  Modes as sqmi_align_counts. */
+ * rows.  Global modes as sqmi_align_counts (SQMI_LOCAL: sqmi_align_local_rows). */
 sqmi_status sqmi_align_rows(sqmi_context *ctx, const sqmi_sequences *set_h,
                             const sqmi_sequences *set_v, const sqmi_scoring *scoring,
                             uint32_t mode, int32_t *scores, int32_t *counts,
                             uint8_t *ops, const int64_t *op_offsets, int64_t *op_lengths,
                             uint32_t flags);
+
+/* Local alignment (Smith-Waterman with affine gaps): score, counts and the
+ * rows of the local window, laid out as in sqmi_align_rows, plus begins
+ * (count x 2 int64, host memory or device memory with SQMI_SCORES_ON_DEVICE):
+ * the positions in H and in V of the window's first residue.  The window
+ * covers H[begins[2i], begins[2i] + #M + #H) and V[begins[2i+1],
+ * begins[2i+1] + #M + #V).  The end cell is the first strict maximum in
+ * column order and the walk stops at the first cell that does not beat 0,
+ * as in the reference.  A pair whose best score is 0 gets an empty window at
+ * (0, 0), zero counts and score 0; an empty pair gets INT32_MIN. */
+sqmi_status sqmi_align_local_rows(sqmi_context *ctx, const sqmi_sequences *set_h,
+                                  const sqmi_sequences *set_v, const sqmi_scoring *scoring,
+                                  int32_t *scores, int32_t *counts, uint8_t *ops,
+                                  const int64_t *op_offsets, int64_t *op_lengths,
+                                  int64_t *begins, uint32_t flags);
 
 sqmi_status sqmi_get_stats(const sqmi_context *ctx, sqmi_stats *out);
 

@@ -6,6 +6,7 @@
  *   globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial>, setH, setV, score [, AlignConfig<T,L,R,B>])
  *   localAlignmentScore(ExecutionPolicy<Parallel, Vectorial>, setH, setV, score)
  *   globalAlignment(ExecutionPolicy<Parallel, Vectorial>, gapsSetH, gapsSetV, score [, AlignConfig<T,L,R,B>])
+ *   localAlignment(ExecutionPolicy<Parallel, Vectorial>, gapsSetH, gapsSetV, score)
  *   globalAlignmentScore(ExecutionPolicy<WavefrontAlignment<S>, V>, setH, setV, score [, AlignConfig])
  *   localAlignmentScore(ExecutionPolicy<WavefrontAlignment<S>, V>, setH, setV, score)
  *
@@ -26,9 +27,12 @@
  *       include/seqan/align_parallel/parallel_align_interface.h:354-365
  *   globalAlignment(ExecutionPolicy<TPar,TVec> const &, TArgs && ...)
  *       include/seqan/align_parallel/parallel_align_interface.h:288-299
- * globalAlignment writes the rows of the reference's traceback into the Gaps
- * objects (the way the reference does: clear gaps and clipping, then insert
- * the gap runs through Gaps iterators) and returns the scores.
+ *   localAlignment(ExecutionPolicy<TPar,TVec> const &, TArgs && ...)
+ *       include/seqan/align_parallel/parallel_align_interface.h:312-321
+ * globalAlignment and localAlignment write the rows of the reference's
+ * traceback into the Gaps objects (the way the reference does: clear gaps and
+ * clipping, then insert the gap runs through Gaps iterators; localAlignment
+ * also clips both rows to the local window) and return the scores.
  * The wavefront policy (intra-pair parallelism for long sequences,
  * parallel_align_interface.h:367-420) returns std::vector<int> like the
  * reference; its block size and parallel-alignment settings are not used:
@@ -225,14 +229,31 @@ void packSources(TGapsSet & set, std::vector<uint8_t> & residues, std::vector<in
 
 /* Rows of one alignment into its two Gaps objects: clear gaps and clipping,
  * then walk both rows with Gaps iterators, inserting each gap run ('V': a
- * run in the H row, 'H': in the V row). */
+ * run in the H row, 'H': in the V row).  A local window (begins given)
+ * starts at source positions begins[0] / begins[1] and the clipping ends
+ * after its last column; an empty window clips both rows to nothing
+ * (the Gaps state dp_traceback_adaptor.h:60-117 leaves). */
 template <typename TGapsH, typename TGapsV>
-void write_rows(TGapsH & gapsH, TGapsV & gapsV, uint8_t const * row, int64_t len)
+void write_rows(TGapsH & gapsH, TGapsV & gapsV, uint8_t const * row, int64_t len,
+                int64_t const * begins = nullptr)
 {
     seqan2::clearGaps(gapsH);
     seqan2::clearClipping(gapsH);
     seqan2::clearGaps(gapsV);
     seqan2::clearClipping(gapsV);
+    if (begins && len == 0)
+    {
+        seqan2::setClippedBeginPosition(gapsH, 0);
+        seqan2::setClippedEndPosition(gapsH, 0);
+        seqan2::setClippedBeginPosition(gapsV, 0);
+        seqan2::setClippedEndPosition(gapsV, 0);
+        return;
+    }
+    if (begins)
+    {
+        seqan2::setBeginPosition(gapsH, begins[0]);
+        seqan2::setBeginPosition(gapsV, begins[1]);
+    }
     auto itH = seqan2::begin(gapsH, seqan2::Standard());
     auto itV = seqan2::begin(gapsV, seqan2::Standard());
     for (int64_t k = 0; k < len;)
@@ -248,6 +269,11 @@ void write_rows(TGapsH & gapsH, TGapsV & gapsV, uint8_t const * row, int64_t len
         seqan2::goFurther(itH, run);
         seqan2::goFurther(itV, run);
         k += run;
+    }
+    if (begins)
+    {
+        seqan2::setClippedEndPosition(gapsH, seqan2::position(itH) + seqan2::clippedBeginPosition(gapsH));
+        seqan2::setClippedEndPosition(gapsV, seqan2::position(itV) + seqan2::clippedBeginPosition(gapsV));
     }
 }
 
@@ -335,6 +361,42 @@ seqan2::String<int> align_rows(TGapsSetH & gapsH, TGapsSetV & gapsV, TScore cons
     return result;
 }
 
+template <typename TGapsSetH, typename TGapsSetV, typename TScore>
+seqan2::String<int> align_local_rows(TGapsSetH & gapsH, TGapsSetV & gapsV, TScore const & scheme)
+{
+    using TAlphabet = typename GapsSetAlphabet<TGapsSetH>::Type;
+    if (seqan2::length(gapsH) != seqan2::length(gapsV))
+        throw Error(SQMI_ERR_LENGTH_MISMATCH, "seqan_amd: gapsSetH and gapsSetV differ in length");
+    seqan2::String<int> result;
+    int64_t const count = static_cast<int64_t>(seqan2::length(gapsH));
+    if (count == 0)
+        return result;
+    std::vector<uint8_t> resH, resV;
+    std::vector<int64_t> offH, offV;
+    packSources(gapsH, resH, offH);
+    packSources(gapsV, resV, offV);
+    std::vector<int32_t> table;
+    sqmi_scoring const sc = make_scoring<TAlphabet>(scheme, table);
+    sqmi_sequences const h{resH.data(), offH.data(), count};
+    sqmi_sequences const v{resV.data(), offV.data(), count};
+    std::vector<int64_t> opOff(count + 1, 0), opLen(count, 0), begins(2 * count, 0);
+    for (int64_t i = 0; i < count; ++i)
+        opOff[i + 1] = opOff[i] + (offH[i + 1] - offH[i]) + (offV[i + 1] - offV[i]);
+    std::vector<uint8_t> ops(static_cast<size_t>(opOff[count]) + 1);
+    std::vector<int32_t> counts(static_cast<size_t>(count) * (sc.alphabet_size * sc.alphabet_size + 2));
+    seqan2::resize(result, count);
+    sqmi_context * ctx = thread_context();
+    check(sqmi_align_local_rows(ctx, &h, &v, &sc, &result[0], counts.data(), ops.data(), opOff.data(),
+                                opLen.data(), begins.data(), 0u),
+          ctx);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (int64_t i = 0; i < count; ++i)
+        write_rows(gapsH[i], gapsV[i], ops.data() + opOff[i + 1] - opLen[i], opLen[i], begins.data() + 2 * i);
+    return result;
+}
+
 }  // namespace detail
 }  // namespace seqan_amd
 
@@ -419,6 +481,13 @@ globalAlignment(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
                 TGapsSetH && gapsSetH, TGapsSetV && gapsSetV, TScore && scoringScheme, TConfig && /*alignConfig*/)
 SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_rows(gapsSetH, gapsSetV, scoringScheme,
                                                       seqan_amd::detail::IsAlignConfig<std::decay_t<TConfig>>::mode))
+
+template <typename TGapsSetH, typename TGapsSetV, typename TScore,
+          std::enable_if_t<seqan_amd::detail::IsGpuRowsCall<TGapsSetH, TGapsSetV, TScore>::value, int> = 0>
+inline String<int>
+localAlignment(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
+               TGapsSetH && gapsSetH, TGapsSetV && gapsSetV, TScore && scoringScheme)
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_local_rows(gapsSetH, gapsSetV, scoringScheme))
 
 }  // namespace seqan2
 #undef SEQAN_AMD_OVERLOAD_BODY

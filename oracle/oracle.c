@@ -216,12 +216,14 @@ void orc_score_batch(const uint8_t *h_res, const int64_t *h_off,
 /* ------------------------------------------------------------------------ */
 int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
                       const orc_scoring *sc, uint32_t mode, int32_t *counts,
-                      char *ops, int64_t *n_ops)
+                      char *ops, int64_t *n_ops, int64_t *begins)
 {
     const int A = sc->alpha;
+    const int local = (mode & ORC_LOCAL) != 0;
     if (counts) memset(counts, 0, sizeof(int32_t) * (size_t)(A * A + 2));
     if (n_ops) *n_ops = 0;
-    if (n <= 0 || m <= 0 || (mode & ORC_LOCAL)) return INT32_MIN;
+    if (begins) begins[0] = begins[1] = 0;
+    if (n <= 0 || m <= 0) return INT32_MIN;
     const int32_t o = sc->gap_open, e = sc->gap_extend;
     const int64_t R = m + 1;
     uint8_t *T = (uint8_t *)malloc((size_t)(n + 1) * (size_t)R);
@@ -244,7 +246,7 @@ int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
         H[0] = 0; E[0] = NEG_INF; T[0] = TB_NONE;
         TRACK(0, 0, 0, 0, H[0], E[0], F);
         for (int64_t i = 1; i <= m; ++i) {
-            if (mode & ORC_FREE_LEFT) {
+            if (local || (mode & ORC_FREE_LEFT)) {
                 H[i] = 0;
                 T[i] = TB_NONE;
             } else {
@@ -255,7 +257,7 @@ int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
                 H[i] = F;
             }
             E[i] = NEG_INF;
-            TRACK(0, i == m ? 2 : 1, i, 0, H[i], E[i], (mode & ORC_FREE_LEFT) ? NEG_INF : F);
+            TRACK(0, i == m ? 2 : 1, i, 0, H[i], E[i], (local || (mode & ORC_FREE_LEFT)) ? NEG_INF : F);
         }
     }
     for (int64_t j = 1; j <= n; ++j) {
@@ -264,7 +266,7 @@ int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
         const uint8_t hj = h[j - 1];
         int32_t diag = H[0];
         int32_t F = NEG_INF;
-        if (mode & ORC_FREE_TOP) {
+        if (local || (mode & ORC_FREE_TOP)) {
             H[0] = 0;
             Tc[0] = TB_NONE;
         } else {
@@ -290,6 +292,10 @@ int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
             /* H = max(diag, max(F,E)): diagonal is the left argument. */
             int32_t x;
             if (d < fe) { x = fe; tv |= tfe; } else { x = d; tv |= TB_DIAG; }
+            /* Local: a cell that does not beat 0 becomes 0 and its whole
+             * trace value NONE (dp_formula_affine.h:110-118, the single-
+             * trace _maxScore of dp_formula.h:181-196). */
+            if (local && !(0 < x)) { x = 0; tv = TB_NONE; }
             Tc[i] = tv;
             H[i] = x;
             up = x;
@@ -314,9 +320,12 @@ int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
     /* Walk back, emitting ops in reverse order. */
     char *rev = (char *)malloc((size_t)(n + m + 2));
     int64_t nr = 0;
-    /* Tail: cells after the best cell are free end gaps (global only). */
-    for (int64_t i = m; i > bi; --i) rev[nr++] = 'V';
-    for (int64_t j = n; j > bj; --j) rev[nr++] = 'H';
+    /* Tail: cells after the best cell are free end gaps (global only;
+     * TraceTail_ is false for local, dp_profile.h:482-483). */
+    if (!local) {
+        for (int64_t i = m; i > bi; --i) rev[nr++] = 'V';
+        for (int64_t j = n; j > bj; --j) rev[nr++] = 'H';
+    }
     int64_t ci = bi, cj = bj;
 #define TV(J, I) T[(J) * R + (I)]
     while (!(cj <= 0 || ci <= 0) && tv != TB_NONE) {
@@ -355,13 +364,21 @@ int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
         }
     }
 #undef TV
-    /* Head: leading gaps up to the origin. */
-    for (int64_t i = ci; i > 0; --i) rev[nr++] = 'V';
-    for (int64_t j = cj; j > 0; --j) rev[nr++] = 'H';
+    /* Head: leading gaps up to the origin (global only, TraceHead_,
+     * dp_profile.h:491-492).  Local: the window starts where the walk
+     * stopped (the begin positions of _adaptTraceSegmentsTo,
+     * dp_traceback_adaptor.h:87-93); no window at all when it is empty. */
+    if (!local) {
+        for (int64_t i = ci; i > 0; --i) rev[nr++] = 'V';
+        for (int64_t j = cj; j > 0; --j) rev[nr++] = 'H';
+    } else if (begins && nr > 0) {
+        begins[0] = cj;
+        begins[1] = ci;
+    }
 
     /* computeAlignmentStats: walk the rows left to right. */
     if (counts) {
-        int64_t hi = 0, vi = 0;
+        int64_t hi = local ? cj : 0, vi = local ? ci : 0;
         int open0 = 0, open1 = 0;
         for (int64_t k = nr - 1; k >= 0; --k) {
             const char op = rev[k];
@@ -401,7 +418,7 @@ void orc_traceback_batch(const uint8_t *h_res, const int64_t *h_off,
     for (int64_t k = 0; k < count; ++k)
         scores[k] = orc_traceback(h_res + h_off[k], h_off[k + 1] - h_off[k],
                                   v_res + v_off[k], v_off[k + 1] - v_off[k], sc, mode,
-                                  counts ? counts + k * stride : NULL, NULL, NULL);
+                                  counts ? counts + k * stride : NULL, NULL, NULL, NULL);
     (void)threads;
 }
 
@@ -409,7 +426,7 @@ void orc_rows_batch(const uint8_t *h_res, const int64_t *h_off,
                     const uint8_t *v_res, const int64_t *v_off, int64_t count,
                     const orc_scoring *sc, uint32_t mode, int32_t *scores,
                     int32_t *counts, char *ops, const int64_t *op_off, int64_t *op_len,
-                    int threads)
+                    int64_t *begins, int threads)
 {
     const int64_t stride = (int64_t)sc->alpha * sc->alpha + 2;
 #ifdef _OPENMP
@@ -419,7 +436,8 @@ void orc_rows_batch(const uint8_t *h_res, const int64_t *h_off,
     for (int64_t k = 0; k < count; ++k)
         scores[k] = orc_traceback(h_res + h_off[k], h_off[k + 1] - h_off[k],
                                   v_res + v_off[k], v_off[k + 1] - v_off[k], sc, mode,
-                                  counts ? counts + k * stride : NULL, ops + op_off[k], op_len + k);
+                                  counts ? counts + k * stride : NULL, ops + op_off[k], op_len + k,
+                                  begins ? begins + 2 * k : NULL);
     (void)threads;
 }
 

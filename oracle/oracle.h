@@ -52,11 +52,13 @@ void orc_score_batch(const uint8_t *h_res, const int64_t *h_off,
  * number of gap opens and of gap extensions (computeAlignmentStats semantics).
  * ops (optional, capacity n+m) receives the path from start to end as
  * 'M' (diagonal), 'H' (horizontal: H residue vs gap) and 'V' (vertical:
- * gap vs V residue); *n_ops its length.  Only global modes (no LOCAL) and
- * sizes whose (n+1)(m+1) trace matrix fits in memory.  Returns the score. */
+ * gap vs V residue); *n_ops its length.  With ORC_LOCAL the ops are those of
+ * the local window and begins (optional, 2 values) receives the positions in
+ * H and in V where it starts (0, 0 for an empty window).  Sizes whose
+ * (n+1)(m+1) trace matrix fits in memory.  Returns the score. */
 int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
                       const orc_scoring *sc, uint32_t mode, int32_t *counts,
-                      char *ops, int64_t *n_ops);
+                      char *ops, int64_t *n_ops, int64_t *begins);
 
 /* Banded global score: cells with lower <= j - i <= upper only; band
  * settings the reference rejects give INT32_MIN. */
@@ -70,12 +72,13 @@ void orc_traceback_batch(const uint8_t *h_res, const int64_t *h_off,
                          int32_t *counts, int threads);
 
 /* Batch form returning the rows too: pair k's ops (as orc_traceback) at
- * ops + op_off[k] (capacity >= n + m + 2), their number in op_len[k]. */
+ * ops + op_off[k] (capacity >= n + m + 2), their number in op_len[k], its
+ * local window's begin positions at begins + 2k (optional). */
 void orc_rows_batch(const uint8_t *h_res, const int64_t *h_off,
                     const uint8_t *v_res, const int64_t *v_off, int64_t count,
                     const orc_scoring *sc, uint32_t mode, int32_t *scores,
                     int32_t *counts, char *ops, const int64_t *op_off, int64_t *op_len,
-                    int threads);
+                    int64_t *begins, int threads);
 
 /* Exhaustive maximum over all alignments (tiny inputs only, n, m <= 8). */
 int32_t orc_brute_force(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,

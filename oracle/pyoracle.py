@@ -54,13 +54,13 @@ def lib():
                                       ctypes.c_uint32, i32p, ctypes.c_int]
         L.orc_traceback.restype = ctypes.c_int32
         L.orc_traceback.argtypes = [u8p, ctypes.c_int64, u8p, ctypes.c_int64, sp, ctypes.c_uint32,
-                                    i32p, ctypes.c_char_p, i64p]
+                                    i32p, ctypes.c_char_p, i64p, i64p]
         L.orc_traceback_batch.restype = None
         L.orc_traceback_batch.argtypes = [u8p, i64p, u8p, i64p, ctypes.c_int64, sp,
                                           ctypes.c_uint32, i32p, i32p, ctypes.c_int]
         L.orc_rows_batch.restype = None
         L.orc_rows_batch.argtypes = [u8p, i64p, u8p, i64p, ctypes.c_int64, sp, ctypes.c_uint32, i32p, i32p,
-                                     u8p, i64p, i64p, ctypes.c_int]
+                                     u8p, i64p, i64p, i64p, ctypes.c_int]
         L.orc_score_banded.restype = ctypes.c_int32
         L.orc_score_banded.argtypes = [u8p, ctypes.c_int64, u8p, ctypes.c_int64, sp, ctypes.c_uint32,
                                        ctypes.c_int64, ctypes.c_int64]
@@ -110,16 +110,19 @@ def brute_force(h, v, sc: Scoring, mode: int) -> int:
                                      len(v), ctypes.byref(sc._c), mode))
 
 
-def traceback(h, v, sc: Scoring, mode: int = 0):
-    """Returns (score, counts[alpha*alpha+2], ops string)."""
+def traceback(h, v, sc: Scoring, mode: int = 0, with_begins: bool = False):
+    """Returns (score, counts[alpha*alpha+2], ops string), plus the local
+    window's begin positions (in H, in V) with with_begins."""
     h, v = _u8(h), _u8(v)
     counts = np.zeros(sc.alpha * sc.alpha + 2, dtype=np.int32)
     buf = ctypes.create_string_buffer(len(h) + len(v) + 2)
     nops = ctypes.c_int64(0)
+    begins = np.zeros(2, dtype=np.int64)
     s = lib().orc_traceback(_ptr(h, ctypes.c_uint8), len(h), _ptr(v, ctypes.c_uint8), len(v),
                             ctypes.byref(sc._c), mode, _ptr(counts, ctypes.c_int32), buf,
-                            ctypes.byref(nops))
-    return int(s), counts, buf.raw[:nops.value].decode()
+                            ctypes.byref(nops), _ptr(begins, ctypes.c_int64))
+    out = (int(s), counts, buf.raw[:nops.value].decode())
+    return out + ((int(begins[0]), int(begins[1])),) if with_begins else out
 
 
 def score_banded(h, v, sc: Scoring, mode: int, lower: int, upper: int) -> int:
@@ -140,10 +143,12 @@ def score_batch(h_res, h_off, v_res, v_off, sc: Scoring, mode: int, threads: int
     return out
 
 
-def rows_batch(h_res, h_off, v_res, v_off, sc: Scoring, mode: int = 0, threads: int = 0):
+def rows_batch(h_res, h_off, v_res, v_off, sc: Scoring, mode: int = 0, threads: int = 0,
+               with_begins: bool = False):
     """Scores, counts and rows of every pair: returns (scores, counts,
     op_off, ops) with pair k's ops ('M', 'V', 'H' bytes, left to right) in
-    ops[op_off[k]:op_off[k + 1]]."""
+    ops[op_off[k]:op_off[k + 1]]; with_begins adds begins [count, 2], the
+    positions in H and V where each local window starts."""
     h_res, v_res = _u8(h_res), _u8(v_res)
     h_off = np.ascontiguousarray(h_off, dtype=np.int64)
     v_off = np.ascontiguousarray(v_off, dtype=np.int64)
@@ -154,15 +159,16 @@ def rows_batch(h_res, h_off, v_res, v_off, sc: Scoring, mode: int = 0, threads: 
     counts = np.zeros((n, sc.alpha * sc.alpha + 2), dtype=np.int32)
     buf = np.zeros(int(cap[-1]), dtype=np.uint8)
     lens = np.zeros(n, dtype=np.int64)
+    begins = np.zeros((n, 2), dtype=np.int64)
     lib().orc_rows_batch(_ptr(h_res, ctypes.c_uint8), _ptr(h_off, ctypes.c_int64),
                          _ptr(v_res, ctypes.c_uint8), _ptr(v_off, ctypes.c_int64), n,
                          ctypes.byref(sc._c), mode, _ptr(scores, ctypes.c_int32),
                          _ptr(counts, ctypes.c_int32), _ptr(buf, ctypes.c_uint8), _ptr(cap, ctypes.c_int64),
-                         _ptr(lens, ctypes.c_int64), threads)
+                         _ptr(lens, ctypes.c_int64), _ptr(begins, ctypes.c_int64), threads)
     op_off = np.zeros(n + 1, dtype=np.int64)
     op_off[1:] = np.cumsum(lens)
     ops = np.concatenate([buf[cap[k]:cap[k] + lens[k]] for k in range(n)]) if n else np.zeros(0, np.uint8)
-    return scores, counts, op_off, ops
+    return (scores, counts, op_off, ops, begins) if with_begins else (scores, counts, op_off, ops)
 
 
 def traceback_batch(h_res, h_off, v_res, v_off, sc: Scoring, mode: int = 0, threads: int = 0):
@@ -209,7 +215,8 @@ def run_reference(op, h_res, h_off, v_res, v_off, mode, *, alphabet="dna", matri
                   match=0, mismatch=0, gap_extend=0, gap_open=0, threads=0, repeats=1,
                   timeout=3600, lower_diag=0, upper_diag=0):
     """Runs the reference library on a batch.  Returns a dict with scores,
-    counts (ops 1/2), best_seconds, simd_lanes, threads."""
+    counts, op_off and ops (ops 1/2; begins [count, 2] too with mode LOCAL),
+    best_seconds, simd_lanes, threads."""
     exe = ref_binary()
     if exe is None:
         raise FileNotFoundError("oracle/_ref is not built")
@@ -247,6 +254,9 @@ def run_reference(op, h_res, h_off, v_res, v_off, mode, *, alphabet="dna", matri
         out["op_off"] = np.frombuffer(raw, dtype=np.int64, count=cnt + 1, offset=off).copy()
         off += 8 * (cnt + 1)
         out["ops"] = np.frombuffer(raw, dtype=np.uint8, count=int(out["op_off"][-1]), offset=off).copy()
+        off += int(out["op_off"][-1])
+        if mode & 16:
+            out["begins"] = np.frombuffer(raw, dtype=np.int64, count=2 * cnt, offset=off).reshape(cnt, 2).copy()
     return out
 
 

@@ -14,6 +14,10 @@
 //       include/seqan/align_parallel/parallel_align_interface.h:288-299
 //   globalAlignment(Gaps, Gaps, score)   (scalar, for very long pairs)
 //       include/seqan/align/global_alignment_unbanded.h:281-306
+//   localAlignment(ExecutionPolicy<Parallel,Vectorial>, gapsH, gapsV, score)
+//       include/seqan/align_parallel/parallel_align_interface.h:312-321
+//   localAlignment(Gaps, Gaps, score)    (scalar)
+//       include/seqan/align/local_alignment_unbanded.h:262-273
 //   computeAlignmentStats(stats, row0, row1, score)
 //       include/seqan/align/evaluate_alignment.h:215-300
 //
@@ -30,7 +34,9 @@
 //   i32 scores[count], then for ops 1/2: i32 counts[count][A*A+2],
 //   i64 op_off[count+1], u8 ops[op_off[count]]: the rows of each alignment
 //   as one byte per column, 'M' (both rows hold a residue), 'V' (a gap in
-//   the H row) or 'H' (a gap in the V row).
+//   the H row) or 'H' (a gap in the V row); with mode LOCAL the rows are
+//   those of the clipped local window, followed by i64 begins[count][2]:
+//   the source positions in H and in V where the window starts.
 #include <seqan/align.h>
 #include <seqan/align_parallel.h>
 #include <seqan/score.h>
@@ -223,6 +229,7 @@ int run(RefInput const & in, TScore const & sc, int alpha, std::FILE * out)
     std::vector<int32_t> counts;
     std::vector<int64_t> opOff;
     std::vector<uint8_t> ops;
+    std::vector<int64_t> begins;
     double best = 1e300;
     if (h.op == 4)
     {
@@ -268,8 +275,9 @@ int run(RefInput const & in, TScore const & sc, int alpha, std::FILE * out)
     }
     else if (h.op == 1 || h.op == 2)
     {
-        if (h.mode & 16u)
-            die("counts are produced for global alignment (AlignConfig) only");
+        bool const local = (h.mode & 16u) != 0;
+        if (local && (h.mode & 15u))
+            die("LOCAL takes no free end gaps");
         counts.assign(h.count * (alpha * alpha + 2), 0);
         using TGaps = Gaps<String<TAlphabet>>;
         StringSet<TGaps> gH, gV;
@@ -283,14 +291,16 @@ int run(RefInput const & in, TScore const & sc, int alpha, std::FILE * out)
         auto t0 = std::chrono::steady_clock::now();
         if (h.op == 1)
         {
-            String<int> res = withConfig(h.mode, [&](auto cfg) { return globalAlignment(ex, gH, gV, sc, cfg); });
+            String<int> res = local ? localAlignment(ex, gH, gV, sc)
+                                    : withConfig(h.mode, [&](auto cfg) { return globalAlignment(ex, gH, gV, sc, cfg); });
             for (int64_t k = 0; k < h.count; ++k)
                 scores[k] = res[k];
         }
         else
         {
             for (int64_t k = 0; k < h.count; ++k)
-                scores[k] = withConfig(h.mode, [&](auto cfg) { return globalAlignment(gH[k], gV[k], sc, cfg); });
+                scores[k] = local ? localAlignment(gH[k], gV[k], sc)
+                                  : withConfig(h.mode, [&](auto cfg) { return globalAlignment(gH[k], gV[k], sc, cfg); });
         }
         auto t1 = std::chrono::steady_clock::now();
         best = std::chrono::duration<double>(t1 - t0).count();
@@ -303,6 +313,11 @@ int run(RefInput const & in, TScore const & sc, int alpha, std::FILE * out)
             for (; it0 != end(gH[k]); ++it0, ++it1)
                 ops.push_back(isGap(it0) ? 'V' : isGap(it1) ? 'H' : 'M');
             opOff.push_back(static_cast<int64_t>(ops.size()));
+            if (local)
+            {
+                begins.push_back(static_cast<int64_t>(beginPosition(gH[k])));
+                begins.push_back(static_cast<int64_t>(beginPosition(gV[k])));
+            }
         }
     }
     else
@@ -319,6 +334,7 @@ int run(RefInput const & in, TScore const & sc, int alpha, std::FILE * out)
         std::fwrite(counts.data(), sizeof(int32_t), counts.size(), out);
         std::fwrite(opOff.data(), sizeof(int64_t), opOff.size(), out);
         std::fwrite(ops.data(), 1, ops.size(), out);
+        std::fwrite(begins.data(), sizeof(int64_t), begins.size(), out);
     }
     return 0;
 }

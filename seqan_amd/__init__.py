@@ -71,11 +71,12 @@ class Stats(ctypes.Structure):
 
 
 # SQMI_ABI_VERSION of include/seqan_amd.h that the structures below mirror.
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # Every symbol the C ABI declares (include/seqan_amd.h).
 ABI_SYMBOLS = (
     "sqmi_create", "sqmi_destroy", "sqmi_set_stream", "sqmi_align_score", "sqmi_align_counts", "sqmi_align_rows",
+    "sqmi_align_local_rows",
     "sqmi_get_stats", "sqmi_plan_stats", "sqmi_blosum62", "sqmi_status_string", "sqmi_last_error", "sqmi_abi_version",
     "sqmi_workload_create", "sqmi_workload_sizes", "sqmi_workload_copy", "sqmi_workload_destroy",
 )
@@ -102,6 +103,8 @@ def lib():
                                         ctypes.POINTER(_Scoring), ctypes.c_uint32, vp, vp, ctypes.c_uint32]
         L.sqmi_align_rows.argtypes = [vp, ctypes.POINTER(_Sequences), ctypes.POINTER(_Sequences),
                                       ctypes.POINTER(_Scoring), ctypes.c_uint32, vp, vp, vp, vp, vp, ctypes.c_uint32]
+        L.sqmi_align_local_rows.argtypes = [vp, ctypes.POINTER(_Sequences), ctypes.POINTER(_Sequences),
+                                            ctypes.POINTER(_Scoring), vp, vp, vp, vp, vp, vp, ctypes.c_uint32]
         L.sqmi_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.sqmi_plan_stats.argtypes = [ctypes.POINTER(_Sequences), ctypes.POINTER(_Sequences),
                                       ctypes.POINTER(_Scoring), ctypes.c_uint32, ctypes.c_uint32,
@@ -350,10 +353,39 @@ class Context:
                                      scores.ctypes.data, counts.ctypes.data, buf.ctypes.data, seg.ctypes.data,
                                      lens.ctypes.data, flags)
         self._check(st, "sqmi_align_rows")
-        op_off = np.zeros(n + 1, np.int64)
-        op_off[1:] = np.cumsum(lens)
-        ops = np.concatenate([buf[seg[i + 1] - lens[i]:seg[i + 1]] for i in range(n)]) if n else buf[:0]
-        return scores, counts, op_off, ops
+        return scores, counts, *_compact_rows(buf, seg, lens)
+
+    def align_local_rows(self, set_h: SeqSet, set_v: SeqSet, score: Score, flags=0):
+        """Local alignment: scores, counts and the rows of every local window.
+        Returns (scores, counts, op_off, ops, begins) with rows as in
+        align_rows and begins[i] = (first H position, first V position) of
+        pair i's window."""
+        ch, keep_h = _seqs(set_h)
+        cv, keep_v = _seqs(set_v)
+        sc = score._c()
+        n = len(set_h.offsets) - 1
+        A = score.alphabet_size
+        scores = np.empty(n, np.int32)
+        counts = np.empty((n, A * A + 2), np.int32)
+        seg = np.zeros(n + 1, np.int64)
+        seg[1:] = np.cumsum(set_h.lengths() + set_v.lengths())
+        buf = np.empty(max(int(seg[-1]), 1), np.uint8)
+        lens = np.zeros(n, np.int64)
+        begins = np.zeros((n, 2), np.int64)
+        st = self._L.sqmi_align_local_rows(self._h, ctypes.byref(ch), ctypes.byref(cv), ctypes.byref(sc),
+                                           scores.ctypes.data, counts.ctypes.data, buf.ctypes.data,
+                                           seg.ctypes.data, lens.ctypes.data, begins.ctypes.data, flags)
+        self._check(st, "sqmi_align_local_rows")
+        return (scores, counts, *_compact_rows(buf, seg, lens), begins)
+
+
+def _compact_rows(buf, seg, lens):
+    """(op_off, ops) from rows written to the ends of the segments seg."""
+    n = len(lens)
+    op_off = np.zeros(n + 1, np.int64)
+    op_off[1:] = np.cumsum(lens)
+    ops = np.concatenate([buf[seg[i + 1] - lens[i]:seg[i + 1]] for i in range(n)]) if n else buf[:0]
+    return op_off, ops
 
 
 # ---------------------------------------------------------------------------
@@ -461,3 +493,19 @@ def local_alignment_score(set_h, set_v, score: Score, alphabet=AMINO_ACIDS, ctx:
     """localAlignmentScore(ExecutionPolicy<Parallel,Vectorial>, setH, setV, score)."""
     ctx = ctx or default_context()
     return ctx.align_score(_as_set(set_h, alphabet), _as_set(set_v, alphabet), score, LOCAL)
+
+
+def global_alignment(set_h, set_v, score: Score, align_config=(False, False, False, False), alphabet=DNA,
+                     ctx: Context = None):
+    """globalAlignment(ExecutionPolicy<Parallel,Vectorial>, gapsSetH, gapsSetV, score, AlignConfig):
+    (scores, counts, op_off, ops), rows as Context.align_rows."""
+    ctx = ctx or default_context()
+    return ctx.align_rows(_as_set(set_h, alphabet), _as_set(set_v, alphabet), score,
+                          _mode_from_config(align_config))
+
+
+def local_alignment(set_h, set_v, score: Score, alphabet=AMINO_ACIDS, ctx: Context = None):
+    """localAlignment(ExecutionPolicy<Parallel,Vectorial>, gapsSetH, gapsSetV, score):
+    (scores, counts, op_off, ops, begins), as Context.align_local_rows."""
+    ctx = ctx or default_context()
+    return ctx.align_local_rows(_as_set(set_h, alphabet), _as_set(set_v, alphabet), score)

@@ -53,8 +53,8 @@ struct sqmi_context
     DevBuf trace, traceOff, locs, endEF, counts;
     // Long pairs (tiled int32 kernels).
     DevBuf longCk, longPairs, longWalk, longTrace;
-    // Alignment rows (sqmi_align_rows).
-    DevBuf ops, opOff, opLen;
+    // Alignment rows (sqmi_align_rows, sqmi_align_local_rows).
+    DevBuf ops, opOff, opLen, begins;
 };
 
 namespace {
@@ -621,6 +621,7 @@ struct RowsDev {
     uint8_t *ops = nullptr;
     const int64_t *opOff = nullptr;
     int64_t *opLen = nullptr;
+    int64_t *begins = nullptr;   // local windows only
 };
 
 sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV, const Planned &P,
@@ -790,7 +791,7 @@ sqmi_status sqmi_destroy(sqmi_context *ctx)
                       &ctx->rowbuf, &ctx->scores, &ctx->tTable, &ctx->dnaProf, &ctx->ids32, &ctx->scratchOff32,
                       &ctx->scratch32, &ctx->table32, &ctx->emptyIds, &ctx->trace, &ctx->traceOff, &ctx->locs,
                       &ctx->endEF, &ctx->counts, &ctx->longCk, &ctx->longPairs, &ctx->longWalk, &ctx->longTrace,
-                      &ctx->ops, &ctx->opOff, &ctx->opLen})
+                      &ctx->ops, &ctx->opOff, &ctx->opLen, &ctx->begins})
         release(*b);
     if (ctx->ev0)
         (void)hipEventDestroy(ctx->ev0);
@@ -1062,11 +1063,12 @@ sqmi_status sqmi_plan_stats(const sqmi_sequences *setH, const sqmi_sequences *se
 // Traceback: the packed forward pass writes 4 trace bits per cell into a
 // buffer of at most traceBudget() bytes, so the work items run in chunks;
 // after each chunk one thread per pair walks its path.  Pairs outside the
-// int16 window run the int32 kernel with one trace byte per cell, chunked by
-// the same budget.
+// int16 window, and every pair of a free-end or local alignment, run the
+// int32 kernel with one trace byte per cell, chunked by the same budget.
 static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
                                  const sqmi_scoring *scoring, uint32_t mode, int32_t *scores, int32_t *counts,
-                                 uint8_t *ops, const int64_t *opOffsets, int64_t *opLengths, uint32_t flags)
+                                 uint8_t *ops, const int64_t *opOffsets, int64_t *opLengths, int64_t *begins,
+                                 uint32_t flags)
 {
     const auto tStart = std::chrono::steady_clock::now();
     const bool wantRows = ops != nullptr;
@@ -1077,8 +1079,6 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
     sqmi_status st;
     if ((st = validate(ctx, setH, setV, scoring, mode)) != SQMI_OK)
         return st;
-    if (mode & SQMI_LOCAL)
-        return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_counts / sqmi_align_rows: global alignment only (no SQMI_LOCAL)");
     if (flags & (SQMI_PLAIN_PER_PAIR | 0x00FFFF00u))
         return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_counts: no plain form or forced shapes");
     const int64_t N = setH->count;
@@ -1197,6 +1197,17 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
         }
         if ((err = hipMemsetAsync(rows.opLen, 0, 8 * N, stream)) != hipSuccess)
             return hipFail(ctx, err, "clearing op lengths");
+        if (begins) {
+            if (flags & SQMI_SCORES_ON_DEVICE) {
+                rows.begins = begins;
+            } else {
+                if ((err = ensure(ctx->begins, 16 * N)) != hipSuccess)
+                    return hipFail(ctx, err, "allocating begins");
+                rows.begins = static_cast<int64_t *>(ctx->begins.p);
+            }
+            if ((err = hipMemsetAsync(rows.begins, 0, 16 * N, stream)) != hipSuccess)
+                return hipFail(ctx, err, "clearing begins");
+        }
     }
     if ((err = hipEventRecord(ctx->ev0, stream)) != hipSuccess)
         return hipFail(ctx, err, "event");
@@ -1312,6 +1323,7 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             a.ops = rows.ops;
             a.opOff = rows.opOff;
             a.opLen = rows.opLen;
+            a.begins = rows.begins;
             if ((err = launchCounts32(a, stream)) != hipSuccess)
                 return hipFail(ctx, err, "launching int32 traceback kernel");
             b = e;
@@ -1339,7 +1351,9 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
         if (wantRows &&
             ((err = hipMemcpyAsync(ops + opOffsets[0], rows.ops + opOffsets[0], (size_t)(opOffsets[N] - opOffsets[0]),
                                    hipMemcpyDeviceToHost, stream)) != hipSuccess ||
-             (err = hipMemcpyAsync(opLengths, rows.opLen, 8 * N, hipMemcpyDeviceToHost, stream)) != hipSuccess))
+             (err = hipMemcpyAsync(opLengths, rows.opLen, 8 * N, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+             (begins && (err = hipMemcpyAsync(begins, rows.begins, 16 * N, hipMemcpyDeviceToHost, stream)) !=
+                            hipSuccess)))
             return hipFail(ctx, err, "copying rows");
     }
     if ((err = hipStreamSynchronize(stream)) != hipSuccess)
@@ -1355,7 +1369,7 @@ sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *setH, con
                               const sqmi_scoring *scoring, uint32_t mode, int32_t *scores, int32_t *counts,
                               uint32_t flags)
 {
-    return alignBackward(ctx, setH, setV, scoring, mode, scores, counts, nullptr, nullptr, nullptr, flags);
+    return alignBackward(ctx, setH, setV, scoring, mode, scores, counts, nullptr, nullptr, nullptr, nullptr, flags);
 }
 
 sqmi_status sqmi_align_rows(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
@@ -1364,7 +1378,21 @@ sqmi_status sqmi_align_rows(sqmi_context *ctx, const sqmi_sequences *setH, const
 {
     if (!ops)
         return ctx ? fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "ops is null") : SQMI_ERR_INVALID_ARGUMENT;
-    return alignBackward(ctx, setH, setV, scoring, mode, scores, counts, ops, op_offsets, op_lengths, flags);
+    if (mode & SQMI_LOCAL)
+        return ctx ? fail(ctx, SQMI_ERR_UNSUPPORTED,
+                          "sqmi_align_rows: global modes only; local rows come from sqmi_align_local_rows")
+                   : SQMI_ERR_UNSUPPORTED;
+    return alignBackward(ctx, setH, setV, scoring, mode, scores, counts, ops, op_offsets, op_lengths, nullptr, flags);
+}
+
+sqmi_status sqmi_align_local_rows(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
+                                  const sqmi_scoring *scoring, int32_t *scores, int32_t *counts, uint8_t *ops,
+                                  const int64_t *op_offsets, int64_t *op_lengths, int64_t *begins, uint32_t flags)
+{
+    if (!ops || !begins)
+        return ctx ? fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "ops or begins is null") : SQMI_ERR_INVALID_ARGUMENT;
+    return alignBackward(ctx, setH, setV, scoring, SQMI_LOCAL, scores, counts, ops, op_offsets, op_lengths, begins,
+                         flags);
 }
 
 }  // extern "C"

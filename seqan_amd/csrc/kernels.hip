@@ -721,6 +721,9 @@ __global__ void kFillScores(int32_t *scores, const int32_t *ids, int n, int32_t 
 // counts can be taken while walking backwards.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kBitEOpen = 1, kBitFOpen = 2, kBitNotDiag = 4, kBitHori = 8;
+// Local alignment (byte trace only): the cell did not beat 0, so its trace
+// value is NONE and a walk that reaches it stops there.
+constexpr uint32_t kBitStop = 16;
 
 __device__ __forceinline__ int dirOf(uint32_t bits)   // 0 diagonal, 1 vertical, 2 horizontal
 {
@@ -730,11 +733,19 @@ __device__ __forceinline__ int dirOf(uint32_t bits)   // 0 diagonal, 1 vertical,
 template <class Bits>
 __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const uint8_t *v, int A, int score, int endE,
                                            int endF, Bits &bits, int32_t *cnt, uint8_t *opEnd = nullptr,
-                                           int64_t *opLen = nullptr, int i0 = -1, int j0 = -1)
+                                           int64_t *opLen = nullptr, int i0 = -1, int j0 = -1,
+                                           bool local = false, int64_t *begins = nullptr)
 {
     // The walk starts at the end cell (i0, j0), (m, n) unless free end gaps
-    // moved it; the cells after it are trailing gaps of the rows.
+    // or a local alignment moved it; in global modes the cells after it are
+    // trailing gaps of the rows.  A local walk stops at the first cell whose
+    // trace value is NONE (kBitStop) and emits neither trailing nor leading
+    // gaps (TraceTail_ / TraceHead_, dp_profile.h:482-492).
     int i = i0 < 0 ? m : i0, j = j0 < 0 ? n : j0;
+    if (local) {
+        m = i;
+        n = j;
+    }
     int prev = 0;   // last emitted operation: 0 none / aligned pair, 1 gap in H (vertical), 2 gap in V
     int opens = 0, exts = 0;
     // Rows: one byte per alignment column, written backwards from opEnd
@@ -760,11 +771,14 @@ __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const
     // (_correctTraceValue, dp_algorithm_impl.h:1190-1207), and a gap start
     // there follows the gap matrix (PreferGapsAtEnd_, dp_traceback_impl.h:115-122).
     int dir;
+    bool walk = true;
     {
-        const uint32_t b0 = bits(i, j);
+        const uint32_t b0 = (i > 0 && j > 0) ? bits(i, j) : kBitStop;
         dir = (endF == score) ? 1 : (endE == score) ? 2 : dirOf(b0);
+        if (local && (b0 & kBitStop))
+            walk = false;
     }
-    while (i > 0 && j > 0) {
+    while (walk && i > 0 && j > 0) {
         if (dir == 0) {
             const int hc = min((int)h[j - 1], A - 1), vc = min((int)v[i - 1], A - 1);
             cnt[hc * A + vc] += 1;
@@ -795,8 +809,21 @@ __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const
                     break;
             }
         }
-        if (i > 0 && j > 0)
-            dir = dirOf(bits(i, j));
+        if (i > 0 && j > 0) {
+            const uint32_t b = bits(i, j);
+            dir = dirOf(b);
+            if (local && (b & kBitStop))
+                break;
+        }
+    }
+    if (local) {
+        // The window starts where the walk stopped (dp_traceback_adaptor.h:
+        // 87-93); an empty window keeps both begins at 0.
+        if (begins && nOps > 0) {
+            begins[0] = j;
+            begins[1] = i;
+        }
+        i = j = 0;
     }
     // Leading gaps down to the origin: first the column, then the row
     // (dp_traceback_impl.h:522-545).
@@ -902,12 +929,17 @@ __global__ __launch_bounds__(256) void kCounts32(Counts32Args a)
     // column is 0; the result is the first strict maximum over the tracked
     // cells in column order: (m, n), the last row if it is free, the last
     // column if it is free (dp_scout.h:169-210, dp_meta_info.h:178-221).
-    const bool freeTop = (a.mode & SQMI_FREE_TOP) != 0, freeLeft = (a.mode & SQMI_FREE_LEFT) != 0;
-    const bool freeRight = (a.mode & SQMI_FREE_RIGHT) != 0, freeBottom = (a.mode & SQMI_FREE_BOTTOM) != 0;
+    // Local: both borders are 0, a cell that does not beat 0 is 0 with trace
+    // NONE (dp_formula_affine.h:110-118), and every cell is tracked
+    // (dp_scout_simd.h:427-440).
+    const bool local = (a.mode & SQMI_LOCAL) != 0;
+    const bool freeTop = local || (a.mode & SQMI_FREE_TOP) != 0, freeLeft = local || (a.mode & SQMI_FREE_LEFT) != 0;
+    const bool freeRight = local || (a.mode & SQMI_FREE_RIGHT) != 0;
+    const bool freeBottom = local || (a.mode & SQMI_FREE_BOTTOM) != 0;
     int32_t best = kNeg32, bestE = kNeg32, bestF = kNeg32;
     int bi = m, bj = n;
     auto track = [&](bool lastCol, int i, int j, int32_t hv, int32_t ev, int32_t fv) {
-        const bool tracked = (i == m) ? (lastCol || freeBottom) : (lastCol && freeRight);
+        const bool tracked = local || ((i == m) ? (lastCol || freeBottom) : (lastCol && freeRight));
         if (tracked && hv > best) {
             best = hv;
             bestE = ev;
@@ -961,6 +993,10 @@ __global__ __launch_bounds__(256) void kCounts32(Counts32Args a)
                 x = fe;
                 tb |= kBitNotDiag | (F < ei ? kBitHori : 0u);
             }
+            if (local && !(0 < x)) {
+                x = 0;
+                tb = kBitStop;
+            }
             Tc[i] = (uint8_t)tb;
             H[i] = x;
             up = x;
@@ -970,7 +1006,7 @@ __global__ __launch_bounds__(256) void kCounts32(Counts32Args a)
     a.scores[pid] = best;
     auto bits = [&](int i, int j) -> uint32_t { return T[(int64_t)j * CS + i]; };
     walkCounts(n, m, h, v, A, best, bestE, bestF, bits, cnt, a.ops ? a.ops + a.opOff[pid + 1] : nullptr,
-               a.ops ? a.opLen + pid : nullptr, bi, bj);
+               a.ops ? a.opLen + pid : nullptr, bi, bj, local, a.begins ? a.begins + 2 * pid : nullptr);
 }
 
 template <int G, int R, bool DNA>

@@ -102,12 +102,13 @@ struct Counts32Args {
     int32_t *counts;
     int32_t gapOpen;
     int32_t gapExtend;
-    uint32_t mode;              // free end gaps (SQMI_FREE_*); no local
+    uint32_t mode;              // free end gaps (SQMI_FREE_*) or SQMI_LOCAL
     int32_t alpha;
     const int32_t *table;
     uint8_t *ops;               // rows, as in WalkArgs (optional)
     const int64_t *opOff;
     int64_t *opLen;
+    int64_t *begins;            // local: per pair the window's first H and V position (optional)
 };
 
 // Simple int32 kernel: one thread per pair, H/E column in global scratch.

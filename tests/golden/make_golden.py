@@ -4,6 +4,7 @@ holds the reference tree and the oracle/_ref build):
     make -C oracle && python tests/golden/make_golden.py
     python tests/golden/make_golden.py --long   # the 100 kbp fixture (~25 GB of RAM)
     python tests/golden/make_golden.py --modes  # only the AlignConfig rows fixtures
+    python tests/golden/make_golden.py --local  # only the local rows fixtures
 
 * ``ref_*.npz``   -- inputs (residue ordinals + offsets) and the outputs of
   the reference library itself (oracle/_ref: SeqAn 2 globalAlignmentScore /
@@ -190,6 +191,56 @@ def mode_rows():
              mode_off=mode_off, **kw)
 
 
+def local_pairs(rng, count, lo, hi, alpha):
+    """Pairs whose V embeds a mutated segment of H between random flanks
+    (a third of them unrelated), so that local windows start and end inside
+    both sequences."""
+    hs, vs = [], []
+    for _ in range(count):
+        h = rng.integers(0, alpha, int(rng.integers(lo, hi + 1))).astype(np.uint8)
+        if rng.random() < 0.33:
+            v = rng.integers(0, alpha, int(rng.integers(lo, hi + 1))).astype(np.uint8)
+        else:
+            a = int(rng.integers(0, len(h)))
+            b = int(rng.integers(a, len(h))) + 1
+            seg = h[a:b].copy()
+            f = rng.random(len(seg)) < 0.15
+            seg[f] = rng.integers(0, alpha, int(f.sum()))
+            cut = np.nonzero(rng.random(len(seg)) < 0.03)[0]
+            seg = np.delete(seg, cut)
+            left = rng.integers(0, alpha, int(rng.integers(0, hi // 4 + 1))).astype(np.uint8)
+            right = rng.integers(0, alpha, int(rng.integers(0, hi // 4 + 1))).astype(np.uint8)
+            v = np.concatenate([left, seg, right]).astype(np.uint8)
+            if len(v) == 0:
+                v = np.array([0], np.uint8)
+        if rng.random() < 0.3:
+            h, v = v, h
+        hs.append(h)
+        vs.append(v)
+    return sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+
+
+def local_rows():
+    """Counts, rows and window begins of local alignment (the reference's
+    localAlignment(ExecutionPolicy<Parallel, Vectorial>, gapsH, gapsV,
+    score)); the scalar localAlignment must agree."""
+    rng = np.random.default_rng(0x5EED0200)
+    for name, count, lo, hi, alpha, kw in (
+            ("ref_dna_local_rows.npz", 160, 1, 400, 4, dict(alphabet="dna", **SIMPLE)),
+            ("ref_dna_local_short_rows.npz", 256, 1, 30, 4,
+             dict(alphabet="dna", match=1, mismatch=-1, gap_extend=-1, gap_open=-2)),
+            ("ref_protein_local_rows.npz", 128, 10, 400, 27,
+             dict(alphabet="aa", matrix="blosum62", gap_extend=-1, gap_open=-11))):
+        H, V = local_pairs(rng, count, lo, hi, 24 if alpha == 27 else alpha)
+        r = po.run_reference(po.OP_COUNTS_SIMD, H.residues, H.offsets, V.residues, V.offsets, 16, **kw)
+        q = po.run_reference(po.OP_COUNTS_SCALAR, H.residues, H.offsets, V.residues, V.offsets, 16, **kw)
+        for key in ("scores", "counts", "op_off", "ops", "begins"):
+            assert np.array_equal(r[key], q[key]), (name, key)
+        extra = {k: v for k, v in kw.items() if k not in ("alphabet", "matrix")}
+        save(name, H, V, scores=r["scores"], counts=r["counts"], op_off=r["op_off"], ops=r["ops"],
+             begins=r["begins"], **extra)
+
+
 def long_pairs():
     """100 kbp fixture: the first two pairs of the 64 x 100 kbp workload at
     full length (run on its own: --long).  Scores come from the reference's
@@ -212,6 +263,9 @@ if __name__ == "__main__":
         long_pairs()
     elif "--modes" in sys.argv:
         mode_rows()
+    elif "--local" in sys.argv:
+        local_rows()
     else:
         main()
         mode_rows()
+        local_rows()

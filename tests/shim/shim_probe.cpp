@@ -29,7 +29,12 @@ template <typename, typename... A> struct RowsCallable : std::false_type {};
 template <typename... A>
 struct RowsCallable<void_t<decltype(globalAlignment(std::declval<A>()...))>, A...> : std::true_type {};
 
+template <typename, typename... A> struct LocalRowsCallable : std::false_type {};
+template <typename... A>
+struct LocalRowsCallable<void_t<decltype(localAlignment(std::declval<A>()...))>, A...> : std::true_type {};
+
 template <typename... A> constexpr bool gpuGlobal() { return !GlobalCallable<void, A...>::value; }
+template <typename... A> constexpr bool gpuLocalRows() { return !LocalRowsCallable<void, A...>::value; }
 template <typename... A> constexpr bool gpuRows() { return !RowsCallable<void, A...>::value; }
 template <typename... A> constexpr bool gpuLocal() { return !LocalCallable<void, A...>::value; }
 
@@ -89,6 +94,10 @@ static_assert(gpuRows<ExPV const &, DnaGaps &, DnaGaps &, Simple &>(), "rows, DN
 static_assert(gpuRows<ExPV, DnaGaps &, DnaGaps &, Simple const &>(), "rows, const score");
 static_assert(gpuRows<ExPV const &, AaGaps &, AaGaps &, Blosum62 &>(), "rows, protein");
 static_assert(gpuRows<ExPV const &, DnaGaps &, DnaGaps &, Simple &, Semi &>(), "rows, AlignConfig");
+// localAlignment on StringSets of Gaps (parallel_align_interface.h:312-321).
+static_assert(gpuLocalRows<ExPV const &, DnaGaps &, DnaGaps &, Simple &>(), "local rows, DNA");
+static_assert(gpuLocalRows<ExPV, AaGaps &, AaGaps &, Blosum62 const &>(), "local rows, protein");
+static_assert(!gpuLocalRows<ExPS const &, AaGaps &, AaGaps &, Blosum62 &>(), "local rows, Parallel, Serial");
 
 // Everything else stays with the reference templates.
 static_assert(!gpuRows<ExPS const &, DnaGaps &, DnaGaps &, Simple &>(), "rows, Parallel, Serial");

@@ -125,6 +125,46 @@ void runRowsCase(char const * name, StringSet<String<TAlphabet>> const & setH,
     }
 }
 
+// localAlignment on StringSets of Gaps: scores, both clipped rows and the
+// clipping of every pair against the reference's own localAlignment.
+template <typename TAlphabet, typename TScore, typename TGpu, typename TCpu>
+void runLocalRowsCase(char const * name, StringSet<String<TAlphabet>> const & setH,
+                      StringSet<String<TAlphabet>> const & setV, TScore const & sc, TGpu const & ex, TCpu const & cpu)
+{
+    using TGaps = Gaps<String<TAlphabet> const>;
+    StringSet<TGaps> gH, gV, rH, rV;
+    for (size_t i = 0; i < length(setH); ++i)
+    {
+        if (length(setH[i]) == 0 || length(setV[i]) == 0)
+            continue;
+        appendValue(gH, TGaps(setH[i]));
+        appendValue(gV, TGaps(setV[i]));
+        appendValue(rH, TGaps(setH[i]));
+        appendValue(rV, TGaps(setV[i]));
+    }
+    String<int> expect = localAlignment(cpu, rH, rV, sc);
+    auto same = [](TGaps & a, TGaps & b) {
+        return rowText(a) == rowText(b) && clippedBeginPosition(a) == clippedBeginPosition(b) &&
+               clippedEndPosition(a) == clippedEndPosition(b) && beginPosition(a) == beginPosition(b) &&
+               endPosition(a) == endPosition(b);
+    };
+    try
+    {
+        String<int> got = localAlignment(ex, gH, gV, sc);
+        for (size_t i = 0; i < length(expect); ++i)
+            if (got[i] != expect[i] || !same(gH[i], rH[i]) || !same(gV[i], rV[i]))
+            {
+                std::printf("case %s mismatch pair %zu: %d vs %d\n", name, i, got[i], expect[i]);
+                return;
+            }
+        std::printf("case %s match %zu\n", name, (size_t)length(got));
+    }
+    catch (seqan_amd::Error const & e)
+    {
+        std::printf("case %s %s (%s)\n", name, e.status() == SQMI_ERR_NO_DEVICE ? "nodevice" : "error", e.what());
+    }
+}
+
 }  // namespace
 
 int main()
@@ -203,6 +243,8 @@ int main()
     runRowsCase("protein_rows_blosum62", ph, pv, b62, ex, cpu);
     runRowsCase("dna_rows_semiglobal", dh, dv, dna, ex, cpu, semi);
     runRowsCase("dna_rows_overlap", dh, dv, dna, ex, cpu, overlap);
+    runLocalRowsCase("dna_local_rows", dh, dv, dna, ex, cpu);
+    runLocalRowsCase("protein_local_rows_blosum62", ph, pv, b62, ex, cpu);
 
     // |setH| != |setV| raises instead of asserting.
     StringSet<String<Dna>> shorter = dv;

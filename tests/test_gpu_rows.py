@@ -7,7 +7,9 @@ Checked against the reference's own rows (fixtures written by the compiled
 reference), the oracle's rows (pinned to those fixtures by the CPU tests),
 and what any alignment's rows must satisfy: they hold every residue of both
 sequences in order, and the counts read from them are the counts
-sqmi_align_counts returns."""
+sqmi_align_counts returns.  Local rows (sqmi_align_local_rows, the
+reference's localAlignment on Gaps sets) are checked the same way, with the
+window begins."""
 import os
 
 import numpy as np
@@ -155,7 +157,7 @@ def test_rows_empty_pairs_and_errors(ctx):
     assert list(np.diff(op_off)) == [3, 0, 0, 1]
     assert bytes(ops[op_off[3]:op_off[4]]) == b"M"
     with pytest.raises(sa.SqmiError):
-        ctx.align_rows(Hs, Vs, score, mode=sa.LOCAL)
+        ctx.align_rows(Hs, Vs, score, mode=sa.LOCAL)   # local rows: align_local_rows
 
 
 @pytest.mark.parametrize("fixture", ["ref_dna_modes_rows.npz", "ref_dna_modes_short_rows.npz"])
@@ -201,3 +203,75 @@ def test_rows_align_configs_match_oracle(ctx):
         assert np.array_equal(s, ws) and np.array_equal(c, wc)
         assert np.array_equal(op_off, woff) and np.array_equal(ops, wops)
         assert np.array_equal(_counts_from_rows(hs, vs, op_off, ops, 4), c)
+
+
+def _oracle_local(Hs, Vs, score):
+    return po.rows_batch(Hs.residues, Hs.offsets, Vs.residues, Vs.offsets,
+                         po.Scoring(score.table, score.gap_open, score.gap_extend), sa.LOCAL, with_begins=True)
+
+
+@pytest.mark.parametrize("fixture", ["ref_dna_local_rows.npz", "ref_dna_local_short_rows.npz",
+                                     "ref_protein_local_rows.npz"])
+def test_local_rows_match_reference_fixtures(ctx, fixture):
+    # localAlignment(ExecutionPolicy<Parallel, Vectorial>, gapsH, gapsV,
+    # score): scores, counts, clipped rows and window begins.
+    z = _load(fixture)
+    score = _score_of(z, fixture)
+    Hs, Vs = sa.SeqSet(z["h_res"], z["h_off"]), sa.SeqSet(z["v_res"], z["v_off"])
+    print("local rows fixture", fixture, flush=True)
+    s, c, op_off, ops, begins = ctx.align_local_rows(Hs, Vs, score)
+    assert np.array_equal(s, z["scores"])
+    assert np.array_equal(c, z["counts"])
+    assert np.array_equal(op_off, z["op_off"])
+    assert np.array_equal(ops, z["ops"])
+    assert np.array_equal(begins, z["begins"])
+    s2, c2 = ctx.align_counts(Hs, Vs, score, mode=sa.LOCAL)
+    assert np.array_equal(s2, s) and np.array_equal(c2, c)
+    assert np.array_equal(ctx.align_score(Hs, Vs, score, sa.LOCAL), s)
+
+
+def test_local_rows_match_oracle(ctx):
+    rng = np.random.default_rng(24)
+    for alpha, score in ((4, sa.simple_score(2, -3, -1, -5)), (4, sa.simple_score(1, -1, -1, -2)),
+                         (27, sa.blosum62(-1, -11))):
+        hs, vs = [], []
+        for _ in range(200):
+            h = rng.integers(0, min(alpha, 24), int(rng.integers(1, 700))).astype(np.uint8)
+            a = int(rng.integers(0, len(h)))
+            seg = h[a:a + int(rng.integers(1, 300))].copy()
+            flip = rng.random(len(seg)) < 0.15
+            seg[flip] = rng.integers(0, min(alpha, 24), int(flip.sum()))
+            flank = lambda: rng.integers(0, min(alpha, 24), int(rng.integers(0, 200))).astype(np.uint8)
+            v = np.concatenate([flank(), seg, flank()]).astype(np.uint8)
+            if rng.random() < 0.25:
+                v = rng.integers(0, min(alpha, 24), int(rng.integers(1, 700))).astype(np.uint8)
+            hs.append(h)
+            vs.append(v)
+        Hs, Vs = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+        print("local rows vs oracle, alphabet", alpha, flush=True)
+        s, c, op_off, ops, begins = ctx.align_local_rows(Hs, Vs, score)
+        ws, wc, woff, wops, wbeg = _oracle_local(Hs, Vs, score)
+        assert np.array_equal(s, ws) and np.array_equal(c, wc)
+        assert np.array_equal(op_off, woff) and np.array_equal(ops, wops)
+        assert np.array_equal(begins, wbeg)
+
+
+def test_local_rows_empty_zero_and_errors(ctx):
+    # Empty pairs: INT32_MIN and no window.  A pair with no positive cell:
+    # score 0, empty window at (0, 0).  Local rows come only from
+    # sqmi_align_local_rows; free-end bits beside LOCAL are ignored, as in
+    # sqmi_align_score (localAlignment has no AlignConfig).
+    Hs = sa.SeqSet.from_list([[0, 1, 2], [], [0, 0, 0], [1, 2, 3, 0]])
+    Vs = sa.SeqSet.from_list([[0, 2], [1, 1], [1, 2, 3], []])
+    score = sa.simple_score(2, -3, -1, -5)
+    s, c, op_off, ops, begins = ctx.align_local_rows(Hs, Vs, score)
+    assert s[1] == np.iinfo(np.int32).min and s[3] == np.iinfo(np.int32).min
+    assert s[2] == 0 and s[0] == 2
+    assert list(np.diff(op_off)) == [1, 0, 0, 0]
+    assert bytes(ops) == b"M"
+    assert begins.tolist() == [[0, 0], [0, 0], [0, 0], [0, 0]]
+    assert (c[1:] == 0).all()
+    with pytest.raises(sa.SqmiError):
+        ctx.align_rows(Hs, Vs, score, mode=sa.LOCAL)
+    s2, c2 = ctx.align_counts(Hs, Vs, score, mode=sa.LOCAL | sa.FREE_TOP)
+    assert np.array_equal(s2, s) and np.array_equal(c2, c)

@@ -71,6 +71,15 @@ def test_kat_local_affine():
     assert po.score(enc("CACACTTAACTTCACAA"), enc("GGGGCTTGAGAGCTTGGGG"), SC_2_1_1_3, po.LOCAL) == 8
 
 
+# tests/align/test_alignment_algorithms_local.h:250-299: the clipped rows of
+# localAlignment on Align and on Gaps objects.
+def test_kat_local_affine_rows():
+    h, v = enc("CACACTTAACTTCACAA"), enc("GGGGCTTGAGAGCTTGGGG")
+    s, c, ops, (bh, bv) = po.traceback(h, v, SC_2_1_1_3, po.LOCAL, with_begins=True)
+    assert s == 8
+    assert rows_from_ops(h[bh:], v[bv:], ops) == ("CTT---AACTT", "CTTGAGAGCTT")
+
+
 def _pair_align():
     with open(os.path.join(GOLDEN, "pair_align.json")) as f:
         return json.load(f)["pairs"]
@@ -230,6 +239,67 @@ def test_oracle_rows_match_reference_every_align_config(fixture):
         assert np.array_equal(c, z["counts"][mode]), mode
         assert np.array_equal(op_off, z["op_off"][mode]), mode
         assert np.array_equal(ops, z["ops"][z["mode_off"][mode]:z["mode_off"][mode + 1]]), mode
+
+
+def _local_scoring(z, fixture):
+    if "match" in z:
+        return po.Scoring.simple(int(z["match"]), int(z["mismatch"]), int(z["gap_extend"]), int(z["gap_open"]))
+    return po.Scoring(_blosum62_fixture(), int(z["gap_open"]), int(z["gap_extend"]))
+
+
+@pytest.mark.parametrize("fixture", ["ref_dna_local_rows.npz", "ref_dna_local_short_rows.npz",
+                                     "ref_protein_local_rows.npz"])
+def test_oracle_local_rows_match_reference(fixture):
+    # localAlignment(ExecutionPolicy<Parallel, Vectorial>, gapsH, gapsV,
+    # score): the clipped rows of each local window and where it begins.
+    z = _load(fixture)
+    s, c, op_off, ops, begins = po.rows_batch(z["h_res"], z["h_off"], z["v_res"], z["v_off"],
+                                              _local_scoring(z, fixture), po.LOCAL, with_begins=True)
+    assert np.array_equal(s, z["scores"])
+    assert np.array_equal(c, z["counts"])
+    assert np.array_equal(op_off, z["op_off"])
+    assert np.array_equal(ops, z["ops"])
+    assert np.array_equal(begins, z["begins"])
+
+
+def _window_score(h, v, ops, bh, bv, sc):
+    """Value of the local window ops starting at (bh, bv), gap runs per row."""
+    t = sc.table
+    total, i, j, prev = 0, bv, bh, ""
+    for op in ops:
+        if op == "M":
+            total += int(t[h[j], v[i]]); i += 1; j += 1
+        else:
+            total += sc.gap_extend if prev == op else sc.gap_open
+            if op == "V":
+                i += 1
+            else:
+                j += 1
+        prev = op if op != "M" else ""
+    return total, i, j
+
+
+def test_oracle_local_traceback_vs_brute_force():
+    # The window is inside both sequences, its value is the best local score
+    # (exhaustive search), it is empty exactly when that score is 0, and the
+    # counts read from it give the score back.
+    rng = np.random.default_rng(77)
+    for trial in range(400):
+        alpha = int(rng.integers(2, 5))
+        o = int(rng.integers(-6, 0))
+        e = int(rng.integers(o, 1))
+        t = rng.integers(-4, 5, size=(alpha, alpha)).astype(np.int32)
+        sc = po.Scoring(t, o, e)
+        n, m = int(rng.integers(1, 7)), int(rng.integers(1, 7))
+        h = rng.integers(0, alpha, n).astype(np.uint8)
+        v = rng.integers(0, alpha, m).astype(np.uint8)
+        s, c, ops, (bh, bv) = po.traceback(h, v, sc, po.LOCAL, with_begins=True)
+        assert s == po.brute_force(h, v, sc, po.LOCAL), (h, v, t, o, e)
+        val, iv, jh = _window_score(h, v, ops, bh, bv, sc)
+        assert val == s and jh <= n and iv <= m
+        assert (ops == "") == (s == 0) and (ops != "" or (bh, bv) == (0, 0))
+        par = np.concatenate([t.reshape(-1), [o, e]]).astype(np.int64)
+        assert int(c.astype(np.int64) @ par) == s
 
 
 def test_100kbp_fixture_is_a_valid_alignment():
