@@ -304,18 +304,29 @@ int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
     }
 #undef TRACK
 
-    /* _correctTraceValue: at the best cell prefer the gap matrices. */
+    /* gap_open == gap_extend: the reference runs its linear-gap algorithm
+     * (_usesAffineGaps, dp_setup.h:244-260; dp_formula_linear.h:62-106).
+     * Its cell values are these, and so is each cell's own direction
+     * (diagonal wins ties, then vertical), but it has no gap matrices to
+     * follow: every step re-reads the direction of the cell it reaches, and
+     * the end cell is taken as it is (no _correctTraceValue for linear gaps,
+     * dp_algorithm_impl.h:1180-1188; PreferGapsAtEnd_ is false,
+     * dp_traceback_impl.h:115-122). */
+    const int linear = (o == e);
     uint8_t tv = T[bj * R + bi];
-    if (bestF == best) {
-        tv = (uint8_t)((tv & ~TB_DIAG) | TB_MAX_VERT);
-    } else if (bestE == best) {
-        tv = (uint8_t)((tv & ~TB_DIAG) | TB_MAX_HORI);
+    if (!linear) {
+        /* _correctTraceValue: at the best cell prefer the gap matrices. */
+        if (bestF == best) {
+            tv = (uint8_t)((tv & ~TB_DIAG) | TB_MAX_VERT);
+        } else if (bestE == best) {
+            tv = (uint8_t)((tv & ~TB_DIAG) | TB_MAX_HORI);
+        }
+        /* _retrieveInitialTraceDirection with PreferGapsAtEnd_ (affine). */
+        if (tv & TB_MAX_VERT)
+            tv &= (uint8_t)(TB_VERT | TB_VERT_OPEN | TB_MAX_VERT);
+        else if (tv & TB_MAX_HORI)
+            tv &= (uint8_t)(TB_HORI | TB_HORI_OPEN | TB_MAX_HORI);
     }
-    /* _retrieveInitialTraceDirection with PreferGapsAtEnd_ (affine). */
-    if (tv & TB_MAX_VERT)
-        tv &= (uint8_t)(TB_VERT | TB_VERT_OPEN | TB_MAX_VERT);
-    else if (tv & TB_MAX_HORI)
-        tv &= (uint8_t)(TB_HORI | TB_HORI_OPEN | TB_MAX_HORI);
 
     /* Walk back, emitting ops in reverse order. */
     char *rev = (char *)malloc((size_t)(n + m + 2));
@@ -329,7 +340,20 @@ int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
     int64_t ci = bi, cj = bj;
 #define TV(J, I) T[(J) * R + (I)]
     while (!(cj <= 0 || ci <= 0) && tv != TB_NONE) {
-        if (tv & TB_DIAG) {
+        if (linear) {
+            /* One step per cell (dp_traceback_impl.h:253-259, 332-338). */
+            if (tv & TB_DIAG) {
+                rev[nr++] = 'M';
+                --ci; --cj;
+            } else if (tv & TB_MAX_VERT) {
+                rev[nr++] = 'V';
+                --ci;
+            } else {
+                rev[nr++] = 'H';
+                --cj;
+            }
+            tv = TV(cj, ci);
+        } else if (tv & TB_DIAG) {
             rev[nr++] = 'M';
             --ci; --cj;
             tv = TV(cj, ci);

@@ -1285,6 +1285,7 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             wa.endEF = a.endEF;
             wa.counts = dCounts;
             wa.alpha = A;
+            wa.linear = si.o == si.e;
             wa.ops = rows.ops;
             wa.opOff = rows.opOff;
             wa.opLen = rows.opLen;

@@ -719,6 +719,14 @@ __global__ void kFillScores(int32_t *scores, const int32_t *ids, int n, int32_t 
 // (index h * alpha + v), and per maximal run of gaps in one row one open and
 // run-1 extensions.  A run is maximal in either reading direction, so the
 // counts can be taken while walking backwards.
+//
+// With gap open == gap extend the reference runs its linear-gap algorithm
+// (_usesAffineGaps, dp_setup.h:244-260; dp_formula_linear.h:62-106).  Its
+// cells hold the same values and the same own direction (diagonal wins ties,
+// then vertical), which the trace bits record; but it keeps no gap matrices,
+// so every gap step re-reads the direction of the cell it reaches, and the
+// end cell is taken as it is (no _correctTraceValue for linear gaps,
+// dp_algorithm_impl.h:1180-1188).
 // ---------------------------------------------------------------------------
 constexpr uint32_t kBitEOpen = 1, kBitFOpen = 2, kBitNotDiag = 4, kBitHori = 8;
 // Local alignment (byte trace only): the cell did not beat 0, so its trace
@@ -734,7 +742,7 @@ template <class Bits>
 __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const uint8_t *v, int A, int score, int endE,
                                            int endF, Bits &bits, int32_t *cnt, uint8_t *opEnd = nullptr,
                                            int64_t *opLen = nullptr, int i0 = -1, int j0 = -1,
-                                           bool local = false, int64_t *begins = nullptr)
+                                           bool local = false, int64_t *begins = nullptr, bool linear = false)
 {
     // The walk starts at the end cell (i0, j0), (m, n) unless free end gaps
     // or a local alignment moved it; in global modes the cells after it are
@@ -774,7 +782,7 @@ __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const
     bool walk = true;
     {
         const uint32_t b0 = (i > 0 && j > 0) ? bits(i, j) : kBitStop;
-        dir = (endF == score) ? 1 : (endE == score) ? 2 : dirOf(b0);
+        dir = linear ? dirOf(b0) : (endF == score) ? 1 : (endE == score) ? 2 : dirOf(b0);
         if (local && (b0 & kBitStop))
             walk = false;
     }
@@ -792,7 +800,7 @@ __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const
                 if (prev == 1) ++exts; else ++opens;
                 prev = 1;
                 emit('V');
-                const bool stop = open || i == 1;
+                const bool stop = linear || open || i == 1;
                 --i;
                 if (stop)
                     break;
@@ -803,7 +811,7 @@ __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const
                 if (prev == 2) ++exts; else ++opens;
                 prev = 2;
                 emit('H');
-                const bool stop = open || j == 1;
+                const bool stop = linear || open || j == 1;
                 --j;
                 if (stop)
                     break;
@@ -897,7 +905,8 @@ __global__ __launch_bounds__(kWalkThreads) void kWalk16(WalkArgs a)
         cnt[q] = 0;
     Bits16 bits{a.trace, a.locs[t]};
     walkCounts(n, m, a.hRes + ho, a.vRes + vo, A, a.scores[pid], a.endEF[2 * pid], a.endEF[2 * pid + 1], bits, cnt,
-               a.ops ? a.ops + a.opOff[pid + 1] : nullptr, a.ops ? a.opLen + pid : nullptr);
+               a.ops ? a.ops + a.opOff[pid + 1] : nullptr, a.ops ? a.opLen + pid : nullptr, -1, -1, false, nullptr,
+               a.linear != 0);
     if constexpr (LDS)
         for (int q = 0; q < S; ++q)
             out[q] = cnt[q];
@@ -1006,7 +1015,7 @@ __global__ __launch_bounds__(256) void kCounts32(Counts32Args a)
     a.scores[pid] = best;
     auto bits = [&](int i, int j) -> uint32_t { return T[(int64_t)j * CS + i]; };
     walkCounts(n, m, h, v, A, best, bestE, bestF, bits, cnt, a.ops ? a.ops + a.opOff[pid + 1] : nullptr,
-               a.ops ? a.opLen + pid : nullptr, bi, bj, local, a.begins ? a.begins + 2 * pid : nullptr);
+               a.ops ? a.opLen + pid : nullptr, bi, bj, local, a.begins ? a.begins + 2 * pid : nullptr, o == e);
 }
 
 template <int G, int R, bool DNA>

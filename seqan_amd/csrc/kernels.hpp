@@ -81,6 +81,7 @@ struct WalkArgs {
     const int32_t *endEF;       // per pair id: E(m, n), F(m, n)
     int32_t *counts;            // per pair id: alpha^2 + 2 counts
     int32_t alpha;
+    int32_t linear;             // gap open == gap extend: the linear-gap walk
     // Rows (optional): pair id p's ops end at ops[opOff[p + 1]] (written
     // backwards), their number goes to opLen[p].
     uint8_t *ops;

@@ -613,8 +613,9 @@ __global__ void kTileWalkInit(TileArgs a)
     ws[WS_I] = P.m;
     ws[WS_J] = P.n;
     // End cell: F, then E, win ties (dp_algorithm_impl.h:1190-1207,
-    // PreferGapsAtEnd_ dp_traceback_impl.h:115-122).
-    ws[WS_MODE] = (endF == score) ? 1 : (endE == score) ? 2 : 0;
+    // PreferGapsAtEnd_ dp_traceback_impl.h:115-122); the linear-gap walk
+    // (gap open == gap extend) takes the cell's own direction.
+    ws[WS_MODE] = (a.gapOpen == a.gapExtend) ? 0 : (endF == score) ? 1 : (endE == score) ? 2 : 0;
     ws[WS_PREV] = 0;
     ws[WS_OPENS] = 0;
     ws[WS_EXTS] = 0;
@@ -646,6 +647,7 @@ __global__ __launch_bounds__(kWalkWGThreads) void kTileWalk32(TileArgs a)
     if (!g0.active)
         return;   // finished (the same for every thread of the workgroup)
     const int A = a.alpha, S = A * A + 2;
+    const bool linear = a.gapOpen == a.gapExtend;
     const uint8_t *h = a.hRes + a.hOff[P.pid];
     const uint8_t *v = a.vRes + a.vOff[P.pid];
     const int tid = threadIdx.x;
@@ -720,12 +722,14 @@ __global__ __launch_bounds__(kWalkWGThreads) void kTileWalk32(TileArgs a)
                 } else {
                     // One gap character; the gap goes on while the cell's gap
                     // extends, down to row / column 1 (dp_traceback_impl.h:220-341).
+                    // Linear gaps (gap open == gap extend) take one step per
+                    // cell and re-read the next cell's own direction.
                     const bool vert = d == 1;
                     const bool open = (bits & (vert ? 2u : 1u)) != 0;
                     if (prev == d) ++exts; else ++opens;
                     prev = d;
                     emit(vert ? 'V' : 'H');
-                    const bool stop = open || (vert ? i == 1 : j == 1);
+                    const bool stop = linear || open || (vert ? i == 1 : j == 1);
                     if (vert)
                         --i;
                     else

@@ -5,6 +5,7 @@ holds the reference tree and the oracle/_ref build):
     python tests/golden/make_golden.py --long   # the 100 kbp fixture (~25 GB of RAM)
     python tests/golden/make_golden.py --modes  # only the AlignConfig rows fixtures
     python tests/golden/make_golden.py --local  # only the local rows fixtures
+    python tests/golden/make_golden.py --linear # only the linear-gap rows fixtures
 
 * ``ref_*.npz``   -- inputs (residue ordinals + offsets) and the outputs of
   the reference library itself (oracle/_ref: SeqAn 2 globalAlignmentScore /
@@ -241,6 +242,30 @@ def local_rows():
              begins=r["begins"], **extra)
 
 
+def linear_rows():
+    """Rows under linear gaps (gap open == gap extend), where the reference
+    switches to its linear-gap algorithm (_usesAffineGaps, dp_setup.h:244-260)
+    with its own tie-breaks: global under each AlignConfig (modes 0-15) and
+    local (mode 16)."""
+    rng = np.random.default_rng(0x5EED0300)
+    for name, count, lo, hi, alpha, kw, modes in (
+            ("ref_dna_linear_rows.npz", 160, 1, 200, 4,
+             dict(alphabet="dna", match=1, mismatch=-1, gap_extend=-1, gap_open=-1), range(17)),
+            ("ref_dna_linear2_rows.npz", 96, 20, 400, 4,
+             dict(alphabet="dna", match=2, mismatch=-3, gap_extend=-2, gap_open=-2), (0, 9, 15, 16)),
+            ("ref_protein_linear_rows.npz", 96, 10, 300, 27,
+             dict(alphabet="aa", matrix="blosum62", gap_extend=-4, gap_open=-4), (0, 9, 16))):
+        H, V = trimmed_pairs(rng, count, lo, hi) if alpha == 4 else local_pairs(rng, count, lo, hi, 24)
+        res = [po.run_reference(po.OP_COUNTS_SIMD, H.residues, H.offsets, V.residues, V.offsets, mode, **kw)
+               for mode in modes]
+        extra = {k: v for k, v in kw.items() if k not in ("alphabet", "matrix")}
+        mode_off = np.cumsum([0] + [len(r["ops"]) for r in res]).astype(np.int64)
+        save(name, H, V, modes=np.array(list(modes), np.int32), scores=np.stack([r["scores"] for r in res]),
+             counts=np.stack([r["counts"] for r in res]), op_off=np.stack([r["op_off"] for r in res]),
+             ops=np.concatenate([r["ops"] for r in res]), mode_off=mode_off,
+             begins=res[-1]["begins"] if modes[-1] == 16 else np.zeros((0, 2), np.int64), **extra)
+
+
 def long_pairs():
     """100 kbp fixture: the first two pairs of the 64 x 100 kbp workload at
     full length (run on its own: --long).  Scores come from the reference's
@@ -265,7 +290,10 @@ if __name__ == "__main__":
         mode_rows()
     elif "--local" in sys.argv:
         local_rows()
+    elif "--linear" in sys.argv:
+        linear_rows()
     else:
         main()
         mode_rows()
         local_rows()
+        linear_rows()

@@ -244,6 +244,10 @@ int main()
     runRowsCase("dna_rows_semiglobal", dh, dv, dna, ex, cpu, semi);
     runRowsCase("dna_rows_overlap", dh, dv, dna, ex, cpu, overlap);
     runLocalRowsCase("dna_local_rows", dh, dv, dna, ex, cpu);
+    // Gap open == gap extend (lin): the reference's linear-gap traceback.
+    runRowsCase("dna_rows_linear_gaps", dh, dv, lin, ex, cpu);
+    runRowsCase("dna_rows_linear_gaps_semiglobal", dh, dv, lin, ex, cpu, semi);
+    runLocalRowsCase("dna_local_rows_linear_gaps", dh, dv, lin, ex, cpu);
     runLocalRowsCase("protein_local_rows_blosum62", ph, pv, b62, ex, cpu);
 
     // |setH| != |setV| raises instead of asserting.

@@ -275,3 +275,26 @@ def test_local_rows_empty_zero_and_errors(ctx):
         ctx.align_rows(Hs, Vs, score, mode=sa.LOCAL)
     s2, c2 = ctx.align_counts(Hs, Vs, score, mode=sa.LOCAL | sa.FREE_TOP)
     assert np.array_equal(s2, s) and np.array_equal(c2, c)
+
+
+@pytest.mark.parametrize("fixture", ["ref_dna_linear_rows.npz", "ref_dna_linear2_rows.npz",
+                                     "ref_protein_linear_rows.npz"])
+@pytest.mark.parametrize("flags", [0, sa.FORCE_INT32], ids=["packed", "int32"])
+def test_linear_gap_rows_match_reference(ctx, fixture, flags):
+    # gap open == gap extend: the reference's linear-gap traceback (its own
+    # tie-breaks) for global modes under AlignConfigs and for local windows.
+    z = _load(fixture)
+    score = _score_of(z, fixture)
+    Hs, Vs = sa.SeqSet(z["h_res"], z["h_off"]), sa.SeqSet(z["v_res"], z["v_off"])
+    for k, mode in enumerate(z["modes"]):
+        mode = int(mode)
+        print("linear-gap rows", fixture, "mode", mode, "flags", flags, flush=True)
+        if mode == sa.LOCAL:
+            s, c, op_off, ops, begins = ctx.align_local_rows(Hs, Vs, score, flags=flags)
+            assert np.array_equal(begins, z["begins"])
+        else:
+            s, c, op_off, ops = ctx.align_rows(Hs, Vs, score, mode=mode, flags=flags)
+        assert np.array_equal(s, z["scores"][k]), mode
+        assert np.array_equal(c, z["counts"][k]), mode
+        assert np.array_equal(op_off, z["op_off"][k]), mode
+        assert np.array_equal(ops, z["ops"][z["mode_off"][k]:z["mode_off"][k + 1]]), mode

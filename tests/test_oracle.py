@@ -262,6 +262,25 @@ def test_oracle_local_rows_match_reference(fixture):
     assert np.array_equal(begins, z["begins"])
 
 
+@pytest.mark.parametrize("fixture", ["ref_dna_linear_rows.npz", "ref_dna_linear2_rows.npz",
+                                     "ref_protein_linear_rows.npz"])
+def test_oracle_linear_gap_rows_match_reference(fixture):
+    # gap open == gap extend: the reference's linear-gap algorithm and its
+    # tie-breaks (one step per cell, no end-cell correction), global under
+    # AlignConfigs and local.
+    z = _load(fixture)
+    sc = _local_scoring(z, fixture)
+    for k, mode in enumerate(z["modes"]):
+        s, c, op_off, ops, begins = po.rows_batch(z["h_res"], z["h_off"], z["v_res"], z["v_off"], sc, int(mode),
+                                                  with_begins=True)
+        assert np.array_equal(s, z["scores"][k]), mode
+        assert np.array_equal(c, z["counts"][k]), mode
+        assert np.array_equal(op_off, z["op_off"][k]), mode
+        assert np.array_equal(ops, z["ops"][z["mode_off"][k]:z["mode_off"][k + 1]]), mode
+        if mode == 16:
+            assert np.array_equal(begins, z["begins"])
+
+
 def _window_score(h, v, ops, bh, bv, sc):
     """Value of the local window ops starting at (bh, bv), gap runs per row."""
     t = sc.table
