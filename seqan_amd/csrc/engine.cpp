@@ -47,6 +47,11 @@ struct sqmi_context
     std::string lastError;
     sqmi_stats stats{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Long-pair forward: side streams (default priority) that sweep groups
+    // of pair couples concurrently, with fork / join events.
+    static constexpr int kMaxLongStreams = 4;
+    hipStream_t longStreams[kMaxLongStreams] = {};
+    hipEvent_t longFork = nullptr, longJoin[kMaxLongStreams] = {};
     DevBuf hRes, vRes, hOff, vOff, order, items, queue, rowbuf, scores, tTable, dnaProf, ids32, scratchOff32,
         scratch32, table32, emptyIds;
     // Traceback (sqmi_align_counts).
@@ -231,6 +236,17 @@ int64_t longMinSide()
     if (env && *env && std::atoll(env) > 0)
         return std::atoll(env);
     return 2048;
+}
+
+// Streams the long-pair forward spreads its pair couples over (1..4).  On
+// 64 pairs of 100 kbp, 2 and 3 streams take 170 ms, 1 takes 220 ms and 4
+// take 260 ms (the 4th shares a hardware queue); starting the groups some
+// anti-diagonals apart only made it slower.
+int longStreams()
+{
+    const char *env = std::getenv("SQMI_LONG_STREAMS");
+    const int v = (env && *env) ? std::atoi(env) : 2;
+    return std::max(1, std::min(v, sqmi_context::kMaxLongStreams));
 }
 
 
@@ -702,10 +718,49 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
             maxT = std::max(maxT, pairs[k].ntiles);
         }
         a.wavesPerPair = (maxB + 3) / 4;
+        // Every launch sweeps one tile anti-diagonal and ends with a barrier
+        // for its whole grid.  Groups of pair couples on separate streams do
+        // not wait for each other's slowest tiles, so their launches fill
+        // the SIMDs the others leave idle (the ramps of the wavefront and
+        // the last waves of the widest anti-diagonals).  Group 0 stays on the
+        // call's stream, so that ns groups use ns hardware queues.
+        const int couples = (int)((e - b + 1) / 2);
+        const int ns = std::min(longStreams(), couples);
+        std::vector<TileArgs> sub(ns, a);
+        for (int s = 0; s < ns; ++s) {
+            const int c0 = couples * s / ns, c1 = couples * (s + 1) / ns;
+            sub[s].pairs = a.pairs + 2 * c0;
+            sub[s].nPairs = std::min((int32_t)(2 * c1), a.nPairs) - 2 * c0;
+        }
+        if (ns > 1) {
+            if (!ctx->longFork && (err = hipEventCreateWithFlags(&ctx->longFork, hipEventDisableTiming)) != hipSuccess)
+                return hipFail(ctx, err, "creating long-pair events");
+            for (int s = 1; s < ns; ++s) {
+                if (!ctx->longStreams[s] && (err = hipStreamCreate(&ctx->longStreams[s])) != hipSuccess)
+                    return hipFail(ctx, err, "creating long-pair streams");
+                if (!ctx->longJoin[s] &&
+                    (err = hipEventCreateWithFlags(&ctx->longJoin[s], hipEventDisableTiming)) != hipSuccess)
+                    return hipFail(ctx, err, "creating long-pair events");
+            }
+            if ((err = hipEventRecord(ctx->longFork, stream)) != hipSuccess)
+                return hipFail(ctx, err, "long-pair fork");
+            for (int s = 1; s < ns; ++s)
+                if ((err = hipStreamWaitEvent(ctx->longStreams[s], ctx->longFork, 0)) != hipSuccess)
+                    return hipFail(ctx, err, "long-pair fork");
+        }
         for (int d = 0; d < maxB + maxT - 1; ++d) {
-            a.diag = d;
-            if ((err = packed ? launchTileFwd16(a, stream) : launchTileFwd32(a, stream)) != hipSuccess)
-                return hipFail(ctx, err, "launching long-pair forward");
+            for (int s = 0; s < ns; ++s) {
+                sub[s].diag = d;
+                hipStream_t st = s > 0 ? ctx->longStreams[s] : stream;
+                if ((err = packed ? launchTileFwd16(sub[s], st) : launchTileFwd32(sub[s], st)) != hipSuccess)
+                    return hipFail(ctx, err, "launching long-pair forward");
+            }
+        }
+        if (ns > 1) {
+            for (int s = 1; s < ns; ++s)
+                if ((err = hipEventRecord(ctx->longJoin[s], ctx->longStreams[s])) != hipSuccess ||
+                    (err = hipStreamWaitEvent(stream, ctx->longJoin[s], 0)) != hipSuccess)
+                    return hipFail(ctx, err, "long-pair join");
         }
         if (dCounts) {
             a.trace = static_cast<uint2 *>(ctx->longTrace.p);
@@ -797,6 +852,16 @@ sqmi_status sqmi_destroy(sqmi_context *ctx)
         (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1)
         (void)hipEventDestroy(ctx->ev1);
+    for (int s = 0; s < sqmi_context::kMaxLongStreams; ++s) {
+        if (ctx->longJoin[s])
+            (void)hipEventDestroy(ctx->longJoin[s]);
+        if (ctx->longStreams[s]) {
+            (void)hipStreamSynchronize(ctx->longStreams[s]);
+            (void)hipStreamDestroy(ctx->longStreams[s]);
+        }
+    }
+    if (ctx->longFork)
+        (void)hipEventDestroy(ctx->longFork);
     if (ctx->ownStream)
         (void)hipStreamDestroy(ctx->ownStream);
     delete ctx;
