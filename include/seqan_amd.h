@@ -21,6 +21,11 @@
  *                           its traceback writes into the Gaps objects
  *                           (dp_traceback_impl.h:115-546, gaps set through
  *                           include/seqan/align/dp_traceback_adaptor.h)
+ *   sqmi_align_score_banded <- globalAlignmentScore(ExecutionPolicy<Parallel,Vectorial>,
+ *                           setH, setV, score [, AlignConfig], lowerDiag, upperDiag)
+ *                           (the executor forwards to the banded set overload,
+ *                           include/seqan/align/global_alignment_banded.h:555-583;
+ *                           one pair: :394-470)
  *   sqmi_align_local_rows <- localAlignment(ExecutionPolicy<Parallel,Vectorial>,
  *                           gapsSetH, gapsSetV, score)
  *                           include/seqan/align_parallel/parallel_align_interface.h:312-321,
@@ -44,7 +49,8 @@ extern "C" {
 #endif
 
 #define SQMI_ABI_VERSION 3   /* 2: sqmi_stats.pairs_long, pairs_long_packed; sqmi_align_rows
-                                3: sqmi_align_local_rows; SQMI_LOCAL in sqmi_align_counts */
+                                3: sqmi_align_local_rows; SQMI_LOCAL in sqmi_align_counts;
+                                   sqmi_align_score_banded */
 
 typedef enum sqmi_status {
     SQMI_OK = 0,
@@ -178,6 +184,24 @@ sqmi_status sqmi_align_local_rows(sqmi_context *ctx, const sqmi_sequences *set_h
                                   int32_t *scores, int32_t *counts, uint8_t *ops,
                                   const int64_t *op_offsets, int64_t *op_lengths,
                                   int64_t *begins, uint32_t flags);
+
+/* Banded global alignment scores: only cells (i, j) with lower_diag <= j - i
+ * <= upper_diag exist (i a position in V, j in H), under the free-end bits of
+ * mode (SQMI_LOCAL is not supported).  Every non-empty pair must have
+ * |H| >= 2 and lower_diag < min(0, |H| - |V|), max(0, |H| - |V|) < upper_diag:
+ * the band holds both corner diagonals with a margin (the reference accepts
+ * only bands holding both corners unless ends are free,
+ * dp_algorithm_impl.h:192-232, and outside this domain its special band
+ * paths depart from the banded recurrence); otherwise the call returns
+ * SQMI_ERR_UNSUPPORTED.  Empty pairs get INT32_MIN.  Every pair follows the
+ * reference's banded DP for one pair (global_alignment_banded.h:394-470);
+ * the reference's vectorised set path agrees with it when all H and all V
+ * have one length each, the case its documentation supports.  Flags: only
+ * the *_ON_DEVICE ones. */
+sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set_h,
+                                    const sqmi_sequences *set_v, const sqmi_scoring *scoring,
+                                    uint32_t mode, int64_t lower_diag, int64_t upper_diag,
+                                    int32_t *scores, uint32_t flags);
 
 sqmi_status sqmi_get_stats(const sqmi_context *ctx, sqmi_stats *out);
 

@@ -76,7 +76,7 @@ ABI_VERSION = 3
 # Every symbol the C ABI declares (include/seqan_amd.h).
 ABI_SYMBOLS = (
     "sqmi_create", "sqmi_destroy", "sqmi_set_stream", "sqmi_align_score", "sqmi_align_counts", "sqmi_align_rows",
-    "sqmi_align_local_rows",
+    "sqmi_align_local_rows", "sqmi_align_score_banded",
     "sqmi_get_stats", "sqmi_plan_stats", "sqmi_blosum62", "sqmi_status_string", "sqmi_last_error", "sqmi_abi_version",
     "sqmi_workload_create", "sqmi_workload_sizes", "sqmi_workload_copy", "sqmi_workload_destroy",
 )
@@ -103,6 +103,9 @@ def lib():
                                         ctypes.POINTER(_Scoring), ctypes.c_uint32, vp, vp, ctypes.c_uint32]
         L.sqmi_align_rows.argtypes = [vp, ctypes.POINTER(_Sequences), ctypes.POINTER(_Sequences),
                                       ctypes.POINTER(_Scoring), ctypes.c_uint32, vp, vp, vp, vp, vp, ctypes.c_uint32]
+        L.sqmi_align_score_banded.argtypes = [vp, ctypes.POINTER(_Sequences), ctypes.POINTER(_Sequences),
+                                              ctypes.POINTER(_Scoring), ctypes.c_uint32, ctypes.c_int64,
+                                              ctypes.c_int64, vp, ctypes.c_uint32]
         L.sqmi_align_local_rows.argtypes = [vp, ctypes.POINTER(_Sequences), ctypes.POINTER(_Sequences),
                                             ctypes.POINTER(_Scoring), vp, vp, vp, vp, vp, vp, ctypes.c_uint32]
         L.sqmi_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
@@ -306,6 +309,18 @@ class Context:
         self._check(st, "sqmi_align_score")
         return out
 
+    def align_score_banded(self, set_h: SeqSet, set_v: SeqSet, score: Score, lower_diag: int, upper_diag: int,
+                           mode=0, flags=0):
+        """Banded global scores: cells with lower_diag <= j - i <= upper_diag."""
+        ch, keep_h = _seqs(set_h)
+        cv, keep_v = _seqs(set_v)
+        sc = score._c()
+        out = np.empty(len(set_h.offsets) - 1, np.int32)
+        st = self._L.sqmi_align_score_banded(self._h, ctypes.byref(ch), ctypes.byref(cv), ctypes.byref(sc), mode,
+                                             int(lower_diag), int(upper_diag), out.ctypes.data, flags)
+        self._check(st, "sqmi_align_score_banded")
+        return out
+
     def align_counts(self, set_h: SeqSet, set_v: SeqSet, score: Score, mode=0, flags=0,
                      h_dev=None, v_dev=None, scores_dev=None, counts_dev=None):
         """Scores and counts (alpha^2 substitution counts, gap opens, gap
@@ -482,9 +497,13 @@ def _as_set(s, alphabet):
 
 
 def global_alignment_score(set_h, set_v, score: Score, align_config=(False, False, False, False),
-                           alphabet=DNA, ctx: Context = None):
-    """globalAlignmentScore(ExecutionPolicy<Parallel,Vectorial>, setH, setV, score, AlignConfig)."""
+                           alphabet=DNA, ctx: Context = None, lower_diag=None, upper_diag=None):
+    """globalAlignmentScore(ExecutionPolicy<Parallel,Vectorial>, setH, setV, score, AlignConfig
+    [, lowerDiag, upperDiag])."""
     ctx = ctx or default_context()
+    if lower_diag is not None or upper_diag is not None:
+        return ctx.align_score_banded(_as_set(set_h, alphabet), _as_set(set_v, alphabet), score, lower_diag,
+                                      upper_diag, _mode_from_config(align_config))
     return ctx.align_score(_as_set(set_h, alphabet), _as_set(set_v, alphabet), score,
                            _mode_from_config(align_config))
 

@@ -19,7 +19,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libseqan_amd.so")
 ARCH = os.environ.get("SQMI_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "long.hip", "engine.cpp", "scoring.cpp", "workload.cpp"]
+SOURCES = ["kernels.hip", "long.hip", "banded.hip", "engine.cpp", "scoring.cpp", "workload.cpp"]
 
 
 def hipcc():

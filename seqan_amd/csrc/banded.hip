@@ -1,0 +1,256 @@
+// banded.hip -- banded global alignment scores (int32) on gfx950.
+//
+// globalAlignmentScore(setH, setV, score[, AlignConfig], lowerDiag, upperDiag)
+// (include/seqan/align/global_alignment_banded.h:394-470 for one pair,
+// :555-583 for sets): only cells (i, j) with lowerDiag <= j - i <= upperDiag
+// exist; the others are minus infinity in H, E and F (DPBandConfig<BandOn>,
+// dp_band.h).  The recurrence, the borders and the tracked cells are those of
+// the unbanded kernels (dp_formula_affine.h:62-126, dp_setup.h:57-182,
+// dp_scout.h:169-210); requests the reference rejects are filtered on the
+// host (_checkBandProperties, dp_algorithm_impl.h:192-232).
+//
+// Layout: the band is swept row by row (row i = V position), one lane per
+// band diagonal d = j - i, so that the diagonal and vertical predecessors of
+// a row come from the previous row (same lane, and the lane of d + 1) and
+// only the horizontal gap chain runs along the row.  That chain is a
+// max-plus prefix scan over the lanes: with g = max(gapExtend, gapOpen) and
+// X = max(diagonal + s, F) the value before the horizontal gap,
+//     E(i, j) = max over q < j of X(i, q) + gapOpen + (j - 1 - q) * g,
+// which is the unrolled E(j) = max(E(j-1) + gapExtend, H(j-1) + gapOpen).
+//   kBandedNarrow: bands of at most 64 diagonals; a wave holds 64 / S pairs
+//                  in segments of S lanes (S = the band width rounded up to a
+//                  power of two), the previous row in registers.
+//   kBandedWide:   wider bands; one wave per pair, the band in chunks of 64
+//                  lanes, the previous row in a per-wave global buffer.
+#include "kernels.hpp"
+#include "seqan_amd.h"
+
+namespace sqmi {
+namespace {
+
+constexpr int32_t kNegB = INT32_MIN / 2;   // DPCellDefaultInfinity (dp_cell.h:141-146)
+
+struct BandPair {
+    int32_t pid, n, m;
+    int32_t lo, w;      // first band diagonal (clipped to the matrix) and band width
+    const uint8_t *h, *v;
+    __device__ bool track(int i, int j, uint32_t mode) const
+    {
+        return (i == m && (j == n || (mode & SQMI_FREE_BOTTOM))) || (j == n && (mode & SQMI_FREE_RIGHT));
+    }
+};
+
+__device__ __forceinline__ BandPair loadPair(const BandedArgs &a, int pid)
+{
+    BandPair p;
+    p.pid = pid;
+    const int64_t ho = a.hOff[pid], vo = a.vOff[pid];
+    p.n = (int32_t)(a.hOff[pid + 1] - ho);
+    p.m = (int32_t)(a.vOff[pid + 1] - vo);
+    const int64_t lo = a.lower > -(int64_t)p.m ? a.lower : -(int64_t)p.m;
+    const int64_t up = a.upper < (int64_t)p.n ? a.upper : (int64_t)p.n;
+    p.lo = (int32_t)lo;
+    p.w = (int32_t)(up - lo + 1);
+    p.h = a.hRes + ho;
+    p.v = a.vRes + vo;
+    return p;
+}
+
+// Row 0 of column j (first row of the matrix, dp_setup.h:57-182): 0 at the
+// origin and under a free first row, else a gap chain growing by
+// max(gapOpen, gapExtend) per character.
+__device__ __forceinline__ int32_t rowZero(int j, bool freeTop, int32_t o, int32_t g)
+{
+    return j == 0 ? 0 : (freeTop ? 0 : o + (j - 1) * g);
+}
+
+// Inclusive prefix max over the S lanes of a segment.
+template <int S>
+__device__ __forceinline__ int32_t prefixMax(int32_t y, int k)
+{
+#pragma unroll
+    for (int d = 1; d < S; d <<= 1) {
+        const int32_t t = __shfl_up(y, d, S);
+        if (k >= d)
+            y = max(y, t);
+    }
+    return y;
+}
+
+template <int S>
+__device__ __forceinline__ int32_t segmentMax(int32_t y)
+{
+#pragma unroll
+    for (int d = S / 2; d >= 1; d >>= 1)
+        y = max(y, __shfl_xor(y, d, S));
+    return y;
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void kBandedNarrow(BandedArgs a)
+{
+    constexpr int kPairsPerWave = 64 / S;
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int k = lane % S, seg = lane / S;
+    const int t = wave * kPairsPerWave + seg;
+    const bool has = t < a.nIds;
+    if (wave * kPairsPerWave >= a.nIds)
+        return;   // the whole wave is past the end
+    BandPair P{};
+    if (has)
+        P = loadPair(a, a.ids[t]);
+    const int32_t o = a.gapOpen, e = a.gapExtend, g = max(o, e);
+    const int A = a.alpha;
+    const bool freeTop = (a.mode & SQMI_FREE_TOP) != 0, freeLeft = (a.mode & SQMI_FREE_LEFT) != 0;
+    const bool inBand = has && k < P.w;
+    const int d = P.lo + k;
+    // Rows of the wave: the longest pair of its segments.
+    int rows = has ? P.m : 0;
+#pragma unroll
+    for (int x = S; x < 64; x <<= 1)
+        rows = max(rows, __shfl_xor(rows, x));
+    int32_t best = kNegB;
+    int32_t hPrev = kNegB, fPrev = kNegB;
+    if (inBand && d >= 0 && d <= P.n) {
+        hPrev = rowZero(d, freeTop, o, g);
+        if (P.track(0, d, a.mode))
+            best = hPrev;
+    }
+    for (int i = 1; i <= rows; ++i) {
+        const int j = i + d;
+        const bool valid = inBand && i <= P.m && j >= 0 && j <= P.n;
+        int32_t hUp = __shfl_down(hPrev, 1, S), fUp = __shfl_down(fPrev, 1, S);
+        if (k + 1 >= P.w) {
+            hUp = kNegB;
+            fUp = kNegB;
+        }
+        int32_t F = max(fUp + e, hUp + o);
+        int32_t X = kNegB;
+        if (valid) {
+            if (j == 0) {
+                X = freeLeft ? 0 : F;
+                if (freeLeft)
+                    F = kNegB;
+            } else {
+                const int hc = min((int)P.h[j - 1], A - 1), vc = min((int)P.v[i - 1], A - 1);
+                X = max(hPrev + a.table[hc * A + vc], F);
+            }
+        }
+        const int32_t incl = prefixMax<S>(valid ? X + o - k * g : kNegB, k);
+        int32_t E = __shfl_up(incl, 1, S) + (k - 1) * g;
+        if (k == 0)
+            E = kNegB;
+        const int32_t H = valid ? (j == 0 ? X : max(X, E)) : kNegB;
+        if (valid && P.track(i, j, a.mode))
+            best = max(best, H);
+        hPrev = H;
+        fPrev = valid ? F : kNegB;
+    }
+    best = segmentMax<S>(best);
+    if (has && k == 0)
+        a.scores[P.pid] = best;
+}
+
+// One wave per pair; rows of the band (H and F, double-buffered) in the
+// wave's slice of `scratch`: 4 * (wmax + 2) int32.  blockDim = 64, so the
+// workgroup barrier between rows orders this wave's own global writes and
+// reads.
+__global__ __launch_bounds__(64) void kBandedWide(BandedArgs a)
+{
+    const int k = threadIdx.x;
+    const int32_t o = a.gapOpen, e = a.gapExtend, g = max(o, e);
+    const int A = a.alpha;
+    const bool freeTop = (a.mode & SQMI_FREE_TOP) != 0, freeLeft = (a.mode & SQMI_FREE_LEFT) != 0;
+    const int stride = a.wideStride;   // wmax + 2, padded
+    int32_t *buf = a.scratch + (int64_t)blockIdx.x * 4 * stride;
+    for (int t = blockIdx.x; t < a.nIds; t += gridDim.x) {
+        const BandPair P = loadPair(a, a.ids[t]);
+        int32_t *hRow[2] = {buf, buf + stride};
+        int32_t *fRow[2] = {buf + 2 * stride, buf + 3 * stride};
+        int32_t best = kNegB;
+        for (int c = k; c < P.w + 1; c += 64) {
+            const int dd = P.lo + c;
+            int32_t hv = kNegB;
+            if (c < P.w && dd >= 0 && dd <= P.n) {
+                hv = rowZero(dd, freeTop, o, g);
+                if (P.track(0, dd, a.mode))
+                    best = max(best, hv);
+            }
+            hRow[0][c] = hv;
+            fRow[0][c] = kNegB;
+        }
+        __syncthreads();
+        for (int i = 1; i <= P.m; ++i) {
+            const int32_t *hp = hRow[(i - 1) & 1], *fp = fRow[(i - 1) & 1];
+            int32_t *hn = hRow[i & 1], *fn = fRow[i & 1];
+            const int vc = min((int)P.v[i - 1], A - 1);
+            int32_t carry = kNegB;   // E at the chunk's first lane from the cells before it
+            for (int base = 0; base < P.w; base += 64) {
+                const int c = base + k;
+                const int j = i + P.lo + c;
+                const bool valid = c < P.w && j >= 0 && j <= P.n;
+                int32_t X = kNegB, F = kNegB;
+                if (valid) {
+                    F = max(fp[c + 1] + e, hp[c + 1] + o);
+                    if (j == 0) {
+                        X = freeLeft ? 0 : F;
+                        if (freeLeft)
+                            F = kNegB;
+                    } else {
+                        const int hc = min((int)P.h[j - 1], A - 1);
+                        X = max(hp[c] + a.table[hc * A + vc], F);
+                    }
+                }
+                const int32_t incl = prefixMax<64>(valid ? X + o - k * g : kNegB, k);
+                int32_t E = __shfl_up(incl, 1, 64) + (k - 1) * g;
+                if (k == 0)
+                    E = kNegB;
+                E = max(E, carry + k * g);
+                const int32_t H = valid ? (j == 0 ? X : max(X, E)) : kNegB;
+                if (valid && P.track(i, j, a.mode))
+                    best = max(best, H);
+                if (c < P.w) {
+                    hn[c] = H;
+                    fn[c] = valid ? F : kNegB;
+                }
+                carry = max(max(carry + 64 * g, __shfl(incl, 63, 64) + 63 * g), kNegB);
+            }
+            if (k == 0)
+                hn[P.w] = fn[P.w] = kNegB;   // the diagonal past the band
+            __syncthreads();
+        }
+        best = segmentMax<64>(best);
+        if (k == 0)
+            a.scores[P.pid] = best;
+        __syncthreads();   // the buffer is reused by the next pair
+    }
+}
+
+}  // namespace
+
+hipError_t launchBanded(const BandedArgs &args, int segLanes, int wideGrid, hipStream_t stream)
+{
+    if (args.nIds <= 0)
+        return hipSuccess;
+    if (segLanes == 0) {
+        hipLaunchKernelGGL(kBandedWide, dim3(wideGrid), dim3(64), 0, stream, args);
+        return hipGetLastError();
+    }
+    const int pairsPerWave = 64 / segLanes;
+    const int waves = (args.nIds + pairsPerWave - 1) / pairsPerWave;
+    const dim3 grid((waves + 3) / 4), block(256);
+    switch (segLanes) {
+        case 1: hipLaunchKernelGGL(kBandedNarrow<1>, grid, block, 0, stream, args); break;
+        case 2: hipLaunchKernelGGL(kBandedNarrow<2>, grid, block, 0, stream, args); break;
+        case 4: hipLaunchKernelGGL(kBandedNarrow<4>, grid, block, 0, stream, args); break;
+        case 8: hipLaunchKernelGGL(kBandedNarrow<8>, grid, block, 0, stream, args); break;
+        case 16: hipLaunchKernelGGL(kBandedNarrow<16>, grid, block, 0, stream, args); break;
+        case 32: hipLaunchKernelGGL(kBandedNarrow<32>, grid, block, 0, stream, args); break;
+        case 64: hipLaunchKernelGGL(kBandedNarrow<64>, grid, block, 0, stream, args); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace sqmi

@@ -60,6 +60,8 @@ struct sqmi_context
     DevBuf longCk, longPairs, longWalk, longTrace;
     // Alignment rows (sqmi_align_rows, sqmi_align_local_rows).
     DevBuf ops, opOff, opLen, begins;
+    // Banded scores (sqmi_align_score_banded).
+    DevBuf bandIds, bandScratch;
 };
 
 namespace {
@@ -846,7 +848,7 @@ sqmi_status sqmi_destroy(sqmi_context *ctx)
                       &ctx->rowbuf, &ctx->scores, &ctx->tTable, &ctx->dnaProf, &ctx->ids32, &ctx->scratchOff32,
                       &ctx->scratch32, &ctx->table32, &ctx->emptyIds, &ctx->trace, &ctx->traceOff, &ctx->locs,
                       &ctx->endEF, &ctx->counts, &ctx->longCk, &ctx->longPairs, &ctx->longWalk, &ctx->longTrace,
-                      &ctx->ops, &ctx->opOff, &ctx->opLen, &ctx->begins})
+                      &ctx->ops, &ctx->opOff, &ctx->opLen, &ctx->begins, &ctx->bandIds, &ctx->bandScratch})
         release(*b);
     if (ctx->ev0)
         (void)hipEventDestroy(ctx->ev0);
@@ -1449,6 +1451,147 @@ sqmi_status sqmi_align_rows(sqmi_context *ctx, const sqmi_sequences *setH, const
                           "sqmi_align_rows: global modes only; local rows come from sqmi_align_local_rows")
                    : SQMI_ERR_UNSUPPORTED;
     return alignBackward(ctx, setH, setV, scoring, mode, scores, counts, ops, op_offsets, op_lengths, nullptr, flags);
+}
+
+// Bands the library computes: the band holds the diagonals of (0, 0) and of
+// (m, n) with at least one more diagonal beyond each, and H has at least two
+// residues.  There the reference's banded code computes the banded
+// recurrence.  Elsewhere it takes special paths of its own
+// (dp_algorithm_impl.h:595-700: one-diagonal bands, single-column matrices,
+// bands whose edge lies on a corner diagonal) whose results differ from that
+// recurrence under free end gaps or for |H| = 1, so such requests are
+// refused.  Without free ends the reference itself accepts only bands that
+// hold both corners (_checkBandProperties, dp_algorithm_impl.h:192-232).
+static bool bandSupported(int64_t n, int64_t m, int64_t lower, int64_t upper)
+{
+    return n >= 2 && lower < std::min<int64_t>(0, n - m) && upper > std::max<int64_t>(0, n - m);
+}
+
+sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
+                                    const sqmi_scoring *scoring, uint32_t mode, int64_t lower_diag,
+                                    int64_t upper_diag, int32_t *scores, uint32_t flags)
+{
+    const auto tStart = std::chrono::steady_clock::now();
+    if (!ctx)
+        return SQMI_ERR_INVALID_ARGUMENT;
+    ctx->lastError.clear();
+    ctx->stats = sqmi_stats{};
+    sqmi_status st;
+    if ((st = validate(ctx, setH, setV, scoring, mode)) != SQMI_OK)
+        return st;
+    if (mode & SQMI_LOCAL)
+        return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_score_banded: global modes only");
+    if (flags & ~(uint32_t)(SQMI_RESIDUES_ON_DEVICE | SQMI_SCORES_ON_DEVICE))
+        return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_score_banded: flags other than the *_ON_DEVICE ones");
+    if (lower_diag > upper_diag)
+        return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "lower_diag > upper_diag");
+    const int64_t N = setH->count;
+    if (N == 0)
+        return SQMI_OK;
+    if (!scores)
+        return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "scores is null");
+    if (hipSetDevice(ctx->device) != hipSuccess)
+        return fail(ctx, SQMI_ERR_DEVICE, "hipSetDevice failed");
+    hipStream_t stream = ctx->stream;
+
+    // Route: empty pairs (INT32_MIN), narrow bands (<= 64 diagonals), wide bands.
+    const auto tPlan = std::chrono::steady_clock::now();
+    std::vector<int32_t> narrow, wide, rejected;
+    int64_t wNarrow = 1, wWide = 0, cells = 0;
+    for (int64_t i = 0; i < N; ++i) {
+        const int64_t n = setH->offsets[i + 1] - setH->offsets[i], m = setV->offsets[i + 1] - setV->offsets[i];
+        if (n == 0 || m == 0) {
+            rejected.push_back((int32_t)i);
+            continue;
+        }
+        if (!bandSupported(n, m, lower_diag, upper_diag))
+            return fail(ctx, SQMI_ERR_UNSUPPORTED,
+                        "sqmi_align_score_banded: pair " + std::to_string(i) +
+                            ": needs |H| >= 2 and lower_diag < min(0, |H| - |V|) < max(0, |H| - |V|) < upper_diag");
+        const int64_t w = std::min<int64_t>(upper_diag, n) - std::max<int64_t>(lower_diag, -m) + 1;
+        cells += n * m;
+        if (w <= 64) {
+            narrow.push_back((int32_t)i);
+            wNarrow = std::max(wNarrow, w);
+        } else {
+            wide.push_back((int32_t)i);
+            wWide = std::max(wWide, w);
+        }
+    }
+    int seg = 1;
+    while (seg < wNarrow)
+        seg <<= 1;
+    ctx->stats.pairs = N;
+    ctx->stats.cells = cells;
+    ctx->stats.pairs_int32 = (int64_t)(narrow.size() + wide.size());
+    ctx->stats.host_plan_ms = msSince(tPlan);
+
+    ScoringInfo si{};
+    si.alpha = scoring->alphabet_size;
+    si.o = scoring->gap_open;
+    si.e = scoring->gap_extend;
+    Inputs in;
+    if ((st = uploadInputs(ctx, setH, setV, scoring, si, scores, flags, in)) != SQMI_OK)
+        return st;
+    hipError_t err;
+    const int wideStride = (int)((wWide + 2 + 63) & ~(int64_t)63);
+    const int wideGrid = wide.empty() ? 0 : (int)std::min<int64_t>((int64_t)wide.size(), (int64_t)ctx->numCUs * 16);
+    const size_t nIds = narrow.size() + wide.size() + rejected.size();
+    if ((err = ensure(ctx->bandIds, 4 * std::max<size_t>(nIds, 1))) != hipSuccess ||
+        (wideGrid && (err = ensure(ctx->bandScratch, (size_t)wideGrid * 4 * wideStride * 4)) != hipSuccess))
+        return hipFail(ctx, err, "allocating banded buffers");
+    std::vector<int32_t> all(narrow);
+    all.insert(all.end(), wide.begin(), wide.end());
+    all.insert(all.end(), rejected.begin(), rejected.end());
+    int32_t *dIds = static_cast<int32_t *>(ctx->bandIds.p);
+    if (!all.empty() &&
+        (err = hipMemcpyAsync(dIds, all.data(), 4 * all.size(), hipMemcpyHostToDevice, stream)) != hipSuccess)
+        return hipFail(ctx, err, "uploading banded pair lists");
+    if ((err = hipEventRecord(ctx->ev0, stream)) != hipSuccess)
+        return hipFail(ctx, err, "event");
+    BandedArgs a{};
+    a.hRes = in.dH;
+    a.vRes = in.dV;
+    a.hOff = static_cast<const int64_t *>(ctx->hOff.p);
+    a.vOff = static_cast<const int64_t *>(ctx->vOff.p);
+    a.scores = in.dScores;
+    a.gapOpen = si.o;
+    a.gapExtend = si.e;
+    a.mode = mode;
+    a.alpha = si.alpha;
+    a.table = static_cast<const int32_t *>(ctx->table32.p);
+    a.lower = lower_diag;
+    a.upper = upper_diag;
+    a.scratch = static_cast<int32_t *>(ctx->bandScratch.p);
+    a.wideStride = wideStride;
+    if (!narrow.empty()) {
+        a.ids = dIds;
+        a.nIds = (int32_t)narrow.size();
+        if ((err = launchBanded(a, seg, 0, stream)) != hipSuccess)
+            return hipFail(ctx, err, "launching banded kernel");
+    }
+    if (!wide.empty()) {
+        a.ids = dIds + narrow.size();
+        a.nIds = (int32_t)wide.size();
+        if ((err = launchBanded(a, 0, wideGrid, stream)) != hipSuccess)
+            return hipFail(ctx, err, "launching banded kernel");
+    }
+    if (!rejected.empty() &&
+        (err = launchFillScores(in.dScores, dIds + narrow.size() + wide.size(), (int)rejected.size(), INT32_MIN,
+                                stream)) != hipSuccess)
+        return hipFail(ctx, err, "launching fill kernel");
+    if ((err = hipEventRecord(ctx->ev1, stream)) != hipSuccess)
+        return hipFail(ctx, err, "event");
+    if (!(flags & SQMI_SCORES_ON_DEVICE) &&
+        (err = hipMemcpyAsync(scores, in.dScores, 4 * N, hipMemcpyDeviceToHost, stream)) != hipSuccess)
+        return hipFail(ctx, err, "copying scores");
+    if ((err = hipStreamSynchronize(stream)) != hipSuccess)
+        return hipFail(ctx, err, "banded kernels");
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess)
+        ctx->stats.device_ms = ms;
+    ctx->stats.total_ms = msSince(tStart);
+    return SQMI_OK;
 }
 
 sqmi_status sqmi_align_local_rows(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,

@@ -147,6 +147,29 @@ int wavesPerBlock16();
 int occupancy16(const Variant16 &v);
 bool variantSupported(const Variant16 &v);
 hipError_t launchSimple32(const Simple32Args &args, hipStream_t stream);
+
+// Banded global scores (banded.hip).  Pairs in `ids` passed the host's band
+// checks; their band, clipped to the matrix, is lower..upper.
+struct BandedArgs {
+    const uint8_t *hRes;
+    const uint8_t *vRes;
+    const int64_t *hOff;
+    const int64_t *vOff;
+    const int32_t *ids;
+    int32_t nIds;
+    int32_t *scores;            // indexed by pair id
+    int32_t gapOpen;
+    int32_t gapExtend;
+    uint32_t mode;              // SQMI_FREE_* bits
+    int32_t alpha;
+    const int32_t *table;       // alpha^2 substitution scores
+    int64_t lower, upper;       // the band: lower <= j - i <= upper
+    int32_t *scratch;           // wide kernel: 4 * wideStride int32 per workgroup
+    int32_t wideStride;
+};
+// segLanes: 1..64 (power of two >= every band width: kBandedNarrow), or 0
+// (kBandedWide on wideGrid workgroups).
+hipError_t launchBanded(const BandedArgs &args, int segLanes, int wideGrid, hipStream_t stream);
 // scores[ids[k]] = value for k < n.
 hipError_t launchFillScores(int32_t *scores, const int32_t *ids, int n, int32_t value, hipStream_t stream);
 

@@ -6,6 +6,7 @@ holds the reference tree and the oracle/_ref build):
     python tests/golden/make_golden.py --modes  # only the AlignConfig rows fixtures
     python tests/golden/make_golden.py --local  # only the local rows fixtures
     python tests/golden/make_golden.py --linear # only the linear-gap rows fixtures
+    python tests/golden/make_golden.py --banded # only the banded score fixtures
 
 * ``ref_*.npz``   -- inputs (residue ordinals + offsets) and the outputs of
   the reference library itself (oracle/_ref: SeqAn 2 globalAlignmentScore /
@@ -266,6 +267,72 @@ def linear_rows():
              begins=res[-1]["begins"] if modes[-1] == 16 else np.zeros((0, 2), np.int64), **extra)
 
 
+def banded_scores():
+    """Banded globalAlignmentScore(seqH, seqV, score, AlignConfig, lowerDiag,
+    upperDiag) for one pair (global_alignment_banded.h:394-470, driven with
+    repeats = -1) under several (mode, band) cases: `scores`.  Beside them,
+    `simd_scores` from the set path the ExecutionPolicy wrapper reaches
+    (:555-583) on the same collections, whose H and V each have one length as
+    its documentation requires (on these cases the two agree).  All bands
+    hold both corner diagonals with a margin and every H has two residues or
+    more: the domain where the reference's band code computes the banded
+    recurrence."""
+    rng = np.random.default_rng(0x5EED0400)
+
+    def equal_length(count, n, m, alpha):
+        hs, vs = [], []
+        for _ in range(count):
+            h = rng.integers(0, alpha, n).astype(np.uint8)
+            v = h[:m].copy() if m <= n else np.concatenate([h, rng.integers(0, alpha, m - n).astype(np.uint8)])
+            f = rng.random(m) < 0.12
+            v[f] = rng.integers(0, alpha, int(f.sum()))
+            hs.append(h)
+            vs.append(v.astype(np.uint8))
+        return sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+
+    def run(H, V, cases, kw, scalar=True):
+        out = []
+        for mode, lo, up in cases:
+            r = po.run_reference(po.OP_SCORE_BANDED, H.residues, H.offsets, V.residues, V.offsets, mode,
+                                 lower_diag=lo, upper_diag=up, repeats=-1 if scalar else 1, **kw)
+            out.append(r["scores"])
+        return np.stack(out)
+
+    dna = dict(alphabet="dna", **SIMPLE)
+    for name, count, n, m, alpha, kw, cases in (
+            ("ref_dna_banded_150.npz", 256, 150, 150, 4, dna,
+             [(0, -5, 5), (0, -1, 1), (0, -20, 7), (9, -10, 10), (15, -3, 12), (6, -31, 32)]),
+            ("ref_dna_banded_300x280.npz", 96, 300, 280, 4, dna,
+             [(0, -60, 80), (0, -10, 100), (9, -150, 150), (15, -5, 25), (0, -500, 500)]),
+            ("ref_dna_banded_linear.npz", 128, 120, 126, 4,
+             dict(alphabet="dna", match=1, mismatch=-1, gap_extend=-1, gap_open=-1),
+             [(0, -8, 8), (9, -40, 30), (15, -7, 2)]),
+            ("ref_protein_banded.npz", 96, 120, 100, 27,
+             dict(alphabet="aa", matrix="blosum62", gap_extend=-1, gap_open=-11),
+             [(0, -30, 40), (0, -1, 21), (9, -5, 70)])):
+        H, V = equal_length(count, n, m, 24 if alpha == 27 else alpha)
+        extra = {k: v for k, v in kw.items() if k not in ("alphabet", "matrix")}
+        save(name, H, V, cases=np.array(cases, np.int64), scores=run(H, V, cases, kw),
+             simd_scores=run(H, V, cases, kw, scalar=False), **extra)
+    # Varying lengths: the per-pair function.
+    hs, vs = [], []
+    for _ in range(200):
+        n = int(rng.integers(2, 260))
+        h = rng.integers(0, 4, n).astype(np.uint8)
+        v = np.delete(h, np.nonzero(rng.random(n) < 0.05)[0])
+        if len(v) == 0:
+            v = h.copy()
+        f = rng.random(len(v)) < 0.1
+        v[f] = rng.integers(0, 4, int(f.sum()))
+        hs.append(h)
+        vs.append(v)
+    H, V = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+    lim = int(max(abs(len(a) - len(b)) for a, b in zip(hs, vs)))
+    cases = [(0, -lim - 3, lim + 3), (9, -lim - 40, lim + 5), (15, -lim - 1, lim + 70)]
+    save("ref_dna_banded_varying.npz", H, V, cases=np.array(cases, np.int64), scores=run(H, V, cases, dna),
+         **SIMPLE)
+
+
 def long_pairs():
     """100 kbp fixture: the first two pairs of the 64 x 100 kbp workload at
     full length (run on its own: --long).  Scores come from the reference's
@@ -292,8 +359,11 @@ if __name__ == "__main__":
         local_rows()
     elif "--linear" in sys.argv:
         linear_rows()
+    elif "--banded" in sys.argv:
+        banded_scores()
     else:
         main()
         mode_rows()
         local_rows()
         linear_rows()
+        banded_scores()

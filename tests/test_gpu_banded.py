@@ -1,0 +1,98 @@
+"""Banded global alignment scores (sqmi_align_score_banded): the reference's
+banded globalAlignmentScore (global_alignment_banded.h:394-470, :555-583).
+
+Checked against fixtures written by the compiled reference (its set path on
+collections of one H length and one V length, the case its documentation
+supports, and its per-pair function on varying lengths), against the oracle
+on random collections of varying lengths, and against the unbanded kernels
+for bands that hold every diagonal.  Bands of up to 64 diagonals run on
+kBandedNarrow (several pairs per wave), wider ones on kBandedWide."""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import seqan_amd as sa
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+BANDED = ["ref_dna_banded_150.npz", "ref_dna_banded_300x280.npz", "ref_dna_banded_linear.npz",
+          "ref_protein_banded.npz", "ref_dna_banded_varying.npz"]
+
+
+def _score_of(z, fixture):
+    if "match" in z:
+        return sa.simple_score(int(z["match"]), int(z["mismatch"]), int(z["gap_extend"]), int(z["gap_open"]))
+    return sa.blosum62(int(z["gap_extend"]), int(z["gap_open"]))
+
+
+@pytest.mark.parametrize("fixture", BANDED)
+def test_banded_matches_reference_fixtures(ctx, fixture):
+    z = np.load(os.path.join(GOLDEN, fixture), allow_pickle=False)
+    score = _score_of(z, fixture)
+    Hs, Vs = sa.SeqSet(z["h_res"], z["h_off"]), sa.SeqSet(z["v_res"], z["v_off"])
+    for (mode, lo, up), want in zip(z["cases"], z["scores"]):
+        print("banded fixture", fixture, "mode", mode, "band", lo, up, flush=True)
+        got = ctx.align_score_banded(Hs, Vs, score, int(lo), int(up), mode=int(mode))
+        assert np.array_equal(got, want), (mode, lo, up, np.nonzero(got != want)[0][:5])
+
+
+@pytest.mark.parametrize("width", [3, 5, 17, 40, 64, 65, 130, 700])
+def test_banded_matches_oracle(ctx, width):
+    # Varying lengths (|H| - |V| up to 6 and inside the band with a margin),
+    # bands of the given width around diagonal 0; several modes; two scorings.
+    rng = np.random.default_rng(width)
+    lim = min(6, (width - 3) // 2)
+    hs, vs = [], []
+    for _ in range(300):
+        n = int(rng.integers(2, 400))
+        h = rng.integers(0, 4, n).astype(np.uint8)
+        v = h.copy()
+        f = rng.random(n) < 0.12
+        v[f] = rng.integers(0, 4, int(f.sum()))
+        d = int(rng.integers(-lim, lim + 1))
+        if d > 0:
+            v = v[:max(1, n - d)]
+        elif d < 0:
+            v = np.concatenate([v, rng.integers(0, 4, -d).astype(np.uint8)])
+        hs.append(h)
+        vs.append(v.astype(np.uint8))
+    Hs, Vs = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+    lo = -((width - 1) // 2)
+    up = lo + width - 1
+    assert all(lo < min(0, len(a) - len(b)) and max(0, len(a) - len(b)) < up for a, b in zip(hs, vs))
+    for score in (sa.simple_score(2, -3, -1, -5), sa.simple_score(3, -2, -3, -2)):
+        sc = po.Scoring(score.table, score.gap_open, score.gap_extend)
+        for mode in (0, 9, 15, 6):
+            print("banded vs oracle, width", width, "mode", mode, flush=True)
+            got = ctx.align_score_banded(Hs, Vs, score, lo, up, mode=mode)
+            want = np.array([po.score_banded(h, v, sc, mode, lo, up) for h, v in zip(hs, vs)], np.int32)
+            assert np.array_equal(got, want), (mode, np.nonzero(got != want)[0][:5])
+
+
+def test_banded_full_band_equals_unbanded(ctx):
+    H, V = sa.workload(sa.WL_DNA150, 0x5EED0401, 0, 2048)
+    score = sa.simple_score(2, -3, -1, -5)
+    for mode in (0, 9):
+        got = ctx.align_score_banded(H, V, score, -10000, 10000, mode=mode)
+        assert np.array_equal(got, ctx.align_score(H, V, score, mode))
+
+
+def test_banded_empty_pairs_and_errors(ctx):
+    Hs = sa.SeqSet.from_list([[0, 1, 2, 3], [], [1, 1]])
+    Vs = sa.SeqSet.from_list([[0, 1, 2], [1], [1, 1]])
+    score = sa.simple_score(2, -3, -1, -5)
+    got = ctx.align_score_banded(Hs, Vs, score, -2, 2)
+    assert got[1] == np.iinfo(np.int32).min
+    sc = po.Scoring(score.table, -5, -1)
+    assert got[0] == po.score_banded(np.array([0, 1, 2, 3], np.uint8), np.array([0, 1, 2], np.uint8), sc, 0, -2, 2)
+    assert got[2] == 4
+    # One diagonal; missing diagonal 0; an edge on diagonal |H| - |V| = 1; an empty band.
+    for lo, up in ((0, 0), (2, 5), (-5, 1), (3, 1)):
+        with pytest.raises(sa.SqmiError):
+            ctx.align_score_banded(Hs, Vs, score, lo, up)
+    with pytest.raises(sa.SqmiError):   # |H| = 1
+        ctx.align_score_banded(sa.SeqSet.from_list([[1]]), sa.SeqSet.from_list([[1, 2]]), score, -3, 3)
+    with pytest.raises(sa.SqmiError):
+        ctx.align_score_banded(Hs, Vs, score, -2, 2, mode=sa.LOCAL)

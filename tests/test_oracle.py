@@ -321,6 +321,34 @@ def test_oracle_local_traceback_vs_brute_force():
         assert int(c.astype(np.int64) @ par) == s
 
 
+BANDED = ["ref_dna_banded_150.npz", "ref_dna_banded_300x280.npz", "ref_dna_banded_linear.npz",
+          "ref_protein_banded.npz", "ref_dna_banded_varying.npz"]
+
+
+@pytest.mark.parametrize("fixture", BANDED)
+def test_oracle_banded_matches_reference(fixture):
+    # Banded globalAlignmentScore: the reference's set path on collections of
+    # one H length and one V length, its per-pair function on varying ones.
+    z = _load(fixture)
+    sc = _local_scoring(z, fixture)
+    hs = [z["h_res"][z["h_off"][k]:z["h_off"][k + 1]] for k in range(len(z["h_off"]) - 1)]
+    vs = [z["v_res"][z["v_off"][k]:z["v_off"][k + 1]] for k in range(len(z["v_off"]) - 1)]
+    for (mode, lo, up), want in zip(z["cases"], z["scores"]):
+        got = np.array([po.score_banded(h, v, sc, int(mode), int(lo), int(up)) for h, v in zip(hs, vs)], np.int32)
+        assert np.array_equal(got, want), (fixture, mode, lo, up)
+
+
+def test_oracle_banded_full_band_is_unbanded():
+    # A band holding every diagonal changes nothing.
+    rng = np.random.default_rng(31)
+    sc = po.Scoring.simple(2, -3, -1, -5)
+    for _ in range(200):
+        h = rng.integers(0, 4, int(rng.integers(1, 40))).astype(np.uint8)
+        v = rng.integers(0, 4, int(rng.integers(1, 40))).astype(np.uint8)
+        mode = int(rng.integers(0, 16))
+        assert po.score_banded(h, v, sc, mode, -len(v) - 3, len(h) + 2) == po.score(h, v, sc, mode)
+
+
 def test_100kbp_fixture_is_a_valid_alignment():
     # Two 100 kbp pairs: reference scores, oracle counts (the reference's own
     # traceback truncates trace-matrix positions past 2^32 at this size).
