@@ -15,6 +15,8 @@ Also measured in the same run, each reported beside the headline figure:
   - protein local: 262,144 Smith-Waterman pairs, Blosum62(-1, -11);
   - long fwd+bwd: score plus traceback counts of 64 DNA pairs of 100 kbp
     (tiled int32 kernels with checkpoints), and the forward alone;
+  - banded: 1,048,576 DNA pairs of 150 bp (8% substitutions), global, band
+    [-16, 16] (int32 lane-per-diagonal kernel), GCUPS over the band's cells;
   - the reference SeqAn 2 CPU path (oracle/_ref, built from the reference
     sources) on bounded samples of each workload, all host cores and one
     core; the GPU results of each sample are bit-checked against it.
@@ -39,6 +41,7 @@ MODE = sa.FREE_TOP | sa.FREE_BOTTOM
 SEED_PROTEIN, N_PROTEIN = 0x5EED0003, 1 << 18
 SEED_DNA1K, N_DNA1K = 0x5EED0004, 1 << 16
 SEED_DNA100K, N_DNA100K = 0x5EED0005, 64
+SEED_BANDED, N_BANDED, L_BANDED, BAND = 0x5EED0006, 1 << 20, 150, (-16, 16)
 # Peak int32 VALU operations (one element per lane): 256 CU x 64 lanes x 2.4 GHz.
 PEAK_TOPS_INT32 = 256 * 64 * 2.4e9 / 1e12
 # Peak packed-int16 VALU element operations: 256 CU x 64 lanes x 2 halves x 2.4 GHz.
@@ -240,6 +243,60 @@ def protein_leg(ctx, steps, warmup, cpu):
     return res
 
 
+def band_cells(n, m, lower, upper):
+    """Cells (i, j), 1 <= i <= m, 1 <= j <= n, with lower <= j - i <= upper."""
+    i = np.arange(1, m + 1)
+    lo = np.maximum(1, i + lower)
+    hi = np.minimum(n, i + upper)
+    return int(np.clip(hi - lo + 1, 0, None).sum())
+
+
+def banded_leg(ctx, steps, warmup, cpu):
+    """Banded global scores: pairs of one length (the collections the
+    reference's banded set path supports), substitutions only."""
+    rng = np.random.default_rng(SEED_BANDED)
+    h = rng.integers(0, 4, (N_BANDED, L_BANDED), dtype=np.uint8)
+    v = h.copy()
+    sub = rng.random(h.shape) < 0.08
+    v[sub] = rng.integers(0, 4, int(sub.sum()), dtype=np.uint8)
+    off = np.arange(N_BANDED + 1, dtype=np.int64) * L_BANDED
+    H, V = sa.SeqSet(h.reshape(-1), off), sa.SeqSet(v.reshape(-1), off.copy())
+    lo, up = BAND
+    cells = band_cells(L_BANDED, L_BANDED, lo, up) * N_BANDED
+    score = sa.simple_score(2, -3, -1, -5)
+    scores = [None]
+    ms = timed(lambda: scores.__setitem__(0, ctx.align_score_banded(H, V, score, lo, up)), steps, warmup, "banded")
+    st = ctx.stats()
+    kern = st["device_ms"]
+    achieved = OPS_PER_CELL * cells / (kern / 1e3) / 1e12
+    res = {"value": round(cells / (ms / 1e3) / 1e9, 2), "unit": "GCUPS (cells inside the band)",
+           "ms_per_step": round(ms, 3), "device_ms": round(kern, 3),
+           "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_TOPS_INT32, 2),
+                        "unit": "TOP/s", "frac": round(achieved / PEAK_TOPS_INT32, 4),
+                        "note": "int32 VALU ops, 8 per cell inside the band; kernel time from HIP events"},
+           "workload": f"{N_BANDED:,} DNA pairs of {L_BANDED} bp, 8% substitutions, global, band [{lo}, {up}], "
+                       "Score<int,Simple>(2,-3,-1,-5); host residues (PCIe copies inside the step)",
+           "pairs": N_BANDED, "cells": cells, "seed": SEED_BANDED}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        if po.ref_binary() is not None:
+            count = 1 << 17
+            hs, vs = head(H, count), head(V, count)
+            r = po.run_reference(po.OP_SCORE_BANDED, hs.residues, hs.offsets, vs.residues, vs.offsets, 0,
+                                 threads=host_cores(), alphabet="dna", match=2, mismatch=-3, gap_extend=-1,
+                                 gap_open=-5, lower_diag=lo, upper_diag=up)
+            c = band_cells(L_BANDED, L_BANDED, lo, up) * count
+            res["cpu_baseline"] = {
+                "value": round(c / r["best_seconds"] / 1e9, 3), "unit": "GCUPS (cells inside the band)",
+                "cores": int(r["threads"]), "kind": "reference",
+                "sample": f"first {count} pairs ({c:.3e} band cells); {r['binary']} banded set path, one timed "
+                          f"call; {cpu_model()}",
+                "parity_vs_gpu": bool(np.array_equal(r["scores"], scores[0][:count]))}
+            log(f"banded cpu baseline: {res['cpu_baseline']}")
+    return res
+
+
 def summarize_rocpd(paths):
     """Prints per-kernel duration statistics and counter totals of rocprofv3
     result databases (the rocpd SQLite output)."""
@@ -278,7 +335,7 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the fwd+bwd and protein legs")
-    ap.add_argument("--leg", choices=["fwd_bwd", "protein_local", "long_fwd_bwd"], default=None,
+    ap.add_argument("--leg", choices=["fwd_bwd", "protein_local", "long_fwd_bwd", "banded"], default=None,
                     help="run only this extra leg and print its JSON (for profiling)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also copy this many MiB device-to-device (counter calibration)")
@@ -288,7 +345,8 @@ def main():
         sys.exit(2)
 
     if args.leg:
-        legs = {"fwd_bwd": backward_leg, "protein_local": protein_leg, "long_fwd_bwd": long_leg}
+        legs = {"fwd_bwd": backward_leg, "protein_local": protein_leg, "long_fwd_bwd": long_leg,
+                "banded": banded_leg}
         print(json.dumps({args.leg: legs[args.leg](sa.Context(0), args.steps, args.warmup, not args.no_cpu)}))
         return
 
@@ -386,6 +444,7 @@ def main():
         extra["fwd_bwd"] = backward_leg(ctx, 3, 1, not args.no_cpu)
         extra["protein_local"] = protein_leg(ctx, 3, 1, not args.no_cpu)
         extra["long_fwd_bwd"] = long_leg(ctx, 3, 1, not args.no_cpu)
+        extra["banded"] = banded_leg(ctx, 3, 1, not args.no_cpu)
 
     res = {
         "metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,

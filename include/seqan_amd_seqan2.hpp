@@ -4,6 +4,8 @@
  * Including this header after (or instead of) <seqan/align_parallel.h> routes
  *
  *   globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial>, setH, setV, score [, AlignConfig<T,L,R,B>])
+ *   globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial>, setH, setV, score [, AlignConfig],
+ *                        lowerDiag, upperDiag)   (banded; see sqmi_align_score_banded for the bands taken)
  *   localAlignmentScore(ExecutionPolicy<Parallel, Vectorial>, setH, setV, score)
  *   globalAlignment(ExecutionPolicy<Parallel, Vectorial>, gapsSetH, gapsSetV, score [, AlignConfig<T,L,R,B>])
  *   localAlignment(ExecutionPolicy<Parallel, Vectorial>, gapsSetH, gapsSetV, score)
@@ -319,6 +321,30 @@ seqan2::String<int> align_scores(TSetH const & setH, TSetV const & setV, TScore 
 }
 
 template <typename TSetH, typename TSetV, typename TScore>
+seqan2::String<int> align_scores_banded(TSetH const & setH, TSetV const & setV, TScore const & scheme, uint32_t mode,
+                                        int64_t lowerDiag, int64_t upperDiag)
+{
+    using TAlphabet = typename SetAlphabet<TSetH>::Type;
+    if (seqan2::length(setH) != seqan2::length(setV))
+        throw Error(SQMI_ERR_LENGTH_MISMATCH, "seqan_amd: setH and setV differ in length");
+    seqan2::String<int> result;
+    if (seqan2::length(setH) == 0)
+        return result;
+    std::vector<uint8_t> resH, resV;
+    std::vector<int64_t> offH, offV;
+    pack(setH, resH, offH);
+    pack(setV, resV, offV);
+    std::vector<int32_t> table;
+    sqmi_scoring const sc = make_scoring<TAlphabet>(scheme, table);
+    sqmi_sequences const h{resH.data(), offH.data(), static_cast<int64_t>(offH.size()) - 1};
+    sqmi_sequences const v{resV.data(), offV.data(), static_cast<int64_t>(offV.size()) - 1};
+    seqan2::resize(result, seqan2::length(setH));
+    sqmi_context * ctx = thread_context();
+    check(sqmi_align_score_banded(ctx, &h, &v, &sc, mode, lowerDiag, upperDiag, &result[0], 0u), ctx);
+    return result;
+}
+
+template <typename TSetH, typename TSetV, typename TScore>
 std::vector<int> align_scores_vector(TSetH const & setH, TSetV const & setV, TScore const & scheme, uint32_t mode)
 {
     seqan2::String<int> s = align_scores(setH, setV, scheme, mode);
@@ -423,6 +449,33 @@ globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/
                      TSetH && setH, TSetV && setV, TScore && scoringScheme, TConfig && /*alignConfig*/)
 SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores(setH, setV, scoringScheme,
                                                         seqan_amd::detail::IsAlignConfig<std::decay_t<TConfig>>::mode))
+
+// Banded: the executor forwards (setH, setV, score [, config], lowerDiag,
+// upperDiag) to global_alignment_banded.h:555-583.  Bands outside the ones
+// sqmi_align_score_banded takes raise seqan_amd::Error(SQMI_ERR_UNSUPPORTED).
+template <typename TSetH, typename TSetV, typename TScore, typename TLower, typename TUpper,
+          std::enable_if_t<seqan_amd::detail::IsGpuCall<TSetH, TSetV, TScore>::value &&
+                           std::is_integral<std::decay_t<TLower>>::value &&
+                           std::is_integral<std::decay_t<TUpper>>::value, int> = 0>
+inline String<int>
+globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
+                     TSetH && setH, TSetV && setV, TScore && scoringScheme, TLower && lowerDiag, TUpper && upperDiag)
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores_banded(setH, setV, scoringScheme, 0u,
+                                                               static_cast<int64_t>(lowerDiag),
+                                                               static_cast<int64_t>(upperDiag)))
+
+template <typename TSetH, typename TSetV, typename TScore, typename TConfig, typename TLower, typename TUpper,
+          std::enable_if_t<seqan_amd::detail::IsGpuCall<TSetH, TSetV, TScore>::value &&
+                           seqan_amd::detail::IsAlignConfig<std::decay_t<TConfig>>::value &&
+                           std::is_integral<std::decay_t<TLower>>::value &&
+                           std::is_integral<std::decay_t<TUpper>>::value, int> = 0>
+inline String<int>
+globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
+                     TSetH && setH, TSetV && setV, TScore && scoringScheme, TConfig && /*alignConfig*/,
+                     TLower && lowerDiag, TUpper && upperDiag)
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores_banded(
+    setH, setV, scoringScheme, seqan_amd::detail::IsAlignConfig<std::decay_t<TConfig>>::mode,
+    static_cast<int64_t>(lowerDiag), static_cast<int64_t>(upperDiag)))
 
 template <typename TSetH, typename TSetV, typename TScore,
           std::enable_if_t<seqan_amd::detail::IsGpuCall<TSetH, TSetV, TScore>::value, int> = 0>

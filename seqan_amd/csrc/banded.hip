@@ -19,7 +19,10 @@
 // which is the unrolled E(j) = max(E(j-1) + gapExtend, H(j-1) + gapOpen).
 //   kBandedNarrow: bands of at most 64 diagonals; a wave holds 64 / S pairs
 //                  in segments of S lanes (S = the band width rounded up to a
-//                  power of two), the previous row in registers.
+//                  power of two), the previous row in registers; lane shifts
+//                  and the scan are DPP moves (wave_shl/shr:1, row_shr,
+//                  row_bcast), the substitution table sits in LDS and the
+//                  residue codes of the next row are loaded one row ahead.
 //   kBandedWide:   wider bands; one wave per pair, the band in chunks of 64
 //                  lanes, the previous row in a per-wave global buffer.
 #include "kernels.hpp"
@@ -77,6 +80,42 @@ __device__ __forceinline__ int32_t prefixMax(int32_t y, int k)
     return y;
 }
 
+// DPP lane moves; lanes without a source lane read kNegB.
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138, kDppRowShr = 0x110, kDppRowBcast15 = 0x142,
+              kDppRowBcast31 = 0x143;
+template <int Ctrl, int RowMask = 0xF>
+__device__ __forceinline__ int32_t dpp(int32_t x)
+{
+    return __builtin_amdgcn_update_dpp(kNegB, x, Ctrl, RowMask, 0xF, false);
+}
+
+// Inclusive prefix max over the S lanes of each segment (S a power of two):
+// row_shr steps inside the 16-lane rows (masked at segment edges for S < 16),
+// then the row broadcasts for segments of 32 and 64 lanes.
+template <int S>
+__device__ __forceinline__ int32_t prefixMaxDpp(int32_t y, int k)
+{
+    if constexpr (S >= 2) {
+        const int32_t t = dpp<kDppRowShr + 1>(y);
+        y = (S >= 16 || k >= 1) ? max(y, t) : y;
+    }
+    if constexpr (S >= 4) {
+        const int32_t t = dpp<kDppRowShr + 2>(y);
+        y = (S >= 16 || k >= 2) ? max(y, t) : y;
+    }
+    if constexpr (S >= 8) {
+        const int32_t t = dpp<kDppRowShr + 4>(y);
+        y = (S >= 16 || k >= 4) ? max(y, t) : y;
+    }
+    if constexpr (S >= 16)
+        y = max(y, dpp<kDppRowShr + 8>(y));
+    if constexpr (S >= 32)
+        y = max(y, dpp<kDppRowBcast15, 0xA>(y));
+    if constexpr (S >= 64)
+        y = max(y, dpp<kDppRowBcast31, 0xC>(y));
+    return y;
+}
+
 template <int S>
 __device__ __forceinline__ int32_t segmentMax(int32_t y)
 {
@@ -90,6 +129,10 @@ template <int S>
 __global__ __launch_bounds__(256) void kBandedNarrow(BandedArgs a)
 {
     constexpr int kPairsPerWave = 64 / S;
+    __shared__ int32_t sTab[32 * 32];
+    for (int x = threadIdx.x; x < a.alpha * a.alpha; x += blockDim.x)
+        sTab[x] = a.table[x];
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int k = lane % S, seg = lane / S;
@@ -117,35 +160,41 @@ __global__ __launch_bounds__(256) void kBandedNarrow(BandedArgs a)
         if (P.track(0, d, a.mode))
             best = hPrev;
     }
+    // Residues of row i, loaded one row ahead (the loads of row i + 1 are in
+    // flight while row i is computed); lanes without a cell read nothing.
+    auto hByte = [&](int i) -> int {
+        const int j = i + d;
+        return (inBand && i <= P.m && j >= 1 && j <= P.n) ? (int)P.h[j - 1] : 0;
+    };
+    auto vByte = [&](int i) -> int { return (has && i <= P.m) ? (int)P.v[i - 1] : 0; };
+    int hb = hByte(1), vb = vByte(1);
+    const bool upOut = k + 1 >= P.w;
     for (int i = 1; i <= rows; ++i) {
         const int j = i + d;
         const bool valid = inBand && i <= P.m && j >= 0 && j <= P.n;
-        int32_t hUp = __shfl_down(hPrev, 1, S), fUp = __shfl_down(fPrev, 1, S);
-        if (k + 1 >= P.w) {
-            hUp = kNegB;
-            fUp = kNegB;
-        }
-        int32_t F = max(fUp + e, hUp + o);
-        int32_t X = kNegB;
-        if (valid) {
-            if (j == 0) {
-                X = freeLeft ? 0 : F;
-                if (freeLeft)
-                    F = kNegB;
-            } else {
-                const int hc = min((int)P.h[j - 1], A - 1), vc = min((int)P.v[i - 1], A - 1);
-                X = max(hPrev + a.table[hc * A + vc], F);
-            }
-        }
-        const int32_t incl = prefixMax<S>(valid ? X + o - k * g : kNegB, k);
-        int32_t E = __shfl_up(incl, 1, S) + (k - 1) * g;
-        if (k == 0)
-            E = kNegB;
-        const int32_t H = valid ? (j == 0 ? X : max(X, E)) : kNegB;
-        if (valid && P.track(i, j, a.mode))
-            best = max(best, H);
+        const int hbNext = hByte(i + 1), vbNext = vByte(i + 1);
+        const int32_t sub = sTab[min(hb, A - 1) * A + min(vb, A - 1)];
+        hb = hbNext;
+        vb = vbNext;
+        // The DPP moves run with every lane enabled (a lane reads its
+        // neighbour's register only while that lane is active); the band
+        // edge is masked afterwards.
+        const int32_t hUpAll = dpp<kDppWaveShl1>(hPrev), fUpAll = dpp<kDppWaveShl1>(fPrev);
+        const int32_t hUp = upOut ? kNegB : hUpAll;
+        const int32_t fUp = upOut ? kNegB : fUpAll;
+        const int32_t F0 = max(fUp + e, hUp + o);
+        // Column 0: H is F, or 0 (and F unused) under a free first column.
+        const bool col0 = j == 0;
+        const int32_t X0 = col0 ? (freeLeft ? 0 : F0) : max(hPrev + sub, F0);
+        const int32_t X = valid ? X0 : kNegB;
+        const int32_t F = (valid && !(col0 && freeLeft)) ? F0 : kNegB;
+        const int32_t incl = prefixMaxDpp<S>(valid ? X + o - k * g : kNegB, k);
+        const int32_t inclLeft = dpp<kDppWaveShr1>(incl);
+        const int32_t E = k == 0 ? kNegB : inclLeft + (k - 1) * g;
+        const int32_t H = valid ? (col0 ? X : max(X, E)) : kNegB;
+        best = (valid && P.track(i, j, a.mode)) ? max(best, H) : best;
         hPrev = H;
-        fPrev = valid ? F : kNegB;
+        fPrev = F;
     }
     best = segmentMax<S>(best);
     if (has && k == 0)

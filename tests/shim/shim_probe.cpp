@@ -99,7 +99,14 @@ static_assert(gpuLocalRows<ExPV const &, DnaGaps &, DnaGaps &, Simple &>(), "loc
 static_assert(gpuLocalRows<ExPV, AaGaps &, AaGaps &, Blosum62 const &>(), "local rows, protein");
 static_assert(!gpuLocalRows<ExPS const &, AaGaps &, AaGaps &, Blosum62 &>(), "local rows, Parallel, Serial");
 
+// Banded scores (lowerDiag, upperDiag after the score or the AlignConfig).
+static_assert(gpuGlobal<ExPV const &, DnaSet &, DnaSet &, Simple &, int, int>(), "banded");
+static_assert(gpuGlobal<ExPV const &, DnaSet const &, DnaSet const &, Simple const &, Semi const &, int, int>(),
+              "banded + config");
+static_assert(gpuGlobal<ExPV, AaSet &, AaSet &, Blosum62 &, long, long>(), "banded, long diagonals");
+
 // Everything else stays with the reference templates.
+static_assert(!gpuGlobal<ExPS const &, DnaSet &, DnaSet &, Simple &, int, int>(), "banded, Parallel, Serial");
 static_assert(!gpuRows<ExPS const &, DnaGaps &, DnaGaps &, Simple &>(), "rows, Parallel, Serial");
 static_assert(!gpuRows<ExPV const &, DnaGaps &, DnaGaps &, Simple &, Semi &, Gotoh &>(), "rows, algorithm tag");
 static_assert(!gpuGlobal<ExPS const &, DnaSet &, DnaSet &, Simple &>(), "Parallel, Serial");

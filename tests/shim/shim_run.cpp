@@ -53,6 +53,27 @@ void randomPairs(StringSet<String<TAlphabet>> & setH, StringSet<String<TAlphabet
     }
 }
 
+// Pairs of one length with substitutions only: collections the reference's
+// banded set path supports (one H length, one V length).
+template <typename TAlphabet>
+void equalPairs(StringSet<String<TAlphabet>> & setH, StringSet<String<TAlphabet>> & setV, unsigned count,
+                unsigned len, unsigned seed)
+{
+    std::mt19937 rng(seed);
+    unsigned const alpha = ValueSize<TAlphabet>::VALUE;
+    for (unsigned i = 0; i < count; ++i)
+    {
+        String<TAlphabet> h, v;
+        for (unsigned k = 0; k < len; ++k)
+        {
+            appendValue(h, TAlphabet(rng() % alpha));
+            appendValue(v, rng() % 100 < 12 ? TAlphabet(rng() % alpha) : h[k]);
+        }
+        appendValue(setH, h);
+        appendValue(setV, v);
+    }
+}
+
 template <typename TFn, typename TRef>
 void runCase(char const * name, TFn && gpu, TRef && ref)
 {
@@ -238,6 +259,19 @@ int main()
             [&] { return globalAlignmentScore(cpu, dh, dv, dna, semi); });
     runCase("wavefront_local_blosum62", [&] { return asString(localAlignmentScore(wave, ph, pv, b62)); },
             [&] { return localAlignmentScore(cpu, ph, pv, b62); });
+
+    // Banded scores (the executor forwards lowerDiag, upperDiag to the banded
+    // set overload, global_alignment_banded.h:555-583).
+    StringSet<String<Dna>> eh, ev;
+    equalPairs(eh, ev, 200, 120, 21);
+    runCase("dna_banded", [&] { return globalAlignmentScore(ex, eh, ev, dna, -5, 5); },
+            [&] { return globalAlignmentScore(cpu, eh, ev, dna, -5, 5); });
+    runCase("dna_banded_semiglobal", [&] { return globalAlignmentScore(ex, eh, ev, dna, semi, -8, 12); },
+            [&] { return globalAlignmentScore(cpu, eh, ev, dna, semi, -8, 12); });
+    StringSet<String<AminoAcid>> eph, epv;
+    equalPairs(eph, epv, 96, 100, 22);
+    runCase("protein_banded_blosum62", [&] { return globalAlignmentScore(ex, eph, epv, b62, -70, 70); },
+            [&] { return globalAlignmentScore(cpu, eph, epv, b62, -70, 70); });
 
     runRowsCase("dna_rows", dh, dv, dna, ex, cpu);
     runRowsCase("protein_rows_blosum62", ph, pv, b62, ex, cpu);

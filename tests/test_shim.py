@@ -24,7 +24,8 @@ CASES = ["dna_global_nonconst", "dna_semiglobal_const", "dna_overlap_rvalue", "d
          "protein_global_pam120", "protein_local_selectable_blosum80", "wavefront_global_semi",
          "wavefront_local_blosum62", "dna_rows", "protein_rows_blosum62", "dna_rows_semiglobal",
          "dna_rows_overlap", "dna_local_rows", "protein_local_rows_blosum62",
-         "dna_rows_linear_gaps", "dna_rows_linear_gaps_semiglobal", "dna_local_rows_linear_gaps", "length_mismatch"]
+         "dna_rows_linear_gaps", "dna_rows_linear_gaps_semiglobal", "dna_local_rows_linear_gaps",
+         "dna_banded", "dna_banded_semiglobal", "protein_banded_blosum62", "length_mismatch"]
 
 
 def _run_cases():
