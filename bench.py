@@ -277,6 +277,23 @@ def banded_leg(ctx, steps, warmup, cpu):
            "workload": f"{N_BANDED:,} DNA pairs of {L_BANDED} bp, 8% substitutions, global, band [{lo}, {up}], "
                        "Score<int,Simple>(2,-3,-1,-5); host residues (PCIe copies inside the step)",
            "pairs": N_BANDED, "cells": cells, "seed": SEED_BANDED}
+    # Wide band (kBandedWide): 65,536 pairs of 1 kbp, band [-100, 100].
+    nw, lw, (wlo, wup) = 1 << 16, 1000, (-100, 100)
+    hw = rng.integers(0, 4, (nw, lw), dtype=np.uint8)
+    vw = hw.copy()
+    sub = rng.random(hw.shape) < 0.08
+    vw[sub] = rng.integers(0, 4, int(sub.sum()), dtype=np.uint8)
+    offw = np.arange(nw + 1, dtype=np.int64) * lw
+    Hw, Vw = sa.SeqSet(hw.reshape(-1), offw), sa.SeqSet(vw.reshape(-1), offw.copy())
+    wcells = band_cells(lw, lw, wlo, wup) * nw
+    wms = timed(lambda: ctx.align_score_banded(Hw, Vw, score, wlo, wup), steps, warmup, "banded wide")
+    wkern = ctx.stats()["device_ms"]
+    wach = OPS_PER_CELL * wcells / (wkern / 1e3) / 1e12
+    res["wide"] = {"value": round(wcells / (wms / 1e3) / 1e9, 2), "unit": "GCUPS (cells inside the band)",
+                   "ms_per_step": round(wms, 3), "device_ms": round(wkern, 3),
+                   "roofline_frac": round(wach / PEAK_TOPS_INT32, 4),
+                   "workload": f"{nw:,} DNA pairs of {lw} bp, 8% substitutions, global, band [{wlo}, {wup}]",
+                   "cells": wcells}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as po

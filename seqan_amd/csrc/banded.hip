@@ -89,30 +89,39 @@ __device__ __forceinline__ int32_t dpp(int32_t x)
     return __builtin_amdgcn_update_dpp(kNegB, x, Ctrl, RowMask, 0xF, false);
 }
 
-// Inclusive prefix max over the S lanes of each segment (S a power of two):
-// row_shr steps inside the 16-lane rows (masked at segment edges for S < 16),
-// then the row broadcasts for segments of 32 and 64 lanes.
+// DPP lane move where lanes without a source lane read 0 (bound_ctrl), so
+// that a max with it folds into one v_max_i32_dpp.
+template <int Ctrl, int RowMask = 0xF>
+__device__ __forceinline__ int32_t dpp0(int32_t x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, Ctrl, RowMask, 0xF, true);
+}
+
+// Inclusive prefix max over the S lanes of each segment (S a power of two)
+// of values biased to y - kNegB >= 0 (0 stands for minus infinity): row_shr
+// steps inside the 16-lane rows (masked at segment edges for S < 16), then
+// the row broadcasts for segments of 32 and 64 lanes.
 template <int S>
 __device__ __forceinline__ int32_t prefixMaxDpp(int32_t y, int k)
 {
     if constexpr (S >= 2) {
-        const int32_t t = dpp<kDppRowShr + 1>(y);
+        const int32_t t = dpp0<kDppRowShr + 1>(y);
         y = (S >= 16 || k >= 1) ? max(y, t) : y;
     }
     if constexpr (S >= 4) {
-        const int32_t t = dpp<kDppRowShr + 2>(y);
+        const int32_t t = dpp0<kDppRowShr + 2>(y);
         y = (S >= 16 || k >= 2) ? max(y, t) : y;
     }
     if constexpr (S >= 8) {
-        const int32_t t = dpp<kDppRowShr + 4>(y);
+        const int32_t t = dpp0<kDppRowShr + 4>(y);
         y = (S >= 16 || k >= 4) ? max(y, t) : y;
     }
     if constexpr (S >= 16)
-        y = max(y, dpp<kDppRowShr + 8>(y));
+        y = max(y, dpp0<kDppRowShr + 8>(y));
     if constexpr (S >= 32)
-        y = max(y, dpp<kDppRowBcast15, 0xA>(y));
+        y = max(y, dpp0<kDppRowBcast15, 0xA>(y));
     if constexpr (S >= 64)
-        y = max(y, dpp<kDppRowBcast31, 0xC>(y));
+        y = max(y, dpp0<kDppRowBcast31, 0xC>(y));
     return y;
 }
 
@@ -125,11 +134,19 @@ __device__ __forceinline__ int32_t segmentMax(int32_t y)
     return y;
 }
 
+// Rows per residue chunk of kBandedNarrow: every kChunkRows rows a segment
+// copies the V residues of the next rows and the H residues they meet into
+// LDS, so the row loop reads residues from LDS only.
+constexpr int kChunkRows = 64;
+
 template <int S>
 __global__ __launch_bounds__(256) void kBandedNarrow(BandedArgs a)
 {
     constexpr int kPairsPerWave = 64 / S;
+    constexpr int kHSpan = kChunkRows + 64;   // H columns a chunk meets: rows + band width - 1
     __shared__ int32_t sTab[32 * 32];
+    __shared__ uint8_t sV[4][kPairsPerWave][kChunkRows];
+    __shared__ uint8_t sH[4][kPairsPerWave][kHSpan];
     for (int x = threadIdx.x; x < a.alpha * a.alpha; x += blockDim.x)
         sTab[x] = a.table[x];
     __syncthreads();
@@ -160,22 +177,32 @@ __global__ __launch_bounds__(256) void kBandedNarrow(BandedArgs a)
         if (P.track(0, d, a.mode))
             best = hPrev;
     }
-    // Residues of row i, loaded one row ahead (the loads of row i + 1 are in
-    // flight while row i is computed); lanes without a cell read nothing.
-    auto hByte = [&](int i) -> int {
-        const int j = i + d;
-        return (inBand && i <= P.m && j >= 1 && j <= P.n) ? (int)P.h[j - 1] : 0;
-    };
-    auto vByte = [&](int i) -> int { return (has && i <= P.m) ? (int)P.v[i - 1] : 0; };
-    int hb = hByte(1), vb = vByte(1);
+    const int wv = threadIdx.x >> 6;
+    uint8_t *myV = sV[wv][seg];
+    uint8_t *myH = sH[wv][seg];
     const bool upOut = k + 1 >= P.w;
     for (int i = 1; i <= rows; ++i) {
+        const int r = (i - 1) % kChunkRows;
+        if (r == 0) {
+            // Chunk rows i .. i + kChunkRows - 1: V[i - 1 + x] and the H
+            // columns i + lo + c (c = 0 .. kChunkRows + w - 2), zero outside
+            // the sequences.  LDS operations of one wave complete in order,
+            // so the wave barriers keep the compiler from moving reads across.
+            __builtin_amdgcn_wave_barrier();
+            for (int x = k; x < kChunkRows; x += S) {
+                const int row = i + x;
+                myV[x] = (has && row <= P.m) ? P.v[row - 1] : 0;
+            }
+            for (int c = k; c < kChunkRows + S - 1; c += S) {
+                const int col = i + P.lo + c;
+                myH[c] = (has && col >= 1 && col <= P.n) ? P.h[col - 1] : 0;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
         const int j = i + d;
         const bool valid = inBand && i <= P.m && j >= 0 && j <= P.n;
-        const int hbNext = hByte(i + 1), vbNext = vByte(i + 1);
+        const int hb = myH[r + k], vb = myV[r];
         const int32_t sub = sTab[min(hb, A - 1) * A + min(vb, A - 1)];
-        hb = hbNext;
-        vb = vbNext;
         // The DPP moves run with every lane enabled (a lane reads its
         // neighbour's register only while that lane is active); the band
         // edge is masked afterwards.
@@ -188,9 +215,12 @@ __global__ __launch_bounds__(256) void kBandedNarrow(BandedArgs a)
         const int32_t X0 = col0 ? (freeLeft ? 0 : F0) : max(hPrev + sub, F0);
         const int32_t X = valid ? X0 : kNegB;
         const int32_t F = (valid && !(col0 && freeLeft)) ? F0 : kNegB;
-        const int32_t incl = prefixMaxDpp<S>(valid ? X + o - k * g : kNegB, k);
-        const int32_t inclLeft = dpp<kDppWaveShr1>(incl);
-        const int32_t E = k == 0 ? kNegB : inclLeft + (k - 1) * g;
+        // The scan runs on values biased by -kNegB (cells that can matter
+        // are far above kNegB; 0 then stands for minus infinity, and a
+        // value derived from minus infinity never wins a cell).
+        const int32_t incl = prefixMaxDpp<S>(valid ? max(X - kNegB + o - k * g, 0) : 0, k);
+        const int32_t inclLeft = dpp0<kDppWaveShr1>(incl);
+        const int32_t E = k == 0 ? kNegB : inclLeft + kNegB + (k - 1) * g;
         const int32_t H = valid ? (col0 ? X : max(X, E)) : kNegB;
         best = (valid && P.track(i, j, a.mode)) ? max(best, H) : best;
         hPrev = H;
