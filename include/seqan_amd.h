@@ -25,7 +25,9 @@
  *                           setH, setV, score [, AlignConfig], lowerDiag, upperDiag)
  *                           (the executor forwards to the banded set overload,
  *                           include/seqan/align/global_alignment_banded.h:555-583;
- *                           one pair: :394-470)
+ *                           one pair: :394-470), and with SQMI_LOCAL
+ *                           localAlignmentScore(..., lowerDiag, upperDiag)
+ *                           (local_alignment_banded.h:317-383)
  *   sqmi_align_local_rows <- localAlignment(ExecutionPolicy<Parallel,Vectorial>,
  *                           gapsSetH, gapsSetV, score)
  *                           include/seqan/align_parallel/parallel_align_interface.h:312-321,
@@ -185,15 +187,17 @@ sqmi_status sqmi_align_local_rows(sqmi_context *ctx, const sqmi_sequences *set_h
                                   const int64_t *op_offsets, int64_t *op_lengths,
                                   int64_t *begins, uint32_t flags);
 
-/* Banded global alignment scores: only cells (i, j) with lower_diag <= j - i
- * <= upper_diag exist (i a position in V, j in H), under the free-end bits of
- * mode (SQMI_LOCAL is not supported).  Every non-empty pair must have
+/* Banded alignment scores: only cells (i, j) with lower_diag <= j - i <=
+ * upper_diag exist (i a position in V, j in H), under the free-end bits of
+ * mode, or SQMI_LOCAL.  Global modes: every non-empty pair must have
  * |H| >= 2 and lower_diag < min(0, |H| - |V|), max(0, |H| - |V|) < upper_diag:
  * the band holds both corner diagonals with a margin (the reference accepts
  * only bands holding both corners unless ends are free,
  * dp_algorithm_impl.h:192-232, and outside this domain its special band
- * paths depart from the banded recurrence); otherwise the call returns
- * SQMI_ERR_UNSUPPORTED.  Empty pairs get INT32_MIN.  Every pair follows the
+ * paths depart from the banded recurrence).  SQMI_LOCAL: |H| >= 2, at least
+ * two diagonals inside the matrix, lower_diag <= |H| - 2 and upper_diag >=
+ * 2 - |V|.  Otherwise the call returns SQMI_ERR_UNSUPPORTED.  Empty pairs get
+ * INT32_MIN.  Every pair follows the
  * reference's banded DP for one pair (global_alignment_banded.h:394-470);
  * the reference's vectorised set path agrees with it when all H and all V
  * have one length each, the case its documentation supports.  Flags: only

@@ -6,6 +6,7 @@
  *   globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial>, setH, setV, score [, AlignConfig<T,L,R,B>])
  *   globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial>, setH, setV, score [, AlignConfig],
  *                        lowerDiag, upperDiag)   (banded; see sqmi_align_score_banded for the bands taken)
+ *   localAlignmentScore(ExecutionPolicy<Parallel, Vectorial>, setH, setV, score, lowerDiag, upperDiag)
  *   localAlignmentScore(ExecutionPolicy<Parallel, Vectorial>, setH, setV, score)
  *   globalAlignment(ExecutionPolicy<Parallel, Vectorial>, gapsSetH, gapsSetV, score [, AlignConfig<T,L,R,B>])
  *   localAlignment(ExecutionPolicy<Parallel, Vectorial>, gapsSetH, gapsSetV, score)
@@ -449,6 +450,18 @@ globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/
                      TSetH && setH, TSetV && setV, TScore && scoringScheme, TConfig && /*alignConfig*/)
 SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores(setH, setV, scoringScheme,
                                                         seqan_amd::detail::IsAlignConfig<std::decay_t<TConfig>>::mode))
+
+template <typename TSetH, typename TSetV, typename TScore, typename TLower, typename TUpper,
+          std::enable_if_t<seqan_amd::detail::IsGpuCall<TSetH, TSetV, TScore>::value &&
+                           std::is_integral<std::decay_t<TLower>>::value &&
+                           std::is_integral<std::decay_t<TUpper>>::value, int> = 0>
+inline String<int>
+localAlignmentScore(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
+                    TSetH && setH, TSetV && setV, TScore && scoringScheme, TLower && lowerDiag, TUpper && upperDiag)
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores_banded(setH, setV, scoringScheme,
+                                                               static_cast<uint32_t>(SQMI_LOCAL),
+                                                               static_cast<int64_t>(lowerDiag),
+                                                               static_cast<int64_t>(upperDiag)))
 
 // Banded: the executor forwards (setH, setV, score [, config], lowerDiag,
 // upperDiag) to global_alignment_banded.h:555-583.  Bands outside the ones

@@ -124,8 +124,12 @@ int32_t orc_score(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
 int32_t orc_score_banded(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
                          const orc_scoring *sc, uint32_t mode, int64_t lower, int64_t upper)
 {
-    if (n <= 0 || m <= 0 || (mode & ORC_LOCAL)) return INT32_MIN;
+    const int local = (mode & ORC_LOCAL) != 0;
+    if (local) mode = ORC_LOCAL | ORC_FREE_TOP | ORC_FREE_LEFT | ORC_FREE_RIGHT | ORC_FREE_BOTTOM;
+    if (n <= 0 || m <= 0) return INT32_MIN;
     if (upper < -m || lower > n) return INT32_MIN;
+    /* Local alignment has free ends on every side (IsFreeEndGap_,
+     * dp_profile.h:604-612), so only the intersection check applies. */
     if (upper < 0 && !(mode & ORC_FREE_LEFT)) return INT32_MIN;
     if (lower > 0 && !(mode & ORC_FREE_TOP)) return INT32_MIN;
     if (upper + m < n && !(mode & ORC_FREE_BOTTOM)) return INT32_MIN;
@@ -184,7 +188,8 @@ int32_t orc_score_banded(const uint8_t *h, int64_t n, const uint8_t *v, int64_t 
             diag = H[i];
             E[i] = max2(E[i] + e, H[i] + o);
             F = max2(F + e, up + o);
-            const int32_t x = max2(d, max2(F, E[i]));
+            int32_t x = max2(d, max2(F, E[i]));
+            if (local) x = max2(x, 0);
             H[i] = x;
             up = x;
             if (tracked(mode, col, i == m ? 2 : 1) && x > best) best = x;

@@ -60,8 +60,9 @@ int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
                       const orc_scoring *sc, uint32_t mode, int32_t *counts,
                       char *ops, int64_t *n_ops, int64_t *begins);
 
-/* Banded global score: cells with lower <= j - i <= upper only; band
- * settings the reference rejects give INT32_MIN. */
+/* Banded score: cells with lower <= j - i <= upper only; band settings the
+ * reference rejects give INT32_MIN.  Global modes, or ORC_LOCAL (the band
+ * need only meet the matrix). */
 int32_t orc_score_banded(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
                          const orc_scoring *sc, uint32_t mode, int64_t lower, int64_t upper);
 

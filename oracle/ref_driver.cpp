@@ -23,7 +23,8 @@
 //
 // Input file (little endian):
 //   char[4] "SQRI", u32 op (0 score, 1 counts-simd, 2 counts-scalar, 3 table,
-//   4 banded score: i64 lower_diag, i64 upper_diag follow the count),
+//   4 banded score, global or LOCAL: i64 lower_diag, i64 upper_diag follow
+//   the count),
 //   u32 mode (1 TOP, 2 LEFT, 4 RIGHT, 8 BOTTOM, 16 LOCAL),
 //   u32 alphabet (0 Dna, 1 AminoAcid), i32 match, i32 mismatch,
 //   i32 gap_extend, i32 gap_open, u32 matrix (0 Simple, 1 Blosum62),
@@ -234,26 +235,28 @@ int run(RefInput const & in, TScore const & sc, int alpha, std::FILE * out)
     if (h.op == 4)
     {
         // Banded globalAlignmentScore(setH, setV, score, AlignConfig, lowerDiag,
-        // upperDiag) (global_alignment_banded.h:555-583), SIMD over the sets.
-        if (h.mode & 16u)
-            die("banded scores are produced for global alignment (AlignConfig) only");
+        // upperDiag) (global_alignment_banded.h:555-583), SIMD over the sets;
+        // with mode LOCAL localAlignmentScore(setH, setV, score, lowerDiag,
+        // upperDiag) (local_alignment_banded.h:344-383).
+        bool const local = (h.mode & 16u) != 0;
+        int const lo = static_cast<int>(h.lowerDiag), up = static_cast<int>(h.upperDiag);
         auto t0 = std::chrono::steady_clock::now();
         if (h.repeats < 0)
         {
             // repeats == -1: the scalar banded function, pair by pair
-            // (global_alignment_banded.h:394-470).
+            // (global_alignment_banded.h:394-470, local_alignment_banded.h:317-342).
             for (int64_t k = 0; k < h.count; ++k)
-                scores[k] = withConfig(h.mode, [&](auto cfg) {
-                    return globalAlignmentScore(setH[k], setV[k], sc, cfg, static_cast<int>(h.lowerDiag),
-                                                static_cast<int>(h.upperDiag));
-                });
+                scores[k] = local ? localAlignmentScore(setH[k], setV[k], sc, lo, up)
+                                  : withConfig(h.mode, [&](auto cfg) {
+                                        return globalAlignmentScore(setH[k], setV[k], sc, cfg, lo, up);
+                                    });
         }
         else
         {
-            String<int> res = withConfig(h.mode, [&](auto cfg) {
-                return globalAlignmentScore(setH, setV, sc, cfg, static_cast<int>(h.lowerDiag),
-                                            static_cast<int>(h.upperDiag));
-            });
+            String<int> res = local ? localAlignmentScore(setH, setV, sc, lo, up)
+                                    : withConfig(h.mode, [&](auto cfg) {
+                                          return globalAlignmentScore(setH, setV, sc, cfg, lo, up);
+                                      });
             for (int64_t k = 0; k < h.count; ++k)
                 scores[k] = res[k];
         }

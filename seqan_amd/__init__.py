@@ -508,9 +508,14 @@ def global_alignment_score(set_h, set_v, score: Score, align_config=(False, Fals
                            _mode_from_config(align_config))
 
 
-def local_alignment_score(set_h, set_v, score: Score, alphabet=AMINO_ACIDS, ctx: Context = None):
-    """localAlignmentScore(ExecutionPolicy<Parallel,Vectorial>, setH, setV, score)."""
+def local_alignment_score(set_h, set_v, score: Score, alphabet=AMINO_ACIDS, ctx: Context = None,
+                          lower_diag=None, upper_diag=None):
+    """localAlignmentScore(ExecutionPolicy<Parallel,Vectorial>, setH, setV, score
+    [, lowerDiag, upperDiag])."""
     ctx = ctx or default_context()
+    if lower_diag is not None or upper_diag is not None:
+        return ctx.align_score_banded(_as_set(set_h, alphabet), _as_set(set_v, alphabet), score, lower_diag,
+                                      upper_diag, LOCAL)
     return ctx.align_score(_as_set(set_h, alphabet), _as_set(set_v, alphabet), score, LOCAL)
 
 

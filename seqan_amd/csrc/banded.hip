@@ -39,7 +39,8 @@ struct BandPair {
     const uint8_t *h, *v;
     __device__ bool track(int i, int j, uint32_t mode) const
     {
-        return (i == m && (j == n || (mode & SQMI_FREE_BOTTOM))) || (j == n && (mode & SQMI_FREE_RIGHT));
+        return (mode & SQMI_LOCAL) || (i == m && (j == n || (mode & SQMI_FREE_BOTTOM))) ||
+               (j == n && (mode & SQMI_FREE_RIGHT));
     }
 };
 
@@ -162,7 +163,11 @@ __global__ __launch_bounds__(256) void kBandedNarrow(BandedArgs a)
         P = loadPair(a, a.ids[t]);
     const int32_t o = a.gapOpen, e = a.gapExtend, g = max(o, e);
     const int A = a.alpha;
-    const bool freeTop = (a.mode & SQMI_FREE_TOP) != 0, freeLeft = (a.mode & SQMI_FREE_LEFT) != 0;
+    // Local: free borders, every cell tracked, H never below 0
+    // (local_alignment_banded.h, dp_formula_affine.h:110-118).
+    const bool freeTop = (a.mode & (SQMI_FREE_TOP | SQMI_LOCAL)) != 0;
+    const bool freeLeft = (a.mode & (SQMI_FREE_LEFT | SQMI_LOCAL)) != 0;
+    const int32_t floorH = (a.mode & SQMI_LOCAL) ? 0 : kNegB;
     const bool inBand = has && k < P.w;
     const int d = P.lo + k;
     // Rows of the wave: the longest pair of its segments.
@@ -212,7 +217,7 @@ __global__ __launch_bounds__(256) void kBandedNarrow(BandedArgs a)
         const int32_t F0 = max(fUp + e, hUp + o);
         // Column 0: H is F, or 0 (and F unused) under a free first column.
         const bool col0 = j == 0;
-        const int32_t X0 = col0 ? (freeLeft ? 0 : F0) : max(hPrev + sub, F0);
+        const int32_t X0 = col0 ? (freeLeft ? 0 : F0) : max(max(hPrev + sub, F0), floorH);
         const int32_t X = valid ? X0 : kNegB;
         const int32_t F = (valid && !(col0 && freeLeft)) ? F0 : kNegB;
         // The scan runs on values biased by -kNegB (cells that can matter
@@ -240,7 +245,11 @@ __global__ __launch_bounds__(64) void kBandedWide(BandedArgs a)
     const int k = threadIdx.x;
     const int32_t o = a.gapOpen, e = a.gapExtend, g = max(o, e);
     const int A = a.alpha;
-    const bool freeTop = (a.mode & SQMI_FREE_TOP) != 0, freeLeft = (a.mode & SQMI_FREE_LEFT) != 0;
+    // Local: free borders, every cell tracked, H never below 0
+    // (local_alignment_banded.h, dp_formula_affine.h:110-118).
+    const bool freeTop = (a.mode & (SQMI_FREE_TOP | SQMI_LOCAL)) != 0;
+    const bool freeLeft = (a.mode & (SQMI_FREE_LEFT | SQMI_LOCAL)) != 0;
+    const int32_t floorH = (a.mode & SQMI_LOCAL) ? 0 : kNegB;
     const int stride = a.wideStride;   // wmax + 2, padded
     int32_t *buf = a.scratch + (int64_t)blockIdx.x * 4 * stride;
     for (int t = blockIdx.x; t < a.nIds; t += gridDim.x) {
@@ -278,7 +287,7 @@ __global__ __launch_bounds__(64) void kBandedWide(BandedArgs a)
                             F = kNegB;
                     } else {
                         const int hc = min((int)P.h[j - 1], A - 1);
-                        X = max(hp[c] + a.table[hc * A + vc], F);
+                        X = max(max(hp[c] + a.table[hc * A + vc], F), floorH);
                     }
                 }
                 const int32_t incl = prefixMax<64>(valid ? X + o - k * g : kNegB, k);

@@ -1462,8 +1462,16 @@ sqmi_status sqmi_align_rows(sqmi_context *ctx, const sqmi_sequences *setH, const
 // recurrence under free end gaps or for |H| = 1, so such requests are
 // refused.  Without free ends the reference itself accepts only bands that
 // hold both corners (_checkBandProperties, dp_algorithm_impl.h:192-232).
-static bool bandSupported(int64_t n, int64_t m, int64_t lower, int64_t upper)
+//
+// Local alignment (every end free): |H| >= 2, at least two diagonals, and
+// the band reaches past the corner cells (0, n - 1) and (m - 1, 0): bands
+// that only touch a corner, or have one diagonal, again take the special
+// paths.
+static bool bandSupported(int64_t n, int64_t m, int64_t lower, int64_t upper, bool local)
 {
+    if (local)
+        return n >= 2 && lower < upper && lower <= n - 2 && upper >= -(m - 2) &&
+               std::min<int64_t>(upper, n) - std::max<int64_t>(lower, -m) >= 1;
     return n >= 2 && lower < std::min<int64_t>(0, n - m) && upper > std::max<int64_t>(0, n - m);
 }
 
@@ -1479,8 +1487,7 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
     sqmi_status st;
     if ((st = validate(ctx, setH, setV, scoring, mode)) != SQMI_OK)
         return st;
-    if (mode & SQMI_LOCAL)
-        return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_score_banded: global modes only");
+    const bool local = (mode & SQMI_LOCAL) != 0;
     if (flags & ~(uint32_t)(SQMI_RESIDUES_ON_DEVICE | SQMI_SCORES_ON_DEVICE))
         return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_score_banded: flags other than the *_ON_DEVICE ones");
     if (lower_diag > upper_diag)
@@ -1504,10 +1511,12 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
             rejected.push_back((int32_t)i);
             continue;
         }
-        if (!bandSupported(n, m, lower_diag, upper_diag))
+        if (!bandSupported(n, m, lower_diag, upper_diag, local))
             return fail(ctx, SQMI_ERR_UNSUPPORTED,
                         "sqmi_align_score_banded: pair " + std::to_string(i) +
-                            ": needs |H| >= 2 and lower_diag < min(0, |H| - |V|) < max(0, |H| - |V|) < upper_diag");
+                            (local ? ": needs |H| >= 2, two diagonals, lower_diag <= |H| - 2 and upper_diag >= 2 - |V|"
+                                   : ": needs |H| >= 2 and lower_diag < min(0, |H| - |V|) < max(0, |H| - |V|) "
+                                     "< upper_diag"));
         const int64_t w = std::min<int64_t>(upper_diag, n) - std::max<int64_t>(lower_diag, -m) + 1;
         cells += n * m;
         if (w <= 64) {

@@ -270,7 +270,9 @@ def linear_rows():
 def banded_scores():
     """Banded globalAlignmentScore(seqH, seqV, score, AlignConfig, lowerDiag,
     upperDiag) for one pair (global_alignment_banded.h:394-470, driven with
-    repeats = -1) under several (mode, band) cases: `scores`.  Beside them,
+    repeats = -1), and localAlignmentScore(..., lowerDiag, upperDiag) for
+    mode 16 (local_alignment_banded.h:317-342), under several (mode, band)
+    cases: `scores`.  Beside them,
     `simd_scores` from the set path the ExecutionPolicy wrapper reaches
     (:555-583) on the same collections, whose H and V each have one length as
     its documentation requires (on these cases the two agree).  All bands
@@ -301,15 +303,16 @@ def banded_scores():
     dna = dict(alphabet="dna", **SIMPLE)
     for name, count, n, m, alpha, kw, cases in (
             ("ref_dna_banded_150.npz", 256, 150, 150, 4, dna,
-             [(0, -5, 5), (0, -1, 1), (0, -20, 7), (9, -10, 10), (15, -3, 12), (6, -31, 32)]),
+             [(0, -5, 5), (0, -1, 1), (0, -20, 7), (9, -10, 10), (15, -3, 12), (6, -31, 32), (16, -5, 5),
+              (16, -20, 7)]),
             ("ref_dna_banded_300x280.npz", 96, 300, 280, 4, dna,
-             [(0, -60, 80), (0, -10, 100), (9, -150, 150), (15, -5, 25), (0, -500, 500)]),
+             [(0, -60, 80), (0, -10, 100), (9, -150, 150), (15, -5, 25), (0, -500, 500), (16, -60, 80)]),
             ("ref_dna_banded_linear.npz", 128, 120, 126, 4,
              dict(alphabet="dna", match=1, mismatch=-1, gap_extend=-1, gap_open=-1),
              [(0, -8, 8), (9, -40, 30), (15, -7, 2)]),
             ("ref_protein_banded.npz", 96, 120, 100, 27,
              dict(alphabet="aa", matrix="blosum62", gap_extend=-1, gap_open=-11),
-             [(0, -30, 40), (0, -1, 21), (9, -5, 70)])):
+             [(0, -30, 40), (0, -1, 21), (9, -5, 70), (16, -30, 40), (16, 3, 90)])):
         H, V = equal_length(count, n, m, 24 if alpha == 27 else alpha)
         extra = {k: v for k, v in kw.items() if k not in ("alphabet", "matrix")}
         save(name, H, V, cases=np.array(cases, np.int64), scores=run(H, V, cases, kw),
@@ -328,7 +331,7 @@ def banded_scores():
         vs.append(v)
     H, V = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
     lim = int(max(abs(len(a) - len(b)) for a, b in zip(hs, vs)))
-    cases = [(0, -lim - 3, lim + 3), (9, -lim - 40, lim + 5), (15, -lim - 1, lim + 70)]
+    cases = [(0, -lim - 3, lim + 3), (9, -lim - 40, lim + 5), (15, -lim - 1, lim + 70), (16, -lim - 3, lim + 3)]
     save("ref_dna_banded_varying.npz", H, V, cases=np.array(cases, np.int64), scores=run(H, V, cases, dna),
          **SIMPLE)
 

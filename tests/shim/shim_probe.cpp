@@ -104,6 +104,7 @@ static_assert(gpuGlobal<ExPV const &, DnaSet &, DnaSet &, Simple &, int, int>(),
 static_assert(gpuGlobal<ExPV const &, DnaSet const &, DnaSet const &, Simple const &, Semi const &, int, int>(),
               "banded + config");
 static_assert(gpuGlobal<ExPV, AaSet &, AaSet &, Blosum62 &, long, long>(), "banded, long diagonals");
+static_assert(gpuLocal<ExPV const &, AaSet &, AaSet &, Blosum62 &, int, int>(), "local banded");
 
 // Everything else stays with the reference templates.
 static_assert(!gpuGlobal<ExPS const &, DnaSet &, DnaSet &, Simple &, int, int>(), "banded, Parallel, Serial");

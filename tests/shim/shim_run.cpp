@@ -272,6 +272,8 @@ int main()
     equalPairs(eph, epv, 96, 100, 22);
     runCase("protein_banded_blosum62", [&] { return globalAlignmentScore(ex, eph, epv, b62, -70, 70); },
             [&] { return globalAlignmentScore(cpu, eph, epv, b62, -70, 70); });
+    runCase("protein_local_banded_blosum62", [&] { return localAlignmentScore(ex, eph, epv, b62, -20, 30); },
+            [&] { return localAlignmentScore(cpu, eph, epv, b62, -20, 30); });
 
     runRowsCase("dna_rows", dh, dv, dna, ex, cpu);
     runRowsCase("protein_rows_blosum62", ph, pv, b62, ex, cpu);

@@ -94,5 +94,46 @@ def test_banded_empty_pairs_and_errors(ctx):
             ctx.align_score_banded(Hs, Vs, score, lo, up)
     with pytest.raises(sa.SqmiError):   # |H| = 1
         ctx.align_score_banded(sa.SeqSet.from_list([[1]]), sa.SeqSet.from_list([[1, 2]]), score, -3, 3)
-    with pytest.raises(sa.SqmiError):
-        ctx.align_score_banded(Hs, Vs, score, -2, 2, mode=sa.LOCAL)
+
+
+@pytest.mark.parametrize("band", [(-4, 4), (-20, 7), (3, 50), (-90, -10), (-300, 300)])
+def test_banded_local_matches_oracle(ctx, band):
+    # Local banded scores (localAlignmentScore with lowerDiag, upperDiag):
+    # bands need not hold diagonal 0; narrow and wide kernels.
+    rng = np.random.default_rng(abs(band[0]) * 1000 + abs(band[1]))
+    hs, vs = [], []
+    for _ in range(200):
+        n = int(rng.integers(100, 400))
+        h = rng.integers(0, 4, n).astype(np.uint8)
+        a = int(rng.integers(0, n // 2))
+        seg = h[a:a + int(rng.integers(20, 200))].copy()
+        f = rng.random(len(seg)) < 0.12
+        seg[f] = rng.integers(0, 4, int(f.sum()))
+        v = np.concatenate([rng.integers(0, 4, int(rng.integers(0, 60))).astype(np.uint8), seg,
+                            rng.integers(0, 4, int(rng.integers(0, 60))).astype(np.uint8)])
+        hs.append(h)
+        vs.append(v.astype(np.uint8))
+    Hs, Vs = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+    lo, up = band
+    for score in (sa.simple_score(2, -3, -1, -5), sa.simple_score(1, -1, -1, -1)):
+        sc = po.Scoring(score.table, score.gap_open, score.gap_extend)
+        print("local banded vs oracle, band", band, flush=True)
+        got = ctx.align_score_banded(Hs, Vs, score, lo, up, mode=sa.LOCAL)
+        want = np.array([po.score_banded(h, v, sc, sa.LOCAL, lo, up) for h, v in zip(hs, vs)], np.int32)
+        assert np.array_equal(got, want), np.nonzero(got != want)[0][:5]
+
+
+def test_banded_local_full_band_equals_unbanded(ctx):
+    H, V = sa.workload(sa.WL_PROTEIN, 0x5EED0402, 0, 2048)
+    score = sa.blosum62(-1, -11)
+    got = ctx.align_score_banded(H, V, score, -5000, 5000, mode=sa.LOCAL)
+    assert np.array_equal(got, ctx.align_score(H, V, score, sa.LOCAL))
+
+
+def test_banded_local_unsupported(ctx):
+    score = sa.simple_score(2, -3, -1, -5)
+    Hs = sa.SeqSet.from_list([[0, 1, 2, 3, 0, 1]])
+    Vs = sa.SeqSet.from_list([[0, 1, 2, 3]])
+    for lo, up in ((2, 2), (5, 9), (-9, -3)):   # one diagonal; touching a corner only
+        with pytest.raises(sa.SqmiError):
+            ctx.align_score_banded(Hs, Vs, score, lo, up, mode=sa.LOCAL)
