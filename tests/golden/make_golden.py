@@ -102,17 +102,17 @@ def main():
         json.dump(pa, f, indent=1, sort_keys=True)
     print("wrote pair_align.json")
 
-    # --- config 1: 4,096 x 150 bp, global, full output -------------------
+    # --- 4,096 DNA pairs of 150 bp, global, every score -----------------
     H, V = sa.workload(sa.WL_DNA150, 0x5EED0001, 0, 4096)
     save("ref_dna150_global.npz", H, V, scores=ref_scores(H, V, 0, alphabet="dna", **SIMPLE),
          seed=0x5EED0001, mode=0)
 
-    # --- config 2 subset: semi-global AlignConfig<1,0,0,1> ---------------
+    # --- uneven log-normal DNA, first 1,500 pairs, AlignConfig<1,0,0,1> --
     H, V = sa.workload(sa.WL_DNA_LOGNORMAL, 0x5EED0002, 0, 1500)
     save("ref_dna_lognormal_semiglobal.npz", H, V,
          scores=ref_scores(H, V, 9, alphabet="dna", **SIMPLE), seed=0x5EED0002, mode=9)
 
-    # --- config 3 subset: protein local, Blosum62(-1, -11) ---------------
+    # --- protein local, first 300 pairs, Blosum62(-1, -11) --------------
     H, V = sa.workload(sa.WL_PROTEIN, 0x5EED0003, 0, 300)
     save("ref_protein_local.npz", H, V,
          scores=ref_scores(H, V, 16, alphabet="aa", matrix="blosum62", gap_extend=-1, gap_open=-11),
@@ -130,7 +130,7 @@ def main():
                      for m in modes])
     save("ref_protein_all_modes.npz", H, V, scores=allm, modes=np.array(modes), gap_extend=-2, gap_open=-4)
 
-    # --- config 4 subset: traceback counts (SIMD and scalar reference) ---
+    # --- 1 kbp DNA, first 96 pairs: counts (SIMD and scalar reference) --
     H, V = sa.workload(sa.WL_DNA1K, 0x5EED0004, 0, 96)
     rs = po.run_reference(po.OP_COUNTS_SIMD, H.residues, H.offsets, V.residues, V.offsets, 0, alphabet="dna",
                           **SIMPLE)

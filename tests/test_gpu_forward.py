@@ -141,20 +141,20 @@ def test_int32_route_for_long_identical(ctx):
     assert np.array_equal(got, _oracle(H, V, SIMPLE, 0))
 
 
-def test_config1_full_vs_oracle(ctx):
+def test_dna150_4096_pairs_vs_oracle(ctx):
     H, V = sa.workload(sa.WL_DNA150, 0x5EED0001, 0, 4096)
     got = ctx.align_score(H, V, SIMPLE, 0)
     assert np.array_equal(got, _oracle(H, V, SIMPLE, 0))
 
 
-def test_config2_window_vs_oracle(ctx):
+def test_lognormal_semiglobal_window_vs_oracle(ctx):
     H, V = sa.workload(sa.WL_DNA_LOGNORMAL, 0x5EED0002, 5000, 6000)
-    _say("config2 window", len(H), int((H.lengths() * V.lengths()).sum()))
+    _say("log-normal semi-global window", len(H), int((H.lengths() * V.lengths()).sum()))
     got = ctx.align_score(H, V, SIMPLE, 9)
     assert np.array_equal(got, _oracle(H, V, SIMPLE, 9))
 
 
-def test_config3_window_vs_oracle(ctx):
+def test_protein_local_window_vs_oracle(ctx):
     H, V = sa.workload(sa.WL_PROTEIN, 0x5EED0003, 1000, 1500)
     got = ctx.align_score(H, V, sa.blosum62(-1, -11), sa.LOCAL)
     assert np.array_equal(got, _oracle(H, V, sa.blosum62(-1, -11), sa.LOCAL))
