@@ -241,13 +241,14 @@ int64_t longMinSide()
 }
 
 // Streams the long-pair forward spreads its pair couples over (1..4).  On
-// 64 pairs of 100 kbp, 2 and 3 streams take 170 ms, 1 takes 220 ms and 4
-// take 260 ms (the 4th shares a hardware queue); starting the groups some
-// anti-diagonals apart only made it slower.
+// 64 pairs of 100 kbp with 128-column tiles: 1 stream 263 ms, 2 streams
+// 164 ms, 3 streams 157 ms (with 256-column tiles: 220, 170, 170 ms; 4
+// streams 260 ms, the 4th sharing a hardware queue); starting the groups
+// some anti-diagonals apart only made it slower.
 int longStreams()
 {
     const char *env = std::getenv("SQMI_LONG_STREAMS");
-    const int v = (env && *env) ? std::atoi(env) : 2;
+    const int v = (env && *env) ? std::atoi(env) : 3;
     return std::max(1, std::min(v, sqmi_context::kMaxLongStreams));
 }
 

@@ -217,10 +217,13 @@ struct TileArgs {
 constexpr int kTileSlack = 24;        // steps per tile beyond its width (skew + prefetch)
 constexpr int kWalkStateInts = 8;
 // Traceback window of one round: kRoundBlocks row blocks x kRoundTiles column
-// tiles recomputed with trace records (tileC = kLongTileC).
-constexpr int kLongTileC = 256;
+// tiles recomputed with trace records (tileC = kLongTileC).  Tiles of 128
+// columns: the forward of 64 pairs of 100 kbp takes 157 ms against 171 ms
+// with 256 columns (a lone tile's sweep, the per-anti-diagonal latency, is
+// halved) and 291 ms with 64 (per-tile overheads take over).
+constexpr int kLongTileC = 128;
 constexpr int kTileRecSteps = kLongTileC + kTileSlack;
-constexpr int kRoundBlocks = 8, kRoundTiles = 8, kRoundSlots = kRoundBlocks * kRoundTiles;
+constexpr int kRoundBlocks = 8, kRoundTiles = 16, kRoundSlots = kRoundBlocks * kRoundTiles;
 hipError_t launchTileFwd32(const TileArgs &args, hipStream_t stream);
 // Packed variant: pair couples (2 c, 2 c + 1), values relative to a per-tile
 // base; valid when kLong16Span x (max|s| + |o| + |e|) fits the int16 window.

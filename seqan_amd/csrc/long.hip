@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // ---------------------------------------------------------------------------
 // Packed forward: two pairs per lane (low / high int16 half), values held
 // relative to a per-tile base.  Within one tile every value lies within
-// (256 rows + 2 x 256 columns + skew) x D of the base, D = max|s| + |o| + |e|
+// (256 rows + 2 x tileC columns + skew) x D of the base, D = max|s| + |o| + |e|
 // (one row or column moves H by at most D; the columns past a shorter
 // pair's end lose at most |o| each), so the host takes this kernel when
 // 800 x D stays inside the int16 window; -inf is kNeg16 as in the packed
