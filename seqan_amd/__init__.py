@@ -310,14 +310,23 @@ class Context:
         return out
 
     def align_score_banded(self, set_h: SeqSet, set_v: SeqSet, score: Score, lower_diag: int, upper_diag: int,
-                           mode=0, flags=0):
-        """Banded global scores: cells with lower_diag <= j - i <= upper_diag."""
-        ch, keep_h = _seqs(set_h)
-        cv, keep_v = _seqs(set_v)
+                           mode=0, flags=0, h_dev=None, v_dev=None, scores_dev=None):
+        """Banded scores: cells with lower_diag <= j - i <= upper_diag (global
+        modes or LOCAL).  Device addresses as in align_score."""
+        ch, keep_h = _seqs(set_h, h_dev)
+        cv, keep_v = _seqs(set_v, v_dev)
+        if h_dev is not None or v_dev is not None:
+            assert h_dev is not None and v_dev is not None
+            flags |= RESIDUES_ON_DEVICE
         sc = score._c()
-        out = np.empty(len(set_h.offsets) - 1, np.int32)
+        if scores_dev is not None:
+            flags |= SCORES_ON_DEVICE
+            out, ptr = None, scores_dev
+        else:
+            out = np.empty(len(set_h.offsets) - 1, np.int32)
+            ptr = out.ctypes.data
         st = self._L.sqmi_align_score_banded(self._h, ctypes.byref(ch), ctypes.byref(cv), ctypes.byref(sc), mode,
-                                             int(lower_diag), int(upper_diag), out.ctypes.data, flags)
+                                             int(lower_diag), int(upper_diag), ptr, flags)
         self._check(st, "sqmi_align_score_banded")
         return out
 

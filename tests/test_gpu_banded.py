@@ -137,3 +137,39 @@ def test_banded_local_unsupported(ctx):
     for lo, up in ((2, 2), (5, 9), (-9, -3)):   # one diagonal; touching a corner only
         with pytest.raises(sa.SqmiError):
             ctx.align_score_banded(Hs, Vs, score, lo, up, mode=sa.LOCAL)
+
+
+def test_banded_device_resident(ctx):
+    # Residues and scores in device memory (SQMI_RESIDUES_ON_DEVICE,
+    # SQMI_SCORES_ON_DEVICE) give the host-buffer results.
+    H, V = sa.workload(sa.WL_DNA150, 0x5EED0403, 0, 3000)
+    score = sa.simple_score(2, -3, -1, -5)
+    want = ctx.align_score_banded(H, V, score, -12, 12)
+    dh, dv = sa.DeviceBuffer.from_array(H.residues), sa.DeviceBuffer.from_array(V.residues)
+    ds = sa.DeviceBuffer(4 * len(H))
+    try:
+        ctx.align_score_banded(H, V, score, -12, 12, h_dev=dh.address, v_dev=dv.address, scores_dev=ds.address)
+        assert np.array_equal(ds.to_array(np.int32, len(H)), want)
+    finally:
+        for b in (dh, dv, ds):
+            b.free()
+
+
+def test_python_mirror_functions(ctx):
+    # The SeqAn-named functions of the Python mirror reach the same entry points.
+    hs = [[0, 1, 2, 3, 0, 1, 2, 3], [3, 3, 2, 1, 0, 0, 1]]
+    vs = [[0, 1, 2, 0, 0, 1, 2, 3], [3, 2, 2, 1, 0, 1]]
+    score = sa.simple_score(2, -3, -1, -5)
+    Hs, Vs = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+    assert np.array_equal(sa.global_alignment_score(hs, vs, score, ctx=ctx), ctx.align_score(Hs, Vs, score, 0))
+    assert np.array_equal(sa.global_alignment_score(hs, vs, score, ctx=ctx, lower_diag=-3, upper_diag=3),
+                          ctx.align_score_banded(Hs, Vs, score, -3, 3))
+    assert np.array_equal(sa.local_alignment_score(hs, vs, score, alphabet=sa.DNA, ctx=ctx, lower_diag=-3,
+                                                   upper_diag=3),
+                          ctx.align_score_banded(Hs, Vs, score, -3, 3, mode=sa.LOCAL))
+    g = sa.global_alignment(hs, vs, score, align_config=(True, False, False, True), ctx=ctx)
+    r = ctx.align_rows(Hs, Vs, score, mode=sa.FREE_TOP | sa.FREE_BOTTOM)
+    assert all(np.array_equal(x, y) for x, y in zip(g, r))
+    l = sa.local_alignment(hs, vs, score, alphabet=sa.DNA, ctx=ctx)
+    lr = ctx.align_local_rows(Hs, Vs, score)
+    assert all(np.array_equal(x, y) for x, y in zip(l, lr))
