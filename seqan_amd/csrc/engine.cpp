@@ -55,7 +55,7 @@ struct sqmi_context
     DevBuf hRes, vRes, hOff, vOff, order, items, queue, rowbuf, scores, tTable, dnaProf, ids32, scratchOff32,
         scratch32, table32, emptyIds;
     // Traceback (sqmi_align_counts).
-    DevBuf trace, traceOff, locs, endEF, counts;
+    DevBuf trace, traceOff, locs, endEF, endIJ, counts;
     // Long pairs (tiled int32 kernels).
     DevBuf longCk, longPairs, longWalk, longTrace;
     // Alignment rows (sqmi_align_rows, sqmi_align_local_rows).
@@ -330,7 +330,7 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
     P.local = local;
     const bool costsOk16 = si.o <= 0 && si.e <= 0 && si.o >= -1024 && si.e >= -1024 && si.sMax <= 1024 &&
                            si.sMin >= -1024 && !(flags & SQMI_FORCE_INT32) &&
-                           !(traceShape && mode != 0);   // free-end tracebacks run in int32
+                           !(traceShape && (mode & SQMI_LOCAL));   // local tracebacks run in int32
     si.dna = si.alpha <= 4 && !local && si.sMin - si.o >= 0 && si.sMax - si.o <= 255;
     uint32_t track;
     if (local)
@@ -848,7 +848,7 @@ sqmi_status sqmi_destroy(sqmi_context *ctx)
     for (DevBuf *b : {&ctx->hRes, &ctx->vRes, &ctx->hOff, &ctx->vOff, &ctx->order, &ctx->items, &ctx->queue,
                       &ctx->rowbuf, &ctx->scores, &ctx->tTable, &ctx->dnaProf, &ctx->ids32, &ctx->scratchOff32,
                       &ctx->scratch32, &ctx->table32, &ctx->emptyIds, &ctx->trace, &ctx->traceOff, &ctx->locs,
-                      &ctx->endEF, &ctx->counts, &ctx->longCk, &ctx->longPairs, &ctx->longWalk, &ctx->longTrace,
+                      &ctx->endEF, &ctx->endIJ, &ctx->counts, &ctx->longCk, &ctx->longPairs, &ctx->longWalk, &ctx->longTrace,
                       &ctx->ops, &ctx->opOff, &ctx->opLen, &ctx->begins, &ctx->bandIds, &ctx->bandScratch})
         release(*b);
     if (ctx->ev0)
@@ -1297,8 +1297,10 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             (err = ensure(ctx->queue, 4 * chunkEnd.size() + 64)) != hipSuccess ||
             (err = ensure(ctx->rowbuf, rbBytes)) != hipSuccess ||
             (err = ensure(ctx->endEF, (size_t)8 * N)) != hipSuccess ||
+            (P.track != kTrackEnd && (err = ensure(ctx->endIJ, (size_t)8 * N)) != hipSuccess) ||
             (err = ensure(ctx->trace, (size_t)traceRecs * 64 * 16)) != hipSuccess)
             return hipFail(ctx, err, "allocating traceback buffers");
+        int32_t *dEndIJ = P.track != kTrackEnd ? static_cast<int32_t *>(ctx->endIJ.p) : nullptr;
         if ((err = hipMemcpyAsync(ctx->order.p, pl->order.data(), 4 * np, hipMemcpyHostToDevice, stream)) !=
                 hipSuccess ||
             (err = hipMemcpyAsync(ctx->items.p, pl->items.data(), sizeof(WorkItem) * nIt, hipMemcpyHostToDevice,
@@ -1335,8 +1337,9 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             a.trace = static_cast<uint4 *>(ctx->trace.p);
             a.traceOff = static_cast<const int64_t *>(ctx->traceOff.p) + b;
             a.endEF = static_cast<int32_t *>(ctx->endEF.p);
+            a.endIJ = dEndIJ;
             const int cg = std::max(1, std::min(grid, (int)((e - b + wpb - 1) / wpb)));
-            if ((err = launchFwdTrace16(si.dna, a, cg, stream)) != hipSuccess)
+            if ((err = launchFwdTrace16(si.dna, P.track, a, cg, stream)) != hipSuccess)
                 return hipFail(ctx, err, "launching trace kernel");
             const int32_t p0 = pl->items[b].first;
             const int32_t p1 = pl->items[e - 1].first + pl->items[e - 1].count;
@@ -1354,6 +1357,7 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             wa.counts = dCounts;
             wa.alpha = A;
             wa.linear = si.o == si.e;
+            wa.endIJ = dEndIJ;
             wa.ops = rows.ops;
             wa.opOff = rows.opOff;
             wa.opLen = rows.opLen;

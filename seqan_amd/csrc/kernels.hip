@@ -55,6 +55,21 @@ __device__ __forceinline__ uint32_t bfi(uint32_t mask, uint32_t a, uint32_t b) {
 // Saturating a - b per half: its sign bit is exactly (a < b), whatever the
 // magnitudes (v_pk_sub_i16 with clamp).
 __device__ __forceinline__ s16x2 subSat(s16x2 a, s16x2 b) { return __builtin_elementwise_sub_sat(a, b); }
+// 0xffff in the halves where a > b.
+__device__ __forceinline__ uint32_t gtMask(s16x2 a, s16x2 b) { return asu(subSat(b, a) >> splat(15)); }
+
+// Free-end traceback (trace kernels with bottom / right tracking): the first
+// strict maximum of the bottom row in column order and of the last column in
+// row order, with E and F of that cell (dp_scout.h:169-210; the end-cell rule
+// of dp_algorithm_impl.h:1190-1207 needs them).  Values in G = H + gapOpen,
+// E and F in H; positions per half (pair) in int32.
+struct EndTrack {
+    s16x2 bV, bE, bF;   // bottom row, held by lane G - 1 of the last block
+    int bJ[2];          // its column
+    s16x2 rV, rE, rF;   // last column, every lane over its own rows
+    int rRow[2];        // its item row
+};
+
 // Four sign bytes from two packed differences: v_perm_b32 selectors 8..11
 // replicate bits 15, 31 (of the second operand) and 47, 63 (of the first)
 // into whole bytes, giving bytes {sign(t0.x), sign(t0.y), sign(t1.x),
@@ -167,7 +182,7 @@ struct Fwd16 {
     static constexpr int BR = G * R;
     static constexpr int GROUPS = 64 / G;
     static constexpr int QD = 4;   // residue prefetch depth (steps)
-    static_assert(!TB || (R == 16 && TRK == kTrackEnd), "trace records hold 16 rows; global alignment only");
+    static_assert(!TB || (R == 16 && !(TRK & kTrackLocal)), "trace records hold 16 rows; global alignment only");
 
     __host__ __device__ __forceinline__ static int stepsFor(int Nw) { return ((Nw + G + QD - 1) / QD) * QD; }
 
@@ -197,7 +212,7 @@ struct Fwd16 {
                                                  const bool last, const bool coop, const uint32_t *rbIn,
                                                  uint32_t *rbOut, uint32_t *prodIn, uint32_t *prodOut,
                                                  const uint32_t *sProf, const int16_t *sT, s16x2 &best,
-                                                 uint4 *trItem, s16x2 &endE, s16x2 &endF)
+                                                 uint4 *trItem, s16x2 &endE, s16x2 &endF, EndTrack &et)
     {
         const int o = a.gapOpen, e = a.gapExtend;
         const s16x2 o2 = splat(o), e2 = splat(e), neg2 = splat(kNeg16);
@@ -255,8 +270,20 @@ struct Fwd16 {
             publish(prodOut, progTag(b, 0));
 
         if constexpr (TRK & kTrackBottom) {
-            if (last && l == G - 1)
+            if (last && l == G - 1) {
                 best = vmax(best, Gk[R - 1]);   // H(m, 0) is a tracked cell
+                if constexpr (TB) {
+                    // (m, 0) is the first bottom cell in column order.
+                    const uint32_t gt = gtMask(Gk[R - 1], et.bV);
+                    et.bV = as16(bfi(gt, asu(Gk[R - 1]), asu(et.bV)));
+                    et.bE = as16(bfi(gt, asu(neg2), asu(et.bE)));
+                    et.bF = as16(bfi(gt, asu(neg2), asu(et.bF)));
+                    if (gt & 0xffffu)
+                        et.bJ[0] = 0;
+                    if (gt >> 16)
+                        et.bJ[1] = 0;
+                }
+            }
         }
 
         uint32_t botG = asu(Gk[R - 1]), botF = asu(neg2);
@@ -340,18 +367,33 @@ struct Fwd16 {
                 botF = asu(uF);
                 if constexpr (TRK & (kTrackEnd | kTrackBottom | kTrackRight)) {
                     const s16x2 rem = remBase - splat(s);   // n - j per half
+                    // eq = 0xffff in halves where rem == 0.
+                    const uint16_t r0 = (uint16_t)rem.x, r1 = (uint16_t)rem.y;
+                    const uint32_t eq = (r0 == 0 ? 0xffffu : 0u) | (r1 == 0 ? 0xffff0000u : 0u);
                     if constexpr (TRK & kTrackBottom) {
                         if (last) {
                             const uint32_t after = asu(rem >> splat(15));   // 0xffff where j > n
                             const s16x2 cand = as16(bfi(after, asu(neg2), asu(uG)));
-                            if (l == G - 1)
+                            if (l == G - 1) {
                                 best = vmax(best, cand);
+                                if constexpr (TB) {
+                                    // With a free last column, (m, n) is the last cell
+                                    // of column n and is tracked there.
+                                    const uint32_t skip = (TRK & kTrackRight) ? (after | eq) : after;
+                                    const s16x2 cb = as16(bfi(skip, asu(neg2), asu(uG)));
+                                    const uint32_t gt = gtMask(cb, et.bV);
+                                    et.bV = as16(bfi(gt, asu(cb), asu(et.bV)));
+                                    et.bE = as16(bfi(gt, asu(Ek[R - 1]), asu(et.bE)));
+                                    et.bF = as16(bfi(gt, asu(uF), asu(et.bF)));
+                                    if (gt & 0xffffu)
+                                        et.bJ[0] = j;
+                                    if (gt >> 16)
+                                        et.bJ[1] = j;
+                                }
+                            }
                         }
                     }
                     if constexpr (TRK & (kTrackEnd | kTrackRight)) {
-                        // eq = 0xffff in halves where rem == 0.
-                        const uint16_t r0 = (uint16_t)rem.x, r1 = (uint16_t)rem.y;
-                        const uint32_t eq = (r0 == 0 ? 0xffffu : 0u) | (r1 == 0 ? 0xffff0000u : 0u);
                         if constexpr (TRK & kTrackEnd) {
                             if (last && l == G - 1) {
                                 best = as16(bfi(eq, asu(uG), asu(best)));
@@ -368,6 +410,25 @@ struct Fwd16 {
                                 for (int k = 1; k < R; ++k)
                                     cm = vmax(cm, Gk[k]);
                                 best = vmax(best, as16(bfi(eq, asu(cm), asu(neg2))));
+                                if constexpr (TB) {
+                                    // The lane's rows of column n in row order; F of
+                                    // each row again from the column above it.
+                                    s16x2 Fr = vmax(upF + e2, upG);
+#pragma unroll
+                                    for (int k = 0; k < R; ++k) {
+                                        if (k > 0)
+                                            Fr = vmax(Fr + e2, Gk[k - 1]);
+                                        const s16x2 cr = as16(bfi(eq, asu(Gk[k]), asu(neg2)));
+                                        const uint32_t gt = gtMask(cr, et.rV);
+                                        et.rV = as16(bfi(gt, asu(cr), asu(et.rV)));
+                                        et.rE = as16(bfi(gt, asu(Ek[k]), asu(et.rE)));
+                                        et.rF = as16(bfi(gt, asu(Fr), asu(et.rF)));
+                                        if (gt & 0xffffu)
+                                            et.rRow[0] = rowbase + k;
+                                        if (gt >> 16)
+                                            et.rRow[1] = rowbase + k;
+                                    }
+                                }
                             }
                         }
                     }
@@ -546,6 +607,10 @@ struct Fwd16 {
             const Couple c = couple(a, w, grp);
             s16x2 best = (TRK & kTrackLocal) ? splat(a.gapOpen) : splat(kNeg16);
             s16x2 endE = splat(kNeg16), endF = splat(kNeg16);
+            EndTrack et;
+            et.bV = et.bE = et.bF = et.rV = et.rE = et.rF = splat(kNeg16);
+            et.bJ[0] = et.bJ[1] = 0;
+            et.rRow[0] = et.rRow[1] = 0x7fffffff;
             const int Nw = w.maxN;
             uint4 *trItem = nullptr;
             if constexpr (TB)
@@ -560,13 +625,13 @@ struct Fwd16 {
             int b = ok ? b0 : w.nblocks;
             if (b == 0) {
                 block<true>(a, c, 0, Nw, l, w.nblocks == 1, coop, rbIn, rowbuf, prodIn, prodOut, sProf, sT, best,
-                            trItem, endE, endF);
+                            trItem, endE, endF, et);
                 b += bs;
             }
             for (; b < w.nblocks; b += bs)
                 block<false>(a, c, b, Nw, l, b == w.nblocks - 1, coop, rbIn, rowbuf, prodIn, prodOut, sProf, sT, best,
-                             trItem, endE, endF);
-            if constexpr (TB) {
+                             trItem, endE, endF, et);
+            if constexpr (TB && TRK == kTrackEnd) {
                 // E and F of the end cell (m, n), for the end-cell rule of
                 // the traceback (dp_algorithm_impl.h:1190-1207).
                 if (ok && l == G - 1) {
@@ -577,6 +642,45 @@ struct Fwd16 {
                     if (c.pid[1] >= 0) {
                         a.endEF[2 * c.pid[1]] = (int)endE.y;
                         a.endEF[2 * c.pid[1] + 1] = (int)endF.y;
+                    }
+                }
+            } else if constexpr (TB) {
+                // Free ends: the end cell is the first strict maximum over the
+                // tracked cells in column order, the bottom row's cells (m, j),
+                // j < n, before column n's cells (i, n).  The last column's
+                // candidates are spread over the group's lanes: reduce them
+                // (larger value, then smaller row), then lane G - 1, which
+                // holds the bottom row's, picks between the two.
+                int rv[2] = {(int)et.rV.x, (int)et.rV.y}, rr[2] = {et.rRow[0], et.rRow[1]};
+                int re[2] = {(int)et.rE.x, (int)et.rE.y}, rf[2] = {(int)et.rF.x, (int)et.rF.y};
+#pragma unroll
+                for (int d = 1; d < G; d <<= 1) {
+#pragma unroll
+                    for (int x = 0; x < 2; ++x) {
+                        const int ov = __shfl_xor(rv[x], d, G), orr = __shfl_xor(rr[x], d, G);
+                        const int oe = __shfl_xor(re[x], d, G), of = __shfl_xor(rf[x], d, G);
+                        const bool take = ov > rv[x] || (ov == rv[x] && orr < rr[x]);
+                        rv[x] = take ? ov : rv[x];
+                        rr[x] = take ? orr : rr[x];
+                        re[x] = take ? oe : re[x];
+                        rf[x] = take ? of : rf[x];
+                    }
+                }
+                const int bv[2] = {(int)et.bV.x, (int)et.bV.y}, be[2] = {(int)et.bE.x, (int)et.bE.y};
+                const int bf[2] = {(int)et.bF.x, (int)et.bF.y};
+                if (ok && l == G - 1) {
+#pragma unroll
+                    for (int x = 0; x < 2; ++x) {
+                        const int pid = c.pid[x];
+                        if (pid < 0)
+                            continue;
+                        bool bottom = (TRK & kTrackBottom) != 0;
+                        if constexpr ((TRK & kTrackBottom) && (TRK & kTrackRight))
+                            bottom = bv[x] >= rv[x];
+                        a.endEF[2 * pid] = bottom ? be[x] : re[x];
+                        a.endEF[2 * pid + 1] = bottom ? bf[x] : rf[x];
+                        a.endIJ[2 * pid] = bottom ? c.m[x] : rr[x] - c.p[x];
+                        a.endIJ[2 * pid + 1] = bottom ? et.bJ[x] : c.n[x];
                     }
                 }
             }
@@ -611,11 +715,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     Fwd16<G, R, DNA, TRK>::run(a);
 }
 
-// Forward pass with trace records (global alignment, G = 16, R = 16).
-template <bool DNA>
+// Forward pass with trace records (global alignment under any AlignConfig,
+// G = 16, R = 16).
+template <bool DNA, uint32_t TRK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kFwdTrace16(FwdArgs a)
 {
-    Fwd16<16, 16, DNA, kTrackEnd, true>::run(a);
+    Fwd16<16, 16, DNA, TRK, true>::run(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -905,8 +1010,8 @@ __global__ __launch_bounds__(kWalkThreads) void kWalk16(WalkArgs a)
         cnt[q] = 0;
     Bits16 bits{a.trace, a.locs[t]};
     walkCounts(n, m, a.hRes + ho, a.vRes + vo, A, a.scores[pid], a.endEF[2 * pid], a.endEF[2 * pid + 1], bits, cnt,
-               a.ops ? a.ops + a.opOff[pid + 1] : nullptr, a.ops ? a.opLen + pid : nullptr, -1, -1, false, nullptr,
-               a.linear != 0);
+               a.ops ? a.ops + a.opOff[pid + 1] : nullptr, a.ops ? a.opLen + pid : nullptr,
+               a.endIJ ? a.endIJ[2 * pid] : -1, a.endIJ ? a.endIJ[2 * pid + 1] : -1, false, nullptr, a.linear != 0);
     if constexpr (LDS)
         for (int q = 0; q < S; ++q)
             out[q] = cnt[q];
@@ -1114,19 +1219,31 @@ hipError_t launchFwd16(const Variant16 &v, const FwdArgs &args, int gridBlocks, 
 
 int traceSteps(int maxN) { return Fwd16<16, 16, true, kTrackEnd, true>::stepsFor(maxN); }
 
-hipError_t launchFwdTrace16(bool dna, const FwdArgs &args, int gridBlocks, hipStream_t stream)
+template <bool DNA>
+hipError_t launchTrace(uint32_t track, const FwdArgs &args, int gridBlocks, hipStream_t stream)
 {
-    if (dna)
-        hipLaunchKernelGGL(kFwdTrace16<true>, dim3(gridBlocks), dim3(256), 0, stream, args);
-    else
-        hipLaunchKernelGGL(kFwdTrace16<false>, dim3(gridBlocks), dim3(256), 0, stream, args);
+    const dim3 grid(gridBlocks), block(256);
+    switch (track) {
+        case kTrackEnd: hipLaunchKernelGGL((kFwdTrace16<DNA, kTrackEnd>), grid, block, 0, stream, args); break;
+        case kTrackBottom: hipLaunchKernelGGL((kFwdTrace16<DNA, kTrackBottom>), grid, block, 0, stream, args); break;
+        case kTrackRight: hipLaunchKernelGGL((kFwdTrace16<DNA, kTrackRight>), grid, block, 0, stream, args); break;
+        case kTrackBottom | kTrackRight:
+            hipLaunchKernelGGL((kFwdTrace16<DNA, kTrackBottom | kTrackRight>), grid, block, 0, stream, args);
+            break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
+}
+
+hipError_t launchFwdTrace16(bool dna, uint32_t track, const FwdArgs &args, int gridBlocks, hipStream_t stream)
+{
+    return dna ? launchTrace<true>(track, args, gridBlocks, stream) : launchTrace<false>(track, args, gridBlocks, stream);
 }
 
 int occupancyTrace16(bool dna)
 {
     int blocks = 0;
-    const void *k = dna ? (const void *)&kFwdTrace16<true> : (const void *)&kFwdTrace16<false>;
+    const void *k = dna ? (const void *)&kFwdTrace16<true, kTrackEnd> : (const void *)&kFwdTrace16<false, kTrackEnd>;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 256, 0) != hipSuccess)
         return 1;
     return blocks > 0 ? blocks : 1;

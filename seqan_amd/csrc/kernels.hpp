@@ -53,7 +53,8 @@ struct FwdArgs {
     // Trace variant only:
     uint4 *trace;               // 16-byte trace records (64 per step)
     const int64_t *traceOff;    // per item: first record (units of 64 records)
-    int32_t *endEF;             // per pair id: E and F of cell (m, n)
+    int32_t *endEF;             // per pair id: E and F of the end cell
+    int32_t *endIJ;             // free ends: per pair id, the end cell (i, j)
 };
 
 // Where the trace records of one pair live: rows are bottom-aligned in the
@@ -82,6 +83,7 @@ struct WalkArgs {
     int32_t *counts;            // per pair id: alpha^2 + 2 counts
     int32_t alpha;
     int32_t linear;             // gap open == gap extend: the linear-gap walk
+    const int32_t *endIJ;       // free ends: per pair id, the end cell (i, j); null: (m, n)
     // Rows (optional): pair id p's ops end at ops[opOff[p + 1]] (written
     // backwards), their number goes to opLen[p].
     uint8_t *ops;
@@ -237,7 +239,8 @@ hipError_t launchTileWalk32(const TileArgs &args, hipStream_t stream);
 constexpr int kTraceRows = 256;       // rows per block of the trace kernel (G = 16, R = 16)
 constexpr int kTraceLanes = 16;
 int traceSteps(int maxN);             // column steps per block for an item of width maxN
-hipError_t launchFwdTrace16(bool dna, const FwdArgs &args, int gridBlocks, hipStream_t stream);
+// track: kTrackEnd (global) or the bottom / right bits of free end gaps.
+hipError_t launchFwdTrace16(bool dna, uint32_t track, const FwdArgs &args, int gridBlocks, hipStream_t stream);
 int occupancyTrace16(bool dna);
 hipError_t launchWalk16(const WalkArgs &args, hipStream_t stream);
 hipError_t launchCounts32(const Counts32Args &args, hipStream_t stream);

@@ -161,20 +161,25 @@ def test_rows_empty_pairs_and_errors(ctx):
 
 
 @pytest.mark.parametrize("fixture", ["ref_dna_modes_rows.npz", "ref_dna_modes_short_rows.npz"])
-def test_rows_every_align_config_match_reference(ctx, fixture):
+@pytest.mark.parametrize("flags", [0, sa.FORCE_INT32], ids=["packed", "int32"])
+def test_rows_every_align_config_match_reference(ctx, fixture, flags):
     # Free end gaps: the end cell moves to the best tracked cell and the rows
-    # carry the free leading / trailing gaps (counted like any gap).
+    # carry the free leading / trailing gaps (counted like any gap).  Packed:
+    # the trace kernel tracks the end cell (kFwdTrace16 with bottom / right
+    # tracking); int32: kCounts32.
     z = _load(fixture)
     score = _score_of(z, fixture)
     Hs, Vs = sa.SeqSet(z["h_res"], z["h_off"]), sa.SeqSet(z["v_res"], z["v_off"])
+    nonempty = int(((Hs.lengths() > 0) & (Vs.lengths() > 0)).sum())
     for mode in range(16):
-        print("rows, AlignConfig mode", mode, flush=True)
-        s, c, op_off, ops = ctx.align_rows(Hs, Vs, score, mode=mode)
+        print("rows, AlignConfig mode", mode, "flags", flags, flush=True)
+        s, c, op_off, ops = ctx.align_rows(Hs, Vs, score, mode=mode, flags=flags)
+        assert ctx.stats()["pairs_int16"] == (0 if flags else nonempty), mode
         assert np.array_equal(s, z["scores"][mode]), mode
         assert np.array_equal(c, z["counts"][mode]), mode
         assert np.array_equal(op_off, z["op_off"][mode]), mode
         assert np.array_equal(ops, z["ops"][z["mode_off"][mode]:z["mode_off"][mode + 1]]), mode
-        s2, c2 = ctx.align_counts(Hs, Vs, score, mode=mode)
+        s2, c2 = ctx.align_counts(Hs, Vs, score, mode=mode, flags=flags)
         assert np.array_equal(s2, s) and np.array_equal(c2, c), mode
         assert np.array_equal(ctx.align_score(Hs, Vs, score, mode), s), mode
 
@@ -298,3 +303,40 @@ def test_linear_gap_rows_match_reference(ctx, fixture, flags):
         assert np.array_equal(c, z["counts"][k]), mode
         assert np.array_equal(op_off, z["op_off"][k]), mode
         assert np.array_equal(ops, z["ops"][z["mode_off"][k]:z["mode_off"][k + 1]]), mode
+
+
+@pytest.mark.parametrize("scoring", [(2, -3, -1, -5), (1, -1, -1, -2), (1, -1, -1, -1), (3, -2, -3, -2)],
+                         ids=["affine", "ties", "linear", "extend-below-open"])
+def test_rows_free_ends_packed_match_oracle(ctx, scoring):
+    # The packed free-end traceback over items of several 256-row blocks,
+    # where the last column's candidates come from several blocks and lanes
+    # and the two pairs of a couple end at different columns.
+    rng = np.random.default_rng(sum(abs(x) for x in scoring) * 7 + scoring[0])
+    hs, vs = [], []
+    for _ in range(160):
+        n = int(rng.integers(1, 1400))
+        h = rng.integers(0, 4, n).astype(np.uint8)
+        lo = int(rng.integers(0, n // 3 + 1))
+        hi = n - int(rng.integers(0, n // 3 + 1))
+        v = h[lo:max(hi, lo + 1)].copy()
+        flip = rng.random(len(v)) < 0.1
+        v[flip] = rng.integers(0, 4, int(flip.sum()))
+        if rng.random() < 0.15:
+            v = rng.integers(0, 4, int(rng.integers(1, 1400))).astype(np.uint8)
+        if rng.random() < 0.5:
+            h, v = v, h
+        hs.append(h)
+        vs.append(v)
+    Hs, Vs = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+    score = sa.simple_score(*scoring)
+    sc = po.Scoring(score.table, score.gap_open, score.gap_extend)
+    for mode in (1, 2, 4, 8, 9, 12, 6, 15):
+        print("packed free-end rows vs oracle, scoring", scoring, "mode", mode, flush=True)
+        s, c, op_off, ops = ctx.align_rows(Hs, Vs, score, mode=mode)
+        assert ctx.stats()["pairs_int16"] == len(hs)
+        ws, wc, woff, wops = po.rows_batch(Hs.residues, Hs.offsets, Vs.residues, Vs.offsets, sc, mode)
+        bad = np.nonzero(s != ws)[0]
+        assert len(bad) == 0, (mode, bad[:5])
+        bad = np.nonzero((c != wc).any(axis=1))[0]
+        assert len(bad) == 0, (mode, bad[:5])
+        assert np.array_equal(op_off, woff) and np.array_equal(ops, wops)
