@@ -154,6 +154,24 @@ def backward_leg(ctx, steps, warmup, cpu):
         res["cpu_baseline"] = reference_run(1, H, V, 4096, 0, host_cores(), 1, scores, counts, alphabet="dna",
                                             match=2, mismatch=-3, gap_extend=-1, gap_open=-5)
         log(f"dna1k cpu baseline: {res['cpu_baseline']}")
+    # The same pairs under AlignConfig<true,false,false,true> (free first
+    # and last row): the trace kernel tracks the end cell on the bottom row.
+    semi = sa.FREE_TOP | sa.FREE_BOTTOM
+    sfwd = timed(lambda: ctx.align_score(H, V, score, semi, scores_dev=ds.address, **dev), steps, warmup,
+                 "dna1k semi-global forward")
+    sbwd = timed(lambda: ctx.align_counts(H, V, score, semi, scores_dev=ds.address, counts_dev=dc.address, **dev),
+                 steps, warmup, "dna1k semi-global fwd+bwd")
+    st = ctx.stats()
+    res["semi_global"] = {"value": round(cells / (sbwd / 1e3) / 1e9, 2), "unit": "GCUPS",
+                          "ms_per_step": round(sbwd, 3), "forward_ms_per_step": round(sfwd, 3),
+                          "backward_over_forward": round(sbwd / sfwd, 3), "pairs_int16": int(st["pairs_int16"]),
+                          "workload": "the same pairs, AlignConfig<true,false,false,true>"}
+    if cpu:
+        sscores = ds.to_array(np.int32, N_DNA1K)
+        scounts = dc.to_array(np.int32, S * N_DNA1K).reshape(N_DNA1K, S)
+        res["semi_global"]["cpu_baseline"] = reference_run(1, H, V, 4096, semi, host_cores(), 1, sscores, scounts,
+                                                           alphabet="dna", match=2, mismatch=-3, gap_extend=-1,
+                                                           gap_open=-5)
     for b in (dh, dv, ds, dc):
         b.free()
     return res
