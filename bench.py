@@ -172,6 +172,23 @@ def backward_leg(ctx, steps, warmup, cpu):
         res["semi_global"]["cpu_baseline"] = reference_run(1, H, V, 4096, semi, host_cores(), 1, sscores, scounts,
                                                            alphabet="dna", match=2, mismatch=-3, gap_extend=-1,
                                                            gap_open=-5)
+    # Local alignment of the same pairs: the trace kernel tracks the end cell
+    # over every cell and the walk stops where H falls to 0.
+    lfwd = timed(lambda: ctx.align_score(H, V, score, sa.LOCAL, scores_dev=ds.address, **dev), steps, warmup,
+                 "dna1k local forward")
+    lbwd = timed(lambda: ctx.align_counts(H, V, score, sa.LOCAL, scores_dev=ds.address, counts_dev=dc.address,
+                                          **dev), steps, warmup, "dna1k local fwd+bwd")
+    st = ctx.stats()
+    res["local"] = {"value": round(cells / (lbwd / 1e3) / 1e9, 2), "unit": "GCUPS",
+                    "ms_per_step": round(lbwd, 3), "forward_ms_per_step": round(lfwd, 3),
+                    "backward_over_forward": round(lbwd / lfwd, 3), "pairs_int16": int(st["pairs_int16"]),
+                    "workload": "the same pairs, localAlignment counts"}
+    if cpu:
+        lscores = ds.to_array(np.int32, N_DNA1K)
+        lcounts = dc.to_array(np.int32, S * N_DNA1K).reshape(N_DNA1K, S)
+        res["local"]["cpu_baseline"] = reference_run(1, H, V, 4096, sa.LOCAL, host_cores(), 1, lscores, lcounts,
+                                                     alphabet="dna", match=2, mismatch=-3, gap_extend=-1,
+                                                     gap_open=-5)
     for b in (dh, dv, ds, dc):
         b.free()
     return res

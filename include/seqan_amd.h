@@ -148,8 +148,8 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *set_h,
  * gaps preferred at the end cell), counted on the rows like
  * computeAlignmentStats (free end gaps included; for SQMI_LOCAL the rows of
  * the local window).  Global alignment with any free-end bits, or SQMI_LOCAL
- * (free-end bits beside it are ignored, as in sqmi_align_score).  With
- * SQMI_LOCAL the pairs run on the int32 one-thread-per-pair kernel. */
+ * (free-end bits beside it are ignored, as in sqmi_align_score).  Pairs whose
+ * values fit int16 run on the packed trace kernel in every mode. */
 sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *set_h,
                               const sqmi_sequences *set_v, const sqmi_scoring *scoring,
                               uint32_t mode, int32_t *scores, int32_t *counts,

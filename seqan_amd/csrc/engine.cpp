@@ -330,7 +330,7 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
     P.local = local;
     const bool costsOk16 = si.o <= 0 && si.e <= 0 && si.o >= -1024 && si.e >= -1024 && si.sMax <= 1024 &&
                            si.sMin >= -1024 && !(flags & SQMI_FORCE_INT32) &&
-                           !(traceShape && (mode & SQMI_LOCAL));   // local tracebacks run in int32
+                           true;
     si.dna = si.alpha <= 4 && !local && si.sMin - si.o >= 0 && si.sMax - si.o <= 255;
     uint32_t track;
     if (local)
@@ -1358,6 +1358,11 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             wa.alpha = A;
             wa.linear = si.o == si.e;
             wa.endIJ = dEndIJ;
+            wa.local = P.local ? 1 : 0;
+            wa.table = static_cast<const int32_t *>(ctx->table32.p);
+            wa.gapOpen = si.o;
+            wa.gapExtend = si.e;
+            wa.begins = rows.begins;
             wa.ops = rows.ops;
             wa.opOff = rows.opOff;
             wa.opLen = rows.opLen;

@@ -63,12 +63,21 @@ __device__ __forceinline__ uint32_t gtMask(s16x2 a, s16x2 b) { return asu(subSat
 // row order, with E and F of that cell (dp_scout.h:169-210; the end-cell rule
 // of dp_algorithm_impl.h:1190-1207 needs them).  Values in G = H + gapOpen,
 // E and F in H; positions per half (pair) in int32.
+// Local alignment uses the b fields for the best cell over every cell, with
+// its item row in bRow.
 struct EndTrack {
     s16x2 bV, bE, bF;   // bottom row, held by lane G - 1 of the last block
     int bJ[2];          // its column
+    int bRow[2];        // local: its item row
     s16x2 rV, rE, rF;   // last column, every lane over its own rows
     int rRow[2];        // its item row
 };
+
+// 0xffff in the halves where a == b.
+__device__ __forceinline__ uint32_t eqMask(s16x2 a, s16x2 b)
+{
+    return ((uint16_t)a.x == (uint16_t)b.x ? 0xffffu : 0u) | ((uint16_t)a.y == (uint16_t)b.y ? 0xffff0000u : 0u);
+}
 
 // Four sign bytes from two packed differences: v_perm_b32 selectors 8..11
 // replicate bits 15, 31 (of the second operand) and 47, 63 (of the first)
@@ -182,7 +191,7 @@ struct Fwd16 {
     static constexpr int BR = G * R;
     static constexpr int GROUPS = 64 / G;
     static constexpr int QD = 4;   // residue prefetch depth (steps)
-    static_assert(!TB || (R == 16 && !(TRK & kTrackLocal)), "trace records hold 16 rows; global alignment only");
+    static_assert(!TB || R == 16, "trace records hold 16 rows");
 
     __host__ __device__ __forceinline__ static int stepsFor(int Nw) { return ((Nw + G + QD - 1) / QD) * QD; }
 
@@ -358,13 +367,47 @@ struct Fwd16 {
                         D = Gk[k] + as16(tOf(k + 1));
                     const s16x2 Gn = H + o2;
                     Gk[k] = Gn;
-                    if constexpr (TRK & kTrackLocal)
+                    if constexpr ((TRK & kTrackLocal) && !TB)
                         best = vmax(best, Gn);
                     uG = Gn;
                     uF = F;
                 }
                 botG = asu(uG);
                 botF = asu(uF);
+                if constexpr ((TRK & kTrackLocal) && TB) {
+                    // Local traceback: the first strict maximum over every
+                    // cell in column order.  The column's maximum first; the
+                    // rows are scanned only when it reaches the best so far
+                    // (equal values count when they lie in an earlier column:
+                    // a later block sweeps columns the earlier ones passed).
+                    s16x2 sm = Gk[0];
+#pragma unroll
+                    for (int k = 1; k < R; ++k)
+                        sm = vmax(sm, Gk[k]);
+                    best = vmax(best, sm);
+                    if (gtMask(sm + splat(1), et.bV) != 0) {
+                        uint32_t earlier = (j < et.bJ[0] ? 0xffffu : 0u) | (j < et.bJ[1] ? 0xffff0000u : 0u);
+                        s16x2 Fr = vmax(upF + e2, upG);
+#pragma unroll
+                        for (int k = 0; k < R; ++k) {
+                            if (k > 0)
+                                Fr = vmax(Fr + e2, Gk[k - 1]);
+                            const uint32_t take = gtMask(Gk[k], et.bV) | (eqMask(Gk[k], et.bV) & earlier);
+                            earlier &= ~take;   // bJ is now j: later rows need a larger value
+                            et.bV = as16(bfi(take, asu(Gk[k]), asu(et.bV)));
+                            et.bE = as16(bfi(take, asu(Ek[k]), asu(et.bE)));
+                            et.bF = as16(bfi(take, asu(Fr), asu(et.bF)));
+                            if (take & 0xffffu) {
+                                et.bJ[0] = j;
+                                et.bRow[0] = rowbase + k;
+                            }
+                            if (take >> 16) {
+                                et.bJ[1] = j;
+                                et.bRow[1] = rowbase + k;
+                            }
+                        }
+                    }
+                }
                 if constexpr (TRK & (kTrackEnd | kTrackBottom | kTrackRight)) {
                     const s16x2 rem = remBase - splat(s);   // n - j per half
                     // eq = 0xffff in halves where rem == 0.
@@ -609,7 +652,10 @@ struct Fwd16 {
             s16x2 endE = splat(kNeg16), endF = splat(kNeg16);
             EndTrack et;
             et.bV = et.bE = et.bF = et.rV = et.rE = et.rF = splat(kNeg16);
-            et.bJ[0] = et.bJ[1] = 0;
+            if constexpr (TRK & kTrackLocal)
+                et.bV = splat(a.gapOpen + 1);   // G of H = 1: a cell must beat 0 (taken while bJ is unset)
+            et.bJ[0] = et.bJ[1] = (TRK & kTrackLocal) ? 0x7fffffff : 0;
+            et.bRow[0] = et.bRow[1] = 0;
             et.rRow[0] = et.rRow[1] = 0x7fffffff;
             const int Nw = w.maxN;
             uint4 *trItem = nullptr;
@@ -642,6 +688,41 @@ struct Fwd16 {
                     if (c.pid[1] >= 0) {
                         a.endEF[2 * c.pid[1]] = (int)endE.y;
                         a.endEF[2 * c.pid[1] + 1] = (int)endF.y;
+                    }
+                }
+            } else if constexpr (TB && (TRK & kTrackLocal)) {
+                // Local: reduce the lanes' candidates (larger value, then
+                // earlier column, then earlier row); no cell above 0 leaves
+                // the end cell at (0, 0), an empty window.
+                int bv[2] = {(int)et.bV.x, (int)et.bV.y}, bj[2] = {et.bJ[0], et.bJ[1]};
+                int br[2] = {et.bRow[0], et.bRow[1]};
+                int be[2] = {(int)et.bE.x, (int)et.bE.y}, bf[2] = {(int)et.bF.x, (int)et.bF.y};
+#pragma unroll
+                for (int d = 1; d < G; d <<= 1) {
+#pragma unroll
+                    for (int x = 0; x < 2; ++x) {
+                        const int ov = __shfl_xor(bv[x], d, G), oj = __shfl_xor(bj[x], d, G);
+                        const int orr = __shfl_xor(br[x], d, G);
+                        const int oe = __shfl_xor(be[x], d, G), of = __shfl_xor(bf[x], d, G);
+                        const bool take = ov > bv[x] || (ov == bv[x] && (oj < bj[x] || (oj == bj[x] && orr < br[x])));
+                        bv[x] = take ? ov : bv[x];
+                        bj[x] = take ? oj : bj[x];
+                        br[x] = take ? orr : br[x];
+                        be[x] = take ? oe : be[x];
+                        bf[x] = take ? of : bf[x];
+                    }
+                }
+                if (ok && l == G - 1) {
+#pragma unroll
+                    for (int x = 0; x < 2; ++x) {
+                        const int pid = c.pid[x];
+                        if (pid < 0)
+                            continue;
+                        const bool any = bj[x] != 0x7fffffff;
+                        a.endEF[2 * pid] = be[x];
+                        a.endEF[2 * pid + 1] = bf[x];
+                        a.endIJ[2 * pid] = any ? br[x] - c.p[x] : 0;
+                        a.endIJ[2 * pid + 1] = any ? bj[x] : 0;
                     }
                 }
             } else if constexpr (TB) {
@@ -847,8 +928,15 @@ template <class Bits>
 __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const uint8_t *v, int A, int score, int endE,
                                            int endF, Bits &bits, int32_t *cnt, uint8_t *opEnd = nullptr,
                                            int64_t *opLen = nullptr, int i0 = -1, int j0 = -1,
-                                           bool local = false, int64_t *begins = nullptr, bool linear = false)
+                                           bool local = false, int64_t *begins = nullptr, bool linear = false,
+                                           const int32_t *lTab = nullptr, int go = 0, int ge = 0)
 {
+    // Local walks over packed trace records (lTab given) carry no stop bit:
+    // they track H along the path instead (the score at the end cell, minus
+    // each diagonal's substitution score, minus gapOpen or gapExtend per gap
+    // character by the record's open bit) and stop on the first cell where
+    // it is 0, which is where the reference's trace value is NONE.
+    int cur = score;
     // The walk starts at the end cell (i0, j0), (m, n) unless free end gaps
     // or a local alignment moved it; in global modes the cells after it are
     // trailing gaps of the rows.  A local walk stops at the first cell whose
@@ -888,13 +976,15 @@ __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const
     {
         const uint32_t b0 = (i > 0 && j > 0) ? bits(i, j) : kBitStop;
         dir = linear ? dirOf(b0) : (endF == score) ? 1 : (endE == score) ? 2 : dirOf(b0);
-        if (local && (b0 & kBitStop))
+        if (local && (lTab ? cur <= 0 : (b0 & kBitStop) != 0))
             walk = false;
     }
     while (walk && i > 0 && j > 0) {
         if (dir == 0) {
             const int hc = min((int)h[j - 1], A - 1), vc = min((int)v[i - 1], A - 1);
             cnt[hc * A + vc] += 1;
+            if (lTab)
+                cur -= lTab[hc * A + vc];
             emit('M');
             prev = 0;
             --i;
@@ -904,6 +994,7 @@ __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const
                 const bool open = (bits(i, j) & kBitFOpen) != 0;
                 if (prev == 1) ++exts; else ++opens;
                 prev = 1;
+                cur -= (open || linear) ? go : ge;
                 emit('V');
                 const bool stop = linear || open || i == 1;
                 --i;
@@ -915,6 +1006,7 @@ __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const
                 const bool open = (bits(i, j) & kBitEOpen) != 0;
                 if (prev == 2) ++exts; else ++opens;
                 prev = 2;
+                cur -= (open || linear) ? go : ge;
                 emit('H');
                 const bool stop = linear || open || j == 1;
                 --j;
@@ -923,9 +1015,11 @@ __device__ __forceinline__ void walkCounts(int n, int m, const uint8_t *h, const
             }
         }
         if (i > 0 && j > 0) {
+            if (local && lTab && cur == 0)
+                break;
             const uint32_t b = bits(i, j);
             dir = dirOf(b);
-            if (local && (b & kBitStop))
+            if (local && !lTab && (b & kBitStop))
                 break;
         }
     }
@@ -1011,7 +1105,9 @@ __global__ __launch_bounds__(kWalkThreads) void kWalk16(WalkArgs a)
     Bits16 bits{a.trace, a.locs[t]};
     walkCounts(n, m, a.hRes + ho, a.vRes + vo, A, a.scores[pid], a.endEF[2 * pid], a.endEF[2 * pid + 1], bits, cnt,
                a.ops ? a.ops + a.opOff[pid + 1] : nullptr, a.ops ? a.opLen + pid : nullptr,
-               a.endIJ ? a.endIJ[2 * pid] : -1, a.endIJ ? a.endIJ[2 * pid + 1] : -1, false, nullptr, a.linear != 0);
+               a.endIJ ? a.endIJ[2 * pid] : -1, a.endIJ ? a.endIJ[2 * pid + 1] : -1, a.local != 0,
+               a.begins ? a.begins + 2 * pid : nullptr, a.linear != 0, a.local ? a.table : nullptr, a.gapOpen,
+               a.gapExtend);
     if constexpr (LDS)
         for (int q = 0; q < S; ++q)
             out[q] = cnt[q];
@@ -1230,6 +1326,7 @@ hipError_t launchTrace(uint32_t track, const FwdArgs &args, int gridBlocks, hipS
         case kTrackBottom | kTrackRight:
             hipLaunchKernelGGL((kFwdTrace16<DNA, kTrackBottom | kTrackRight>), grid, block, 0, stream, args);
             break;
+        case kTrackLocal: hipLaunchKernelGGL((kFwdTrace16<DNA, kTrackLocal>), grid, block, 0, stream, args); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -83,7 +83,12 @@ struct WalkArgs {
     int32_t *counts;            // per pair id: alpha^2 + 2 counts
     int32_t alpha;
     int32_t linear;             // gap open == gap extend: the linear-gap walk
-    const int32_t *endIJ;       // free ends: per pair id, the end cell (i, j); null: (m, n)
+    const int32_t *endIJ;       // free ends, local: per pair id, the end cell (i, j); null: (m, n)
+    int32_t local;              // local walk: stops where H falls to 0, no leading / trailing gaps
+    const int32_t *table;       // local: alpha^2 substitution scores (H along the path)
+    int32_t gapOpen;
+    int32_t gapExtend;
+    int64_t *begins;            // local: per pair id, the window's first H and V position (optional)
     // Rows (optional): pair id p's ops end at ops[opOff[p + 1]] (written
     // backwards), their number goes to opLen[p].
     uint8_t *ops;

@@ -217,14 +217,19 @@ def _oracle_local(Hs, Vs, score):
 
 @pytest.mark.parametrize("fixture", ["ref_dna_local_rows.npz", "ref_dna_local_short_rows.npz",
                                      "ref_protein_local_rows.npz"])
-def test_local_rows_match_reference_fixtures(ctx, fixture):
+@pytest.mark.parametrize("flags", [0, sa.FORCE_INT32], ids=["packed", "int32"])
+def test_local_rows_match_reference_fixtures(ctx, fixture, flags):
     # localAlignment(ExecutionPolicy<Parallel, Vectorial>, gapsH, gapsV,
-    # score): scores, counts, clipped rows and window begins.
+    # score): scores, counts, clipped rows and window begins, from the packed
+    # trace kernel (end cell tracked over every cell, walk stopped where H
+    # falls to 0) and from the int32 kernel.
     z = _load(fixture)
     score = _score_of(z, fixture)
     Hs, Vs = sa.SeqSet(z["h_res"], z["h_off"]), sa.SeqSet(z["v_res"], z["v_off"])
-    print("local rows fixture", fixture, flush=True)
-    s, c, op_off, ops, begins = ctx.align_local_rows(Hs, Vs, score)
+    print("local rows fixture", fixture, "flags", flags, flush=True)
+    s, c, op_off, ops, begins = ctx.align_local_rows(Hs, Vs, score, flags=flags)
+    nonempty = int(((np.diff(z["h_off"]) > 0) & (np.diff(z["v_off"]) > 0)).sum())
+    assert ctx.stats()["pairs_int16"] == (0 if flags else nonempty)
     assert np.array_equal(s, z["scores"])
     assert np.array_equal(c, z["counts"])
     assert np.array_equal(op_off, z["op_off"])
@@ -235,7 +240,8 @@ def test_local_rows_match_reference_fixtures(ctx, fixture):
     assert np.array_equal(ctx.align_score(Hs, Vs, score, sa.LOCAL), s)
 
 
-def test_local_rows_match_oracle(ctx):
+@pytest.mark.parametrize("flags", [0, sa.FORCE_INT32], ids=["packed", "int32"])
+def test_local_rows_match_oracle(ctx, flags):
     rng = np.random.default_rng(24)
     for alpha, score in ((4, sa.simple_score(2, -3, -1, -5)), (4, sa.simple_score(1, -1, -1, -2)),
                          (27, sa.blosum62(-1, -11))):
@@ -253,15 +259,17 @@ def test_local_rows_match_oracle(ctx):
             hs.append(h)
             vs.append(v)
         Hs, Vs = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
-        print("local rows vs oracle, alphabet", alpha, flush=True)
-        s, c, op_off, ops, begins = ctx.align_local_rows(Hs, Vs, score)
+        print("local rows vs oracle, alphabet", alpha, "flags", flags, flush=True)
+        s, c, op_off, ops, begins = ctx.align_local_rows(Hs, Vs, score, flags=flags)
+        assert ctx.stats()["pairs_int16"] == (0 if flags else len(hs))
         ws, wc, woff, wops, wbeg = _oracle_local(Hs, Vs, score)
         assert np.array_equal(s, ws) and np.array_equal(c, wc)
         assert np.array_equal(op_off, woff) and np.array_equal(ops, wops)
         assert np.array_equal(begins, wbeg)
 
 
-def test_local_rows_empty_zero_and_errors(ctx):
+@pytest.mark.parametrize("flags", [0, sa.FORCE_INT32], ids=["packed", "int32"])
+def test_local_rows_empty_zero_and_errors(ctx, flags):
     # Empty pairs: INT32_MIN and no window.  A pair with no positive cell:
     # score 0, empty window at (0, 0).  Local rows come only from
     # sqmi_align_local_rows; free-end bits beside LOCAL are ignored, as in
@@ -269,7 +277,7 @@ def test_local_rows_empty_zero_and_errors(ctx):
     Hs = sa.SeqSet.from_list([[0, 1, 2], [], [0, 0, 0], [1, 2, 3, 0]])
     Vs = sa.SeqSet.from_list([[0, 2], [1, 1], [1, 2, 3], []])
     score = sa.simple_score(2, -3, -1, -5)
-    s, c, op_off, ops, begins = ctx.align_local_rows(Hs, Vs, score)
+    s, c, op_off, ops, begins = ctx.align_local_rows(Hs, Vs, score, flags=flags)
     assert s[1] == np.iinfo(np.int32).min and s[3] == np.iinfo(np.int32).min
     assert s[2] == 0 and s[0] == 2
     assert list(np.diff(op_off)) == [1, 0, 0, 0]
@@ -278,7 +286,7 @@ def test_local_rows_empty_zero_and_errors(ctx):
     assert (c[1:] == 0).all()
     with pytest.raises(sa.SqmiError):
         ctx.align_rows(Hs, Vs, score, mode=sa.LOCAL)
-    s2, c2 = ctx.align_counts(Hs, Vs, score, mode=sa.LOCAL | sa.FREE_TOP)
+    s2, c2 = ctx.align_counts(Hs, Vs, score, mode=sa.LOCAL | sa.FREE_TOP, flags=flags)
     assert np.array_equal(s2, s) and np.array_equal(c2, c)
 
 
