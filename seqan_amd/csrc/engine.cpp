@@ -329,9 +329,12 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
     const bool local = (mode & SQMI_LOCAL) != 0;
     P.local = local;
     const bool costsOk16 = si.o <= 0 && si.e <= 0 && si.o >= -1024 && si.e >= -1024 && si.sMax <= 1024 &&
-                           si.sMin >= -1024 && !(flags & SQMI_FORCE_INT32) &&
-                           true;
-    si.dna = si.alpha <= 4 && !local && si.sMin - si.o >= 0 && si.sMax - si.o <= 255;
+                           si.sMin >= -1024 && !(flags & SQMI_FORCE_INT32);
+    // DNA byte profile: t(h, v) - gapOpen in 0..255.  Its padding column
+    // (past n) is 0, i.e. t = gapOpen <= 0, so under local alignment the
+    // cells past n stay at or below the maximum over columns 1..n, and an
+    // equal one follows its source in column order.
+    si.dna = si.alpha <= 4 && si.sMin - si.o >= 0 && si.sMax - si.o <= 255;
     uint32_t track;
     if (local)
         track = kTrackLocal;

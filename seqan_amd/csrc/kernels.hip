@@ -1238,11 +1238,8 @@ hipError_t launchTrk(const Variant16 &v, const FwdArgs &args, int gridBlocks, hi
             hipLaunchKernelGGL((kFwd16<G, R, DNA, kTrackBottom | kTrackRight>), grid, block, 0, stream, args);
             break;
         case kTrackLocal:
-            if constexpr (!DNA) {
-                hipLaunchKernelGGL((kFwd16<G, R, DNA, kTrackLocal>), grid, block, 0, stream, args);
-                break;
-            }
-            return hipErrorInvalidValue;
+            hipLaunchKernelGGL((kFwd16<G, R, DNA, kTrackLocal>), grid, block, 0, stream, args);
+            break;
         default:
             return hipErrorInvalidValue;
     }
@@ -1257,10 +1254,7 @@ const void *kernelTrk(uint32_t track)
         case kTrackBottom: return (const void *)&kFwd16<G, R, DNA, kTrackBottom>;
         case kTrackRight: return (const void *)&kFwd16<G, R, DNA, kTrackRight>;
         case kTrackBottom | kTrackRight: return (const void *)&kFwd16<G, R, DNA, kTrackBottom | kTrackRight>;
-        case kTrackLocal:
-            if constexpr (!DNA)
-                return (const void *)&kFwd16<G, R, DNA, kTrackLocal>;
-            return nullptr;
+        case kTrackLocal: return (const void *)&kFwd16<G, R, DNA, kTrackLocal>;
         default: return nullptr;
     }
 }
