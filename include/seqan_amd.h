@@ -28,6 +28,12 @@
  *                           one pair: :394-470), and with SQMI_LOCAL
  *                           localAlignmentScore(..., lowerDiag, upperDiag)
  *                           (local_alignment_banded.h:317-383)
+ *   sqmi_align_rows_banded <- globalAlignment(gapsH, gapsV, score [, AlignConfig],
+ *                           lowerDiag, upperDiag)
+ *                           include/seqan/align/global_alignment_banded.h:152-181
+ *                           (sets: :655-760), and localAlignment(gapsH, gapsV,
+ *                           score, lowerDiag, upperDiag)
+ *                           include/seqan/align/local_alignment_banded.h:242-280
  *   sqmi_align_local_rows <- localAlignment(ExecutionPolicy<Parallel,Vectorial>,
  *                           gapsSetH, gapsSetV, score)
  *                           include/seqan/align_parallel/parallel_align_interface.h:312-321,
@@ -50,9 +56,10 @@
 extern "C" {
 #endif
 
-#define SQMI_ABI_VERSION 3   /* 2: sqmi_stats.pairs_long, pairs_long_packed; sqmi_align_rows
+#define SQMI_ABI_VERSION 4   /* 2: sqmi_stats.pairs_long, pairs_long_packed; sqmi_align_rows
                                 3: sqmi_align_local_rows; SQMI_LOCAL in sqmi_align_counts;
-                                   sqmi_align_score_banded */
+                                   sqmi_align_score_banded
+                                4: sqmi_align_rows_banded */
 
 typedef enum sqmi_status {
     SQMI_OK = 0,
@@ -205,6 +212,22 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
                                     const sqmi_sequences *set_v, const sqmi_scoring *scoring,
                                     uint32_t mode, int64_t lower_diag, int64_t upper_diag,
                                     int32_t *scores, uint32_t flags);
+
+/* Banded alignment rows: sqmi_align_rows (global modes) or
+ * sqmi_align_local_rows (SQMI_LOCAL) restricted to the band of
+ * sqmi_align_score_banded, with the same supported bands.  ops may be NULL
+ * for scores and counts only (op_offsets, op_lengths and begins are then
+ * not read); begins is required with SQMI_LOCAL and ops, and not used in
+ * global modes.  Paths follow the reference's banded traceback for one pair,
+ * including its end cell on a band whose lower diagonal ends on the last
+ * row, where it takes the cell's own direction.  Flags: only the
+ * *_ON_DEVICE ones. */
+sqmi_status sqmi_align_rows_banded(sqmi_context *ctx, const sqmi_sequences *set_h,
+                                   const sqmi_sequences *set_v, const sqmi_scoring *scoring,
+                                   uint32_t mode, int64_t lower_diag, int64_t upper_diag,
+                                   int32_t *scores, int32_t *counts, uint8_t *ops,
+                                   const int64_t *op_offsets, int64_t *op_lengths,
+                                   int64_t *begins, uint32_t flags);
 
 sqmi_status sqmi_get_stats(const sqmi_context *ctx, sqmi_stats *out);
 

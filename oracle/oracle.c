@@ -219,9 +219,17 @@ void orc_score_batch(const uint8_t *h_res, const int64_t *h_off,
 /* ------------------------------------------------------------------------ */
 /* Traceback: full trace matrix (column-major, (n+1) x (m+1) bytes).          */
 /* ------------------------------------------------------------------------ */
-int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
-                      const orc_scoring *sc, uint32_t mode, int32_t *counts,
-                      char *ops, int64_t *n_ops, int64_t *begins)
+/* Banded tracebacks (globalAlignment / localAlignment with lowerDiag,   */
+/* upperDiag, global_alignment_banded.h:152-181, local_alignment_banded.h: */
+/* 242-280) run the same recurrence with every cell outside               */
+/* lower <= j - i <= upper at -inf: the reference's band-edge cells leave   */
+/* out the direction that comes from outside the band                      */
+/* (RecursionDirectionUpperDiagonal / LowerDiagonal,                       */
+/* dp_formula_affine.h:129-230), which is what a -inf predecessor gives.   */
+static int32_t traceback_impl(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
+                              const orc_scoring *sc, uint32_t mode, int32_t *counts,
+                              char *ops, int64_t *n_ops, int64_t *begins,
+                              int banded, int64_t lower, int64_t upper)
 {
     const int A = sc->alpha;
     const int local = (mode & ORC_LOCAL) != 0;
@@ -229,6 +237,16 @@ int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
     if (n_ops) *n_ops = 0;
     if (begins) begins[0] = begins[1] = 0;
     if (n <= 0 || m <= 0) return INT32_MIN;
+    if (banded) {
+        /* The band checks of orc_score_banded. */
+        const uint32_t fm = local ? (ORC_FREE_TOP | ORC_FREE_LEFT | ORC_FREE_RIGHT | ORC_FREE_BOTTOM) : mode;
+        if (upper < -m || lower > n) return INT32_MIN;
+        if (upper < 0 && !(fm & ORC_FREE_LEFT)) return INT32_MIN;
+        if (lower > 0 && !(fm & ORC_FREE_TOP)) return INT32_MIN;
+        if (upper + m < n && !(fm & ORC_FREE_BOTTOM)) return INT32_MIN;
+        if (lower + m > n && !(fm & ORC_FREE_RIGHT)) return INT32_MIN;
+    }
+#define OUT(I, J) (banded && ((J) - (I) < lower || (J) - (I) > upper))
     const int32_t o = sc->gap_open, e = sc->gap_extend;
     const int64_t R = m + 1;
     uint8_t *T = (uint8_t *)malloc((size_t)(n + 1) * (size_t)R);
@@ -248,9 +266,16 @@ int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
     /* Initial column. */
     {
         int32_t F = NEG_INF;
-        H[0] = 0; E[0] = NEG_INF; T[0] = TB_NONE;
+        H[0] = OUT(0, 0) ? NEG_INF : 0; E[0] = NEG_INF; T[0] = TB_NONE;
         TRACK(0, 0, 0, 0, H[0], E[0], F);
         for (int64_t i = 1; i <= m; ++i) {
+            if (OUT(i, 0)) {
+                H[i] = NEG_INF;
+                E[i] = NEG_INF;
+                F = NEG_INF;
+                T[i] = TB_NONE;
+                continue;
+            }
             if (local || (mode & ORC_FREE_LEFT)) {
                 H[i] = 0;
                 T[i] = TB_NONE;
@@ -271,7 +296,11 @@ int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
         const uint8_t hj = h[j - 1];
         int32_t diag = H[0];
         int32_t F = NEG_INF;
-        if (local || (mode & ORC_FREE_TOP)) {
+        if (OUT(0, j)) {
+            H[0] = NEG_INF;
+            E[0] = NEG_INF;
+            Tc[0] = TB_NONE;
+        } else if (local || (mode & ORC_FREE_TOP)) {
             H[0] = 0;
             Tc[0] = TB_NONE;
         } else {
@@ -283,6 +312,15 @@ int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
         TRACK(col, 0, 0, j, H[0], E[0], F);
         int32_t up = H[0];
         for (int64_t i = 1; i <= m; ++i) {
+            if (OUT(i, j)) {
+                diag = H[i];
+                H[i] = NEG_INF;
+                E[i] = NEG_INF;
+                F = NEG_INF;
+                up = NEG_INF;
+                Tc[i] = TB_NONE;
+                continue;
+            }
             const int32_t d = diag + subst(sc, hj, v[i - 1]);
             diag = H[i];
             int32_t ext = E[i] + e, opn = H[i] + o;
@@ -320,6 +358,12 @@ int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
     const int linear = (o == e);
     uint8_t tv = T[bj * R + bi];
     if (!linear) {
+        /* A band whose lower diagonal ends on the last row: that cell is
+         * tracked from the next column's code path (dp_algorithm_impl.h:
+         * 843-862), and the reference's traceback takes its own direction
+         * there, with no preference for the vertical gap. */
+        if (banded && bi == m && bj - bi == lower)
+            bestF = NEG_INF;
         /* _correctTraceValue: at the best cell prefer the gap matrices. */
         if (bestF == best) {
             tv = (uint8_t)((tv & ~TB_DIAG) | TB_MAX_VERT);
@@ -432,6 +476,21 @@ int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
     free(H);
     free(E);
     return best;
+#undef OUT
+}
+
+int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
+                      const orc_scoring *sc, uint32_t mode, int32_t *counts,
+                      char *ops, int64_t *n_ops, int64_t *begins)
+{
+    return traceback_impl(h, n, v, m, sc, mode, counts, ops, n_ops, begins, 0, 0, 0);
+}
+
+int32_t orc_traceback_banded(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
+                             const orc_scoring *sc, uint32_t mode, int64_t lower, int64_t upper,
+                             int32_t *counts, char *ops, int64_t *n_ops, int64_t *begins)
+{
+    return traceback_impl(h, n, v, m, sc, mode, counts, ops, n_ops, begins, 1, lower, upper);
 }
 
 void orc_traceback_batch(const uint8_t *h_res, const int64_t *h_off,
@@ -448,6 +507,25 @@ void orc_traceback_batch(const uint8_t *h_res, const int64_t *h_off,
         scores[k] = orc_traceback(h_res + h_off[k], h_off[k + 1] - h_off[k],
                                   v_res + v_off[k], v_off[k + 1] - v_off[k], sc, mode,
                                   counts ? counts + k * stride : NULL, NULL, NULL, NULL);
+    (void)threads;
+}
+
+void orc_rows_batch_banded(const uint8_t *h_res, const int64_t *h_off,
+                           const uint8_t *v_res, const int64_t *v_off, int64_t count,
+                           const orc_scoring *sc, uint32_t mode, int64_t lower, int64_t upper,
+                           int32_t *scores, int32_t *counts, char *ops, const int64_t *op_off,
+                           int64_t *op_len, int64_t *begins, int threads)
+{
+    const int64_t stride = (int64_t)sc->alpha * sc->alpha + 2;
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads_wrapper();
+#pragma omp parallel for schedule(dynamic, 4) num_threads(threads)
+#endif
+    for (int64_t k = 0; k < count; ++k)
+        scores[k] = orc_traceback_banded(h_res + h_off[k], h_off[k + 1] - h_off[k],
+                                         v_res + v_off[k], v_off[k + 1] - v_off[k], sc, mode, lower, upper,
+                                         counts ? counts + k * stride : NULL, ops + op_off[k], op_len + k,
+                                         begins ? begins + 2 * k : NULL);
     (void)threads;
 }
 

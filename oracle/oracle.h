@@ -66,6 +66,20 @@ int32_t orc_traceback(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
 int32_t orc_score_banded(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
                          const orc_scoring *sc, uint32_t mode, int64_t lower, int64_t upper);
 
+/* orc_traceback inside the band lower <= j - i <= upper (cells outside at
+ * -inf); band settings the reference rejects give INT32_MIN, as in
+ * orc_score_banded. */
+int32_t orc_traceback_banded(const uint8_t *h, int64_t n, const uint8_t *v, int64_t m,
+                             const orc_scoring *sc, uint32_t mode, int64_t lower, int64_t upper,
+                             int32_t *counts, char *ops, int64_t *n_ops, int64_t *begins);
+
+/* orc_rows_batch for banded alignments. */
+void orc_rows_batch_banded(const uint8_t *h_res, const int64_t *h_off,
+                           const uint8_t *v_res, const int64_t *v_off, int64_t count,
+                           const orc_scoring *sc, uint32_t mode, int64_t lower, int64_t upper,
+                           int32_t *scores, int32_t *counts, char *ops, const int64_t *op_off,
+                           int64_t *op_len, int64_t *begins, int threads);
+
 /* Batch form of orc_traceback (counts stride alpha*alpha + 2). */
 void orc_traceback_batch(const uint8_t *h_res, const int64_t *h_off,
                          const uint8_t *v_res, const int64_t *v_off, int64_t count,

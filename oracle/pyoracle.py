@@ -61,6 +61,10 @@ def lib():
         L.orc_rows_batch.restype = None
         L.orc_rows_batch.argtypes = [u8p, i64p, u8p, i64p, ctypes.c_int64, sp, ctypes.c_uint32, i32p, i32p,
                                      u8p, i64p, i64p, i64p, ctypes.c_int]
+        L.orc_rows_batch_banded.restype = None
+        L.orc_rows_batch_banded.argtypes = [u8p, i64p, u8p, i64p, ctypes.c_int64, sp, ctypes.c_uint32,
+                                            ctypes.c_int64, ctypes.c_int64, i32p, i32p, u8p, i64p, i64p, i64p,
+                                            ctypes.c_int]
         L.orc_score_banded.restype = ctypes.c_int32
         L.orc_score_banded.argtypes = [u8p, ctypes.c_int64, u8p, ctypes.c_int64, sp, ctypes.c_uint32,
                                        ctypes.c_int64, ctypes.c_int64]
@@ -144,11 +148,12 @@ def score_batch(h_res, h_off, v_res, v_off, sc: Scoring, mode: int, threads: int
 
 
 def rows_batch(h_res, h_off, v_res, v_off, sc: Scoring, mode: int = 0, threads: int = 0,
-               with_begins: bool = False):
+               with_begins: bool = False, band=None):
     """Scores, counts and rows of every pair: returns (scores, counts,
     op_off, ops) with pair k's ops ('M', 'V', 'H' bytes, left to right) in
     ops[op_off[k]:op_off[k + 1]]; with_begins adds begins [count, 2], the
-    positions in H and V where each local window starts."""
+    positions in H and V where each local window starts.  band =
+    (lower_diag, upper_diag) restricts every pair to that band."""
     h_res, v_res = _u8(h_res), _u8(v_res)
     h_off = np.ascontiguousarray(h_off, dtype=np.int64)
     v_off = np.ascontiguousarray(v_off, dtype=np.int64)
@@ -160,11 +165,20 @@ def rows_batch(h_res, h_off, v_res, v_off, sc: Scoring, mode: int = 0, threads: 
     buf = np.zeros(int(cap[-1]), dtype=np.uint8)
     lens = np.zeros(n, dtype=np.int64)
     begins = np.zeros((n, 2), dtype=np.int64)
-    lib().orc_rows_batch(_ptr(h_res, ctypes.c_uint8), _ptr(h_off, ctypes.c_int64),
-                         _ptr(v_res, ctypes.c_uint8), _ptr(v_off, ctypes.c_int64), n,
-                         ctypes.byref(sc._c), mode, _ptr(scores, ctypes.c_int32),
-                         _ptr(counts, ctypes.c_int32), _ptr(buf, ctypes.c_uint8), _ptr(cap, ctypes.c_int64),
-                         _ptr(lens, ctypes.c_int64), _ptr(begins, ctypes.c_int64), threads)
+    if band is None:
+        lib().orc_rows_batch(_ptr(h_res, ctypes.c_uint8), _ptr(h_off, ctypes.c_int64),
+                             _ptr(v_res, ctypes.c_uint8), _ptr(v_off, ctypes.c_int64), n,
+                             ctypes.byref(sc._c), mode, _ptr(scores, ctypes.c_int32),
+                             _ptr(counts, ctypes.c_int32), _ptr(buf, ctypes.c_uint8), _ptr(cap, ctypes.c_int64),
+                             _ptr(lens, ctypes.c_int64), _ptr(begins, ctypes.c_int64), threads)
+    else:
+        lib().orc_rows_batch_banded(_ptr(h_res, ctypes.c_uint8), _ptr(h_off, ctypes.c_int64),
+                                    _ptr(v_res, ctypes.c_uint8), _ptr(v_off, ctypes.c_int64), n,
+                                    ctypes.byref(sc._c), mode, ctypes.c_int64(int(band[0])),
+                                    ctypes.c_int64(int(band[1])), _ptr(scores, ctypes.c_int32),
+                                    _ptr(counts, ctypes.c_int32), _ptr(buf, ctypes.c_uint8),
+                                    _ptr(cap, ctypes.c_int64), _ptr(lens, ctypes.c_int64),
+                                    _ptr(begins, ctypes.c_int64), threads)
     op_off = np.zeros(n + 1, dtype=np.int64)
     op_off[1:] = np.cumsum(lens)
     ops = np.concatenate([buf[cap[k]:cap[k] + lens[k]] for k in range(n)]) if n else np.zeros(0, np.uint8)
@@ -189,7 +203,7 @@ def traceback_batch(h_res, h_off, v_res, v_off, sc: Scoring, mode: int = 0, thre
 # The compiled reference (oracle/_ref).
 # --------------------------------------------------------------------------
 REF_DIR = os.path.join(HERE, "_ref")
-OP_SCORE, OP_COUNTS_SIMD, OP_COUNTS_SCALAR, OP_SCORE_BANDED = 0, 1, 2, 4
+OP_SCORE, OP_COUNTS_SIMD, OP_COUNTS_SCALAR, OP_SCORE_BANDED, OP_ROWS_BANDED = 0, 1, 2, 4, 5
 
 
 def _cpu_has_avx512():
@@ -232,7 +246,7 @@ def run_reference(op, h_res, h_off, v_res, v_off, mode, *, alphabet="dna", matri
             f.write(b"SQRI")
             f.write(struct.pack("<IIIiiiiIiiq", op, mode, alpha_id, match, mismatch, gap_extend,
                                 gap_open, mat_id, threads, repeats, n))
-            if op == OP_SCORE_BANDED:
+            if op in (OP_SCORE_BANDED, OP_ROWS_BANDED):
                 f.write(struct.pack("<qq", lower_diag, upper_diag))
             f.write(h_off.tobytes())
             f.write(v_off.tobytes())
@@ -246,7 +260,7 @@ def run_reference(op, h_res, h_off, v_res, v_off, mode, *, alphabet="dna", matri
     off += 4 * cnt
     out = {"scores": scores, "best_seconds": secs, "simd_lanes": lanes, "threads": thr,
            "binary": os.path.basename(exe)}
-    if op in (OP_COUNTS_SIMD, OP_COUNTS_SCALAR):
+    if op in (OP_COUNTS_SIMD, OP_COUNTS_SCALAR, OP_ROWS_BANDED):
         alpha = 4 if alphabet == "dna" else 27
         S = alpha * alpha + 2
         out["counts"] = np.frombuffer(raw, dtype=np.int32, count=cnt * S, offset=off).reshape(cnt, S).copy()

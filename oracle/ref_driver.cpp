@@ -18,13 +18,17 @@
 //       include/seqan/align_parallel/parallel_align_interface.h:312-321
 //   localAlignment(Gaps, Gaps, score)    (scalar)
 //       include/seqan/align/local_alignment_unbanded.h:262-273
+//   globalAlignment(Gaps, Gaps, score, AlignConfig, lowerDiag, upperDiag)
+//       include/seqan/align/global_alignment_banded.h:152-181
+//   localAlignment(Gaps, Gaps, score, lowerDiag, upperDiag)
+//       include/seqan/align/local_alignment_banded.h:242-259
 //   computeAlignmentStats(stats, row0, row1, score)
 //       include/seqan/align/evaluate_alignment.h:215-300
 //
 // Input file (little endian):
 //   char[4] "SQRI", u32 op (0 score, 1 counts-simd, 2 counts-scalar, 3 table,
 //   4 banded score, global or LOCAL: i64 lower_diag, i64 upper_diag follow
-//   the count),
+//   the count; 5 banded rows, pair by pair, the same two values follow),
 //   u32 mode (1 TOP, 2 LEFT, 4 RIGHT, 8 BOTTOM, 16 LOCAL),
 //   u32 alphabet (0 Dna, 1 AminoAcid), i32 match, i32 mismatch,
 //   i32 gap_extend, i32 gap_open, u32 matrix (0 Simple, 1 Blosum62),
@@ -32,7 +36,7 @@
 //   i64 h_off[count+1], i64 v_off[count+1], u8 h_res[h_off[count]],
 //   u8 v_res[v_off[count]]   (residue ordinals).
 // Output file: i64 count, f64 best_seconds, i32 simd_lanes, i32 threads,
-//   i32 scores[count], then for ops 1/2: i32 counts[count][A*A+2],
+//   i32 scores[count], then for ops 1/2/5: i32 counts[count][A*A+2],
 //   i64 op_off[count+1], u8 ops[op_off[count]]: the rows of each alignment
 //   as one byte per column, 'M' (both rows hold a residue), 'V' (a gap in
 //   the H row) or 'H' (a gap in the V row); with mode LOCAL the rows are
@@ -107,7 +111,7 @@ RefInput readInput(char const * path)
     readOrDie(f, &h.threads, 1);
     readOrDie(f, &h.repeats, 1);
     readOrDie(f, &h.count, 1);
-    if (h.op == 4)
+    if (h.op == 4 || h.op == 5)
     {
         readOrDie(f, &h.lowerDiag, 1);
         readOrDie(f, &h.upperDiag, 1);
@@ -276,7 +280,7 @@ int run(RefInput const & in, TScore const & sc, int alpha, std::FILE * out)
                 scores[k] = res[k];
         }
     }
-    else if (h.op == 1 || h.op == 2)
+    else if (h.op == 1 || h.op == 2 || h.op == 5)
     {
         bool const local = (h.mode & 16u) != 0;
         if (local && (h.mode & 15u))
@@ -299,11 +303,20 @@ int run(RefInput const & in, TScore const & sc, int alpha, std::FILE * out)
             for (int64_t k = 0; k < h.count; ++k)
                 scores[k] = res[k];
         }
-        else
+        else if (h.op == 2)
         {
             for (int64_t k = 0; k < h.count; ++k)
                 scores[k] = local ? localAlignment(gH[k], gV[k], sc)
                                   : withConfig(h.mode, [&](auto cfg) { return globalAlignment(gH[k], gV[k], sc, cfg); });
+        }
+        else
+        {
+            int const lo = static_cast<int>(h.lowerDiag), up = static_cast<int>(h.upperDiag);
+            for (int64_t k = 0; k < h.count; ++k)
+                scores[k] = local ? localAlignment(gH[k], gV[k], sc, lo, up)
+                                  : withConfig(h.mode, [&](auto cfg) {
+                                        return globalAlignment(gH[k], gV[k], sc, cfg, lo, up);
+                                    });
         }
         auto t1 = std::chrono::steady_clock::now();
         best = std::chrono::duration<double>(t1 - t0).count();

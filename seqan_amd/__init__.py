@@ -71,12 +71,12 @@ class Stats(ctypes.Structure):
 
 
 # SQMI_ABI_VERSION of include/seqan_amd.h that the structures below mirror.
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 # Every symbol the C ABI declares (include/seqan_amd.h).
 ABI_SYMBOLS = (
     "sqmi_create", "sqmi_destroy", "sqmi_set_stream", "sqmi_align_score", "sqmi_align_counts", "sqmi_align_rows",
-    "sqmi_align_local_rows", "sqmi_align_score_banded",
+    "sqmi_align_local_rows", "sqmi_align_score_banded", "sqmi_align_rows_banded",
     "sqmi_get_stats", "sqmi_plan_stats", "sqmi_blosum62", "sqmi_status_string", "sqmi_last_error", "sqmi_abi_version",
     "sqmi_workload_create", "sqmi_workload_sizes", "sqmi_workload_copy", "sqmi_workload_destroy",
 )
@@ -108,6 +108,9 @@ def lib():
                                               ctypes.c_int64, vp, ctypes.c_uint32]
         L.sqmi_align_local_rows.argtypes = [vp, ctypes.POINTER(_Sequences), ctypes.POINTER(_Sequences),
                                             ctypes.POINTER(_Scoring), vp, vp, vp, vp, vp, vp, ctypes.c_uint32]
+        L.sqmi_align_rows_banded.argtypes = [vp, ctypes.POINTER(_Sequences), ctypes.POINTER(_Sequences),
+                                             ctypes.POINTER(_Scoring), ctypes.c_uint32, ctypes.c_int64,
+                                             ctypes.c_int64, vp, vp, vp, vp, vp, vp, ctypes.c_uint32]
         L.sqmi_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.sqmi_plan_stats.argtypes = [ctypes.POINTER(_Sequences), ctypes.POINTER(_Sequences),
                                       ctypes.POINTER(_Scoring), ctypes.c_uint32, ctypes.c_uint32,
@@ -401,6 +404,39 @@ class Context:
                                            seg.ctypes.data, lens.ctypes.data, begins.ctypes.data, flags)
         self._check(st, "sqmi_align_local_rows")
         return (scores, counts, *_compact_rows(buf, seg, lens), begins)
+
+    def align_rows_banded(self, set_h: SeqSet, set_v: SeqSet, score: Score, lower_diag: int, upper_diag: int,
+                          mode=0, flags=0, rows=True):
+        """Banded alignment (cells with lower_diag <= j - i <= upper_diag):
+        (scores, counts, op_off, ops) as align_rows, plus begins as in
+        align_local_rows with mode LOCAL; rows=False returns (scores,
+        counts) only."""
+        ch, keep_h = _seqs(set_h)
+        cv, keep_v = _seqs(set_v)
+        sc = score._c()
+        n = len(set_h.offsets) - 1
+        A = score.alphabet_size
+        scores = np.empty(n, np.int32)
+        counts = np.empty((n, A * A + 2), np.int32)
+        local = bool(mode & LOCAL)
+        if not rows:
+            st = self._L.sqmi_align_rows_banded(self._h, ctypes.byref(ch), ctypes.byref(cv), ctypes.byref(sc), mode,
+                                                lower_diag, upper_diag, scores.ctypes.data, counts.ctypes.data,
+                                                None, None, None, None, flags)
+            self._check(st, "sqmi_align_rows_banded")
+            return scores, counts
+        seg = np.zeros(n + 1, np.int64)
+        seg[1:] = np.cumsum(set_h.lengths() + set_v.lengths())
+        buf = np.empty(max(int(seg[-1]), 1), np.uint8)
+        lens = np.zeros(n, np.int64)
+        begins = np.zeros((n, 2), np.int64)
+        st = self._L.sqmi_align_rows_banded(self._h, ctypes.byref(ch), ctypes.byref(cv), ctypes.byref(sc), mode,
+                                            lower_diag, upper_diag, scores.ctypes.data, counts.ctypes.data,
+                                            buf.ctypes.data, seg.ctypes.data, lens.ctypes.data,
+                                            begins.ctypes.data if local else None, flags)
+        self._check(st, "sqmi_align_rows_banded")
+        out = (scores, counts, *_compact_rows(buf, seg, lens))
+        return (*out, begins) if local else out
 
 
 def _compact_rows(buf, seg, lens):

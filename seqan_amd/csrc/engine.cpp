@@ -312,7 +312,7 @@ inline int32_t blocksOf(uint32_t key) { return (int32_t)(255u - ((key >> kKeyNBi
 // Routing (empty / int16 / int32) and length bucketing into work items.
 // Pure host code: needs no device.
 void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &scoring, uint32_t mode,
-          uint32_t flags, Planned &P, bool traceShape = false)
+          uint32_t flags, Planned &P, bool traceShape = false, bool noLong = false)
 {
     P.flags = flags;
     const int64_t N = H.count;
@@ -361,7 +361,7 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
     }
     std::vector<Key> keys((size_t)N);
     int64_t cells = 0;
-    const bool allLong = longAll() && si.dna && mode == 0;
+    const bool allLong = longAll() && si.dna && mode == 0 && !noLong;
     const int64_t longMin0 = longMinSide();
 #pragma omp parallel for schedule(static) reduction(+ : cells)
     for (int64_t i = 0; i < N; ++i) {
@@ -399,7 +399,7 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
     while (e32 < keys.size() && (keys[e32].key >> kKeyClassShift) == 2)
         ++e32;
     const int64_t longMin = longMinSide();
-    const bool longOk = si.dna && mode == 0;
+    const bool longOk = si.dna && mode == 0 && !noLong;
     for (size_t k = n16; k < e32; ++k) {
         const int32_t i = keys[k].id;
         const int64_t n = H.offsets[i + 1] - H.offsets[i];
@@ -1139,7 +1139,7 @@ sqmi_status sqmi_plan_stats(const sqmi_sequences *setH, const sqmi_sequences *se
 static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
                                  const sqmi_scoring *scoring, uint32_t mode, int32_t *scores, int32_t *counts,
                                  uint8_t *ops, const int64_t *opOffsets, int64_t *opLengths, int64_t *begins,
-                                 uint32_t flags)
+                                 uint32_t flags, const int64_t *band = nullptr)
 {
     const auto tStart = std::chrono::steady_clock::now();
     const bool wantRows = ops != nullptr;
@@ -1174,7 +1174,8 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
 
     Planned P;
     const auto tPlan = std::chrono::steady_clock::now();
-    plan(*setH, *setV, *scoring, mode, flags, P, true);
+    // Banded pairs run on the int32 kernel, band-masked.
+    plan(*setH, *setV, *scoring, mode, band ? (flags | SQMI_FORCE_INT32) : flags, P, true, band != nullptr);
     ctx->stats.host_plan_ms = msSince(tPlan);
     fillStats(P, ctx->stats);
     const ScoringInfo &si = P.si;
@@ -1405,6 +1406,11 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             a.opOff = rows.opOff;
             a.opLen = rows.opLen;
             a.begins = rows.begins;
+            if (band) {
+                a.banded = 1;
+                a.lower = band[0];
+                a.upper = band[1];
+            }
             if ((err = launchCounts32(a, stream)) != hipSuccess)
                 return hipFail(ctx, err, "launching int32 traceback kernel");
             b = e;
@@ -1624,6 +1630,35 @@ sqmi_status sqmi_align_local_rows(sqmi_context *ctx, const sqmi_sequences *setH,
         return ctx ? fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "ops or begins is null") : SQMI_ERR_INVALID_ARGUMENT;
     return alignBackward(ctx, setH, setV, scoring, SQMI_LOCAL, scores, counts, ops, op_offsets, op_lengths, begins,
                          flags);
+}
+
+sqmi_status sqmi_align_rows_banded(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
+                                   const sqmi_scoring *scoring, uint32_t mode, int64_t lower_diag,
+                                   int64_t upper_diag, int32_t *scores, int32_t *counts, uint8_t *ops,
+                                   const int64_t *op_offsets, int64_t *op_lengths, int64_t *begins, uint32_t flags)
+{
+    if (!ctx)
+        return SQMI_ERR_INVALID_ARGUMENT;
+    ctx->lastError.clear();
+    sqmi_status st;
+    if ((st = validate(ctx, setH, setV, scoring, mode)) != SQMI_OK)
+        return st;
+    if (flags & ~(uint32_t)(SQMI_RESIDUES_ON_DEVICE | SQMI_SCORES_ON_DEVICE))
+        return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_rows_banded: flags other than the *_ON_DEVICE ones");
+    if (lower_diag > upper_diag)
+        return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "lower_diag > upper_diag");
+    const bool local = (mode & SQMI_LOCAL) != 0;
+    if (local && ops && !begins)
+        return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "begins is null");
+    for (int64_t i = 0; i < setH->count; ++i) {
+        const int64_t n = setH->offsets[i + 1] - setH->offsets[i], m = setV->offsets[i + 1] - setV->offsets[i];
+        if (n != 0 && m != 0 && !bandSupported(n, m, lower_diag, upper_diag, local))
+            return fail(ctx, SQMI_ERR_UNSUPPORTED,
+                        "sqmi_align_rows_banded: pair " + std::to_string(i) + " is outside the supported bands");
+    }
+    const int64_t band[2] = {lower_diag, upper_diag};
+    return alignBackward(ctx, setH, setV, scoring, local ? (uint32_t)SQMI_LOCAL : mode, scores, counts, ops,
+                         op_offsets, op_lengths, local ? begins : nullptr, flags, band);
 }
 
 }  // extern "C"

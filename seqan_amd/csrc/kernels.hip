@@ -1146,6 +1146,14 @@ __global__ __launch_bounds__(256) void kCounts32(Counts32Args a)
     const bool freeTop = local || (a.mode & SQMI_FREE_TOP) != 0, freeLeft = local || (a.mode & SQMI_FREE_LEFT) != 0;
     const bool freeRight = local || (a.mode & SQMI_FREE_RIGHT) != 0;
     const bool freeBottom = local || (a.mode & SQMI_FREE_BOTTOM) != 0;
+    // Banded (globalAlignment / localAlignment with lowerDiag, upperDiag,
+    // global_alignment_banded.h:152-181, local_alignment_banded.h:242-259):
+    // column j holds rows [j - upper, j - lower] only; the rest are -inf,
+    // which is what the reference's band-edge recursions give
+    // (dp_formula_affine.h:129-230).  Rows below the band keep the -inf
+    // they start with; rows above it are never read again.
+    const bool banded = a.banded != 0;
+    const int64_t lo = a.lower, hi = a.upper;
     int32_t best = kNeg32, bestE = kNeg32, bestF = kNeg32;
     int bi = m, bj = n;
     auto track = [&](bool lastCol, int i, int j, int32_t hv, int32_t ev, int32_t fv) {
@@ -1160,9 +1168,15 @@ __global__ __launch_bounds__(256) void kCounts32(Counts32Args a)
     };
     {
         int32_t F = kNeg32;
-        H[0] = 0;
+        H[0] = (banded && (lo > 0 || hi < 0)) ? kNeg32 : 0;
         E[0] = kNeg32;
         for (int i = 1; i <= m; ++i) {
+            if (banded && (-i < lo || -i > hi)) {
+                H[i] = kNeg32;
+                E[i] = kNeg32;
+                F = kNeg32;
+                continue;
+            }
             if (freeLeft) {
                 H[i] = 0;
             } else {
@@ -1178,7 +1192,10 @@ __global__ __launch_bounds__(256) void kCounts32(Counts32Args a)
         const int32_t *row = a.table + hj * A;
         int32_t diag = H[0];
         int32_t F = kNeg32;
-        if (freeTop) {
+        if (banded && (j < lo || j > hi)) {
+            H[0] = kNeg32;
+            E[0] = kNeg32;
+        } else if (freeTop) {
             H[0] = 0;
         } else {
             E[0] = max(E[0] + e, H[0] + o);
@@ -1187,7 +1204,16 @@ __global__ __launch_bounds__(256) void kCounts32(Counts32Args a)
         track(j == n, 0, j, H[0], E[0], kNeg32);
         int32_t up = H[0];
         uint8_t *Tc = T + (int64_t)j * CS;
-        for (int i = 1; i <= m; ++i) {
+        int i0 = 1, i1 = m;
+        if (banded) {
+            i0 = (int)max<int64_t>(1, j - hi);
+            i1 = (int)min<int64_t>(m, j - lo);
+            if (i0 > 1) {
+                diag = H[i0 - 1];   // column j - 1's value: (i0 - 1, j - 1) is in the band
+                up = kNeg32;
+            }
+        }
+        for (int i = i0; i <= i1; ++i) {
             const int32_t d = diag + row[min((int)v[i - 1], A - 1)];
             diag = H[i];
             const int32_t extE = E[i] + e, opnE = H[i] + o;
@@ -1214,6 +1240,12 @@ __global__ __launch_bounds__(256) void kCounts32(Counts32Args a)
         }
     }
     a.scores[pid] = best;
+    // A band whose lower diagonal ends on the last row: the reference tracks
+    // that cell from the next column's code path (dp_algorithm_impl.h:
+    // 843-862) and its traceback then takes the cell's own direction, with
+    // no preference for the vertical gap.
+    if (banded && bi == m && (int64_t)bj - bi == lo)
+        bestF = kNeg32;
     auto bits = [&](int i, int j) -> uint32_t { return T[(int64_t)j * CS + i]; };
     walkCounts(n, m, h, v, A, best, bestE, bestF, bits, cnt, a.ops ? a.ops + a.opOff[pid + 1] : nullptr,
                a.ops ? a.opLen + pid : nullptr, bi, bj, local, a.begins ? a.begins + 2 * pid : nullptr, o == e);

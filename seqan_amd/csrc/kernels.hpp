@@ -117,6 +117,9 @@ struct Counts32Args {
     const int64_t *opOff;
     int64_t *opLen;
     int64_t *begins;            // local: per pair the window's first H and V position (optional)
+    int32_t banded;             // cells outside lower <= j - i <= upper are -inf
+    int64_t lower;
+    int64_t upper;
 };
 
 // Simple int32 kernel: one thread per pair, H/E column in global scratch.

@@ -336,6 +336,106 @@ def banded_scores():
          **SIMPLE)
 
 
+def banded_rows():
+    """Banded rows, pair by pair: globalAlignment(gapsH, gapsV, score,
+    AlignConfig, lowerDiag, upperDiag) (global_alignment_banded.h:152-181)
+    and, for mode 16, localAlignment(gapsH, gapsV, score, lowerDiag,
+    upperDiag) (local_alignment_banded.h:242-259), under several
+    (mode, lower, upper) cases (`cases`): per case the scores, counts and
+    rows (ops of case k in ops[case_off[k]:case_off[k + 1]]), and the
+    window begins of local cases (`begins`, one [count, 2] block per case,
+    zeros for global ones).  Varying lengths; every band is in the domain
+    the library takes (the one of the banded score fixtures)."""
+    rng = np.random.default_rng(0x5EED0500)
+
+    def pairs(count, lo_len, hi_len, alpha, dele=0.05, sub=0.12):
+        hs, vs = [], []
+        for _ in range(count):
+            n = int(rng.integers(lo_len, hi_len + 1))
+            h = rng.integers(0, alpha, n).astype(np.uint8)
+            v = np.delete(h, np.nonzero(rng.random(n) < dele)[0])
+            if len(v) == 0:
+                v = h.copy()
+            f = rng.random(len(v)) < sub
+            v[f] = rng.integers(0, alpha, int(f.sum()))
+            hs.append(h)
+            vs.append(v.astype(np.uint8))
+        return sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+
+    def limit(H, V):
+        return int(np.max(np.abs(H.lengths() - V.lengths())))
+
+    for name, count, lo_len, hi_len, alpha, kw, spans in (
+            ("ref_dna_banded_rows.npz", 160, 2, 220, 4, dict(alphabet="dna", **SIMPLE),
+             [(0, 3, 3), (8, 2, 10), (9, 40, 5), (15, 1, 70), (4, 6, 1), (6, 2, 2), (16, 3, 3), (16, 12, 30)]),
+            ("ref_dna_banded_ties_rows.npz", 300, 2, 30, 4,
+             dict(alphabet="dna", match=1, mismatch=-1, gap_extend=-1, gap_open=-2),
+             [(0, 1, 1), (8, 1, 1), (8, 2, 4), (9, 1, 2), (15, 1, 1), (2, 3, 1), (16, 1, 1), (16, 4, 6)]),
+            ("ref_dna_banded_linear_rows.npz", 160, 2, 60, 4,
+             dict(alphabet="dna", match=1, mismatch=-1, gap_extend=-1, gap_open=-1),
+             [(0, 2, 2), (8, 1, 3), (15, 1, 1), (16, 2, 2)]),
+            ("ref_protein_banded_rows.npz", 96, 2, 150, 24,
+             dict(alphabet="aa", matrix="blosum62", gap_extend=-1, gap_open=-11),
+             [(0, 4, 4), (9, 20, 10), (16, 5, 5)])):
+        H, V = pairs(count, lo_len, hi_len, alpha)
+        lim = limit(H, V)
+        # (mode, a, b) -> band [-lim - a, lim + b]: both corner diagonals inside.
+        cases = [(mode, -lim - a, lim + b) for mode, a, b in spans]
+        res = [po.run_reference(po.OP_ROWS_BANDED, H.residues, H.offsets, V.residues, V.offsets, mode,
+                                lower_diag=lo, upper_diag=up, **kw) for mode, lo, up in cases]
+        extra = {k: v for k, v in kw.items() if k not in ("alphabet", "matrix")}
+        case_off = np.cumsum([0] + [len(r["ops"]) for r in res]).astype(np.int64)
+        begins = np.stack([r["begins"] if mode == 16 else np.zeros((count, 2), np.int64)
+                           for r, (mode, _, _) in zip(res, cases)])
+        save(name, H, V, cases=np.array(cases, np.int64), scores=np.stack([r["scores"] for r in res]),
+             counts=np.stack([r["counts"] for r in res]), op_off=np.stack([r["op_off"] for r in res]),
+             ops=np.concatenate([r["ops"] for r in res]), case_off=case_off, begins=begins, **extra)
+
+
+def banded_end_rows():
+    """Banded rows whose end cell lies on the band's lower diagonal in the
+    last row (free last row), where the reference's traceback takes the
+    cell's own direction rather than preferring the vertical gap: short DNA
+    pairs under three scorings, the band [-3, 6] (|H| - |V| in [-2, 5]), the
+    reference's rows pair by pair, kept when its end cell (m, n - trailing
+    gap columns) has j - i == -3.  Per scoring s: cases[s] = (mode, lower,
+    upper, match, mismatch, gap_extend, gap_open), pairs sel[s]."""
+    rng = np.random.default_rng(0x5EED0600)
+    lo, up = -3, 6
+    hs_all, vs_all, rows = [], [], []
+    for s, (ma, mi, ge, go) in enumerate(((1, -1, -1, -2), (2, -3, -1, -5), (3, -2, -3, -2))):
+        for mode in (8, 9):
+            hs, vs = [], []
+            for _ in range(3000):
+                m = int(rng.integers(4, 24))
+                n = m + int(rng.integers(-2, 6))
+                n = max(n, 2)
+                hs.append(rng.integers(0, 4, n).astype(np.uint8))
+                vs.append(rng.integers(0, 4, m).astype(np.uint8))
+            H, V = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+            r = po.run_reference(po.OP_ROWS_BANDED, H.residues, H.offsets, V.residues, V.offsets, mode,
+                                 lower_diag=lo, upper_diag=up, alphabet="dna", match=ma, mismatch=mi,
+                                 gap_extend=ge, gap_open=go)
+            kept = 0
+            for k in range(len(hs)):
+                ops = bytes(r["ops"][r["op_off"][k]:r["op_off"][k + 1]])
+                tail = len(ops) - len(ops.rstrip(b"H"))
+                if ops.endswith(b"V") or len(hs[k]) - tail - len(vs[k]) != lo:
+                    continue
+                hs_all.append(hs[k])
+                vs_all.append(vs[k])
+                rows.append((mode, lo, up, ma, mi, ge, go, int(r["scores"][k]), r["counts"][k], ops))
+                kept += 1
+                if kept == 60:
+                    break
+    H, V = sa.SeqSet.from_list(hs_all), sa.SeqSet.from_list(vs_all)
+    lens = [len(x[9]) for x in rows]
+    save("ref_dna_banded_end_rows.npz", H, V, cases=np.array([x[:7] for x in rows], np.int64),
+         scores=np.array([x[7] for x in rows], np.int32), counts=np.stack([x[8] for x in rows]),
+         op_off=np.cumsum([0] + lens).astype(np.int64),
+         ops=np.frombuffer(b"".join(x[9] for x in rows), np.uint8))
+
+
 def long_pairs():
     """100 kbp fixture: the first two pairs of the 64 x 100 kbp workload at
     full length (run on its own: --long).  Scores come from the reference's
@@ -362,6 +462,9 @@ if __name__ == "__main__":
         local_rows()
     elif "--linear" in sys.argv:
         linear_rows()
+    elif "--banded-rows" in sys.argv:
+        banded_rows()
+        banded_end_rows()
     elif "--banded" in sys.argv:
         banded_scores()
     else:
@@ -370,3 +473,5 @@ if __name__ == "__main__":
         local_rows()
         linear_rows()
         banded_scores()
+        banded_rows()
+        banded_end_rows()

@@ -173,3 +173,109 @@ def test_python_mirror_functions(ctx):
     l = sa.local_alignment(hs, vs, score, alphabet=sa.DNA, ctx=ctx)
     lr = ctx.align_local_rows(Hs, Vs, score)
     assert all(np.array_equal(x, y) for x, y in zip(l, lr))
+
+
+# ---------------------------------------------------------------------------
+# Banded rows (sqmi_align_rows_banded): globalAlignment / localAlignment on
+# Gaps with lowerDiag, upperDiag (global_alignment_banded.h:152-181,
+# local_alignment_banded.h:242-259), on the int32 traceback kernel with the
+# band masked in.
+# ---------------------------------------------------------------------------
+BANDED_ROWS = ["ref_dna_banded_rows.npz", "ref_dna_banded_ties_rows.npz", "ref_dna_banded_linear_rows.npz",
+               "ref_protein_banded_rows.npz"]
+
+
+@pytest.mark.parametrize("fixture", BANDED_ROWS)
+def test_banded_rows_match_reference_fixtures(ctx, fixture):
+    z = np.load(os.path.join(GOLDEN, fixture), allow_pickle=False)
+    score = _score_of(z, fixture)
+    Hs, Vs = sa.SeqSet(z["h_res"], z["h_off"]), sa.SeqSet(z["v_res"], z["v_off"])
+    for k, (mode, lo, up) in enumerate(z["cases"]):
+        mode, lo, up = int(mode), int(lo), int(up)
+        print("banded rows fixture", fixture, "mode", mode, "band", lo, up, flush=True)
+        out = ctx.align_rows_banded(Hs, Vs, score, lo, up, mode=mode)
+        s, c, off, ops = out[:4]
+        assert np.array_equal(s, z["scores"][k]), (mode, lo, up)
+        assert np.array_equal(c, z["counts"][k]), (mode, lo, up)
+        assert np.array_equal(off, z["op_off"][k]), (mode, lo, up)
+        assert np.array_equal(ops, z["ops"][z["case_off"][k]:z["case_off"][k + 1]]), (mode, lo, up)
+        if mode == sa.LOCAL:
+            assert np.array_equal(out[4], z["begins"][k]), (lo, up)
+        s2, c2 = ctx.align_rows_banded(Hs, Vs, score, lo, up, mode=mode, rows=False)
+        assert np.array_equal(s2, s) and np.array_equal(c2, c)
+        assert np.array_equal(ctx.align_score_banded(Hs, Vs, score, lo, up, mode=mode), s)
+
+
+def test_banded_rows_end_cell_on_lower_diagonal(ctx):
+    # End cells on the band's lower diagonal in the last row, where the
+    # reference takes the cell's own direction (one pair per call: each has
+    # its own scoring).
+    z = np.load(os.path.join(GOLDEN, "ref_dna_banded_end_rows.npz"), allow_pickle=False)
+    print("banded end-cell rows", len(z["cases"]), "pairs", flush=True)
+    for k, (mode, lo, up, ma, mi, ge, go) in enumerate(z["cases"]):
+        h = z["h_res"][z["h_off"][k]:z["h_off"][k + 1]]
+        v = z["v_res"][z["v_off"][k]:z["v_off"][k + 1]]
+        score = sa.simple_score(int(ma), int(mi), int(ge), int(go))
+        s, c, off, ops = ctx.align_rows_banded(sa.SeqSet.from_list([h]), sa.SeqSet.from_list([v]), score,
+                                               int(lo), int(up), mode=int(mode))
+        assert s[0] == z["scores"][k] and np.array_equal(c[0], z["counts"][k]), k
+        assert np.array_equal(ops, z["ops"][z["op_off"][k]:z["op_off"][k + 1]]), k
+
+
+@pytest.mark.parametrize("mode", [0, 8, 9, 15, 4, sa.LOCAL])
+def test_banded_rows_match_oracle(ctx, mode):
+    # Varying lengths up to 600, bands of several widths, two scorings.
+    rng = np.random.default_rng(100 + mode)
+    for ma, mi, ge, go in ((2, -3, -1, -5), (1, -1, -1, -2)):
+        score = sa.simple_score(ma, mi, ge, go)
+        sc = po.Scoring(score.table, go, ge)
+        hs, vs = [], []
+        for _ in range(150):
+            n = int(rng.integers(2, 600))
+            h = rng.integers(0, 4, n).astype(np.uint8)
+            v = np.delete(h, np.nonzero(rng.random(n) < 0.03)[0])
+            if len(v) == 0:
+                v = h.copy()
+            f = rng.random(len(v)) < 0.1
+            v[f] = rng.integers(0, 4, int(f.sum()))
+            hs.append(h)
+            vs.append(v.astype(np.uint8))
+        Hs, Vs = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+        lim = int(np.max(np.abs(Hs.lengths() - Vs.lengths())))
+        for lo, up in ((-lim - 2, lim + 3), (-lim - 15, lim + 1), (-lim - 1, lim + 40)):
+            print("banded rows vs oracle, mode", mode, "scoring", (ma, mi, ge, go), "band", lo, up, flush=True)
+            out = ctx.align_rows_banded(Hs, Vs, score, lo, up, mode=mode)
+            want = po.rows_batch(Hs.residues, Hs.offsets, Vs.residues, Vs.offsets, sc, mode,
+                                 with_begins=True, band=(lo, up))
+            for a, b in zip(out, want):
+                assert np.array_equal(a, b), (mode, lo, up)
+
+
+def test_banded_rows_full_band_is_unbanded(ctx):
+    rng = np.random.default_rng(5)
+    hs = [rng.integers(0, 4, int(rng.integers(2, 300))).astype(np.uint8) for _ in range(100)]
+    vs = [rng.integers(0, 4, int(rng.integers(2, 300))).astype(np.uint8) for _ in range(100)]
+    Hs, Vs = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+    score = sa.simple_score(2, -3, -1, -5)
+    for mode in (0, 9):
+        got = ctx.align_rows_banded(Hs, Vs, score, -400, 400, mode=mode)
+        want = ctx.align_rows(Hs, Vs, score, mode=mode)
+        for a, b in zip(got, want):
+            assert np.array_equal(a, b), mode
+    got = ctx.align_rows_banded(Hs, Vs, score, -400, 400, mode=sa.LOCAL)
+    want = ctx.align_local_rows(Hs, Vs, score)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+
+
+def test_banded_rows_errors_and_empty(ctx):
+    Hs = sa.SeqSet.from_list([[0, 1, 2, 3], [], [1, 1, 1]])
+    Vs = sa.SeqSet.from_list([[0, 1, 2], [1], [1, 1, 1]])
+    score = sa.simple_score(2, -3, -1, -5)
+    s, c, off, ops = ctx.align_rows_banded(Hs, Vs, score, -2, 3)
+    assert s[1] == np.iinfo(np.int32).min and (c[1] == 0).all() and off[2] - off[1] == 0
+    assert s[2] == 6 and bytes(ops[off[2]:off[3]]) == b"MMM"
+    with pytest.raises(sa.SqmiError):
+        ctx.align_rows_banded(Hs, Vs, score, 0, 3)        # misses the (0, 0) corner diagonal margin
+    with pytest.raises(sa.SqmiError):
+        ctx.align_rows_banded(Hs, Vs, score, 3, -2)       # lower > upper

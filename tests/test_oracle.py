@@ -349,6 +349,60 @@ def test_oracle_banded_full_band_is_unbanded():
         assert po.score_banded(h, v, sc, mode, -len(v) - 3, len(h) + 2) == po.score(h, v, sc, mode)
 
 
+BANDED_ROWS = ["ref_dna_banded_rows.npz", "ref_dna_banded_ties_rows.npz", "ref_dna_banded_linear_rows.npz",
+               "ref_protein_banded_rows.npz"]
+
+
+@pytest.mark.parametrize("fixture", BANDED_ROWS)
+def test_oracle_banded_rows_match_reference(fixture):
+    # Banded globalAlignment / localAlignment on Gaps, pair by pair: scores,
+    # counts, rows and local begins, under every (mode, band) case of the
+    # fixture.  The ties fixture (short pairs, match 1) holds end cells on the
+    # band's lower diagonal in the last row, where the reference does not
+    # prefer the vertical gap; the oracle without that rule fails it.
+    z = _load(fixture)
+    sc = _local_scoring(z, fixture)
+    for k, (mode, lo, up) in enumerate(z["cases"]):
+        s, c, off, ops, beg = po.rows_batch(z["h_res"], z["h_off"], z["v_res"], z["v_off"], sc, int(mode),
+                                            with_begins=True, band=(int(lo), int(up)))
+        assert np.array_equal(s, z["scores"][k]), (fixture, mode, lo, up)
+        assert np.array_equal(c, z["counts"][k]), (fixture, mode, lo, up)
+        assert np.array_equal(off, z["op_off"][k]), (fixture, mode, lo, up)
+        assert np.array_equal(ops, z["ops"][z["case_off"][k]:z["case_off"][k + 1]]), (fixture, mode, lo, up)
+        if mode == 16:
+            assert np.array_equal(beg, z["begins"][k]), (fixture, lo, up)
+
+
+def test_oracle_banded_end_cell_rows_match_reference():
+    # Pairs whose reference end cell lies on the band's lower diagonal in the
+    # last row (free last row), each with its own scoring: the reference takes
+    # that cell's own direction there; an oracle that prefers the vertical gap
+    # at the end cell, as it does elsewhere, gets 48 of these 360 rows wrong.
+    z = _load("ref_dna_banded_end_rows.npz")
+    for k, (mode, lo, up, ma, mi, ge, go) in enumerate(z["cases"]):
+        h = z["h_res"][z["h_off"][k]:z["h_off"][k + 1]]
+        v = z["v_res"][z["v_off"][k]:z["v_off"][k + 1]]
+        sc = po.Scoring.simple(int(ma), int(mi), int(ge), int(go))
+        s, c, off, ops = po.rows_batch(h, np.array([0, len(h)]), v, np.array([0, len(v)]), sc, int(mode),
+                                       band=(int(lo), int(up)))
+        assert s[0] == z["scores"][k] and np.array_equal(c[0], z["counts"][k]), k
+        assert np.array_equal(ops, z["ops"][z["op_off"][k]:z["op_off"][k + 1]]), k
+
+
+def test_oracle_banded_rows_full_band_is_unbanded():
+    # A band holding every diagonal gives the unbanded rows, in every mode.
+    rng = np.random.default_rng(32)
+    sc = po.Scoring.simple(1, -1, -1, -2)
+    for _ in range(150):
+        h = rng.integers(0, 4, int(rng.integers(2, 30))).astype(np.uint8)
+        v = rng.integers(0, 4, int(rng.integers(2, 30))).astype(np.uint8)
+        ho, vo = np.array([0, len(h)]), np.array([0, len(v)])
+        for mode in (0, 8, 9, 15, 16):
+            full = po.rows_batch(h, ho, v, vo, sc, mode, with_begins=True)
+            band = po.rows_batch(h, ho, v, vo, sc, mode, with_begins=True, band=(-len(v) - 2, len(h) + 2))
+            assert all(np.array_equal(a, b) for a, b in zip(full, band)), mode
+
+
 def test_100kbp_fixture_is_a_valid_alignment():
     # Two 100 kbp pairs: reference scores, oracle counts (the reference's own
     # traceback truncates trace-matrix positions past 2^32 at this size).
