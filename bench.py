@@ -329,6 +329,19 @@ def banded_leg(ctx, steps, warmup, cpu):
                    "roofline_frac": round(wach / PEAK_TOPS_INT32, 4),
                    "workload": f"{nw:,} DNA pairs of {lw} bp, 8% substitutions, global, band [{wlo}, {wup}]",
                    "cells": wcells}
+    # Banded rows (sqmi_align_rows_banded, int32 traceback, one thread per
+    # pair): the first 65,536 pairs of the narrow workload.
+    nr = 1 << 16
+    Hr, Vr = head(H, nr), head(V, nr)
+    rcells = band_cells(L_BANDED, L_BANDED, lo, up) * nr
+    rout = [None]
+    rms = timed(lambda: rout.__setitem__(0, ctx.align_rows_banded(Hr, Vr, score, lo, up)), steps, warmup,
+                "banded rows")
+    res["rows"] = {"value": round(rcells / (rms / 1e3) / 1e9, 2), "unit": "GCUPS (cells inside the band)",
+                   "ms_per_step": round(rms, 3), "device_ms": round(ctx.stats()["device_ms"], 3),
+                   "scores_equal_banded_scores": bool(np.array_equal(rout[0][0], scores[0][:nr])),
+                   "workload": f"first {nr:,} pairs of the narrow workload: scores, counts and rows "
+                               "(host buffers, rows compacted on the host)", "cells": rcells}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as po

@@ -424,6 +424,45 @@ seqan2::String<int> align_local_rows(TGapsSetH & gapsH, TGapsSetV & gapsV, TScor
     return result;
 }
 
+template <typename TGapsSetH, typename TGapsSetV, typename TScore>
+seqan2::String<int> align_rows_banded(TGapsSetH & gapsH, TGapsSetV & gapsV, TScore const & scheme, uint32_t mode,
+                                      int64_t lower, int64_t upper)
+{
+    using TAlphabet = typename GapsSetAlphabet<TGapsSetH>::Type;
+    if (seqan2::length(gapsH) != seqan2::length(gapsV))
+        throw Error(SQMI_ERR_LENGTH_MISMATCH, "seqan_amd: gapsSetH and gapsSetV differ in length");
+    seqan2::String<int> result;
+    int64_t const count = static_cast<int64_t>(seqan2::length(gapsH));
+    if (count == 0)
+        return result;
+    bool const local = (mode & SQMI_LOCAL) != 0;
+    std::vector<uint8_t> resH, resV;
+    std::vector<int64_t> offH, offV;
+    packSources(gapsH, resH, offH);
+    packSources(gapsV, resV, offV);
+    std::vector<int32_t> table;
+    sqmi_scoring const sc = make_scoring<TAlphabet>(scheme, table);
+    sqmi_sequences const h{resH.data(), offH.data(), count};
+    sqmi_sequences const v{resV.data(), offV.data(), count};
+    std::vector<int64_t> opOff(count + 1, 0), opLen(count, 0), begins(2 * count, 0);
+    for (int64_t i = 0; i < count; ++i)
+        opOff[i + 1] = opOff[i] + (offH[i + 1] - offH[i]) + (offV[i + 1] - offV[i]);
+    std::vector<uint8_t> ops(static_cast<size_t>(opOff[count]) + 1);
+    std::vector<int32_t> counts(static_cast<size_t>(count) * (sc.alphabet_size * sc.alphabet_size + 2));
+    seqan2::resize(result, count);
+    sqmi_context * ctx = thread_context();
+    check(sqmi_align_rows_banded(ctx, &h, &v, &sc, mode, lower, upper, &result[0], counts.data(), ops.data(),
+                                 opOff.data(), opLen.data(), local ? begins.data() : nullptr, 0u),
+          ctx);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (int64_t i = 0; i < count; ++i)
+        write_rows(gapsH[i], gapsV[i], ops.data() + opOff[i + 1] - opLen[i], opLen[i],
+                   local ? begins.data() + 2 * i : nullptr);
+    return result;
+}
+
 }  // namespace detail
 }  // namespace seqan_amd
 
@@ -554,6 +593,49 @@ inline String<int>
 localAlignment(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
                TGapsSetH && gapsSetH, TGapsSetV && gapsSetV, TScore && scoringScheme)
 SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_local_rows(gapsSetH, gapsSetV, scoringScheme))
+
+// Banded rows: the executor forwards (gapsSetH, gapsSetV, score [, config],
+// lowerDiag, upperDiag) to global_alignment_banded.h:655-760 and
+// local_alignment_banded.h:261-280.  Bands outside the ones
+// sqmi_align_rows_banded takes raise seqan_amd::Error(SQMI_ERR_UNSUPPORTED).
+template <typename TGapsSetH, typename TGapsSetV, typename TScore, typename TLower, typename TUpper,
+          std::enable_if_t<seqan_amd::detail::IsGpuRowsCall<TGapsSetH, TGapsSetV, TScore>::value &&
+                           std::is_integral<std::decay_t<TLower>>::value &&
+                           std::is_integral<std::decay_t<TUpper>>::value, int> = 0>
+inline String<int>
+globalAlignment(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
+                TGapsSetH && gapsSetH, TGapsSetV && gapsSetV, TScore && scoringScheme, TLower && lowerDiag,
+                TUpper && upperDiag)
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_rows_banded(gapsSetH, gapsSetV, scoringScheme, 0u,
+                                                             static_cast<int64_t>(lowerDiag),
+                                                             static_cast<int64_t>(upperDiag)))
+
+template <typename TGapsSetH, typename TGapsSetV, typename TScore, typename TConfig, typename TLower,
+          typename TUpper,
+          std::enable_if_t<seqan_amd::detail::IsGpuRowsCall<TGapsSetH, TGapsSetV, TScore>::value &&
+                           seqan_amd::detail::IsAlignConfig<std::decay_t<TConfig>>::value &&
+                           std::is_integral<std::decay_t<TLower>>::value &&
+                           std::is_integral<std::decay_t<TUpper>>::value, int> = 0>
+inline String<int>
+globalAlignment(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
+                TGapsSetH && gapsSetH, TGapsSetV && gapsSetV, TScore && scoringScheme, TConfig && /*alignConfig*/,
+                TLower && lowerDiag, TUpper && upperDiag)
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_rows_banded(
+    gapsSetH, gapsSetV, scoringScheme, seqan_amd::detail::IsAlignConfig<std::decay_t<TConfig>>::mode,
+    static_cast<int64_t>(lowerDiag), static_cast<int64_t>(upperDiag)))
+
+template <typename TGapsSetH, typename TGapsSetV, typename TScore, typename TLower, typename TUpper,
+          std::enable_if_t<seqan_amd::detail::IsGpuRowsCall<TGapsSetH, TGapsSetV, TScore>::value &&
+                           std::is_integral<std::decay_t<TLower>>::value &&
+                           std::is_integral<std::decay_t<TUpper>>::value, int> = 0>
+inline String<int>
+localAlignment(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
+               TGapsSetH && gapsSetH, TGapsSetV && gapsSetV, TScore && scoringScheme, TLower && lowerDiag,
+               TUpper && upperDiag)
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_rows_banded(gapsSetH, gapsSetV, scoringScheme,
+                                                             static_cast<uint32_t>(SQMI_LOCAL),
+                                                             static_cast<int64_t>(lowerDiag),
+                                                             static_cast<int64_t>(upperDiag)))
 
 }  // namespace seqan2
 #undef SEQAN_AMD_OVERLOAD_BODY

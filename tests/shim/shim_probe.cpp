@@ -97,6 +97,10 @@ static_assert(gpuRows<ExPV const &, DnaGaps &, DnaGaps &, Simple &, Semi &>(), "
 // localAlignment on StringSets of Gaps (parallel_align_interface.h:312-321).
 static_assert(gpuLocalRows<ExPV const &, DnaGaps &, DnaGaps &, Simple &>(), "local rows, DNA");
 static_assert(gpuLocalRows<ExPV, AaGaps &, AaGaps &, Blosum62 const &>(), "local rows, protein");
+static_assert(gpuRows<ExPV const &, DnaGaps &, DnaGaps &, Simple &, int, int>(), "banded rows");
+static_assert(gpuRows<ExPV const &, DnaGaps &, DnaGaps &, Simple &, Semi &, int, int>(), "banded rows, AlignConfig");
+static_assert(gpuLocalRows<ExPV const &, AaGaps &, AaGaps &, Blosum62 &, int, int>(), "banded local rows");
+static_assert(!gpuRows<ExPS const &, DnaGaps &, DnaGaps &, Simple &, int, int>(), "banded rows, Parallel, Serial");
 static_assert(!gpuLocalRows<ExPS const &, AaGaps &, AaGaps &, Blosum62 &>(), "local rows, Parallel, Serial");
 
 // Banded scores (lowerDiag, upperDiag after the score or the AlignConfig).

@@ -186,6 +186,49 @@ void runLocalRowsCase(char const * name, StringSet<String<TAlphabet>> const & se
     }
 }
 
+// Banded globalAlignment / localAlignment on StringSets of Gaps against the
+// reference's banded function for one pair (global_alignment_banded.h:
+// 152-181, local_alignment_banded.h:242-259), pair by pair.
+template <typename TAlphabet, typename TScore, typename TGpu, typename... TConfig>
+void runBandedRowsCase(char const * name, bool local, StringSet<String<TAlphabet>> const & setH,
+                       StringSet<String<TAlphabet>> const & setV, TScore const & sc, TGpu const & ex, int lo, int up,
+                       TConfig const &... cfg)
+{
+    using TGaps = Gaps<String<TAlphabet> const>;
+    StringSet<TGaps> gH, gV, rH, rV;
+    String<int> expect;
+    for (size_t i = 0; i < length(setH); ++i)
+    {
+        appendValue(gH, TGaps(setH[i]));
+        appendValue(gV, TGaps(setV[i]));
+        appendValue(rH, TGaps(setH[i]));
+        appendValue(rV, TGaps(setV[i]));
+    }
+    for (size_t i = 0; i < length(rH); ++i)
+        appendValue(expect, local ? localAlignment(rH[i], rV[i], sc, lo, up)
+                                  : globalAlignment(rH[i], rV[i], sc, cfg..., lo, up));
+    auto same = [](TGaps & a, TGaps & b) {
+        return rowText(a) == rowText(b) && clippedBeginPosition(a) == clippedBeginPosition(b) &&
+               clippedEndPosition(a) == clippedEndPosition(b) && beginPosition(a) == beginPosition(b) &&
+               endPosition(a) == endPosition(b);
+    };
+    try
+    {
+        String<int> got = local ? localAlignment(ex, gH, gV, sc, lo, up) : globalAlignment(ex, gH, gV, sc, cfg..., lo, up);
+        for (size_t i = 0; i < length(expect); ++i)
+            if (got[i] != expect[i] || !same(gH[i], rH[i]) || !same(gV[i], rV[i]))
+            {
+                std::printf("case %s mismatch pair %zu: %d vs %d\n", name, i, got[i], expect[i]);
+                return;
+            }
+        std::printf("case %s match %zu\n", name, (size_t)length(got));
+    }
+    catch (seqan_amd::Error const & e)
+    {
+        std::printf("case %s %s (%s)\n", name, e.status() == SQMI_ERR_NO_DEVICE ? "nodevice" : "error", e.what());
+    }
+}
+
 }  // namespace
 
 int main()
@@ -285,6 +328,11 @@ int main()
     runRowsCase("dna_rows_linear_gaps_semiglobal", dh, dv, lin, ex, cpu, semi);
     runLocalRowsCase("dna_local_rows_linear_gaps", dh, dv, lin, ex, cpu);
     runLocalRowsCase("protein_local_rows_blosum62", ph, pv, b62, ex, cpu);
+    // Banded rows (globalAlignment / localAlignment with lowerDiag, upperDiag).
+    runBandedRowsCase("dna_banded_rows", false, eh, ev, dna, ex, -5, 5);
+    runBandedRowsCase("dna_banded_rows_semiglobal", false, eh, ev, dna, ex, -8, 12, semi);
+    runBandedRowsCase("dna_local_banded_rows", true, eh, ev, dna, ex, -6, 9);
+    runBandedRowsCase("protein_local_banded_rows_blosum62", true, eph, epv, b62, ex, -20, 30);
 
     // |setH| != |setV| raises instead of asserting.
     StringSet<String<Dna>> shorter = dv;

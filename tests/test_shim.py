@@ -26,7 +26,8 @@ CASES = ["dna_global_nonconst", "dna_semiglobal_const", "dna_overlap_rvalue", "d
          "dna_rows_overlap", "dna_local_rows", "protein_local_rows_blosum62",
          "dna_rows_linear_gaps", "dna_rows_linear_gaps_semiglobal", "dna_local_rows_linear_gaps",
          "dna_banded", "dna_banded_semiglobal", "protein_banded_blosum62", "protein_local_banded_blosum62",
-         "length_mismatch"]
+         "dna_banded_rows", "dna_banded_rows_semiglobal", "dna_local_banded_rows",
+         "protein_local_banded_rows_blosum62", "length_mismatch"]
 
 
 def _run_cases():
