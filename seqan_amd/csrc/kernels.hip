@@ -386,24 +386,38 @@ struct Fwd16 {
                         sm = vmax(sm, Gk[k]);
                     best = vmax(best, sm);
                     if (gtMask(sm + splat(1), et.bV) != 0) {
-                        uint32_t earlier = (j < et.bJ[0] ? 0xffffu : 0u) | (j < et.bJ[1] ? 0xffff0000u : 0u);
-                        s16x2 Fr = vmax(upF + e2, upG);
+                        // The column is taken (per half) when its maximum beats
+                        // the best so far, or equals it in an earlier column;
+                        // then its first row holding the maximum is the cell.
+                        const uint32_t earlier = (j < et.bJ[0] ? 0xffffu : 0u) | (j < et.bJ[1] ? 0xffff0000u : 0u);
+                        const uint32_t take = gtMask(sm, et.bV) | (eqMask(sm, et.bV) & earlier);
+                        if (take != 0) {
+                            uint32_t open = 0xffffffffu;   // halves whose first maximal row is still ahead
+                            uint32_t rowK = 0;
+                            s16x2 selE = neg2, selF = neg2;
+                            s16x2 Fr = vmax(upF + e2, upG);
 #pragma unroll
-                        for (int k = 0; k < R; ++k) {
-                            if (k > 0)
-                                Fr = vmax(Fr + e2, Gk[k - 1]);
-                            const uint32_t take = gtMask(Gk[k], et.bV) | (eqMask(Gk[k], et.bV) & earlier);
-                            earlier &= ~take;   // bJ is now j: later rows need a larger value
-                            et.bV = as16(bfi(take, asu(Gk[k]), asu(et.bV)));
-                            et.bE = as16(bfi(take, asu(Ek[k]), asu(et.bE)));
-                            et.bF = as16(bfi(take, asu(Fr), asu(et.bF)));
+                            for (int k = 0; k < R; ++k) {
+                                if (k > 0)
+                                    Fr = vmax(Fr + e2, Gk[k - 1]);
+                                // 0xffff in halves where Gk[k] < sm.
+                                const uint32_t below = asu(subSat(Gk[k], sm) >> splat(15));
+                                const uint32_t first = ~below & open;
+                                open &= below;
+                                rowK = bfi(first, (uint32_t)k * 0x10001u, rowK);
+                                selE = as16(bfi(first, asu(Ek[k]), asu(selE)));
+                                selF = as16(bfi(first, asu(Fr), asu(selF)));
+                            }
+                            et.bV = as16(bfi(take, asu(sm), asu(et.bV)));
+                            et.bE = as16(bfi(take, asu(selE), asu(et.bE)));
+                            et.bF = as16(bfi(take, asu(selF), asu(et.bF)));
                             if (take & 0xffffu) {
                                 et.bJ[0] = j;
-                                et.bRow[0] = rowbase + k;
+                                et.bRow[0] = rowbase + (int)(rowK & 0xffffu);
                             }
                             if (take >> 16) {
                                 et.bJ[1] = j;
-                                et.bRow[1] = rowbase + k;
+                                et.bRow[1] = rowbase + (int)(rowK >> 16);
                             }
                         }
                     }
