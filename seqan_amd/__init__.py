@@ -565,16 +565,28 @@ def local_alignment_score(set_h, set_v, score: Score, alphabet=AMINO_ACIDS, ctx:
 
 
 def global_alignment(set_h, set_v, score: Score, align_config=(False, False, False, False), alphabet=DNA,
-                     ctx: Context = None):
-    """globalAlignment(ExecutionPolicy<Parallel,Vectorial>, gapsSetH, gapsSetV, score, AlignConfig):
-    (scores, counts, op_off, ops), rows as Context.align_rows."""
+                     ctx: Context = None, lower_diag=None, upper_diag=None):
+    """globalAlignment(ExecutionPolicy<Parallel,Vectorial>, gapsSetH, gapsSetV, score, AlignConfig
+    [, lowerDiag, upperDiag]): (scores, counts, op_off, ops), rows as Context.align_rows; with a band,
+    as Context.align_rows_banded."""
     ctx = ctx or default_context()
-    return ctx.align_rows(_as_set(set_h, alphabet), _as_set(set_v, alphabet), score,
-                          _mode_from_config(align_config))
+    H, V = _as_set(set_h, alphabet), _as_set(set_v, alphabet)
+    if (lower_diag is None) != (upper_diag is None):
+        raise ValueError("give both lower_diag and upper_diag, or neither")
+    if lower_diag is not None:
+        return ctx.align_rows_banded(H, V, score, lower_diag, upper_diag, mode=_mode_from_config(align_config))
+    return ctx.align_rows(H, V, score, _mode_from_config(align_config))
 
 
-def local_alignment(set_h, set_v, score: Score, alphabet=AMINO_ACIDS, ctx: Context = None):
-    """localAlignment(ExecutionPolicy<Parallel,Vectorial>, gapsSetH, gapsSetV, score):
-    (scores, counts, op_off, ops, begins), as Context.align_local_rows."""
+def local_alignment(set_h, set_v, score: Score, alphabet=AMINO_ACIDS, ctx: Context = None, lower_diag=None,
+                    upper_diag=None):
+    """localAlignment(ExecutionPolicy<Parallel,Vectorial>, gapsSetH, gapsSetV, score
+    [, lowerDiag, upperDiag]): (scores, counts, op_off, ops, begins), as Context.align_local_rows;
+    with a band, as Context.align_rows_banded with mode LOCAL."""
     ctx = ctx or default_context()
-    return ctx.align_local_rows(_as_set(set_h, alphabet), _as_set(set_v, alphabet), score)
+    H, V = _as_set(set_h, alphabet), _as_set(set_v, alphabet)
+    if (lower_diag is None) != (upper_diag is None):
+        raise ValueError("give both lower_diag and upper_diag, or neither")
+    if lower_diag is not None:
+        return ctx.align_rows_banded(H, V, score, lower_diag, upper_diag, mode=LOCAL)
+    return ctx.align_local_rows(H, V, score)

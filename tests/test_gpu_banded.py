@@ -173,6 +173,14 @@ def test_python_mirror_functions(ctx):
     l = sa.local_alignment(hs, vs, score, alphabet=sa.DNA, ctx=ctx)
     lr = ctx.align_local_rows(Hs, Vs, score)
     assert all(np.array_equal(x, y) for x, y in zip(l, lr))
+    gb = sa.global_alignment(hs, vs, score, ctx=ctx, lower_diag=-3, upper_diag=3)
+    rb = ctx.align_rows_banded(Hs, Vs, score, -3, 3)
+    assert all(np.array_equal(x, y) for x, y in zip(gb, rb))
+    lb = sa.local_alignment(hs, vs, score, alphabet=sa.DNA, ctx=ctx, lower_diag=-3, upper_diag=3)
+    lrb = ctx.align_rows_banded(Hs, Vs, score, -3, 3, mode=sa.LOCAL)
+    assert all(np.array_equal(x, y) for x, y in zip(lb, lrb))
+    with pytest.raises(ValueError):
+        sa.global_alignment(hs, vs, score, ctx=ctx, lower_diag=-3)
 
 
 # ---------------------------------------------------------------------------
