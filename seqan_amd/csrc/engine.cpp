@@ -1134,8 +1134,9 @@ sqmi_status sqmi_plan_stats(const sqmi_sequences *setH, const sqmi_sequences *se
 // Traceback: the packed forward pass writes 4 trace bits per cell into a
 // buffer of at most traceBudget() bytes, so the work items run in chunks;
 // after each chunk one thread per pair walks its path.  Pairs outside the
-// int16 window, and every pair of a free-end or local alignment, run the
-// int32 kernel with one trace byte per cell, chunked by the same budget.
+// int16 window, every pair under SQMI_FORCE_INT32 and every banded pair
+// (band given) run the int32 kernel with one trace byte per cell, chunked by
+// the same budget.
 static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
                                  const sqmi_scoring *scoring, uint32_t mode, int32_t *scores, int32_t *counts,
                                  uint8_t *ops, const int64_t *opOffsets, int64_t *opLengths, int64_t *begins,
