@@ -348,3 +348,33 @@ def test_rows_free_ends_packed_match_oracle(ctx, scoring):
         bad = np.nonzero((c != wc).any(axis=1))[0]
         assert len(bad) == 0, (mode, bad[:5])
         assert np.array_equal(op_off, woff) and np.array_equal(ops, wops)
+
+
+@pytest.mark.parametrize("scoring", [(1, -1, -1, -2), (1, -1, -1, -1), (2, -1, -2, -2), (3, -2, -3, -2)],
+                         ids=["ties", "linear", "cheap-gaps", "extend-below-open"])
+def test_local_rows_packed_ties_multi_block(ctx, scoring):
+    # Local windows on the packed trace kernel with many equal maxima: short
+    # repeats of a small alphabet, V up to 1,400 residues (several 256-row
+    # blocks, so equal maxima in one column come from different blocks and
+    # lanes), compared with the oracle; the packed route is asserted.
+    rng = np.random.default_rng(sum(scoring) + 50)
+    score = sa.simple_score(*scoring)
+    hs, vs = [], []
+    for _ in range(160):
+        unit = rng.integers(0, 4, int(rng.integers(1, 5))).astype(np.uint8)
+        n, m = int(rng.integers(1, 500)), int(rng.integers(1, 1400))
+        h = np.resize(unit, n).copy()
+        v = np.resize(unit, m).copy()
+        for s in (h, v):
+            f = rng.random(len(s)) < 0.05
+            s[f] = rng.integers(0, 4, int(f.sum()))
+        hs.append(h)
+        vs.append(v)
+    Hs, Vs = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+    print("packed local ties, scoring", scoring, flush=True)
+    s, c, off, ops, begins = ctx.align_local_rows(Hs, Vs, score)
+    assert ctx.stats()["pairs_int16"] == len(hs)
+    ws, wc, woff, wops, wbeg = _oracle_local(Hs, Vs, score)
+    assert np.array_equal(s, ws) and np.array_equal(c, wc)
+    assert np.array_equal(off, woff) and np.array_equal(ops, wops)
+    assert np.array_equal(begins, wbeg)
