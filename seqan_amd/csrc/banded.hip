@@ -236,6 +236,151 @@ __global__ __launch_bounds__(256) void kBandedNarrow(BandedArgs a)
         a.scores[P.pid] = best;
 }
 
+// kBandedNarrow plus one trace byte per cell, in the encoding of the int32
+// traceback (bit 0 E opened, 1 F opened, 2 not diagonal, 3 E beats F, 16
+// local NONE), and the end cell: every lane keeps the first strict maximum
+// of its tracked cells (a lane's cells come in column order), then the
+// segment takes the largest value, the smallest column, the smallest row.
+// E and F of that cell go with it (_correctTraceValue).
+template <int S>
+__global__ __launch_bounds__(256) void kBandedTrace(BandedArgs a)
+{
+    constexpr int kPairsPerWave = 64 / S;
+    constexpr int kHSpan = kChunkRows + 64;
+    constexpr uint32_t kEOpen = 1, kFOpen = 2, kNotDiag = 4, kHori = 8, kStop = 16;
+    __shared__ int32_t sTab[32 * 32];
+    __shared__ uint8_t sV[4][kPairsPerWave][kChunkRows];
+    __shared__ uint8_t sH[4][kPairsPerWave][kHSpan];
+    for (int x = threadIdx.x; x < a.alpha * a.alpha; x += blockDim.x)
+        sTab[x] = a.table[x];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int k = lane % S, seg = lane / S;
+    const int t = wave * kPairsPerWave + seg;
+    const bool has = t < a.nIds;
+    if (wave * kPairsPerWave >= a.nIds)
+        return;
+    BandPair P{};
+    if (has)
+        P = loadPair(a, a.ids[t]);
+    const int32_t o = a.gapOpen, e = a.gapExtend, g = max(o, e);
+    const int A = a.alpha;
+    const bool local = (a.mode & SQMI_LOCAL) != 0;
+    const bool freeTop = (a.mode & (SQMI_FREE_TOP | SQMI_LOCAL)) != 0;
+    const bool freeLeft = (a.mode & (SQMI_FREE_LEFT | SQMI_LOCAL)) != 0;
+    const int32_t floorH = local ? 0 : kNegB;
+    const bool inBand = has && k < P.w;
+    const int d = P.lo + k;
+    uint8_t *rowT = has ? a.trace + a.traceOff[t] : nullptr;
+    int rows = has ? P.m : 0;
+#pragma unroll
+    for (int x = S; x < 64; x <<= 1)
+        rows = max(rows, __shfl_xor(rows, x));
+    // End cell candidate of this lane.
+    int32_t bv = kNegB, be = kNegB, bf = kNegB;
+    int bi = 0x7fffffff, bj = 0x7fffffff;
+    int32_t hPrev = kNegB, fPrev = kNegB;
+    if (inBand && d >= 0 && d <= P.n) {
+        hPrev = rowZero(d, freeTop, o, g);
+        if (P.track(0, d, a.mode)) {
+            bv = hPrev;
+            be = (d == 0 || freeTop) ? kNegB : hPrev;
+            bi = 0;
+            bj = d;
+        }
+    }
+    const int wv = threadIdx.x >> 6;
+    uint8_t *myV = sV[wv][seg];
+    uint8_t *myH = sH[wv][seg];
+    const bool upOut = k + 1 >= P.w;
+    for (int i = 1; i <= rows; ++i) {
+        const int r = (i - 1) % kChunkRows;
+        if (r == 0) {
+            __builtin_amdgcn_wave_barrier();
+            for (int x = k; x < kChunkRows; x += S) {
+                const int row = i + x;
+                myV[x] = (has && row <= P.m) ? P.v[row - 1] : 0;
+            }
+            for (int c = k; c < kChunkRows + S - 1; c += S) {
+                const int col = i + P.lo + c;
+                myH[c] = (has && col >= 1 && col <= P.n) ? P.h[col - 1] : 0;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        const int j = i + d;
+        const bool valid = inBand && i <= P.m && j >= 0 && j <= P.n;
+        const int hb = myH[r + k], vb = myV[r];
+        const int32_t sub = sTab[min(hb, A - 1) * A + min(vb, A - 1)];
+        const int32_t hUpAll = dpp<kDppWaveShl1>(hPrev), fUpAll = dpp<kDppWaveShl1>(fPrev);
+        const int32_t hUp = upOut ? kNegB : hUpAll;
+        const int32_t fUp = upOut ? kNegB : fUpAll;
+        const int32_t F0 = max(fUp + e, hUp + o);
+        const bool col0 = j == 0;
+        const int32_t D = hPrev + sub;
+        const int32_t X0 = col0 ? (freeLeft ? 0 : F0) : max(max(D, F0), floorH);
+        const int32_t X = valid ? X0 : kNegB;
+        const int32_t F = (valid && !(col0 && freeLeft)) ? F0 : kNegB;
+        const int32_t incl = prefixMaxDpp<S>(valid ? max(X - kNegB + o - k * g, 0) : 0, k);
+        const int32_t inclLeft = dpp0<kDppWaveShr1>(incl);
+        const int32_t E = k == 0 ? kNegB : inclLeft + kNegB + (k - 1) * g;
+        const int32_t H = valid ? (col0 ? X : max(X, E)) : kNegB;
+        // The left neighbour's H and E (lane k - 1, this row); every lane
+        // runs the moves, the segment edge is masked afterwards.
+        const int32_t hLeftAll = dpp<kDppWaveShr1>(H), eLeftAll = dpp<kDppWaveShr1>(k == 0 ? kNegB : E);
+        const int32_t hLeft = k == 0 ? kNegB : hLeftAll, eLeft = k == 0 ? kNegB : eLeftAll;
+        if (valid && !col0) {
+            const int32_t extE = eLeft + e, opnE = hLeft + o;
+            uint32_t tb = extE < opnE ? kEOpen : 0u;
+            const int32_t ei = max(extE, opnE);
+            tb |= (fUp + e) < (hUp + o) ? kFOpen : 0u;
+            const int32_t fe = F0 < ei ? ei : F0;
+            int32_t x = D;
+            if (D < fe) {
+                x = fe;
+                tb |= kNotDiag | (F0 < ei ? kHori : 0u);
+            }
+            if (local && !(0 < x))
+                tb = kStop;
+            rowT[(int64_t)(i - 1) * S + k] = (uint8_t)tb;
+        }
+        if (valid && P.track(i, j, a.mode) && H > bv) {
+            bv = H;
+            be = col0 ? kNegB : E;
+            bf = F;
+            bi = i;
+            bj = j;
+        }
+        hPrev = H;
+        fPrev = F;
+    }
+    // Segment reduction: larger value, then smaller column, then smaller row.
+#pragma unroll
+    for (int x = S / 2; x >= 1; x >>= 1) {
+        const int32_t ov = __shfl_xor(bv, x, S), oe = __shfl_xor(be, x, S), of = __shfl_xor(bf, x, S);
+        const int oi = __shfl_xor(bi, x, S), oj = __shfl_xor(bj, x, S);
+        const bool take = ov > bv || (ov == bv && (oj < bj || (oj == bj && oi < bi)));
+        bv = take ? ov : bv;
+        be = take ? oe : be;
+        bf = take ? of : bf;
+        bi = take ? oi : bi;
+        bj = take ? oj : bj;
+    }
+    if (has && k == 0) {
+        // A band whose lower diagonal ends on the last row: the reference
+        // tracks that cell from the next column's code path
+        // (dp_algorithm_impl.h:843-862) and its traceback takes the cell's
+        // own direction there, with no preference for the vertical gap.
+        if (bi == P.m && (int64_t)bj - bi == a.lower)
+            bf = kNegB;
+        a.scores[P.pid] = bv;
+        a.endIJ[2 * P.pid] = bi;
+        a.endIJ[2 * P.pid + 1] = bj;
+        a.endEF[2 * P.pid] = be;
+        a.endEF[2 * P.pid + 1] = bf;
+    }
+}
+
 // One wave per pair; rows of the band (H and F, double-buffered) in the
 // wave's slice of `scratch`: 4 * (wmax + 2) int32.  blockDim = 64, so the
 // workgroup barrier between rows orders this wave's own global writes and
@@ -336,6 +481,26 @@ hipError_t launchBanded(const BandedArgs &args, int segLanes, int wideGrid, hipS
         case 16: hipLaunchKernelGGL(kBandedNarrow<16>, grid, block, 0, stream, args); break;
         case 32: hipLaunchKernelGGL(kBandedNarrow<32>, grid, block, 0, stream, args); break;
         case 64: hipLaunchKernelGGL(kBandedNarrow<64>, grid, block, 0, stream, args); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchBandedTrace(const BandedArgs &args, int segLanes, hipStream_t stream)
+{
+    if (args.nIds <= 0)
+        return hipSuccess;
+    const int pairsPerWave = 64 / segLanes;
+    const int waves = (args.nIds + pairsPerWave - 1) / pairsPerWave;
+    const dim3 grid((waves + 3) / 4), block(256);
+    switch (segLanes) {
+        case 1: hipLaunchKernelGGL(kBandedTrace<1>, grid, block, 0, stream, args); break;
+        case 2: hipLaunchKernelGGL(kBandedTrace<2>, grid, block, 0, stream, args); break;
+        case 4: hipLaunchKernelGGL(kBandedTrace<4>, grid, block, 0, stream, args); break;
+        case 8: hipLaunchKernelGGL(kBandedTrace<8>, grid, block, 0, stream, args); break;
+        case 16: hipLaunchKernelGGL(kBandedTrace<16>, grid, block, 0, stream, args); break;
+        case 32: hipLaunchKernelGGL(kBandedTrace<32>, grid, block, 0, stream, args); break;
+        case 64: hipLaunchKernelGGL(kBandedTrace<64>, grid, block, 0, stream, args); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -232,6 +232,12 @@ bool longAll()
     return env && *env == '1';
 }
 
+bool envFlag(const char *name)
+{
+    const char *env = std::getenv(name);
+    return env && *env == '1';
+}
+
 int64_t longMinSide()
 {
     const char *env = std::getenv("SQMI_LONG_MIN_SIDE");
@@ -1376,7 +1382,89 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             b = e;
         }
     }
-    if (!P.ids32.empty()) {
+    // Banded pairs whose band, clipped to the matrix, has at most 64
+    // diagonals: the lane-per-diagonal trace kernel (S trace bytes per row,
+    // S = the widest band rounded up to a power of two) and a walk.
+    // SQMI_BANDED_TRACE_INT32=1 keeps them on the int32 kernel.
+    int bandLanes = 0;
+    std::vector<int64_t> bandOff;
+    int64_t bandBytes = 0;
+    if (band && !P.ids32.empty() && !envFlag("SQMI_BANDED_TRACE_INT32")) {
+        int64_t wmax = 1;
+        for (int32_t id : P.ids32) {
+            const int64_t n = setH->offsets[id + 1] - setH->offsets[id], m = setV->offsets[id + 1] - setV->offsets[id];
+            wmax = std::max(wmax, std::min(band[1], n) - std::max(band[0], -m) + 1);
+        }
+        if (wmax <= 64) {
+            bandLanes = 1;
+            while (bandLanes < wmax)
+                bandLanes <<= 1;
+            bandOff.resize(P.ids32.size());
+            for (size_t k = 0; k < P.ids32.size(); ++k) {
+                const int32_t id = P.ids32[k];
+                bandOff[k] = bandBytes;
+                bandBytes += (setV->offsets[id + 1] - setV->offsets[id]) * bandLanes;
+            }
+            if (bandBytes > budget)
+                bandLanes = 0;
+        }
+    }
+    if (bandLanes > 0) {
+        const size_t n32 = P.ids32.size();
+        if ((err = ensure(ctx->ids32, 4 * n32)) != hipSuccess || (err = ensure(ctx->scratchOff32, 8 * n32)) != hipSuccess ||
+            (err = ensure(ctx->scratch32, (size_t)std::max<int64_t>(bandBytes, 1))) != hipSuccess ||
+            (err = ensure(ctx->endEF, (size_t)8 * N)) != hipSuccess || (err = ensure(ctx->endIJ, (size_t)8 * N)) != hipSuccess)
+            return hipFail(ctx, err, "allocating banded traceback buffers");
+        if ((err = hipMemcpyAsync(ctx->ids32.p, P.ids32.data(), 4 * n32, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+            (err = hipMemcpyAsync(ctx->scratchOff32.p, bandOff.data(), 8 * n32, hipMemcpyHostToDevice, stream)) !=
+                hipSuccess)
+            return hipFail(ctx, err, "uploading banded traceback lists");
+        BandedArgs ba{};
+        ba.hRes = in.dH;
+        ba.vRes = in.dV;
+        ba.hOff = static_cast<const int64_t *>(ctx->hOff.p);
+        ba.vOff = static_cast<const int64_t *>(ctx->vOff.p);
+        ba.ids = static_cast<const int32_t *>(ctx->ids32.p);
+        ba.nIds = (int32_t)n32;
+        ba.scores = in.dScores;
+        ba.gapOpen = si.o;
+        ba.gapExtend = si.e;
+        ba.mode = mode;
+        ba.alpha = A;
+        ba.table = static_cast<const int32_t *>(ctx->table32.p);
+        ba.lower = band[0];
+        ba.upper = band[1];
+        ba.trace = static_cast<uint8_t *>(ctx->scratch32.p);
+        ba.traceOff = static_cast<const int64_t *>(ctx->scratchOff32.p);
+        ba.endIJ = static_cast<int32_t *>(ctx->endIJ.p);
+        ba.endEF = static_cast<int32_t *>(ctx->endEF.p);
+        if ((err = launchBandedTrace(ba, bandLanes, stream)) != hipSuccess)
+            return hipFail(ctx, err, "launching banded trace kernel");
+        BandedWalkArgs wa{};
+        wa.hRes = in.dH;
+        wa.vRes = in.dV;
+        wa.hOff = ba.hOff;
+        wa.vOff = ba.vOff;
+        wa.ids = ba.ids;
+        wa.nIds = ba.nIds;
+        wa.segLanes = bandLanes;
+        wa.lower = band[0];
+        wa.trace = ba.trace;
+        wa.traceOff = ba.traceOff;
+        wa.scores = in.dScores;
+        wa.endIJ = ba.endIJ;
+        wa.endEF = ba.endEF;
+        wa.counts = dCounts;
+        wa.alpha = A;
+        wa.local = (mode & SQMI_LOCAL) ? 1 : 0;
+        wa.linear = si.o == si.e;
+        wa.ops = rows.ops;
+        wa.opOff = rows.opOff;
+        wa.opLen = rows.opLen;
+        wa.begins = rows.begins;
+        if ((err = launchBandedWalk(wa, stream)) != hipSuccess)
+            return hipFail(ctx, err, "launching banded walk kernel");
+    } else if (!P.ids32.empty()) {
         const size_t n32 = P.ids32.size();
         if ((err = ensure(ctx->ids32, 4 * n32)) != hipSuccess || (err = ensure(ctx->scratchOff32, 8 * n32)) != hipSuccess ||
             (err = ensure(ctx->scratch32, (size_t)scratch32)) != hipSuccess)

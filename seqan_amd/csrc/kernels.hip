@@ -1091,6 +1091,39 @@ struct Bits16 {
     }
 };
 
+// Trace bytes of a banded pair (kBandedTrace): row i - 1, lane j - i - lo.
+struct BandBits {
+    const uint8_t *trace;
+    int S;
+    int lo;
+    __device__ __forceinline__ uint32_t operator()(int i, int j) const
+    {
+        // An optimal path stays inside the band (cells outside are -inf);
+        // the guard keeps any read inside the pair's rows regardless.
+        const int c = j - i - lo;
+        if (c < 0 || c >= S)
+            return kBitStop;
+        return trace[(int64_t)(i - 1) * S + c];
+    }
+};
+
+__global__ __launch_bounds__(64) void kBandedWalk(BandedWalkArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nIds)
+        return;
+    const int pid = a.ids[t];
+    const int A = a.alpha, S = A * A + 2;
+    const int64_t ho = a.hOff[pid], vo = a.vOff[pid];
+    const int n = (int)(a.hOff[pid + 1] - ho), m = (int)(a.vOff[pid + 1] - vo);
+    int32_t *cnt = a.counts + (int64_t)pid * S;
+    const int lo = (int)(a.lower > -(int64_t)m ? a.lower : -(int64_t)m);   // the band clipped to the matrix
+    BandBits bits{a.trace + a.traceOff[t], a.segLanes, lo};
+    walkCounts(n, m, a.hRes + ho, a.vRes + vo, A, a.scores[pid], a.endEF[2 * pid], a.endEF[2 * pid + 1], bits, cnt,
+               a.ops ? a.ops + a.opOff[pid + 1] : nullptr, a.ops ? a.opLen + pid : nullptr, a.endIJ[2 * pid],
+               a.endIJ[2 * pid + 1], a.local != 0, a.begins ? a.begins + 2 * pid : nullptr, a.linear != 0);
+}
+
 constexpr int kWalkThreads = 64;
 // Counts live in LDS while walking when alpha^2 + 2 <= kWalkLdsStride.
 constexpr int kWalkLdsStride = 51;
@@ -1396,6 +1429,14 @@ hipError_t launchWalk16(const WalkArgs &args, hipStream_t stream)
         hipLaunchKernelGGL(kWalk16<true>, grid, block, (size_t)kWalkThreads * S * sizeof(int32_t), stream, args);
     else
         hipLaunchKernelGGL(kWalk16<false>, grid, block, 0, stream, args);
+    return hipGetLastError();
+}
+
+hipError_t launchBandedWalk(const BandedWalkArgs &args, hipStream_t stream)
+{
+    if (args.nIds <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kBandedWalk, dim3((args.nIds + 63) / 64), dim3(64), 0, stream, args);
     return hipGetLastError();
 }
 

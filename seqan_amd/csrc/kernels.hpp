@@ -176,10 +176,44 @@ struct BandedArgs {
     int64_t lower, upper;       // the band: lower <= j - i <= upper
     int32_t *scratch;           // wide kernel: 4 * wideStride int32 per workgroup
     int32_t wideStride;
+    // Banded traceback (kBandedTrace): per ids index k, rows 1..m of S trace
+    // bytes at trace + traceOff[k] (byte of cell (i, j): row i - 1, lane
+    // j - i - clipped lower); per pair id the end cell and its E and F.
+    uint8_t *trace;
+    const int64_t *traceOff;
+    int32_t *endIJ;
+    int32_t *endEF;
 };
 // segLanes: 1..64 (power of two >= every band width: kBandedNarrow), or 0
 // (kBandedWide on wideGrid workgroups).
 hipError_t launchBanded(const BandedArgs &args, int segLanes, int wideGrid, hipStream_t stream);
+// Forward with trace bytes for bands of at most 64 diagonals (segLanes as in
+// launchBanded), then one thread per pair walks (kBandedWalk).
+hipError_t launchBandedTrace(const BandedArgs &args, int segLanes, hipStream_t stream);
+struct BandedWalkArgs {
+    const uint8_t *hRes;
+    const uint8_t *vRes;
+    const int64_t *hOff;
+    const int64_t *vOff;
+    const int32_t *ids;
+    int32_t nIds;
+    int32_t segLanes;
+    int64_t lower;
+    const uint8_t *trace;
+    const int64_t *traceOff;
+    const int32_t *scores;
+    const int32_t *endIJ;
+    const int32_t *endEF;
+    int32_t *counts;
+    int32_t alpha;
+    int32_t local;
+    int32_t linear;
+    uint8_t *ops;
+    const int64_t *opOff;
+    int64_t *opLen;
+    int64_t *begins;
+};
+hipError_t launchBandedWalk(const BandedWalkArgs &args, hipStream_t stream);
 // scores[ids[k]] = value for k < n.
 hipError_t launchFillScores(int32_t *scores, const int32_t *ids, int n, int32_t value, hipStream_t stream);
 

@@ -193,8 +193,19 @@ BANDED_ROWS = ["ref_dna_banded_rows.npz", "ref_dna_banded_ties_rows.npz", "ref_d
                "ref_protein_banded_rows.npz"]
 
 
+@pytest.fixture(params=["lanes", "int32"])
+def band_route(request, monkeypatch):
+    # Bands of at most 64 diagonals run on kBandedTrace + kBandedWalk;
+    # SQMI_BANDED_TRACE_INT32=1 keeps them on the int32 traceback kernel.
+    if request.param == "int32":
+        monkeypatch.setenv("SQMI_BANDED_TRACE_INT32", "1")
+    else:
+        monkeypatch.delenv("SQMI_BANDED_TRACE_INT32", raising=False)
+    return request.param
+
+
 @pytest.mark.parametrize("fixture", BANDED_ROWS)
-def test_banded_rows_match_reference_fixtures(ctx, fixture):
+def test_banded_rows_match_reference_fixtures(ctx, fixture, band_route):
     z = np.load(os.path.join(GOLDEN, fixture), allow_pickle=False)
     score = _score_of(z, fixture)
     Hs, Vs = sa.SeqSet(z["h_res"], z["h_off"]), sa.SeqSet(z["v_res"], z["v_off"])
@@ -214,7 +225,7 @@ def test_banded_rows_match_reference_fixtures(ctx, fixture):
         assert np.array_equal(ctx.align_score_banded(Hs, Vs, score, lo, up, mode=mode), s)
 
 
-def test_banded_rows_end_cell_on_lower_diagonal(ctx):
+def test_banded_rows_end_cell_on_lower_diagonal(ctx, band_route):
     # End cells on the band's lower diagonal in the last row, where the
     # reference takes the cell's own direction (one pair per call: each has
     # its own scoring).
