@@ -25,6 +25,9 @@
 //                  residue codes of the next row are loaded one row ahead.
 //   kBandedWide:   wider bands; one wave per pair, the band in chunks of 64
 //                  lanes, the previous row in a per-wave global buffer.
+//   kBandedTrace:  kBandedNarrow with one trace byte per cell and the end
+//                  cell's position, E and F, for banded rows and counts
+//                  (walked by kBandedWalk in kernels.hip).
 #include "kernels.hpp"
 #include "seqan_amd.h"
 
