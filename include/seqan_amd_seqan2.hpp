@@ -20,8 +20,10 @@
  *   - score is Score<int, Simple> or Score<int, ScoreMatrix<A, Spec>> with A
  *     the sets' alphabet (Blosum62 and the other built-in matrices included).
  * Every other combination (another policy, another alphabet, a non-int score,
- * a banded call, an algorithm tag) resolves to the reference templates as
- * before.  The results are the reference's, value for value.
+ * an algorithm tag) resolves to the reference templates as before, and a
+ * banded call with a band the library does not take (sqmi_align_score_banded
+ * lists the ones it takes) runs the reference's executor from inside the
+ * overload.  The results are the reference's, value for value.
  *
  * The reference entry points these overloads take precedence over:
  *   globalAlignmentScore(ExecutionPolicy<TPar,TVec> const &, TArgs && ...)
@@ -321,9 +323,15 @@ seqan2::String<int> align_scores(TSetH const & setH, TSetV const & setV, TScore 
     return result;
 }
 
-template <typename TSetH, typename TSetV, typename TScore>
+// Banded calls: the library takes the bands sqmi_align_score_banded
+// documents and answers SQMI_ERR_UNSUPPORTED, before touching any output,
+// for every other one.  Those calls go to `fallback`, which runs the
+// reference's own executor (parallel_align_interface.h:100-145) on the same
+// arguments, so a program that works with SeqAn keeps working, value for
+// value.
+template <typename TSetH, typename TSetV, typename TScore, typename TFallback>
 seqan2::String<int> align_scores_banded(TSetH const & setH, TSetV const & setV, TScore const & scheme, uint32_t mode,
-                                        int64_t lowerDiag, int64_t upperDiag)
+                                        int64_t lowerDiag, int64_t upperDiag, TFallback && fallback)
 {
     using TAlphabet = typename SetAlphabet<TSetH>::Type;
     if (seqan2::length(setH) != seqan2::length(setV))
@@ -341,7 +349,10 @@ seqan2::String<int> align_scores_banded(TSetH const & setH, TSetV const & setV, 
     sqmi_sequences const v{resV.data(), offV.data(), static_cast<int64_t>(offV.size()) - 1};
     seqan2::resize(result, seqan2::length(setH));
     sqmi_context * ctx = thread_context();
-    check(sqmi_align_score_banded(ctx, &h, &v, &sc, mode, lowerDiag, upperDiag, &result[0], 0u), ctx);
+    sqmi_status const st = sqmi_align_score_banded(ctx, &h, &v, &sc, mode, lowerDiag, upperDiag, &result[0], 0u);
+    if (st == SQMI_ERR_UNSUPPORTED)
+        return fallback();
+    check(st, ctx);
     return result;
 }
 
@@ -424,9 +435,9 @@ seqan2::String<int> align_local_rows(TGapsSetH & gapsH, TGapsSetV & gapsV, TScor
     return result;
 }
 
-template <typename TGapsSetH, typename TGapsSetV, typename TScore>
+template <typename TGapsSetH, typename TGapsSetV, typename TScore, typename TFallback>
 seqan2::String<int> align_rows_banded(TGapsSetH & gapsH, TGapsSetV & gapsV, TScore const & scheme, uint32_t mode,
-                                      int64_t lower, int64_t upper)
+                                      int64_t lower, int64_t upper, TFallback && fallback)
 {
     using TAlphabet = typename GapsSetAlphabet<TGapsSetH>::Type;
     if (seqan2::length(gapsH) != seqan2::length(gapsV))
@@ -451,9 +462,12 @@ seqan2::String<int> align_rows_banded(TGapsSetH & gapsH, TGapsSetV & gapsV, TSco
     std::vector<int32_t> counts(static_cast<size_t>(count) * (sc.alphabet_size * sc.alphabet_size + 2));
     seqan2::resize(result, count);
     sqmi_context * ctx = thread_context();
-    check(sqmi_align_rows_banded(ctx, &h, &v, &sc, mode, lower, upper, &result[0], counts.data(), ops.data(),
-                                 opOff.data(), opLen.data(), local ? begins.data() : nullptr, 0u),
-          ctx);
+    sqmi_status const st = sqmi_align_rows_banded(ctx, &h, &v, &sc, mode, lower, upper, &result[0], counts.data(),
+                                                  ops.data(), opOff.data(), opLen.data(),
+                                                  local ? begins.data() : nullptr, 0u);
+    if (st == SQMI_ERR_UNSUPPORTED)
+        return fallback();   // the Gaps objects are untouched until here
+    check(st, ctx);
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 16)
 #endif
@@ -495,26 +509,33 @@ template <typename TSetH, typename TSetV, typename TScore, typename TLower, type
                            std::is_integral<std::decay_t<TLower>>::value &&
                            std::is_integral<std::decay_t<TUpper>>::value, int> = 0>
 inline String<int>
-localAlignmentScore(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
+localAlignmentScore(ExecutionPolicy<Parallel, Vectorial> const & execPolicy,
                     TSetH && setH, TSetV && setV, TScore && scoringScheme, TLower && lowerDiag, TUpper && upperDiag)
-SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores_banded(setH, setV, scoringScheme,
-                                                               static_cast<uint32_t>(SQMI_LOCAL),
-                                                               static_cast<int64_t>(lowerDiag),
-                                                               static_cast<int64_t>(upperDiag)))
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores_banded(
+    setH, setV, scoringScheme, static_cast<uint32_t>(SQMI_LOCAL), static_cast<int64_t>(lowerDiag),
+    static_cast<int64_t>(upperDiag), [&]() {
+        return seqan2::localAlignmentScore<Parallel, Vectorial>(execPolicy, setH, setV, scoringScheme, lowerDiag,
+                                                                upperDiag);
+    }))
 
 // Banded: the executor forwards (setH, setV, score [, config], lowerDiag,
-// upperDiag) to global_alignment_banded.h:555-583.  Bands outside the ones
-// sqmi_align_score_banded takes raise seqan_amd::Error(SQMI_ERR_UNSUPPORTED).
+// upperDiag) to global_alignment_banded.h:555-583.  Calls with a pair whose
+// band sqmi_align_score_banded does not take run the reference's executor
+// instead: explicit template arguments <Parallel, Vectorial> make these
+// overloads non-viable (their first template parameters are the sets), so
+// the call resolves to the reference's template.
 template <typename TSetH, typename TSetV, typename TScore, typename TLower, typename TUpper,
           std::enable_if_t<seqan_amd::detail::IsGpuCall<TSetH, TSetV, TScore>::value &&
                            std::is_integral<std::decay_t<TLower>>::value &&
                            std::is_integral<std::decay_t<TUpper>>::value, int> = 0>
 inline String<int>
-globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
+globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial> const & execPolicy,
                      TSetH && setH, TSetV && setV, TScore && scoringScheme, TLower && lowerDiag, TUpper && upperDiag)
-SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores_banded(setH, setV, scoringScheme, 0u,
-                                                               static_cast<int64_t>(lowerDiag),
-                                                               static_cast<int64_t>(upperDiag)))
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores_banded(
+    setH, setV, scoringScheme, 0u, static_cast<int64_t>(lowerDiag), static_cast<int64_t>(upperDiag), [&]() {
+        return seqan2::globalAlignmentScore<Parallel, Vectorial>(execPolicy, setH, setV, scoringScheme, lowerDiag,
+                                                                 upperDiag);
+    }))
 
 template <typename TSetH, typename TSetV, typename TScore, typename TConfig, typename TLower, typename TUpper,
           std::enable_if_t<seqan_amd::detail::IsGpuCall<TSetH, TSetV, TScore>::value &&
@@ -522,12 +543,15 @@ template <typename TSetH, typename TSetV, typename TScore, typename TConfig, typ
                            std::is_integral<std::decay_t<TLower>>::value &&
                            std::is_integral<std::decay_t<TUpper>>::value, int> = 0>
 inline String<int>
-globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
-                     TSetH && setH, TSetV && setV, TScore && scoringScheme, TConfig && /*alignConfig*/,
+globalAlignmentScore(ExecutionPolicy<Parallel, Vectorial> const & execPolicy,
+                     TSetH && setH, TSetV && setV, TScore && scoringScheme, TConfig && alignConfig,
                      TLower && lowerDiag, TUpper && upperDiag)
 SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_scores_banded(
     setH, setV, scoringScheme, seqan_amd::detail::IsAlignConfig<std::decay_t<TConfig>>::mode,
-    static_cast<int64_t>(lowerDiag), static_cast<int64_t>(upperDiag)))
+    static_cast<int64_t>(lowerDiag), static_cast<int64_t>(upperDiag), [&]() {
+        return seqan2::globalAlignmentScore<Parallel, Vectorial>(execPolicy, setH, setV, scoringScheme, alignConfig,
+                                                                 lowerDiag, upperDiag);
+    }))
 
 template <typename TSetH, typename TSetV, typename TScore,
           std::enable_if_t<seqan_amd::detail::IsGpuCall<TSetH, TSetV, TScore>::value, int> = 0>
@@ -596,19 +620,22 @@ SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_local_rows(gapsSetH, gapsSetV, 
 
 // Banded rows: the executor forwards (gapsSetH, gapsSetV, score [, config],
 // lowerDiag, upperDiag) to global_alignment_banded.h:655-760 and
-// local_alignment_banded.h:261-280.  Bands outside the ones
-// sqmi_align_rows_banded takes raise seqan_amd::Error(SQMI_ERR_UNSUPPORTED).
+// local_alignment_banded.h:261-280.  Calls with a pair whose band
+// sqmi_align_rows_banded does not take run the reference's executor, as the
+// banded scores do.
 template <typename TGapsSetH, typename TGapsSetV, typename TScore, typename TLower, typename TUpper,
           std::enable_if_t<seqan_amd::detail::IsGpuRowsCall<TGapsSetH, TGapsSetV, TScore>::value &&
                            std::is_integral<std::decay_t<TLower>>::value &&
                            std::is_integral<std::decay_t<TUpper>>::value, int> = 0>
 inline String<int>
-globalAlignment(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
+globalAlignment(ExecutionPolicy<Parallel, Vectorial> const & execPolicy,
                 TGapsSetH && gapsSetH, TGapsSetV && gapsSetV, TScore && scoringScheme, TLower && lowerDiag,
                 TUpper && upperDiag)
-SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_rows_banded(gapsSetH, gapsSetV, scoringScheme, 0u,
-                                                             static_cast<int64_t>(lowerDiag),
-                                                             static_cast<int64_t>(upperDiag)))
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_rows_banded(
+    gapsSetH, gapsSetV, scoringScheme, 0u, static_cast<int64_t>(lowerDiag), static_cast<int64_t>(upperDiag), [&]() {
+        return seqan2::globalAlignment<Parallel, Vectorial>(execPolicy, gapsSetH, gapsSetV, scoringScheme,
+                                                            lowerDiag, upperDiag);
+    }))
 
 template <typename TGapsSetH, typename TGapsSetV, typename TScore, typename TConfig, typename TLower,
           typename TUpper,
@@ -617,25 +644,30 @@ template <typename TGapsSetH, typename TGapsSetV, typename TScore, typename TCon
                            std::is_integral<std::decay_t<TLower>>::value &&
                            std::is_integral<std::decay_t<TUpper>>::value, int> = 0>
 inline String<int>
-globalAlignment(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
-                TGapsSetH && gapsSetH, TGapsSetV && gapsSetV, TScore && scoringScheme, TConfig && /*alignConfig*/,
+globalAlignment(ExecutionPolicy<Parallel, Vectorial> const & execPolicy,
+                TGapsSetH && gapsSetH, TGapsSetV && gapsSetV, TScore && scoringScheme, TConfig && alignConfig,
                 TLower && lowerDiag, TUpper && upperDiag)
 SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_rows_banded(
     gapsSetH, gapsSetV, scoringScheme, seqan_amd::detail::IsAlignConfig<std::decay_t<TConfig>>::mode,
-    static_cast<int64_t>(lowerDiag), static_cast<int64_t>(upperDiag)))
+    static_cast<int64_t>(lowerDiag), static_cast<int64_t>(upperDiag), [&]() {
+        return seqan2::globalAlignment<Parallel, Vectorial>(execPolicy, gapsSetH, gapsSetV, scoringScheme,
+                                                            alignConfig, lowerDiag, upperDiag);
+    }))
 
 template <typename TGapsSetH, typename TGapsSetV, typename TScore, typename TLower, typename TUpper,
           std::enable_if_t<seqan_amd::detail::IsGpuRowsCall<TGapsSetH, TGapsSetV, TScore>::value &&
                            std::is_integral<std::decay_t<TLower>>::value &&
                            std::is_integral<std::decay_t<TUpper>>::value, int> = 0>
 inline String<int>
-localAlignment(ExecutionPolicy<Parallel, Vectorial> const & /*execPolicy*/,
+localAlignment(ExecutionPolicy<Parallel, Vectorial> const & execPolicy,
                TGapsSetH && gapsSetH, TGapsSetV && gapsSetV, TScore && scoringScheme, TLower && lowerDiag,
                TUpper && upperDiag)
-SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_rows_banded(gapsSetH, gapsSetV, scoringScheme,
-                                                             static_cast<uint32_t>(SQMI_LOCAL),
-                                                             static_cast<int64_t>(lowerDiag),
-                                                             static_cast<int64_t>(upperDiag)))
+SEQAN_AMD_OVERLOAD_BODY(seqan_amd::detail::align_rows_banded(
+    gapsSetH, gapsSetV, scoringScheme, static_cast<uint32_t>(SQMI_LOCAL), static_cast<int64_t>(lowerDiag),
+    static_cast<int64_t>(upperDiag), [&]() {
+        return seqan2::localAlignment<Parallel, Vectorial>(execPolicy, gapsSetH, gapsSetV, scoringScheme, lowerDiag,
+                                                           upperDiag);
+    }))
 
 }  // namespace seqan2
 #undef SEQAN_AMD_OVERLOAD_BODY

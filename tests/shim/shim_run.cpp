@@ -317,6 +317,16 @@ int main()
             [&] { return globalAlignmentScore(cpu, eph, epv, b62, -70, 70); });
     runCase("protein_local_banded_blosum62", [&] { return localAlignmentScore(ex, eph, epv, b62, -20, 30); },
             [&] { return localAlignmentScore(cpu, eph, epv, b62, -20, 30); });
+    // Bands the library does not take (dp_algorithm_impl.h:192-232, 595-700):
+    // the overloads hand them to the reference's executor.  A one-diagonal
+    // band, a band whose edge is the corner diagonal under free ends, and a
+    // local band that only reaches the corner column.
+    runCase("dna_banded_one_diagonal", [&] { return globalAlignmentScore(ex, eh, ev, dna, 0, 0); },
+            [&] { return globalAlignmentScore(cpu, eh, ev, dna, 0, 0); });
+    runCase("dna_banded_corner_semiglobal", [&] { return globalAlignmentScore(ex, eh, ev, dna, semi, 0, 12); },
+            [&] { return globalAlignmentScore(cpu, eh, ev, dna, semi, 0, 12); });
+    runCase("dna_local_banded_corner", [&] { return localAlignmentScore(ex, eh, ev, dna, 119, 125); },
+            [&] { return localAlignmentScore(cpu, eh, ev, dna, 119, 125); });
 
     runRowsCase("dna_rows", dh, dv, dna, ex, cpu);
     runRowsCase("protein_rows_blosum62", ph, pv, b62, ex, cpu);
@@ -333,6 +343,8 @@ int main()
     runBandedRowsCase("dna_banded_rows_semiglobal", false, eh, ev, dna, ex, -8, 12, semi);
     runBandedRowsCase("dna_local_banded_rows", true, eh, ev, dna, ex, -6, 9);
     runBandedRowsCase("protein_local_banded_rows_blosum62", true, eph, epv, b62, ex, -20, 30);
+    runBandedRowsCase("dna_banded_rows_one_diagonal", false, eh, ev, dna, ex, 0, 0);
+    runBandedRowsCase("dna_banded_rows_corner_semiglobal", false, eh, ev, dna, ex, -9, 0, semi);
 
     // |setH| != |setV| raises instead of asserting.
     StringSet<String<Dna>> shorter = dv;
