@@ -332,19 +332,25 @@ __global__ __launch_bounds__(256) void kBandedTrace(BandedArgs a)
         // runs the moves, the segment edge is masked afterwards.
         const int32_t hLeftAll = dpp<kDppWaveShr1>(H), eLeftAll = dpp<kDppWaveShr1>(k == 0 ? kNegB : E);
         const int32_t hLeft = k == 0 ? kNegB : hLeftAll, eLeft = k == 0 ? kNegB : eLeftAll;
-        if (valid && !col0) {
-            const int32_t extE = eLeft + e, opnE = hLeft + o;
-            uint32_t tb = extE < opnE ? kEOpen : 0u;
-            const int32_t ei = max(extE, opnE);
-            tb |= (fUp + e) < (hUp + o) ? kFOpen : 0u;
-            const int32_t fe = F0 < ei ? ei : F0;
-            int32_t x = D;
-            if (D < fe) {
-                x = fe;
-                tb |= kNotDiag | (F0 < ei ? kHori : 0u);
+        // Every byte of the pair's rows is written: cells outside the band
+        // or the matrix, and column 0 (which the walk leaves as a leading
+        // gap run), get the stop byte, so no path can read a stale byte.
+        if (has && i <= P.m) {
+            uint32_t tb = kStop;
+            if (valid && !col0) {
+                const int32_t extE = eLeft + e, opnE = hLeft + o;
+                tb = extE < opnE ? kEOpen : 0u;
+                const int32_t ei = max(extE, opnE);
+                tb |= (fUp + e) < (hUp + o) ? kFOpen : 0u;
+                const int32_t fe = F0 < ei ? ei : F0;
+                int32_t x = D;
+                if (D < fe) {
+                    x = fe;
+                    tb |= kNotDiag | (F0 < ei ? kHori : 0u);
+                }
+                if (local && !(0 < x))
+                    tb = kStop;
             }
-            if (local && !(0 < x))
-                tb = kStop;
             rowT[(int64_t)(i - 1) * S + k] = (uint8_t)tb;
         }
         if (valid && P.track(i, j, a.mode) && H > bv) {

@@ -298,3 +298,57 @@ def test_banded_rows_errors_and_empty(ctx):
         ctx.align_rows_banded(Hs, Vs, score, 0, 3)        # misses the (0, 0) corner diagonal margin
     with pytest.raises(sa.SqmiError):
         ctx.align_rows_banded(Hs, Vs, score, 3, -2)       # lower > upper
+
+
+@pytest.mark.parametrize("band", [(-1, 2), (-2, 0), (0, 1), (-3, 4), (-4, 3)])
+def test_banded_rows_narrow_local_bands(ctx, band):
+    # Local bands of 2-8 diagonals: segments of 2, 4 and 8 lanes of the
+    # lane-per-diagonal trace kernel, against the oracle's banded traceback
+    # (rows, counts, window begins), with many ties under the second scoring.
+    lo, up = band
+    rng = np.random.default_rng(300 + 10 * lo + up)
+    for ma, mi, ge, go in ((2, -3, -1, -5), (1, -1, -1, -1)):
+        score = sa.simple_score(ma, mi, ge, go)
+        sc = po.Scoring(score.table, go, ge)
+        hs, vs = [], []
+        for _ in range(200):
+            n = int(rng.integers(max(2, lo + 2), 260))
+            h = rng.integers(0, 4, n).astype(np.uint8)
+            v = h[:max(2, n + int(rng.integers(-2, 3)))].copy()
+            f = rng.random(len(v)) < 0.15
+            v[f] = rng.integers(0, 4, int(f.sum()))
+            hs.append(h)
+            vs.append(v)
+        Hs, Vs = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+        print("narrow local band", band, "scoring", (ma, mi, ge, go), flush=True)
+        out = ctx.align_rows_banded(Hs, Vs, score, lo, up, mode=sa.LOCAL)
+        want = po.rows_batch(Hs.residues, Hs.offsets, Vs.residues, Vs.offsets, sc, sa.LOCAL,
+                             with_begins=True, band=(lo, up))
+        for a, b in zip(out, want):
+            assert np.array_equal(a, b), (band, (ma, mi, ge, go))
+
+
+@pytest.mark.parametrize("band", [(-1, 1), (-2, 2), (-3, 3)])
+def test_banded_rows_narrow_global_bands(ctx, band):
+    # Global bands of 3-7 diagonals on pairs of equal length (both corner
+    # diagonals inside the band): segments of 4 and 8 lanes.
+    lo, up = band
+    rng = np.random.default_rng(400 + up)
+    score = sa.simple_score(2, -3, -1, -5)
+    sc = po.Scoring(score.table, -5, -1)
+    hs, vs = [], []
+    for _ in range(200):
+        n = int(rng.integers(2, 260))
+        h = rng.integers(0, 4, n).astype(np.uint8)
+        v = h.copy()
+        f = rng.random(n) < 0.15
+        v[f] = rng.integers(0, 4, int(f.sum()))
+        hs.append(h)
+        vs.append(v)
+    Hs, Vs = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+    for mode in (0, 9, 15):
+        print("narrow global band", band, "mode", mode, flush=True)
+        out = ctx.align_rows_banded(Hs, Vs, score, lo, up, mode=mode)
+        want = po.rows_batch(Hs.residues, Hs.offsets, Vs.residues, Vs.offsets, sc, mode, band=(lo, up))
+        for a, b in zip(out, want):
+            assert np.array_equal(a, b), (band, mode)
