@@ -47,6 +47,7 @@
 #include <seqan/score.h>
 #include <seqan/sequence.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -269,6 +270,20 @@ int run(RefInput const & in, TScore const & sc, int alpha, std::FILE * out)
     }
     else if (h.op == 0)
     {
+        // One untimed warm-up call on the first pairs (at most 4,096): page
+        // faults of the first allocations and the thread team's start-up stay
+        // outside the timed calls.
+        {
+            int64_t const w = std::min<int64_t>(h.count, 4096);
+            StringSet<String<TAlphabet>> wH, wV;
+            for (int64_t k = 0; k < w; ++k)
+            {
+                appendValue(wH, setH[k]);
+                appendValue(wV, setV[k]);
+            }
+            String<int> warm = runScore(ex, wH, wV, sc, h.mode);
+            (void)warm;
+        }
         int reps = h.repeats > 0 ? h.repeats : 1;
         for (int r = 0; r < reps; ++r)
         {
@@ -294,6 +309,22 @@ int run(RefInput const & in, TScore const & sc, int alpha, std::FILE * out)
         {
             gH[k] = TGaps(setH[k]);
             gV[k] = TGaps(setV[k]);
+        }
+        if (h.op == 1)
+        {
+            // Untimed warm-up on the first pairs (at most 256), as for op 0.
+            int64_t const w = std::min<int64_t>(h.count, 256);
+            StringSet<TGaps> wH, wV;
+            resize(wH, w);
+            resize(wV, w);
+            for (int64_t k = 0; k < w; ++k)
+            {
+                wH[k] = TGaps(setH[k]);
+                wV[k] = TGaps(setV[k]);
+            }
+            String<int> warm = local ? localAlignment(ex, wH, wV, sc)
+                                     : withConfig(h.mode, [&](auto cfg) { return globalAlignment(ex, wH, wV, sc, cfg); });
+            (void)warm;
         }
         auto t0 = std::chrono::steady_clock::now();
         if (h.op == 1)
