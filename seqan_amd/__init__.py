@@ -64,14 +64,15 @@ class Stats(ctypes.Structure):
                 ("pairs_int32", ctypes.c_int64), ("work_items", ctypes.c_int64),
                 ("padded_cells", ctypes.c_int64), ("host_plan_ms", ctypes.c_double),
                 ("device_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("pairs_long", ctypes.c_int64), ("pairs_long_packed", ctypes.c_int64)]
+                ("pairs_long", ctypes.c_int64), ("pairs_long_packed", ctypes.c_int64),
+                ("pairs_lane", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 # SQMI_ABI_VERSION of include/seqan_amd.h that the structures below mirror.
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 # Every symbol the C ABI declares (include/seqan_amd.h).
 ABI_SYMBOLS = (

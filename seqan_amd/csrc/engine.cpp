@@ -29,10 +29,44 @@ using namespace sqmi;
 
 namespace {
 
+// A bump allocator over caller-free device memory (the no-allocation entry
+// point's scratch array): buffers carved from it are never freed.
+struct Arena
+{
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+};
+
 struct DevBuf
 {
     void *p = nullptr;
     size_t cap = 0;
+    Arena *arena = nullptr;   // set: carve from this arena instead of hipMalloc
+};
+
+// Every device buffer a context keeps between calls.
+struct Buffers
+{
+    DevBuf hRes, vRes, hOff, vOff, order, items, queue, rowbuf, scores, tTable, dnaProf, ids32, scratchOff32,
+        scratch32, table32, emptyIds;
+    // Traceback (sqmi_align_counts).
+    DevBuf trace, traceOff, locs, endEF, endIJ, counts;
+    // Long pairs (tiled int32 kernels).
+    DevBuf longCk, longPairs, longWalk, longTrace;
+    // Alignment rows (sqmi_align_rows, sqmi_align_local_rows).
+    DevBuf ops, opOff, opLen, begins;
+    // Banded scores (sqmi_align_score_banded).
+    DevBuf bandIds, bandScratch;
+
+    template <class F>
+    void forEach(F f)
+    {
+        for (DevBuf *b : {&hRes, &vRes, &hOff, &vOff, &order, &items, &queue, &rowbuf, &scores, &tTable, &dnaProf,
+                          &ids32, &scratchOff32, &scratch32, &table32, &emptyIds, &trace, &traceOff, &locs, &endEF,
+                          &endIJ, &counts, &longCk, &longPairs, &longWalk, &longTrace, &ops, &opOff, &opLen, &begins,
+                          &bandIds, &bandScratch})
+            f(*b);
+    }
 };
 
 }  // namespace
@@ -52,16 +86,12 @@ struct sqmi_context
     static constexpr int kMaxLongStreams = 4;
     hipStream_t longStreams[kMaxLongStreams] = {};
     hipEvent_t longFork = nullptr, longJoin[kMaxLongStreams] = {};
-    DevBuf hRes, vRes, hOff, vOff, order, items, queue, rowbuf, scores, tTable, dnaProf, ids32, scratchOff32,
-        scratch32, table32, emptyIds;
-    // Traceback (sqmi_align_counts).
-    DevBuf trace, traceOff, locs, endEF, endIJ, counts;
-    // Long pairs (tiled int32 kernels).
-    DevBuf longCk, longPairs, longWalk, longTrace;
-    // Alignment rows (sqmi_align_rows, sqmi_align_local_rows).
-    DevBuf ops, opOff, opLen, begins;
-    // Banded scores (sqmi_align_score_banded).
-    DevBuf bandIds, bandScratch;
+    Buffers b;
+    // The no-allocation entry point: this device's scratch array (its device
+    // address, from hipGetSymbolAddress at creation) and, during such a
+    // call, the arena carved from it.
+    void *scratchPtr = nullptr;
+    Arena arena;
 };
 
 namespace {
@@ -85,6 +115,16 @@ hipError_t ensure(DevBuf &b, size_t bytes)
         bytes = 16;
     if (b.cap >= bytes)
         return hipSuccess;
+    if (b.arena) {
+        // 256-byte aligned pieces of the arena; an outgrown piece stays unused.
+        const size_t need = (bytes + 255) & ~(size_t)255;
+        if (b.arena->used + need > b.arena->cap)
+            return hipErrorOutOfMemory;
+        b.p = b.arena->base + b.arena->used;
+        b.arena->used += need;
+        b.cap = need;
+        return hipSuccess;
+    }
     if (b.p)
         (void)hipFree(b.p);
     b.p = nullptr;
@@ -97,7 +137,7 @@ hipError_t ensure(DevBuf &b, size_t bytes)
 
 void release(DevBuf &b)
 {
-    if (b.p)
+    if (b.p && !b.arena)
         (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
@@ -145,18 +185,38 @@ struct Plan16
 
 int pairsPerWave(const Variant16 &v) { return 2 * (64 / v.G); }
 
-// Kernel shape for a pair with m rows (V) and n columns (H).
-Variant16 chooseVariant(bool dna, uint32_t track, int64_t n, int64_t m, uint32_t flags)
+// Shapes compiled for the packed forward: groups of 16 lanes (kFwd16) or one
+// pair couple per lane (kLane16).
+bool shapeSupported(const Variant16 &v) { return v.G == 1 ? laneSupported(v) : variantSupported(v); }
+
+// The planner's shape table (seqan_amd/tuning/shapes.json, turned into C++
+// by the build): per category (DNA byte profile or substitution table,
+// global or local), length bin b (2^(b-1) < |V| + 1 <= 2^b) and count class
+// (the bin's pairs in this call, at least kTabCountClasses[c]), an index
+// into kTabShapes.
+#include "shape_table.inc"
+
+int lengthBin(int64_t m)
 {
-    (void)n;
-    const int forcedG = (int)((flags >> 8) & 0xFF), forcedR = (int)((flags >> 16) & 0xFF);
-    if (forcedG && forcedR)
-        return Variant16{forcedG, forcedR, dna, track};
-    Variant16 v{16, 16, dna, track};
-    if (m + 1 <= 128)
-        v.R = 8;
-    return v;
+    int b = 0;
+    while (b < 15 && (int64_t(1) << b) < m + 1)
+        ++b;
+    return b;
 }
+
+int shapeCategory(bool dna, bool local) { return (dna ? 0 : 2) + (local ? 1 : 0); }
+
+Variant16 tableShape(int category, int bin, int64_t pairsInBin, bool dna, uint32_t track)
+{
+    int cls = 0;
+    while (cls + 1 < kTabNClasses && pairsInBin >= kTabCountClasses[cls + 1])
+        ++cls;
+    const int idx = kTabChoice[category][bin][cls];
+    return Variant16{kTabShapes[idx][0], kTabShapes[idx][1], dna, track};
+}
+
+// Row blocks of a pair with m rows under shape v (at most 255 fit the keys).
+int64_t blocksFor(const Variant16 &v, int64_t m) { return (m + 1 + (int64_t)v.G * v.R - 1) / ((int64_t)v.G * v.R); }
 
 // One pair per launch: one wavefront, whose first 16-lane group sweeps the
 // pair (the other pair slots of the wavefront stay empty).
@@ -208,7 +268,8 @@ struct Planned
     std::vector<int64_t> scratchOff32;
     int64_t scratch32 = 0;
     std::vector<Key> keys16;        // int16 pairs, sorted by bucket key
-    std::vector<Plan16> plans;      // one per kernel variant
+    std::vector<Variant16> shapes;  // int16 shape slots (key classes 0..)
+    std::vector<Plan16> plans;      // one per kernel variant in use
     int64_t cells = 0;
 };
 
@@ -306,11 +367,12 @@ void radixSort(std::vector<Key> &keys)
     }
 }
 
-// Bucket keys (26 bits): class (2 bits: 0 / 1 int16 shape index, 2 int32,
-// 3 empty) | 255 - row blocks (8 bits) | 32767 - |H| (15 bits).  Ascending
+// Bucket keys (26 bits): class (3 bits: 0..5 int16 shape slot, 6 int32,
+// 7 empty) | 255 - row blocks (8 bits) | 32767 - |H| (15 bits).  Ascending
 // order puts int16 pairs first, grouped by shape and row blocks (most first),
 // widest first, then the int32 and the empty pairs in input order.
 constexpr int kKeyNBits = 15, kKeyBlockBits = 8, kKeyClassShift = kKeyNBits + kKeyBlockBits;
+constexpr uint32_t kClassInt32 = 6, kClassEmpty = 7, kMaxShapeSlots = 6;
 inline uint32_t bucketOf(uint32_t key) { return key >> kKeyNBits; }
 inline int32_t widthOf(uint32_t key) { return (int32_t)((1u << kKeyNBits) - 1 - (key & ((1u << kKeyNBits) - 1))); }
 inline int32_t blocksOf(uint32_t key) { return (int32_t)(255u - ((key >> kKeyNBits) & 255u)); }
@@ -354,37 +416,83 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
             track = kTrackEnd;
     }
     P.track = track;
-    // Route every pair and give it a bucket key (see bucketOf); shape index
-    // 0 is the forced shape, the trace shape or R = 16, index 1 is R = 8.
-    std::vector<Variant16> shapes;
-    if (traceShape)
-        shapes.push_back(Variant16{kTraceLanes, kTraceRows / kTraceLanes, si.dna, track});
-    else if (((flags >> 8) & 0xFF) && ((flags >> 16) & 0xFF))
-        shapes.push_back(chooseVariant(si.dna, track, 1, 1, flags));
-    else {
-        shapes.push_back(Variant16{16, 16, si.dna, track});
-        shapes.push_back(Variant16{16, 8, si.dna, track});
-    }
+    // Route every pair and give it a bucket key (see bucketOf).  The int16
+    // pairs' shape slots: the trace shape, a forced shape, or per length bin
+    // the shape table's choice for the number of the call's pairs in that
+    // bin (a pair that needs more than 255 row blocks of its bin's shape
+    // takes G = 16, R = 16).
     std::vector<Key> keys((size_t)N);
     int64_t cells = 0;
     const bool allLong = longAll() && si.dna && mode == 0 && !noLong;
     const int64_t longMin0 = longMinSide();
+    auto route16 = [&](int64_t n, int64_t m) {
+        return !(allLong && n >= longMin0 && m >= longMin0) && costsOk16 && fitsInt16(si, n, m, local) &&
+               n < (1 << kKeyNBits);
+    };
+    std::vector<Variant16> &shapes = P.shapes;
+    shapes.clear();
+    uint8_t binSlot[16];
+    int fallbackSlot = -1;
+    const int forcedG = (int)((flags >> 8) & 0xFF), forcedR = (int)((flags >> 16) & 0xFF);
+    if (traceShape || (forcedG && forcedR)) {
+        shapes.push_back(traceShape ? Variant16{kTraceLanes, kTraceRows / kTraceLanes, si.dna, track}
+                                    : Variant16{forcedG, forcedR, si.dna, track});
+        std::fill(binSlot, binSlot + 16, (uint8_t)0);
+        if (shapes[0].G == 1) {
+            // Pairs taller than 255 blocks of the forced lane shape.
+            shapes.push_back(Variant16{16, 16, si.dna, track});
+            fallbackSlot = 1;
+        }
+    } else {
+        // Pairs per length bin (int16 route only), then one slot per shape.
+        int64_t binCount[16] = {};
+#pragma omp parallel
+        {
+            int64_t local16[16] = {};
+#pragma omp for schedule(static) nowait
+            for (int64_t i = 0; i < N; ++i) {
+                const int64_t n = H.offsets[i + 1] - H.offsets[i];
+                const int64_t m = V.offsets[i + 1] - V.offsets[i];
+                if (n != 0 && m != 0 && route16(n, m))
+                    ++local16[lengthBin(m)];
+            }
+#pragma omp critical
+            for (int b = 0; b < 16; ++b)
+                binCount[b] += local16[b];
+        }
+        const int cat = shapeCategory(si.dna, local);
+        auto slotOf = [&](const Variant16 &v) {
+            for (size_t k = 0; k < shapes.size(); ++k)
+                if (shapes[k].G == v.G && shapes[k].R == v.R)
+                    return (int)k;
+            shapes.push_back(v);
+            return (int)shapes.size() - 1;
+        };
+        for (int b = 0; b < 16; ++b) {
+            Variant16 v = tableShape(cat, b, binCount[b], si.dna, track);
+            if (!shapeSupported(v))
+                v = Variant16{16, 16, si.dna, track};
+            binSlot[b] = (uint8_t)(binCount[b] ? slotOf(v) : 0);
+        }
+        fallbackSlot = slotOf(Variant16{16, 16, si.dna, track});
+    }
 #pragma omp parallel for schedule(static) reduction(+ : cells)
     for (int64_t i = 0; i < N; ++i) {
         const int64_t n = H.offsets[i + 1] - H.offsets[i];
         const int64_t m = V.offsets[i + 1] - V.offsets[i];
-        uint32_t key = 3u << kKeyClassShift;
+        uint32_t key = kClassEmpty << kKeyClassShift;
         if (n != 0 && m != 0) {
             cells += n * m;
-            key = 2u << kKeyClassShift;
-            if (allLong && n >= longMin0 && m >= longMin0) {
-                // long route (see longAll)
-            } else if (costsOk16 && fitsInt16(si, n, m, local)) {
-                const uint32_t vidx = shapes.size() == 2 && m + 1 <= 128 ? 1u : 0u;
-                const int64_t BR = (int64_t)shapes[vidx].G * shapes[vidx].R;
-                const int64_t nblocks = (m + 1 + BR - 1) / BR;
-                if (nblocks <= 255 && n < (1 << kKeyNBits))
-                    key = (vidx << kKeyClassShift) | ((255u - (uint32_t)nblocks) << kKeyNBits) |
+            key = kClassInt32 << kKeyClassShift;
+            if (route16(n, m)) {
+                uint32_t slot = binSlot[lengthBin(m)];
+                int64_t nblocks = blocksFor(shapes[slot], m);
+                if (nblocks > 255 && fallbackSlot >= 0) {
+                    slot = (uint32_t)fallbackSlot;
+                    nblocks = blocksFor(shapes[slot], m);
+                }
+                if (nblocks <= 255)
+                    key = (slot << kKeyClassShift) | ((255u - (uint32_t)nblocks) << kKeyNBits) |
                           ((1u << kKeyNBits) - 1 - (uint32_t)n);
             }
         }
@@ -394,7 +502,7 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
     radixSort(keys);
     // Segments of the sorted order: int16 (by shape), int32, empty.
     size_t n16 = 0;
-    while (n16 < keys.size() && (keys[n16].key >> kKeyClassShift) < 2) {
+    while (n16 < keys.size() && (keys[n16].key >> kKeyClassShift) < kClassInt32) {
         // Jump by bucket boundaries found with a binary search.
         const uint32_t cls = keys[n16].key >> kKeyClassShift;
         n16 = (size_t)(std::upper_bound(keys.begin() + n16, keys.end(), ((cls + 1) << kKeyClassShift) - 1,
@@ -402,7 +510,7 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
                        keys.begin());
     }
     size_t e32 = n16;
-    while (e32 < keys.size() && (keys[e32].key >> kKeyClassShift) == 2)
+    while (e32 < keys.size() && (keys[e32].key >> kKeyClassShift) == kClassInt32)
         ++e32;
     const int64_t longMin = longMinSide();
     const bool longOk = si.dna && mode == 0 && !noLong;
@@ -458,7 +566,7 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
         itemOff[b + 1] = itemOff[b] + (starts[b + 1] - starts[b] + PP - 1) / PP;
     }
     std::vector<Plan16> &plans = P.plans;
-    for (size_t v = 0; v < shapes.size(); ++v) {
+    for (uint32_t v = 0; v < (uint32_t)shapes.size(); ++v) {
         // The shape's contiguous range of buckets and of sorted pairs.
         size_t b0 = 0;
         while (b0 < nb && (keys16[starts[b0]].key >> kKeyClassShift) < v)
@@ -505,8 +613,11 @@ void fillStats(const Planned &P, sqmi_stats &st)
     st.pairs_long_packed = longPacked(P.si, P.flags) ? st.pairs_long : 0;
     st.pairs_int16 = (int64_t)P.keys16.size();
     int64_t items = 0, padded = 0;
+    st.pairs_lane = 0;
     for (const Plan16 &pl : P.plans) {
         items += (int64_t)pl.items.size();
+        if (pl.var.G == 1)
+            st.pairs_lane += (int64_t)pl.order.size();
         for (const WorkItem &w : pl.items)
             padded += 2LL * (64 / pl.var.G) * (int64_t)w.maxN * w.nblocks * pl.var.G * pl.var.R;
     }
@@ -537,7 +648,7 @@ sqmi_status checkForced(sqmi_context *ctx, uint32_t flags)
     if (!g && !r)
         return SQMI_OK;
     for (bool dna : {true, false})
-        if (!variantSupported(Variant16{g, r, dna, kTrackEnd}))
+        if (!shapeSupported(Variant16{g, r, dna, kTrackEnd}))
             return fail(ctx, SQMI_ERR_UNSUPPORTED, "forced kernel variant is not compiled");
     return SQMI_OK;
 }
@@ -567,7 +678,7 @@ int coopMinBlocks()
     const char *env = std::getenv("SQMI_COOP_MIN_BLOCKS");
     if (env && *env)
         return std::max(0, std::atoi(env));
-    return 10;
+    return kTabCoopMinBlocks;
 }
 
 // Device copies of what every entry point needs: residues (unless already on
@@ -581,7 +692,10 @@ struct Inputs
     const uint8_t *dH = nullptr, *dV = nullptr;
     int32_t *dScores = nullptr;
     std::vector<int16_t> tTab;
-    uint32_t prof[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // DNA profiles: dwords 0..7 by column residue h (byte v = t(h, v)),
+    // dwords 8..15 by row residue v (byte h = t(h, v)); the padding code
+    // (alpha) maps to zero bytes.
+    uint32_t prof[16] = {};
 };
 
 sqmi_status uploadInputs(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
@@ -594,24 +708,24 @@ sqmi_status uploadInputs(sqmi_context *ctx, const sqmi_sequences *setH, const sq
     const int64_t hTot = setH->offsets[N], vTot = setV->offsets[N];
     const uint8_t *dH = setH->residues, *dV = setV->residues;
     if (!(flags & SQMI_RESIDUES_ON_DEVICE)) {
-        if ((err = ensure(ctx->hRes, hTot + 64)) != hipSuccess || (err = ensure(ctx->vRes, vTot + 64)) != hipSuccess)
+        if ((err = ensure(ctx->b.hRes, hTot + 64)) != hipSuccess || (err = ensure(ctx->b.vRes, vTot + 64)) != hipSuccess)
             return hipFail(ctx, err, "allocating residues");
-        if ((err = hipMemcpyAsync(ctx->hRes.p, setH->residues, hTot, hipMemcpyHostToDevice, stream)) != hipSuccess ||
-            (err = hipMemcpyAsync(ctx->vRes.p, setV->residues, vTot, hipMemcpyHostToDevice, stream)) != hipSuccess)
+        if ((err = hipMemcpyAsync(ctx->b.hRes.p, setH->residues, hTot, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+            (err = hipMemcpyAsync(ctx->b.vRes.p, setV->residues, vTot, hipMemcpyHostToDevice, stream)) != hipSuccess)
             return hipFail(ctx, err, "copying residues");
-        dH = static_cast<const uint8_t *>(ctx->hRes.p);
-        dV = static_cast<const uint8_t *>(ctx->vRes.p);
+        dH = static_cast<const uint8_t *>(ctx->b.hRes.p);
+        dV = static_cast<const uint8_t *>(ctx->b.vRes.p);
     }
-    if ((err = ensure(ctx->hOff, 8 * (N + 1))) != hipSuccess || (err = ensure(ctx->vOff, 8 * (N + 1))) != hipSuccess)
+    if ((err = ensure(ctx->b.hOff, 8 * (N + 1))) != hipSuccess || (err = ensure(ctx->b.vOff, 8 * (N + 1))) != hipSuccess)
         return hipFail(ctx, err, "allocating offsets");
-    if ((err = hipMemcpyAsync(ctx->hOff.p, setH->offsets, 8 * (N + 1), hipMemcpyHostToDevice, stream)) != hipSuccess ||
-        (err = hipMemcpyAsync(ctx->vOff.p, setV->offsets, 8 * (N + 1), hipMemcpyHostToDevice, stream)) != hipSuccess)
+    if ((err = hipMemcpyAsync(ctx->b.hOff.p, setH->offsets, 8 * (N + 1), hipMemcpyHostToDevice, stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(ctx->b.vOff.p, setV->offsets, 8 * (N + 1), hipMemcpyHostToDevice, stream)) != hipSuccess)
         return hipFail(ctx, err, "copying offsets");
     int32_t *dScores = scores;
     if (!(flags & SQMI_SCORES_ON_DEVICE)) {
-        if ((err = ensure(ctx->scores, 4 * N)) != hipSuccess)
+        if ((err = ensure(ctx->b.scores, 4 * N)) != hipSuccess)
             return hipFail(ctx, err, "allocating scores");
-        dScores = static_cast<int32_t *>(ctx->scores.p);
+        dScores = static_cast<int32_t *>(ctx->b.scores.p);
     }
 
     // Substitution tables: t(h, v) = s(h, v) - gapOpen with a padding code.
@@ -624,14 +738,16 @@ sqmi_status uploadInputs(sqmi_context *ctx, const sqmi_sequences *setH, const sq
     uint32_t *prof = in.prof;
     if (si.dna)
         for (int h = 0; h < A; ++h)
-            for (int v = 0; v < A; ++v)
+            for (int v = 0; v < A; ++v) {
                 prof[h] |= (uint32_t)(scoring->table[h * A + v] - si.o) << (8 * v);
-    if ((err = ensure(ctx->tTable, tTab.size() * 2)) != hipSuccess || (err = ensure(ctx->dnaProf, 32)) != hipSuccess ||
-        (err = ensure(ctx->table32, 4 * A * A)) != hipSuccess)
+                prof[8 + v] |= (uint32_t)(scoring->table[h * A + v] - si.o) << (8 * h);
+            }
+    if ((err = ensure(ctx->b.tTable, tTab.size() * 2)) != hipSuccess || (err = ensure(ctx->b.dnaProf, 64)) != hipSuccess ||
+        (err = ensure(ctx->b.table32, 4 * A * A)) != hipSuccess)
         return hipFail(ctx, err, "allocating tables");
-    if ((err = hipMemcpyAsync(ctx->tTable.p, tTab.data(), tTab.size() * 2, hipMemcpyHostToDevice, stream)) != hipSuccess ||
-        (err = hipMemcpyAsync(ctx->dnaProf.p, prof, 32, hipMemcpyHostToDevice, stream)) != hipSuccess ||
-        (err = hipMemcpyAsync(ctx->table32.p, scoring->table, 4 * A * A, hipMemcpyHostToDevice, stream)) != hipSuccess)
+    if ((err = hipMemcpyAsync(ctx->b.tTable.p, tTab.data(), tTab.size() * 2, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(ctx->b.dnaProf.p, prof, 64, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+        (err = hipMemcpyAsync(ctx->b.table32.p, scoring->table, 4 * A * A, hipMemcpyHostToDevice, stream)) != hipSuccess)
         return hipFail(ctx, err, "copying tables");
     in.dH = dH;
     in.dV = dV;
@@ -696,15 +812,15 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
     chunkEnd.push_back(pairs.size());
     const size_t np = pairs.size();
     hipError_t err;
-    if ((err = ensure(ctx->longCk, (size_t)maxRun * 4)) != hipSuccess ||
-        (err = ensure(ctx->longPairs, sizeof(LongPair) * np)) != hipSuccess ||
-        (err = ensure(ctx->endEF, (size_t)8 * N)) != hipSuccess)
+    if ((err = ensure(ctx->b.longCk, (size_t)maxRun * 4)) != hipSuccess ||
+        (err = ensure(ctx->b.longPairs, sizeof(LongPair) * np)) != hipSuccess ||
+        (err = ensure(ctx->b.endEF, (size_t)8 * N)) != hipSuccess)
         return hipFail(ctx, err, "allocating long-pair buffers");
-    if (dCounts && ((err = ensure(ctx->longWalk, (size_t)4 * kWalkStateInts * np)) != hipSuccess ||
-                    (err = ensure(ctx->longTrace, sizeof(uint2) * (size_t)kTileRecSteps * 16 * kRoundSlots * np)) !=
+    if (dCounts && ((err = ensure(ctx->b.longWalk, (size_t)4 * kWalkStateInts * np)) != hipSuccess ||
+                    (err = ensure(ctx->b.longTrace, sizeof(uint2) * (size_t)kTileRecSteps * 16 * kRoundSlots * np)) !=
                         hipSuccess))
         return hipFail(ctx, err, "allocating long-pair traceback buffers");
-    if ((err = hipMemcpyAsync(ctx->longPairs.p, pairs.data(), sizeof(LongPair) * np, hipMemcpyHostToDevice,
+    if ((err = hipMemcpyAsync(ctx->b.longPairs.p, pairs.data(), sizeof(LongPair) * np, hipMemcpyHostToDevice,
                               stream)) != hipSuccess)
         return hipFail(ctx, err, "uploading long pairs");
     size_t b = 0;
@@ -712,17 +828,17 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
         TileArgs a{};
         a.hRes = in.dH;
         a.vRes = in.dV;
-        a.hOff = static_cast<const int64_t *>(ctx->hOff.p);
-        a.vOff = static_cast<const int64_t *>(ctx->vOff.p);
-        a.pairs = static_cast<const LongPair *>(ctx->longPairs.p) + b;
+        a.hOff = static_cast<const int64_t *>(ctx->b.hOff.p);
+        a.vOff = static_cast<const int64_t *>(ctx->b.vOff.p);
+        a.pairs = static_cast<const LongPair *>(ctx->b.longPairs.p) + b;
         a.nPairs = (int32_t)(e - b);
         a.tileC = C;
-        a.ck = static_cast<int32_t *>(ctx->longCk.p);
+        a.ck = static_cast<int32_t *>(ctx->b.longCk.p);
         a.gapOpen = P.si.o;
         a.gapExtend = P.si.e;
-        a.dnaProfile = static_cast<const uint32_t *>(ctx->dnaProf.p);
+        a.dnaProfile = static_cast<const uint32_t *>(ctx->b.dnaProf.p);
         a.scores = in.dScores;
-        a.endEF = static_cast<int32_t *>(ctx->endEF.p);
+        a.endEF = static_cast<int32_t *>(ctx->b.endEF.p);
         a.alpha = P.si.alpha;
         int maxB = 0, maxT = 0;
         for (size_t k = b; k < e; ++k) {
@@ -775,8 +891,8 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
                     return hipFail(ctx, err, "long-pair join");
         }
         if (dCounts) {
-            a.trace = static_cast<uint2 *>(ctx->longTrace.p);
-            a.walk = static_cast<int32_t *>(ctx->longWalk.p);
+            a.trace = static_cast<uint2 *>(ctx->b.longTrace.p);
+            a.walk = static_cast<int32_t *>(ctx->b.longWalk.p);
             a.counts = dCounts;
             a.ops = rows.ops;
             a.opOff = rows.opOff;
@@ -854,12 +970,7 @@ sqmi_status sqmi_destroy(sqmi_context *ctx)
         return SQMI_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (DevBuf *b : {&ctx->hRes, &ctx->vRes, &ctx->hOff, &ctx->vOff, &ctx->order, &ctx->items, &ctx->queue,
-                      &ctx->rowbuf, &ctx->scores, &ctx->tTable, &ctx->dnaProf, &ctx->ids32, &ctx->scratchOff32,
-                      &ctx->scratch32, &ctx->table32, &ctx->emptyIds, &ctx->trace, &ctx->traceOff, &ctx->locs,
-                      &ctx->endEF, &ctx->endIJ, &ctx->counts, &ctx->longCk, &ctx->longPairs, &ctx->longWalk, &ctx->longTrace,
-                      &ctx->ops, &ctx->opOff, &ctx->opLen, &ctx->begins, &ctx->bandIds, &ctx->bandScratch})
-        release(*b);
+    ctx->b.forEach([](DevBuf &b) { release(b); });
     if (ctx->ev0)
         (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1)
@@ -960,16 +1071,16 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
             maxN = std::max<int>(maxN, (int)n);
         }
         const size_t np = ids16.size();
-        if ((err = ensure(ctx->order, 4 * np)) != hipSuccess ||
-            (err = ensure(ctx->items, sizeof(WorkItem) * np)) != hipSuccess ||
-            (err = ensure(ctx->queue, 4 * np + 64)) != hipSuccess ||
-            (err = ensure(ctx->rowbuf, (size_t)4 * (maxN + 18) * 2 * sizeof(uint32_t))) != hipSuccess)
+        if ((err = ensure(ctx->b.order, 4 * np)) != hipSuccess ||
+            (err = ensure(ctx->b.items, sizeof(WorkItem) * np)) != hipSuccess ||
+            (err = ensure(ctx->b.queue, 4 * np + 64)) != hipSuccess ||
+            (err = ensure(ctx->b.rowbuf, (size_t)4 * (maxN + 18) * 2 * sizeof(uint32_t))) != hipSuccess)
             return hipFail(ctx, err, "allocating plain work lists");
-        if (np && ((err = hipMemcpyAsync(ctx->order.p, ids16.data(), 4 * np, hipMemcpyHostToDevice, stream)) !=
+        if (np && ((err = hipMemcpyAsync(ctx->b.order.p, ids16.data(), 4 * np, hipMemcpyHostToDevice, stream)) !=
                        hipSuccess ||
-                   (err = hipMemcpyAsync(ctx->items.p, items.data(), sizeof(WorkItem) * np, hipMemcpyHostToDevice,
+                   (err = hipMemcpyAsync(ctx->b.items.p, items.data(), sizeof(WorkItem) * np, hipMemcpyHostToDevice,
                                          stream)) != hipSuccess ||
-                   (err = hipMemsetAsync(ctx->queue.p, 0, 4 * np, stream)) != hipSuccess))
+                   (err = hipMemsetAsync(ctx->b.queue.p, 0, 4 * np, stream)) != hipSuccess))
             return hipFail(ctx, err, "uploading plain work lists");
         for (size_t k = 0; k < np; ++k) {
             const int32_t id = ids16[k];
@@ -977,21 +1088,21 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
             FwdArgs a{};
             a.hRes = dH;
             a.vRes = dV;
-            a.hOff = static_cast<const int64_t *>(ctx->hOff.p);
-            a.vOff = static_cast<const int64_t *>(ctx->vOff.p);
-            a.order = static_cast<const int32_t *>(ctx->order.p);
-            a.items = static_cast<const WorkItem *>(ctx->items.p) + k;
+            a.hOff = static_cast<const int64_t *>(ctx->b.hOff.p);
+            a.vOff = static_cast<const int64_t *>(ctx->b.vOff.p);
+            a.order = static_cast<const int32_t *>(ctx->b.order.p);
+            a.items = static_cast<const WorkItem *>(ctx->b.items.p) + k;
             a.nItems = 1;
-            a.queue = static_cast<uint32_t *>(ctx->queue.p) + k;
-            a.rowbuf = static_cast<uint32_t *>(ctx->rowbuf.p);
+            a.queue = static_cast<uint32_t *>(ctx->b.queue.p) + k;
+            a.rowbuf = static_cast<uint32_t *>(ctx->b.rowbuf.p);
             a.rowCols = maxN + 18;
             a.scores = dScores;
             a.gapOpen = si.o;
             a.gapExtend = si.e;
             a.mode = mode;
             a.alpha = A;
-            a.tTable = static_cast<const int16_t *>(ctx->tTable.p);
-            a.dnaProfile = static_cast<const uint32_t *>(ctx->dnaProf.p);
+            a.tTable = static_cast<const int16_t *>(ctx->b.tTable.p);
+            a.dnaProfile = static_cast<const uint32_t *>(ctx->b.dnaProf.p);
             if ((err = launchFwd16(v, a, 1, stream, 64)) != hipSuccess)
                 return hipFail(ctx, err, "launching plain kernel");
         }
@@ -999,95 +1110,112 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
     }
 
     // ---- packed int16 kernels -------------------------------------------
-    int64_t padded = 0, nItems = 0;
-    for (Plan16 &pl : plans) {
-        const Variant16 &v = pl.var;
-        if (!variantSupported(v))
-            return fail(ctx, SQMI_ERR_UNSUPPORTED, "kernel variant not compiled");
-        const int GROUPS = 64 / v.G;
+    // One launch per shape in use, in order on the stream.  Each plan has its
+    // own slices of the work lists and its own queue counters (cleared by one
+    // fill kernel), so the uploads of a later plan never touch what an
+    // earlier launch still reads; the launches share the row buffers, which a
+    // launch only uses while it runs.
+    if (!plans.empty()) {
         const int wpb = wavesPerBlock16();
-        const int occ = occupancy16(v);
-        const int64_t nIt = (int64_t)pl.items.size();
-        int grid = (int)std::min<int64_t>((int64_t)ctx->numCUs * occ, (nIt + wpb - 1) / wpb);
-        grid = std::max(grid, 1);
-        const int rowCols = pl.maxN + v.G + 2;
-        const size_t rbBytes = (size_t)grid * wpb * GROUPS * rowCols * 2 * sizeof(uint32_t);
-        if ((err = ensure(ctx->order, 4 * pl.order.size())) != hipSuccess ||
-            (err = ensure(ctx->items, sizeof(WorkItem) * pl.items.size())) != hipSuccess ||
-            (err = ensure(ctx->queue, 64)) != hipSuccess || (err = ensure(ctx->rowbuf, rbBytes)) != hipSuccess)
-            return hipFail(ctx, err, "allocating work lists");
-        if ((err = hipMemcpyAsync(ctx->order.p, pl.order.data(), 4 * pl.order.size(), hipMemcpyHostToDevice,
-                                  stream)) != hipSuccess ||
-            (err = hipMemcpyAsync(ctx->items.p, pl.items.data(), sizeof(WorkItem) * pl.items.size(),
-                                  hipMemcpyHostToDevice, stream)) != hipSuccess ||
-            (err = hipMemsetAsync(ctx->queue.p, 0, 64, stream)) != hipSuccess)
-            return hipFail(ctx, err, "uploading work lists");
-        FwdArgs a{};
-        a.hRes = dH;
-        a.vRes = dV;
-        a.hOff = static_cast<const int64_t *>(ctx->hOff.p);
-        a.vOff = static_cast<const int64_t *>(ctx->vOff.p);
-        a.order = static_cast<const int32_t *>(ctx->order.p);
-        a.items = static_cast<const WorkItem *>(ctx->items.p);
-        a.nItems = (int32_t)nIt;
-        a.queue = static_cast<uint32_t *>(ctx->queue.p);
-        // Items are ordered by row blocks, most first: the leading ones with
-        // at least coopMinBlocks() blocks are shared by a workgroup's waves.
-        {
-            const int minBlocks = coopMinBlocks();
-            int32_t nLong = 0;
-            while (minBlocks > 0 && nLong < (int32_t)nIt && pl.items[nLong].nblocks >= minBlocks)
-                ++nLong;
-            a.nLong = nLong;
-            a.queueLong = static_cast<uint32_t *>(ctx->queue.p) + 1;
+        std::vector<size_t> orderOff(plans.size() + 1, 0), itemOff(plans.size() + 1, 0);
+        std::vector<int> grids(plans.size()), rowColsOf(plans.size());
+        size_t rbMax = 0;
+        for (size_t k = 0; k < plans.size(); ++k) {
+            const Plan16 &pl = plans[k];
+            const Variant16 &v = pl.var;
+            if (!shapeSupported(v))
+                return fail(ctx, SQMI_ERR_UNSUPPORTED, "kernel variant not compiled");
+            const int occ = v.G == 1 ? occupancyLane16(v) : occupancy16(v);
+            const int64_t nIt = (int64_t)pl.items.size();
+            grids[k] = std::max(1, (int)std::min<int64_t>((int64_t)ctx->numCUs * occ, (nIt + wpb - 1) / wpb));
+            // Row buffers per wave slot: G = 16, one row of (G, F) per group;
+            // G = 1, one per lane, column-major across the 64 lanes.
+            rowColsOf[k] = v.G == 1 ? laneRowCols(pl.maxN, v.R) : pl.maxN + v.G + 2;
+            const size_t rb = (size_t)grids[k] * wpb * 64 / (v.G == 1 ? 1 : v.G) * rowColsOf[k] * 8;
+            rbMax = std::max(rbMax, rb);
+            orderOff[k + 1] = orderOff[k] + pl.order.size();
+            itemOff[k + 1] = itemOff[k] + pl.items.size();
         }
-        a.rowbuf = static_cast<uint32_t *>(ctx->rowbuf.p);
-        a.rowCols = rowCols;
-        a.scores = dScores;
-        a.gapOpen = si.o;
-        a.gapExtend = si.e;
-        a.mode = mode;
-        a.alpha = A;
-        a.tTable = static_cast<const int16_t *>(ctx->tTable.p);
-        a.dnaProfile = static_cast<const uint32_t *>(ctx->dnaProf.p);
-        if ((err = launchFwd16(v, a, grid, stream)) != hipSuccess)
-            return hipFail(ctx, err, "launching packed kernel");
-        for (const WorkItem &w : pl.items)
-            padded += 2LL * (64 / v.G) * (int64_t)w.maxN * w.nblocks * v.G * v.R;
-        nItems += nIt;
-        // The work lists are reused by the next variant: wait for this one.
-        if (plans.size() > 1 && (err = hipStreamSynchronize(stream)) != hipSuccess)
-            return hipFail(ctx, err, "packed kernel");
+        const size_t nq = 16 * plans.size();   // queue counters: 16 dwords per plan
+        if ((err = ensure(ctx->b.order, 4 * orderOff.back())) != hipSuccess ||
+            (err = ensure(ctx->b.items, sizeof(WorkItem) * itemOff.back())) != hipSuccess ||
+            (err = ensure(ctx->b.queue, 4 * nq)) != hipSuccess || (err = ensure(ctx->b.rowbuf, rbMax)) != hipSuccess)
+            return hipFail(ctx, err, "allocating work lists");
+        if ((err = launchZero32(static_cast<uint32_t *>(ctx->b.queue.p), nq, stream)) != hipSuccess)
+            return hipFail(ctx, err, "clearing work-queue counters");
+        for (size_t k = 0; k < plans.size(); ++k) {
+            const Plan16 &pl = plans[k];
+            const Variant16 &v = pl.var;
+            int32_t *dOrder = static_cast<int32_t *>(ctx->b.order.p) + orderOff[k];
+            WorkItem *dItems = static_cast<WorkItem *>(ctx->b.items.p) + itemOff[k];
+            uint32_t *dQueue = static_cast<uint32_t *>(ctx->b.queue.p) + 16 * k;
+            if ((err = hipMemcpyAsync(dOrder, pl.order.data(), 4 * pl.order.size(), hipMemcpyHostToDevice,
+                                      stream)) != hipSuccess ||
+                (err = hipMemcpyAsync(dItems, pl.items.data(), sizeof(WorkItem) * pl.items.size(),
+                                      hipMemcpyHostToDevice, stream)) != hipSuccess)
+                return hipFail(ctx, err, "uploading work lists");
+            const int64_t nIt = (int64_t)pl.items.size();
+            FwdArgs a{};
+            a.hRes = dH;
+            a.vRes = dV;
+            a.hOff = static_cast<const int64_t *>(ctx->b.hOff.p);
+            a.vOff = static_cast<const int64_t *>(ctx->b.vOff.p);
+            a.order = dOrder;
+            a.items = dItems;
+            a.nItems = (int32_t)nIt;
+            a.queue = dQueue;
+            // Items are ordered by row blocks, most first: the leading ones with
+            // at least coopMinBlocks() blocks are shared by a workgroup's waves
+            // (groups of 16 lanes only).
+            {
+                const int minBlocks = v.G == 1 ? 0 : coopMinBlocks();
+                int32_t nLong = 0;
+                while (minBlocks > 0 && nLong < (int32_t)nIt && pl.items[nLong].nblocks >= minBlocks)
+                    ++nLong;
+                a.nLong = nLong;
+                a.queueLong = dQueue + 1;
+            }
+            a.rowbuf = static_cast<uint32_t *>(ctx->b.rowbuf.p);
+            a.rowCols = rowColsOf[k];
+            a.scores = dScores;
+            a.gapOpen = si.o;
+            a.gapExtend = si.e;
+            a.mode = mode;
+            a.alpha = A;
+            a.tTable = static_cast<const int16_t *>(ctx->b.tTable.p);
+            a.dnaProfile = static_cast<const uint32_t *>(ctx->b.dnaProf.p);
+            if ((err = v.G == 1 ? launchLane16(v, a, grids[k], stream) : launchFwd16(v, a, grids[k], stream)) !=
+                hipSuccess)
+                return hipFail(ctx, err, "launching packed kernel");
+        }
     }
-    (void)padded;
-    (void)nItems;
 
     // ---- int32 pairs -------------------------------------------------------
     if (!ids32.empty()) {
         const size_t n32 = ids32.size();
-        if ((err = ensure(ctx->ids32, 4 * n32)) != hipSuccess ||
-            (err = ensure(ctx->scratchOff32, 8 * n32)) != hipSuccess ||
-            (err = ensure(ctx->scratch32, (size_t)8 * scratch32)) != hipSuccess)
+        if ((err = ensure(ctx->b.ids32, 4 * n32)) != hipSuccess ||
+            (err = ensure(ctx->b.scratchOff32, 8 * n32)) != hipSuccess ||
+            (err = ensure(ctx->b.scratch32, (size_t)8 * scratch32)) != hipSuccess)
             return hipFail(ctx, err, "allocating int32 lists");
-        if ((err = hipMemcpyAsync(ctx->ids32.p, ids32.data(), 4 * n32, hipMemcpyHostToDevice, stream)) != hipSuccess ||
-            (err = hipMemcpyAsync(ctx->scratchOff32.p, scratchOff32.data(), 8 * n32, hipMemcpyHostToDevice, stream)) !=
+        if ((err = hipMemcpyAsync(ctx->b.ids32.p, ids32.data(), 4 * n32, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+            (err = hipMemcpyAsync(ctx->b.scratchOff32.p, scratchOff32.data(), 8 * n32, hipMemcpyHostToDevice, stream)) !=
                 hipSuccess)
             return hipFail(ctx, err, "uploading int32 lists");
         Simple32Args a{};
         a.hRes = dH;
         a.vRes = dV;
-        a.hOff = static_cast<const int64_t *>(ctx->hOff.p);
-        a.vOff = static_cast<const int64_t *>(ctx->vOff.p);
-        a.ids = static_cast<const int32_t *>(ctx->ids32.p);
-        a.scratchOff = static_cast<const int64_t *>(ctx->scratchOff32.p);
+        a.hOff = static_cast<const int64_t *>(ctx->b.hOff.p);
+        a.vOff = static_cast<const int64_t *>(ctx->b.vOff.p);
+        a.ids = static_cast<const int32_t *>(ctx->b.ids32.p);
+        a.scratchOff = static_cast<const int64_t *>(ctx->b.scratchOff32.p);
         a.nIds = (int32_t)n32;
-        a.scratch = static_cast<int32_t *>(ctx->scratch32.p);
+        a.scratch = static_cast<int32_t *>(ctx->b.scratch32.p);
         a.scores = dScores;
         a.gapOpen = si.o;
         a.gapExtend = si.e;
         a.mode = mode;
         a.alpha = A;
-        a.table = static_cast<const int32_t *>(ctx->table32.p);
+        a.table = static_cast<const int32_t *>(ctx->b.table32.p);
         if ((err = launchSimple32(a, stream)) != hipSuccess)
             return hipFail(ctx, err, "launching int32 kernel");
     }
@@ -1095,13 +1223,13 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
     if ((st = runLong(ctx, setH, setV, P, in, flags, nullptr, RowsDev{}, longPairs)) != SQMI_OK)
         return st;
     if (!emptyIds.empty()) {
-        if ((err = ensure(ctx->emptyIds, 4 * emptyIds.size())) != hipSuccess)
+        if ((err = ensure(ctx->b.emptyIds, 4 * emptyIds.size())) != hipSuccess)
             return hipFail(ctx, err, "allocating empty list");
-        if ((err = hipMemcpyAsync(ctx->emptyIds.p, emptyIds.data(), 4 * emptyIds.size(), hipMemcpyHostToDevice,
+        if ((err = hipMemcpyAsync(ctx->b.emptyIds.p, emptyIds.data(), 4 * emptyIds.size(), hipMemcpyHostToDevice,
                                   stream)) != hipSuccess)
             return hipFail(ctx, err, "uploading empty list");
         const int ne = (int)emptyIds.size();
-        if ((err = launchFillScores(dScores, static_cast<const int32_t *>(ctx->emptyIds.p), ne, INT32_MIN,
+        if ((err = launchFillScores(dScores, static_cast<const int32_t *>(ctx->b.emptyIds.p), ne, INT32_MIN,
                                     stream)) != hipSuccess)
             return hipFail(ctx, err, "launching fill kernel");
     }
@@ -1248,9 +1376,9 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
         return st;
     int32_t *dCounts = counts;
     if (!(flags & SQMI_SCORES_ON_DEVICE)) {
-        if ((err = ensure(ctx->counts, (size_t)4 * S * N)) != hipSuccess)
+        if ((err = ensure(ctx->b.counts, (size_t)4 * S * N)) != hipSuccess)
             return hipFail(ctx, err, "allocating counts");
-        dCounts = static_cast<int32_t *>(ctx->counts.p);
+        dCounts = static_cast<int32_t *>(ctx->b.counts.p);
     }
     if ((err = hipMemsetAsync(dCounts, 0, (size_t)4 * S * N, stream)) != hipSuccess)
         return hipFail(ctx, err, "clearing counts");
@@ -1259,20 +1387,20 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
         // Offsets always come from the host; ops and lengths live where the
         // scores do.
         const int64_t opBytes = opOffsets[N];
-        if ((err = ensure(ctx->opOff, 8 * (N + 1))) != hipSuccess)
+        if ((err = ensure(ctx->b.opOff, 8 * (N + 1))) != hipSuccess)
             return hipFail(ctx, err, "allocating op offsets");
-        if ((err = hipMemcpyAsync(ctx->opOff.p, opOffsets, 8 * (N + 1), hipMemcpyHostToDevice, stream)) != hipSuccess)
+        if ((err = hipMemcpyAsync(ctx->b.opOff.p, opOffsets, 8 * (N + 1), hipMemcpyHostToDevice, stream)) != hipSuccess)
             return hipFail(ctx, err, "uploading op offsets");
-        rows.opOff = static_cast<const int64_t *>(ctx->opOff.p);
+        rows.opOff = static_cast<const int64_t *>(ctx->b.opOff.p);
         if (flags & SQMI_SCORES_ON_DEVICE) {
             rows.ops = ops;
             rows.opLen = opLengths;
         } else {
-            if ((err = ensure(ctx->ops, (size_t)std::max<int64_t>(opBytes, 1))) != hipSuccess ||
-                (err = ensure(ctx->opLen, 8 * N)) != hipSuccess)
+            if ((err = ensure(ctx->b.ops, (size_t)std::max<int64_t>(opBytes, 1))) != hipSuccess ||
+                (err = ensure(ctx->b.opLen, 8 * N)) != hipSuccess)
                 return hipFail(ctx, err, "allocating rows");
-            rows.ops = static_cast<uint8_t *>(ctx->ops.p);
-            rows.opLen = static_cast<int64_t *>(ctx->opLen.p);
+            rows.ops = static_cast<uint8_t *>(ctx->b.ops.p);
+            rows.opLen = static_cast<int64_t *>(ctx->b.opLen.p);
         }
         if ((err = hipMemsetAsync(rows.opLen, 0, 8 * N, stream)) != hipSuccess)
             return hipFail(ctx, err, "clearing op lengths");
@@ -1280,9 +1408,9 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             if (flags & SQMI_SCORES_ON_DEVICE) {
                 rows.begins = begins;
             } else {
-                if ((err = ensure(ctx->begins, 16 * N)) != hipSuccess)
+                if ((err = ensure(ctx->b.begins, 16 * N)) != hipSuccess)
                     return hipFail(ctx, err, "allocating begins");
-                rows.begins = static_cast<int64_t *>(ctx->begins.p);
+                rows.begins = static_cast<int64_t *>(ctx->b.begins.p);
             }
             if ((err = hipMemsetAsync(rows.begins, 0, 16 * N, stream)) != hipSuccess)
                 return hipFail(ctx, err, "clearing begins");
@@ -1301,53 +1429,53 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             maxChunkItems = std::max<int64_t>(maxChunkItems, (int64_t)(chunkEnd[c] - b));
         const int grid = std::max(1, (int)std::min<int64_t>((int64_t)ctx->numCUs * occ, (maxChunkItems + wpb - 1) / wpb));
         const size_t rbBytes = (size_t)grid * wpb * (64 / kTraceLanes) * rowCols * 2 * sizeof(uint32_t);
-        if ((err = ensure(ctx->order, 4 * np)) != hipSuccess ||
-            (err = ensure(ctx->items, sizeof(WorkItem) * nIt)) != hipSuccess ||
-            (err = ensure(ctx->traceOff, 8 * nIt)) != hipSuccess ||
-            (err = ensure(ctx->locs, sizeof(TraceLoc) * np)) != hipSuccess ||
-            (err = ensure(ctx->queue, 4 * chunkEnd.size() + 64)) != hipSuccess ||
-            (err = ensure(ctx->rowbuf, rbBytes)) != hipSuccess ||
-            (err = ensure(ctx->endEF, (size_t)8 * N)) != hipSuccess ||
-            (P.track != kTrackEnd && (err = ensure(ctx->endIJ, (size_t)8 * N)) != hipSuccess) ||
-            (err = ensure(ctx->trace, (size_t)traceRecs * 64 * 16)) != hipSuccess)
+        if ((err = ensure(ctx->b.order, 4 * np)) != hipSuccess ||
+            (err = ensure(ctx->b.items, sizeof(WorkItem) * nIt)) != hipSuccess ||
+            (err = ensure(ctx->b.traceOff, 8 * nIt)) != hipSuccess ||
+            (err = ensure(ctx->b.locs, sizeof(TraceLoc) * np)) != hipSuccess ||
+            (err = ensure(ctx->b.queue, 4 * chunkEnd.size() + 64)) != hipSuccess ||
+            (err = ensure(ctx->b.rowbuf, rbBytes)) != hipSuccess ||
+            (err = ensure(ctx->b.endEF, (size_t)8 * N)) != hipSuccess ||
+            (P.track != kTrackEnd && (err = ensure(ctx->b.endIJ, (size_t)8 * N)) != hipSuccess) ||
+            (err = ensure(ctx->b.trace, (size_t)traceRecs * 64 * 16)) != hipSuccess)
             return hipFail(ctx, err, "allocating traceback buffers");
-        int32_t *dEndIJ = P.track != kTrackEnd ? static_cast<int32_t *>(ctx->endIJ.p) : nullptr;
-        if ((err = hipMemcpyAsync(ctx->order.p, pl->order.data(), 4 * np, hipMemcpyHostToDevice, stream)) !=
+        int32_t *dEndIJ = P.track != kTrackEnd ? static_cast<int32_t *>(ctx->b.endIJ.p) : nullptr;
+        if ((err = hipMemcpyAsync(ctx->b.order.p, pl->order.data(), 4 * np, hipMemcpyHostToDevice, stream)) !=
                 hipSuccess ||
-            (err = hipMemcpyAsync(ctx->items.p, pl->items.data(), sizeof(WorkItem) * nIt, hipMemcpyHostToDevice,
+            (err = hipMemcpyAsync(ctx->b.items.p, pl->items.data(), sizeof(WorkItem) * nIt, hipMemcpyHostToDevice,
                                   stream)) != hipSuccess ||
-            (err = hipMemcpyAsync(ctx->traceOff.p, itemOff.data(), 8 * nIt, hipMemcpyHostToDevice, stream)) !=
+            (err = hipMemcpyAsync(ctx->b.traceOff.p, itemOff.data(), 8 * nIt, hipMemcpyHostToDevice, stream)) !=
                 hipSuccess ||
-            (err = hipMemcpyAsync(ctx->locs.p, locs.data(), sizeof(TraceLoc) * np, hipMemcpyHostToDevice, stream)) !=
+            (err = hipMemcpyAsync(ctx->b.locs.p, locs.data(), sizeof(TraceLoc) * np, hipMemcpyHostToDevice, stream)) !=
                 hipSuccess ||
-            (err = hipMemsetAsync(ctx->queue.p, 0, 4 * chunkEnd.size() + 64, stream)) != hipSuccess)
+            (err = hipMemsetAsync(ctx->b.queue.p, 0, 4 * chunkEnd.size() + 64, stream)) != hipSuccess)
             return hipFail(ctx, err, "uploading traceback work lists");
-        const WorkItem *dItems = static_cast<const WorkItem *>(ctx->items.p);
-        const int32_t *dOrder = static_cast<const int32_t *>(ctx->order.p);
+        const WorkItem *dItems = static_cast<const WorkItem *>(ctx->b.items.p);
+        const int32_t *dOrder = static_cast<const int32_t *>(ctx->b.order.p);
         size_t b = 0;
         for (size_t c = 0; c < chunkEnd.size(); ++c) {
             const size_t e = chunkEnd[c];
             FwdArgs a{};
             a.hRes = in.dH;
             a.vRes = in.dV;
-            a.hOff = static_cast<const int64_t *>(ctx->hOff.p);
-            a.vOff = static_cast<const int64_t *>(ctx->vOff.p);
+            a.hOff = static_cast<const int64_t *>(ctx->b.hOff.p);
+            a.vOff = static_cast<const int64_t *>(ctx->b.vOff.p);
             a.order = dOrder;
             a.items = dItems + b;
             a.nItems = (int32_t)(e - b);
-            a.queue = static_cast<uint32_t *>(ctx->queue.p) + c;
-            a.rowbuf = static_cast<uint32_t *>(ctx->rowbuf.p);
+            a.queue = static_cast<uint32_t *>(ctx->b.queue.p) + c;
+            a.rowbuf = static_cast<uint32_t *>(ctx->b.rowbuf.p);
             a.rowCols = rowCols;
             a.scores = in.dScores;
             a.gapOpen = si.o;
             a.gapExtend = si.e;
             a.mode = mode;
             a.alpha = A;
-            a.tTable = static_cast<const int16_t *>(ctx->tTable.p);
-            a.dnaProfile = static_cast<const uint32_t *>(ctx->dnaProf.p);
-            a.trace = static_cast<uint4 *>(ctx->trace.p);
-            a.traceOff = static_cast<const int64_t *>(ctx->traceOff.p) + b;
-            a.endEF = static_cast<int32_t *>(ctx->endEF.p);
+            a.tTable = static_cast<const int16_t *>(ctx->b.tTable.p);
+            a.dnaProfile = static_cast<const uint32_t *>(ctx->b.dnaProf.p);
+            a.trace = static_cast<uint4 *>(ctx->b.trace.p);
+            a.traceOff = static_cast<const int64_t *>(ctx->b.traceOff.p) + b;
+            a.endEF = static_cast<int32_t *>(ctx->b.endEF.p);
             a.endIJ = dEndIJ;
             const int cg = std::max(1, std::min(grid, (int)((e - b + wpb - 1) / wpb)));
             if ((err = launchFwdTrace16(si.dna, P.track, a, cg, stream)) != hipSuccess)
@@ -1360,7 +1488,7 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             wa.hOff = a.hOff;
             wa.vOff = a.vOff;
             wa.ids = dOrder + p0;
-            wa.locs = static_cast<const TraceLoc *>(ctx->locs.p) + p0;
+            wa.locs = static_cast<const TraceLoc *>(ctx->b.locs.p) + p0;
             wa.nPairs = p1 - p0;
             wa.trace = a.trace;
             wa.scores = in.dScores;
@@ -1370,7 +1498,7 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             wa.linear = si.o == si.e;
             wa.endIJ = dEndIJ;
             wa.local = P.local ? 1 : 0;
-            wa.table = static_cast<const int32_t *>(ctx->table32.p);
+            wa.table = static_cast<const int32_t *>(ctx->b.table32.p);
             wa.gapOpen = si.o;
             wa.gapExtend = si.e;
             wa.begins = rows.begins;
@@ -1411,33 +1539,33 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
     }
     if (bandLanes > 0) {
         const size_t n32 = P.ids32.size();
-        if ((err = ensure(ctx->ids32, 4 * n32)) != hipSuccess || (err = ensure(ctx->scratchOff32, 8 * n32)) != hipSuccess ||
-            (err = ensure(ctx->scratch32, (size_t)std::max<int64_t>(bandBytes, 1))) != hipSuccess ||
-            (err = ensure(ctx->endEF, (size_t)8 * N)) != hipSuccess || (err = ensure(ctx->endIJ, (size_t)8 * N)) != hipSuccess)
+        if ((err = ensure(ctx->b.ids32, 4 * n32)) != hipSuccess || (err = ensure(ctx->b.scratchOff32, 8 * n32)) != hipSuccess ||
+            (err = ensure(ctx->b.scratch32, (size_t)std::max<int64_t>(bandBytes, 1))) != hipSuccess ||
+            (err = ensure(ctx->b.endEF, (size_t)8 * N)) != hipSuccess || (err = ensure(ctx->b.endIJ, (size_t)8 * N)) != hipSuccess)
             return hipFail(ctx, err, "allocating banded traceback buffers");
-        if ((err = hipMemcpyAsync(ctx->ids32.p, P.ids32.data(), 4 * n32, hipMemcpyHostToDevice, stream)) != hipSuccess ||
-            (err = hipMemcpyAsync(ctx->scratchOff32.p, bandOff.data(), 8 * n32, hipMemcpyHostToDevice, stream)) !=
+        if ((err = hipMemcpyAsync(ctx->b.ids32.p, P.ids32.data(), 4 * n32, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+            (err = hipMemcpyAsync(ctx->b.scratchOff32.p, bandOff.data(), 8 * n32, hipMemcpyHostToDevice, stream)) !=
                 hipSuccess)
             return hipFail(ctx, err, "uploading banded traceback lists");
         BandedArgs ba{};
         ba.hRes = in.dH;
         ba.vRes = in.dV;
-        ba.hOff = static_cast<const int64_t *>(ctx->hOff.p);
-        ba.vOff = static_cast<const int64_t *>(ctx->vOff.p);
-        ba.ids = static_cast<const int32_t *>(ctx->ids32.p);
+        ba.hOff = static_cast<const int64_t *>(ctx->b.hOff.p);
+        ba.vOff = static_cast<const int64_t *>(ctx->b.vOff.p);
+        ba.ids = static_cast<const int32_t *>(ctx->b.ids32.p);
         ba.nIds = (int32_t)n32;
         ba.scores = in.dScores;
         ba.gapOpen = si.o;
         ba.gapExtend = si.e;
         ba.mode = mode;
         ba.alpha = A;
-        ba.table = static_cast<const int32_t *>(ctx->table32.p);
+        ba.table = static_cast<const int32_t *>(ctx->b.table32.p);
         ba.lower = band[0];
         ba.upper = band[1];
-        ba.trace = static_cast<uint8_t *>(ctx->scratch32.p);
-        ba.traceOff = static_cast<const int64_t *>(ctx->scratchOff32.p);
-        ba.endIJ = static_cast<int32_t *>(ctx->endIJ.p);
-        ba.endEF = static_cast<int32_t *>(ctx->endEF.p);
+        ba.trace = static_cast<uint8_t *>(ctx->b.scratch32.p);
+        ba.traceOff = static_cast<const int64_t *>(ctx->b.scratchOff32.p);
+        ba.endIJ = static_cast<int32_t *>(ctx->b.endIJ.p);
+        ba.endEF = static_cast<int32_t *>(ctx->b.endEF.p);
         if ((err = launchBandedTrace(ba, bandLanes, stream)) != hipSuccess)
             return hipFail(ctx, err, "launching banded trace kernel");
         BandedWalkArgs wa{};
@@ -1466,11 +1594,11 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             return hipFail(ctx, err, "launching banded walk kernel");
     } else if (!P.ids32.empty()) {
         const size_t n32 = P.ids32.size();
-        if ((err = ensure(ctx->ids32, 4 * n32)) != hipSuccess || (err = ensure(ctx->scratchOff32, 8 * n32)) != hipSuccess ||
-            (err = ensure(ctx->scratch32, (size_t)scratch32)) != hipSuccess)
+        if ((err = ensure(ctx->b.ids32, 4 * n32)) != hipSuccess || (err = ensure(ctx->b.scratchOff32, 8 * n32)) != hipSuccess ||
+            (err = ensure(ctx->b.scratch32, (size_t)scratch32)) != hipSuccess)
             return hipFail(ctx, err, "allocating int32 traceback buffers");
-        if ((err = hipMemcpyAsync(ctx->ids32.p, P.ids32.data(), 4 * n32, hipMemcpyHostToDevice, stream)) != hipSuccess ||
-            (err = hipMemcpyAsync(ctx->scratchOff32.p, off32.data(), 8 * n32, hipMemcpyHostToDevice, stream)) !=
+        if ((err = hipMemcpyAsync(ctx->b.ids32.p, P.ids32.data(), 4 * n32, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+            (err = hipMemcpyAsync(ctx->b.scratchOff32.p, off32.data(), 8 * n32, hipMemcpyHostToDevice, stream)) !=
                 hipSuccess)
             return hipFail(ctx, err, "uploading int32 traceback lists");
         size_t b = 0;
@@ -1478,19 +1606,19 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             Counts32Args a{};
             a.hRes = in.dH;
             a.vRes = in.dV;
-            a.hOff = static_cast<const int64_t *>(ctx->hOff.p);
-            a.vOff = static_cast<const int64_t *>(ctx->vOff.p);
-            a.ids = static_cast<const int32_t *>(ctx->ids32.p) + b;
-            a.scratchOff = static_cast<const int64_t *>(ctx->scratchOff32.p) + b;
+            a.hOff = static_cast<const int64_t *>(ctx->b.hOff.p);
+            a.vOff = static_cast<const int64_t *>(ctx->b.vOff.p);
+            a.ids = static_cast<const int32_t *>(ctx->b.ids32.p) + b;
+            a.scratchOff = static_cast<const int64_t *>(ctx->b.scratchOff32.p) + b;
             a.nIds = (int32_t)(e - b);
-            a.scratch = static_cast<uint8_t *>(ctx->scratch32.p);
+            a.scratch = static_cast<uint8_t *>(ctx->b.scratch32.p);
             a.scores = in.dScores;
             a.counts = dCounts;
             a.gapOpen = si.o;
             a.gapExtend = si.e;
             a.mode = mode;
             a.alpha = A;
-            a.table = static_cast<const int32_t *>(ctx->table32.p);
+            a.table = static_cast<const int32_t *>(ctx->b.table32.p);
             a.ops = rows.ops;
             a.opOff = rows.opOff;
             a.opLen = rows.opLen;
@@ -1509,12 +1637,12 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
     if ((st = runLong(ctx, setH, setV, P, in, flags, dCounts, rows, longPairs)) != SQMI_OK)
         return st;
     if (!P.emptyIds.empty()) {
-        if ((err = ensure(ctx->emptyIds, 4 * P.emptyIds.size())) != hipSuccess)
+        if ((err = ensure(ctx->b.emptyIds, 4 * P.emptyIds.size())) != hipSuccess)
             return hipFail(ctx, err, "allocating empty list");
-        if ((err = hipMemcpyAsync(ctx->emptyIds.p, P.emptyIds.data(), 4 * P.emptyIds.size(), hipMemcpyHostToDevice,
+        if ((err = hipMemcpyAsync(ctx->b.emptyIds.p, P.emptyIds.data(), 4 * P.emptyIds.size(), hipMemcpyHostToDevice,
                                   stream)) != hipSuccess)
             return hipFail(ctx, err, "uploading empty list");
-        if ((err = launchFillScores(in.dScores, static_cast<const int32_t *>(ctx->emptyIds.p), (int)P.emptyIds.size(),
+        if ((err = launchFillScores(in.dScores, static_cast<const int32_t *>(ctx->b.emptyIds.p), (int)P.emptyIds.size(),
                                     INT32_MIN, stream)) != hipSuccess)
             return hipFail(ctx, err, "launching fill kernel");
     }
@@ -1654,13 +1782,13 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
     const int wideStride = (int)((wWide + 2 + 63) & ~(int64_t)63);
     const int wideGrid = wide.empty() ? 0 : (int)std::min<int64_t>((int64_t)wide.size(), (int64_t)ctx->numCUs * 16);
     const size_t nIds = narrow.size() + wide.size() + rejected.size();
-    if ((err = ensure(ctx->bandIds, 4 * std::max<size_t>(nIds, 1))) != hipSuccess ||
-        (wideGrid && (err = ensure(ctx->bandScratch, (size_t)wideGrid * 4 * wideStride * 4)) != hipSuccess))
+    if ((err = ensure(ctx->b.bandIds, 4 * std::max<size_t>(nIds, 1))) != hipSuccess ||
+        (wideGrid && (err = ensure(ctx->b.bandScratch, (size_t)wideGrid * 4 * wideStride * 4)) != hipSuccess))
         return hipFail(ctx, err, "allocating banded buffers");
     std::vector<int32_t> all(narrow);
     all.insert(all.end(), wide.begin(), wide.end());
     all.insert(all.end(), rejected.begin(), rejected.end());
-    int32_t *dIds = static_cast<int32_t *>(ctx->bandIds.p);
+    int32_t *dIds = static_cast<int32_t *>(ctx->b.bandIds.p);
     if (!all.empty() &&
         (err = hipMemcpyAsync(dIds, all.data(), 4 * all.size(), hipMemcpyHostToDevice, stream)) != hipSuccess)
         return hipFail(ctx, err, "uploading banded pair lists");
@@ -1669,17 +1797,17 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
     BandedArgs a{};
     a.hRes = in.dH;
     a.vRes = in.dV;
-    a.hOff = static_cast<const int64_t *>(ctx->hOff.p);
-    a.vOff = static_cast<const int64_t *>(ctx->vOff.p);
+    a.hOff = static_cast<const int64_t *>(ctx->b.hOff.p);
+    a.vOff = static_cast<const int64_t *>(ctx->b.vOff.p);
     a.scores = in.dScores;
     a.gapOpen = si.o;
     a.gapExtend = si.e;
     a.mode = mode;
     a.alpha = si.alpha;
-    a.table = static_cast<const int32_t *>(ctx->table32.p);
+    a.table = static_cast<const int32_t *>(ctx->b.table32.p);
     a.lower = lower_diag;
     a.upper = upper_diag;
-    a.scratch = static_cast<int32_t *>(ctx->bandScratch.p);
+    a.scratch = static_cast<int32_t *>(ctx->b.bandScratch.p);
     a.wideStride = wideStride;
     if (!narrow.empty()) {
         a.ids = dIds;

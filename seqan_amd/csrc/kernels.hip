@@ -22,41 +22,21 @@
 // (a "group") sweep one pair-couple; each lane owns R consecutive rows of a
 // block of G*R rows and the group walks the columns as a skewed wavefront:
 // at step s lane l computes column j = s - l, receiving the bottom cell of
-// lane l-1 (column j, one step old) through a lane shift (DPP row_shr:1 for
-// G = 16, ds_bpermute for G = 64).  The first lane of the group reads the
+// lane l-1 (column j, one step old) through a lane shift (DPP row_shr:1,
+// G = 16).  The first lane of the group reads the
 // bottom row of the previous block from a global row buffer that the last
 // lane writes.  Rows are bottom-aligned: row m of every pair is the last row
 // of the last block, so the rows above a pair's row 0 are "virtual" (floored
 // at the -inf sentinel) and the border row emerges from the recurrence
 // itself (TOP free: floored at 0).
+#include <algorithm>
+
 #include "kernels.hpp"
+#include "packed16.hpp"
 #include "seqan_amd.h"
 
 namespace sqmi {
 namespace {
-
-typedef short s16x2 __attribute__((ext_vector_type(2)));
-typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ s16x2 as16(uint32_t x) { return __builtin_bit_cast(s16x2, x); }
-__device__ __forceinline__ uint32_t asu(s16x2 x) { return __builtin_bit_cast(uint32_t, x); }
-__device__ __forceinline__ s16x2 vmax(s16x2 a, s16x2 b) { return __builtin_elementwise_max(a, b); }
-__device__ __forceinline__ s16x2 splat(int x)
-{
-    s16x2 r = {(short)x, (short)x};
-    return r;
-}
-__device__ __forceinline__ s16x2 pack(int lo, int hi)
-{
-    s16x2 r = {(short)lo, (short)hi};
-    return r;
-}
-__device__ __forceinline__ uint32_t bfi(uint32_t mask, uint32_t a, uint32_t b) { return (mask & a) | (~mask & b); }
-// Saturating a - b per half: its sign bit is exactly (a < b), whatever the
-// magnitudes (v_pk_sub_i16 with clamp).
-__device__ __forceinline__ s16x2 subSat(s16x2 a, s16x2 b) { return __builtin_elementwise_sub_sat(a, b); }
-// 0xffff in the halves where a > b.
-__device__ __forceinline__ uint32_t gtMask(s16x2 a, s16x2 b) { return asu(subSat(b, a) >> splat(15)); }
 
 // Free-end traceback (trace kernels with bottom / right tracking): the first
 // strict maximum of the bottom row in column order and of the last column in
@@ -73,12 +53,6 @@ struct EndTrack {
     int rRow[2];        // its item row
 };
 
-// 0xffff in the halves where a == b.
-__device__ __forceinline__ uint32_t eqMask(s16x2 a, s16x2 b)
-{
-    return ((uint16_t)a.x == (uint16_t)b.x ? 0xffffu : 0u) | ((uint16_t)a.y == (uint16_t)b.y ? 0xffff0000u : 0u);
-}
-
 // Four sign bytes from two packed differences: v_perm_b32 selectors 8..11
 // replicate bits 15, 31 (of the second operand) and 47, 63 (of the first)
 // into whole bytes, giving bytes {sign(t0.x), sign(t0.y), sign(t1.x),
@@ -90,25 +64,12 @@ __device__ __forceinline__ uint32_t signBytes(s16x2 t0, s16x2 t1)
 
 // Lane l of every group of G lanes receives v from lane l-1; lane 0 of the
 // group receives `first`.
+// G = 16: DPP row_shr:1 (lane 0 of each row of 16 keeps `first`).
 template <int G>
 __device__ __forceinline__ uint32_t shiftIn(uint32_t v, uint32_t first)
 {
-    if constexpr (G == 1) {
-        return first;
-    } else if constexpr (G == 16) {
-        return (uint32_t)__builtin_amdgcn_update_dpp((int)first, (int)v, 0x111, 0xf, 0xf, false);
-    } else if constexpr (G == 64) {
-        // Whole-wave shift through the LDS crossbar (ds_bpermute).  The DPP
-        // wavefront shift (wave_shr:1) assembles for gfx950 but does not run
-        // there: kernels using it never completed.
-        const int lane = (int)(threadIdx.x & 63);
-        const uint32_t s = (uint32_t)__builtin_amdgcn_ds_bpermute(((lane + 63) & 63) << 2, (int)v);
-        return lane == 0 ? first : s;
-    } else {
-        static_assert(G == 2 || G == 4 || G == 8, "unsupported group size");
-        const uint32_t s = (uint32_t)__builtin_amdgcn_update_dpp((int)first, (int)v, 0x111, 0xf, 0xf, false);
-        return ((threadIdx.x % G) == 0) ? first : s;
-    }
+    static_assert(G == 16, "the packed family is built for groups of 16 lanes");
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)first, (int)v, 0x111, 0xf, 0xf, false);
 }
 
 template <int G>
@@ -909,6 +870,12 @@ __global__ void kFillScores(int32_t *scores, const int32_t *ids, int n, int32_t 
         scores[ids[t]] = value;
 }
 
+__global__ void kZero32(uint32_t *p, size_t n)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        p[t] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // Traceback.  The walk follows _computeTraceback / _doTracebackGoDiagonal /
 // _doTracebackGoVertical / _doTracebackGoHorizontal (dp_traceback_impl.h:
@@ -1338,11 +1305,12 @@ const void *kernelTrk(uint32_t track)
     }
 }
 
-// The compiled (G, R) family.  G = 64 (one pair couple per wavefront, all
-// per-pair values wave-uniform) is not built: the compiler turns its
-// work-queue loop into a divergent-exit loop nest, and those kernels never
-// completed on gfx950, while the G = 16 and G = 1 builds get a uniform loop
-// exit and run correctly.
+// The compiled (G, R) family of this template: groups of 16 lanes (the LDS
+// profile ring assumes them).  An earlier build of this template with G = 64
+// (one pair couple per wavefront) did not complete on gfx950 in one test
+// run; its cause was never established (its loop control, item loads and
+// lane shifts read correct in its assembly), so no G = 64 build exists.  The
+// one-pair-per-lane form is a separate kernel (lane.hip).
 #define SQMI_FOR_EACH_GR(X) X(16, 16) X(16, 8)
 
 template <bool DNA>
@@ -1454,6 +1422,15 @@ hipError_t launchSimple32(const Simple32Args &args, hipStream_t stream)
         return hipSuccess;
     dim3 grid((args.nIds + 255) / 256), block(256);
     hipLaunchKernelGGL(kSimple32, grid, block, 0, stream, args);
+    return hipGetLastError();
+}
+
+hipError_t launchZero32(uint32_t *p, size_t n, hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(kZero32, dim3((unsigned)blocks), dim3(256), 0, stream, p, n);
     return hipGetLastError();
 }
 

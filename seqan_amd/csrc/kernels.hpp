@@ -49,7 +49,8 @@ struct FwdArgs {
     uint32_t mode;              // SQMI_* mode bits
     int32_t alpha;              // alphabet size; residue code `alpha` is padding
     const int16_t *tTable;      // (alpha+1)^2 entries: s(h,v) - gapOpen, pad row/col
-    const uint32_t *dnaProfile; // 5 dwords: bytes v=0..3 of s(h,v)-gapOpen, h=0..3, pad
+    const uint32_t *dnaProfile; // dwords 0..4: bytes v=0..3 of s(h,v)-gapOpen, h=0..3, pad;
+                                // dwords 8..12: bytes h=0..3 of s(h,v)-gapOpen, v=0..3, pad
     // Trace variant only:
     uint4 *trace;               // 16-byte trace records (64 per step)
     const int64_t *traceOff;    // per item: first record (units of 64 records)
@@ -158,6 +159,15 @@ int occupancy16(const Variant16 &v);
 bool variantSupported(const Variant16 &v);
 hipError_t launchSimple32(const Simple32Args &args, hipStream_t stream);
 
+// One pair couple per lane (lane.hip): Variant16 with G = 1 and R in {16,
+// 32}.  Work items hold up to 128 pairs with the same number of R-row
+// blocks; the row buffer holds, per wave slot, laneRowCols(maxN, R) columns
+// x 64 lanes of (G, F) pairs (8 bytes each).
+bool laneSupported(const Variant16 &v);
+int laneRowCols(int maxN, int R);
+int occupancyLane16(const Variant16 &v);
+hipError_t launchLane16(const Variant16 &v, const FwdArgs &args, int gridBlocks, hipStream_t stream);
+
 // Banded global scores (banded.hip).  Pairs in `ids` passed the host's band
 // checks; their band, clipped to the matrix, is lower..upper.
 struct BandedArgs {
@@ -216,6 +226,9 @@ struct BandedWalkArgs {
 hipError_t launchBandedWalk(const BandedWalkArgs &args, hipStream_t stream);
 // scores[ids[k]] = value for k < n.
 hipError_t launchFillScores(int32_t *scores, const int32_t *ids, int n, int32_t value, hipStream_t stream);
+// p[0 .. n) = 0 by a kernel (no memset node, so the call can be captured in a
+// HIP graph).
+hipError_t launchZero32(uint32_t *p, size_t n, hipStream_t stream);
 
 // Long pairs (int32, tiled).  A pair with m rows and n columns is cut into
 // row blocks of kTileRows (bottom-aligned like the packed kernels: item row

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_status_strings_and_version():
     L = sa.lib()
-    assert L.sqmi_abi_version() == 4
+    assert L.sqmi_abi_version() == 5 == sa.ABI_VERSION
     assert L.sqmi_status_string(2) == b"sequence sets differ in length"
 
 
@@ -72,6 +72,32 @@ def test_plan_buckets_cover_all_pairs_with_bounded_padding():
     assert st["cells"] == int((H.lengths() * V.lengths()).sum())
     assert st["work_items"] >= 50000 // 8
     assert st["cells"] <= st["padded_cells"] <= 1.5 * st["cells"]
+
+
+def test_plan_one_pair_per_lane_shapes():
+    # Forced G = 1 shapes put every packed pair on the one-pair-per-lane
+    # kernel, in items of up to 128 pairs with rows padded to R only; pairs
+    # that would need more than 255 blocks of R rows fall back to G = 16.
+    H, V = sa.workload(sa.WL_DNA_LOGNORMAL, 0x5EED0002, 0, 20000)
+    mode = sa.FREE_TOP | sa.FREE_BOTTOM
+    for R in (16, 32):
+        st = sa.plan_stats(H, V, SIMPLE, mode, flags=sa.force_shape(1, R))
+        m = V.lengths()
+        fits = int(((m + 1 + R - 1) // R <= 255).sum())
+        assert st["pairs_int16"] == 20000 and st["pairs_lane"] == fits
+        assert st["cells"] <= st["padded_cells"]
+    auto = sa.plan_stats(H, V, SIMPLE, mode)
+    assert auto["pairs_int16"] == 20000
+    with pytest.raises(sa.SqmiError) as e:
+        sa.plan_stats(H, V, SIMPLE, mode, flags=sa.force_shape(1, 8))   # not compiled
+    assert e.value.status == 3
+
+
+def test_shape_table_json_is_well_formed():
+    # The committed table the build turns into the planner's C++ table.
+    from seqan_amd import build as b
+    src = b.shape_table_source()
+    assert "kTabChoice[4][16][4]" in src and "kTabProvenance" in src
 
 
 def test_workload_windows_are_reproducible():
