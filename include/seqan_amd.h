@@ -60,7 +60,8 @@ extern "C" {
                                 3: sqmi_align_local_rows; SQMI_LOCAL in sqmi_align_counts;
                                    sqmi_align_score_banded
                                 4: sqmi_align_rows_banded
-                                5: sqmi_stats.pairs_lane; one-pair-per-lane shapes (G = 1) */
+                                5: sqmi_stats.pairs_lane; one-pair-per-lane shapes (G = 1);
+                                   sqmi_align_score_noalloc */
 
 typedef enum sqmi_status {
     SQMI_OK = 0,
@@ -234,6 +235,24 @@ sqmi_status sqmi_align_rows_banded(sqmi_context *ctx, const sqmi_sequences *set_
                                    int32_t *scores, int32_t *counts, uint8_t *ops,
                                    const int64_t *op_offsets, int64_t *op_lengths,
                                    int64_t *begins, uint32_t flags);
+
+/* sqmi_align_score without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as sqmi_align_score.  Every buffer the call
+ * needs beyond the caller's inputs and outputs (device copies of host
+ * residues, offsets and scores, work lists, row buffers, int32 scratch) is
+ * carved from one device array of SQMI_NOALLOC_SCRATCH_BYTES per GPU that the
+ * library holds for the life of the process; the call makes no hipMalloc or
+ * hipFree.  A request that needs more returns SQMI_ERR_OUT_OF_MEMORY before
+ * anything is copied or launched; pairs on the long-pair route and
+ * SQMI_PLAIN_PER_PAIR return SQMI_ERR_UNSUPPORTED.  All calls on one GPU
+ * share its array and leave their contents in it: calls that overlap on one
+ * GPU (other threads, other contexts) race. */
+#define SQMI_NOALLOC_SCRATCH_BYTES (64u << 20)
+sqmi_status sqmi_align_score_noalloc(sqmi_context *ctx, const sqmi_sequences *set_h,
+                                     const sqmi_sequences *set_v, const sqmi_scoring *scoring,
+                                     uint32_t mode, int32_t *scores, uint32_t flags);
 
 sqmi_status sqmi_get_stats(const sqmi_context *ctx, sqmi_stats *out);
 

@@ -33,6 +33,9 @@ def force_shape(G, R):
     """Flag bits forcing the packed kernel shape (SQMI_FORCE_SHAPE)."""
     return ((G & 0xFF) << 8) | ((R & 0xFF) << 16)
 
+# SQMI_NOALLOC_SCRATCH_BYTES: the device scratch of sqmi_align_score_noalloc.
+NOALLOC_SCRATCH_BYTES = 64 << 20
+
 WL_DNA150, WL_DNA_LOGNORMAL, WL_PROTEIN, WL_DNA1K, WL_DNA100K = 1, 2, 3, 4, 5
 
 DNA = "ACGT"
@@ -76,7 +79,8 @@ ABI_VERSION = 5
 
 # Every symbol the C ABI declares (include/seqan_amd.h).
 ABI_SYMBOLS = (
-    "sqmi_create", "sqmi_destroy", "sqmi_set_stream", "sqmi_align_score", "sqmi_align_counts", "sqmi_align_rows",
+    "sqmi_create", "sqmi_destroy", "sqmi_set_stream", "sqmi_align_score", "sqmi_align_score_noalloc",
+    "sqmi_align_counts", "sqmi_align_rows",
     "sqmi_align_local_rows", "sqmi_align_score_banded", "sqmi_align_rows_banded",
     "sqmi_get_stats", "sqmi_plan_stats", "sqmi_blosum62", "sqmi_status_string", "sqmi_last_error", "sqmi_abi_version",
     "sqmi_workload_create", "sqmi_workload_sizes", "sqmi_workload_copy", "sqmi_workload_destroy",
@@ -99,6 +103,8 @@ def lib():
         L.sqmi_destroy.argtypes = [vp]
         L.sqmi_set_stream.argtypes = [vp, vp]
         L.sqmi_align_score.argtypes = [vp, ctypes.POINTER(_Sequences), ctypes.POINTER(_Sequences),
+                                       ctypes.POINTER(_Scoring), ctypes.c_uint32, vp, ctypes.c_uint32]
+        L.sqmi_align_score_noalloc.argtypes = [vp, ctypes.POINTER(_Sequences), ctypes.POINTER(_Sequences),
                                        ctypes.POINTER(_Scoring), ctypes.c_uint32, vp, ctypes.c_uint32]
         L.sqmi_align_counts.argtypes = [vp, ctypes.POINTER(_Sequences), ctypes.POINTER(_Sequences),
                                         ctypes.POINTER(_Scoring), ctypes.c_uint32, vp, vp, ctypes.c_uint32]
@@ -311,6 +317,28 @@ class Context:
         st = self._L.sqmi_align_score(self._h, ctypes.byref(ch), ctypes.byref(cv), ctypes.byref(sc),
                                       mode, ptr, flags)
         self._check(st, "sqmi_align_score")
+        return out
+
+    def align_score_noalloc(self, set_h: SeqSet, set_v: SeqSet, score: Score, mode=0, flags=0,
+                            h_dev=None, v_dev=None, scores_dev=None):
+        """align_score through sqmi_align_score_noalloc: no device allocation,
+        every buffer beyond the inputs and outputs carved from the library's
+        per-GPU scratch array (SQMI_NOALLOC_SCRATCH_BYTES)."""
+        ch, keep_h = _seqs(set_h, h_dev)
+        cv, keep_v = _seqs(set_v, v_dev)
+        if h_dev is not None or v_dev is not None:
+            assert h_dev is not None and v_dev is not None
+            flags |= RESIDUES_ON_DEVICE
+        sc = score._c()
+        if scores_dev is not None:
+            flags |= SCORES_ON_DEVICE
+            out, ptr = None, scores_dev
+        else:
+            out = np.empty(len(set_h.offsets) - 1, np.int32)
+            ptr = out.ctypes.data
+        st = self._L.sqmi_align_score_noalloc(self._h, ctypes.byref(ch), ctypes.byref(cv), ctypes.byref(sc),
+                                              mode, ptr, flags)
+        self._check(st, "sqmi_align_score_noalloc")
         return out
 
     def align_score_banded(self, set_h: SeqSet, set_v: SeqSet, score: Score, lower_diag: int, upper_diag: int,

@@ -92,6 +92,7 @@ struct sqmi_context
     // call, the arena carved from it.
     void *scratchPtr = nullptr;
     Arena arena;
+    bool noAlloc = false;   // inside sqmi_align_score_noalloc
 };
 
 namespace {
@@ -134,6 +135,9 @@ hipError_t ensure(DevBuf &b, size_t bytes)
         b.cap = bytes;
     return e;
 }
+
+// Bytes an arena piece of `bytes` takes (ensure() rounds to 256).
+size_t arenaBytes(size_t bytes) { return ((bytes ? bytes : 16) + 255) & ~(size_t)255; }
 
 void release(DevBuf &b)
 {
@@ -955,7 +959,14 @@ sqmi_status sqmi_create(int device, sqmi_context **out)
     ctx->numCUs = prop.multiProcessorCount;
     ctx->totalMem = (int64_t)prop.totalGlobalMem;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->ownStream) != hipSuccess ||
-        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        scratchAddress(device, &ctx->scratchPtr) != hipSuccess) {
+        if (ctx->ev0)
+            (void)hipEventDestroy(ctx->ev0);
+        if (ctx->ev1)
+            (void)hipEventDestroy(ctx->ev1);
+        if (ctx->ownStream)
+            (void)hipStreamDestroy(ctx->ownStream);
         delete ctx;
         return SQMI_ERR_DEVICE;
     }
@@ -1007,8 +1018,14 @@ sqmi_status sqmi_get_stats(const sqmi_context *ctx, sqmi_stats *out)
     return SQMI_OK;
 }
 
-sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
-                             const sqmi_scoring *scoring, uint32_t mode, int32_t *scores, uint32_t flags)
+}  // extern "C"
+
+// sqmi_align_score and, with ctx->noAlloc, sqmi_align_score_noalloc (every
+// ensure() then carves from ctx->arena; the size of the whole request is
+// checked before anything is copied or launched, and the row buffers'
+// grid is cut to what the arena holds).
+static sqmi_status alignScore(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
+                              const sqmi_scoring *scoring, uint32_t mode, int32_t *scores, uint32_t flags)
 {
     const auto tStart = std::chrono::steady_clock::now();
     if (!ctx)
@@ -1039,6 +1056,44 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
     const int64_t scratch32 = P.scratch32;
     const auto &keys16 = P.keys16;
     const uint32_t track = P.track;
+
+    // The no-allocation call: everything but the row buffers has a fixed
+    // size; the row buffers need at least one workgroup's worth per shape.
+    size_t rowbufBudget = 0;
+    if (ctx->noAlloc) {
+        if (!P.idsLong.empty())
+            return fail(ctx, SQMI_ERR_UNSUPPORTED,
+                        "sqmi_align_score_noalloc: pairs on the long-pair route need sqmi_align_score");
+        const int64_t A2 = (int64_t)si.alpha * si.alpha, hTot = setH->offsets[N], vTot = setV->offsets[N];
+        size_t need = 0;
+        if (!(flags & SQMI_RESIDUES_ON_DEVICE))
+            need += arenaBytes(hTot + 64) + arenaBytes(vTot + 64);
+        need += 2 * arenaBytes(8 * (N + 1));
+        if (!(flags & SQMI_SCORES_ON_DEVICE))
+            need += arenaBytes(4 * N);
+        need += arenaBytes(2 * (si.alpha + 1) * (si.alpha + 1)) + arenaBytes(64) + arenaBytes(4 * A2);
+        size_t nOrder = 0, nItems = 0, minRb = 0;
+        for (const Plan16 &pl : plans) {
+            nOrder += pl.order.size();
+            nItems += pl.items.size();
+            const Variant16 &v = pl.var;
+            const int rc = v.G == 1 ? laneRowCols(pl.maxN, v.R) : pl.maxN + v.G + 2;
+            minRb = std::max(minRb, (size_t)wavesPerBlock16() * 64 / (v.G == 1 ? 1 : v.G) * rc * 8);
+        }
+        if (!plans.empty())
+            need += arenaBytes(4 * nOrder) + arenaBytes(sizeof(WorkItem) * nItems) + arenaBytes(64 * plans.size());
+        if (!ids32.empty())
+            need += arenaBytes(4 * ids32.size()) + arenaBytes(8 * ids32.size()) + arenaBytes((size_t)8 * scratch32);
+        if (!emptyIds.empty())
+            need += arenaBytes(4 * emptyIds.size());
+        const size_t cap = ctx->arena.cap - ctx->arena.used;
+        if (need + arenaBytes(minRb) > cap)
+            return fail(ctx, SQMI_ERR_OUT_OF_MEMORY,
+                        "sqmi_align_score_noalloc: the request needs " + std::to_string(need + arenaBytes(minRb)) +
+                            " bytes of device scratch, more than the " + std::to_string(cap) +
+                            " of SQMI_NOALLOC_SCRATCH_BYTES; use sqmi_align_score");
+        rowbufBudget = (cap - need) & ~(size_t)255;   // holds arenaBytes(minRb), see above
+    }
 
     // ---- device buffers --------------------------------------------------
     hipError_t err;
@@ -1131,8 +1186,10 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
             // Row buffers per wave slot: G = 16, one row of (G, F) per group;
             // G = 1, one per lane, column-major across the 64 lanes.
             rowColsOf[k] = v.G == 1 ? laneRowCols(pl.maxN, v.R) : pl.maxN + v.G + 2;
-            const size_t rb = (size_t)grids[k] * wpb * 64 / (v.G == 1 ? 1 : v.G) * rowColsOf[k] * 8;
-            rbMax = std::max(rbMax, rb);
+            const size_t perBlock = (size_t)wpb * 64 / (v.G == 1 ? 1 : v.G) * rowColsOf[k] * 8;
+            if (ctx->noAlloc)   // the persistent grid shrinks to the scratch that is left
+                grids[k] = (int)std::max<size_t>(1, std::min<size_t>((size_t)grids[k], rowbufBudget / perBlock));
+            rbMax = std::max(rbMax, (size_t)grids[k] * perBlock);
             orderOff[k + 1] = orderOff[k] + pl.order.size();
             itemOff[k + 1] = itemOff[k] + pl.items.size();
         }
@@ -1246,6 +1303,39 @@ sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, cons
         ctx->stats.device_ms = ms;
     ctx->stats.total_ms = msSince(tStart);
     return SQMI_OK;
+}
+
+extern "C" {
+
+sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
+                             const sqmi_scoring *scoring, uint32_t mode, int32_t *scores, uint32_t flags)
+{
+    return alignScore(ctx, setH, setV, scoring, mode, scores, flags);
+}
+
+// The context's buffers are set aside for the call and replaced by empty ones
+// that carve from the device's scratch array (ctx->scratchPtr, looked up at
+// sqmi_create); afterwards the arena pieces are dropped (nothing to free)
+// and the context's own buffers come back untouched.
+sqmi_status sqmi_align_score_noalloc(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV,
+                                     const sqmi_scoring *scoring, uint32_t mode, int32_t *scores, uint32_t flags)
+{
+    if (!ctx)
+        return SQMI_ERR_INVALID_ARGUMENT;
+    if (flags & SQMI_PLAIN_PER_PAIR)
+        return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_score_noalloc: no plain form");
+    if (!ctx->scratchPtr)
+        return fail(ctx, SQMI_ERR_DEVICE, "sqmi_align_score_noalloc: no device scratch on this context");
+    Buffers own;
+    std::swap(own, ctx->b);
+    ctx->arena = Arena{static_cast<char *>(ctx->scratchPtr), kNoAllocScratchBytes, 0};
+    ctx->b.forEach([&](DevBuf &d) { d.arena = &ctx->arena; });
+    ctx->noAlloc = true;
+    const sqmi_status st = alignScore(ctx, setH, setV, scoring, mode, scores, flags);
+    ctx->noAlloc = false;
+    std::swap(own, ctx->b);
+    ctx->arena = Arena{};
+    return st;
 }
 
 sqmi_status sqmi_plan_stats(const sqmi_sequences *setH, const sqmi_sequences *setV, const sqmi_scoring *scoring,

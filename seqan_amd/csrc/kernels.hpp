@@ -230,6 +230,12 @@ hipError_t launchFillScores(int32_t *scores, const int32_t *ids, int n, int32_t 
 // HIP graph).
 hipError_t launchZero32(uint32_t *p, size_t n, hipStream_t stream);
 
+// The no-allocation entry point's device array (scratch.hip): its size and
+// its device address on `device`, which must be the current device; the
+// address is looked up once per device.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+hipError_t scratchAddress(int device, void **out);
+
 // Long pairs (int32, tiled).  A pair with m rows and n columns is cut into
 // row blocks of kTileRows (bottom-aligned like the packed kernels: item row
 // p + i holds pair row i, p = nblocks * kTileRows - 1 - m) and column tiles of
