@@ -56,12 +56,14 @@
 extern "C" {
 #endif
 
-#define SQMI_ABI_VERSION 5   /* 2: sqmi_stats.pairs_long, pairs_long_packed; sqmi_align_rows
+#define SQMI_ABI_VERSION 6   /* 2: sqmi_stats.pairs_long, pairs_long_packed; sqmi_align_rows
                                 3: sqmi_align_local_rows; SQMI_LOCAL in sqmi_align_counts;
                                    sqmi_align_score_banded
                                 4: sqmi_align_rows_banded
                                 5: sqmi_stats.pairs_lane; one-pair-per-lane shapes (G = 1);
-                                   sqmi_align_score_noalloc */
+                                   sqmi_align_score_noalloc
+                                6: sqmi_stats.replayed: identical sqmi_align_score calls relaunch
+                                   the previous call's HIP graph */
 
 typedef enum sqmi_status {
     SQMI_OK = 0,
@@ -138,6 +140,9 @@ typedef struct sqmi_stats {
     int64_t pairs_long;         /* of pairs_int32: long pairs, tiled kernels    */
     int64_t pairs_long_packed;  /* of pairs_long: forward in packed int16       */
     int64_t pairs_lane;         /* of pairs_int16: one pair couple per lane      */
+    int64_t replayed;           /* 1: the call relaunched the previous call's    */
+                                /* graph (same lengths, scoring, mode, flags and */
+                                /* device addresses; host_plan_ms = key time)    */
 } sqmi_stats;
 
 typedef struct sqmi_context sqmi_context;
@@ -151,7 +156,12 @@ sqmi_status sqmi_set_stream(sqmi_context *ctx, void *hip_stream);
 
 /* Optimal alignment score of every pair (setH[i], setV[i]).  Blocks until the
  * scores are written.  scores: count int32 (host, or device with
- * SQMI_SCORES_ON_DEVICE). */
+ * SQMI_SCORES_ON_DEVICE).  The call's kernel launches are captured in a HIP
+ * graph; the next call on the context with the same lengths (offsets),
+ * scoring, mode and flags, the same device addresses (residues and scores on
+ * the device) and the same planner switches skips planning and relaunches
+ * that graph (the residues themselves may differ).  Any other call on the
+ * context drops it; SQMI_GRAPH=0 in the environment turns this off. */
 sqmi_status sqmi_align_score(sqmi_context *ctx, const sqmi_sequences *set_h,
                              const sqmi_sequences *set_v, const sqmi_scoring *scoring,
                              uint32_t mode, int32_t *scores, uint32_t flags);

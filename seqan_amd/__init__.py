@@ -68,14 +68,14 @@ class Stats(ctypes.Structure):
                 ("padded_cells", ctypes.c_int64), ("host_plan_ms", ctypes.c_double),
                 ("device_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
                 ("pairs_long", ctypes.c_int64), ("pairs_long_packed", ctypes.c_int64),
-                ("pairs_lane", ctypes.c_int64)]
+                ("pairs_lane", ctypes.c_int64), ("replayed", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 # SQMI_ABI_VERSION of include/seqan_amd.h that the structures below mirror.
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 # Every symbol the C ABI declares (include/seqan_amd.h).
 ABI_SYMBOLS = (

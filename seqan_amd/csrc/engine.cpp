@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <new>
 #include <string>
 #include <vector>
@@ -93,6 +94,16 @@ struct sqmi_context
     void *scratchPtr = nullptr;
     Arena arena;
     bool noAlloc = false;   // inside sqmi_align_score_noalloc
+    // The last sqmi_align_score call, kept for an identical next call: the
+    // key of its inputs, its statistics and its launches as an instantiated
+    // HIP graph.  Its plan lives on in the context's device buffers, so any
+    // other call on the context drops it.
+    struct Replay {
+        bool valid = false;
+        uint64_t key = 0;
+        hipGraphExec_t exec = nullptr;
+        sqmi_stats stats{};
+    } replay;
 };
 
 namespace {
@@ -134,6 +145,45 @@ hipError_t ensure(DevBuf &b, size_t bytes)
     if (e == hipSuccess)
         b.cap = bytes;
     return e;
+}
+
+// Launches collected during a call's preparation and run (or captured into
+// a HIP graph) once every upload is enqueued.
+using Deferred = std::vector<std::function<hipError_t()>>;
+
+// SQMI_GRAPH=0 turns off the replay of identical sqmi_align_score calls.
+bool graphsEnabled()
+{
+    const char *env = std::getenv("SQMI_GRAPH");
+    return !(env && *env == '0');
+}
+
+void dropReplay(sqmi_context *ctx)
+{
+    if (ctx->replay.exec)
+        (void)hipGraphExecDestroy(ctx->replay.exec);
+    ctx->replay = sqmi_context::Replay{};
+}
+
+uint64_t mix64(uint64_t x)
+{
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33;
+    return x;
+}
+
+// Order-sensitive 64-bit digest of n int64 values (a sum of position-tagged
+// mixes, so threads can take contiguous ranges).
+uint64_t digest64(const int64_t *a, int64_t n)
+{
+    uint64_t h = 0;
+#pragma omp parallel for schedule(static) reduction(+ : h) if (n > 65536)
+    for (int64_t i = 0; i < n; ++i)
+        h += mix64((uint64_t)a[i] ^ ((uint64_t)i * 0x9e3779b97f4a7c15ULL));
+    return mix64(h ^ (uint64_t)n);
 }
 
 // Bytes an arena piece of `bytes` takes (ensure() rounds to 256).
@@ -210,8 +260,18 @@ int lengthBin(int64_t m)
 
 int shapeCategory(bool dna, bool local) { return (dna ? 0 : 2) + (local ? 1 : 0); }
 
+// SQMI_SHAPE_RULE=fixed: the rule the table replaced (G = 16; R = 8 when
+// |V| + 1 <= 128, else 16), for comparing tuned and untuned plans.
+bool fixedShapeRule()
+{
+    const char *env = std::getenv("SQMI_SHAPE_RULE");
+    return env && std::strcmp(env, "fixed") == 0;
+}
+
 Variant16 tableShape(int category, int bin, int64_t pairsInBin, bool dna, uint32_t track)
 {
+    if (fixedShapeRule())
+        return Variant16{16, bin <= 7 ? 8 : 16, dna, track};
     int cls = 0;
     while (cls + 1 < kTabNClasses && pairsInBin >= kTabCountClasses[cls + 1])
         ++cls;
@@ -323,6 +383,10 @@ int longStreams()
     return std::max(1, std::min(v, sqmi_context::kMaxLongStreams));
 }
 
+
+// Calls with at most this many pairs plan on one thread: waking the thread
+// team costs more than it saves.
+constexpr int64_t kParallelMin = 32768;
 
 // Stable LSD radix sort of 26-bit keys in two passes of 13 bits, each pass
 // split into contiguous per-thread ranges so that equal digits keep their
@@ -450,7 +514,7 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
     } else {
         // Pairs per length bin (int16 route only), then one slot per shape.
         int64_t binCount[16] = {};
-#pragma omp parallel
+#pragma omp parallel if (N > kParallelMin)
         {
             int64_t local16[16] = {};
 #pragma omp for schedule(static) nowait
@@ -480,7 +544,7 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
         }
         fallbackSlot = slotOf(Variant16{16, 16, si.dna, track});
     }
-#pragma omp parallel for schedule(static) reduction(+ : cells)
+#pragma omp parallel for schedule(static) reduction(+ : cells) if (N > kParallelMin)
     for (int64_t i = 0; i < N; ++i) {
         const int64_t n = H.offsets[i + 1] - H.offsets[i];
         const int64_t m = V.offsets[i + 1] - V.offsets[i];
@@ -589,13 +653,13 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
         pl.order.resize(p1 - p0);
         pl.items.resize(itemOff[b1] - i0);
         int32_t maxN = 0;
-#pragma omp parallel for schedule(static) reduction(max : maxN)
+#pragma omp parallel for schedule(static) reduction(max : maxN) if (p1 - p0 > (size_t)kParallelMin)
         for (int64_t k = (int64_t)p0; k < (int64_t)p1; ++k) {
             pl.order[k - p0] = keys16[k].id;
             maxN = std::max(maxN, widthOf(keys16[k].key));
         }
         pl.maxN = maxN;
-#pragma omp parallel for schedule(dynamic, 8)
+#pragma omp parallel for schedule(dynamic, 8) if (p1 - p0 > (size_t)kParallelMin)
         for (int64_t b = (int64_t)b0; b < (int64_t)b1; ++b) {
             size_t it = itemOff[b] - i0;
             for (size_t k = starts[b]; k < starts[b + 1]; k += PP, ++it) {
@@ -774,8 +838,18 @@ struct RowsDev {
 
 sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV, const Planned &P,
                     const Inputs &in, uint32_t flags, int32_t *dCounts, const RowsDev &rows,
-                    std::vector<LongPair> &pairs)
+                    std::vector<LongPair> &pairs, Deferred *deferred = nullptr)
 {
+    // Stream operations run at once, or are appended to `deferred` (all
+    // uploads and allocations still happen here).
+    hipError_t opErr = hipSuccess;
+    auto op = [&](std::function<hipError_t()> f) {
+        if (deferred)
+            deferred->push_back(std::move(f));
+        else if (opErr == hipSuccess)
+            opErr = f();
+        return opErr;
+    };
     if (P.idsLong.empty())
         return SQMI_OK;
     hipStream_t stream = ctx->stream;
@@ -874,25 +948,35 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
                     (err = hipEventCreateWithFlags(&ctx->longJoin[s], hipEventDisableTiming)) != hipSuccess)
                     return hipFail(ctx, err, "creating long-pair events");
             }
-            if ((err = hipEventRecord(ctx->longFork, stream)) != hipSuccess)
+            hipEvent_t fork = ctx->longFork;
+            if ((err = op([=] { return hipEventRecord(fork, stream); })) != hipSuccess)
                 return hipFail(ctx, err, "long-pair fork");
-            for (int s = 1; s < ns; ++s)
-                if ((err = hipStreamWaitEvent(ctx->longStreams[s], ctx->longFork, 0)) != hipSuccess)
+            for (int s = 1; s < ns; ++s) {
+                hipStream_t side = ctx->longStreams[s];
+                if ((err = op([=] { return hipStreamWaitEvent(side, fork, 0); })) != hipSuccess)
                     return hipFail(ctx, err, "long-pair fork");
+            }
         }
         for (int d = 0; d < maxB + maxT - 1; ++d) {
             for (int s = 0; s < ns; ++s) {
-                sub[s].diag = d;
+                TileArgs t = sub[s];
+                t.diag = d;
                 hipStream_t st = s > 0 ? ctx->longStreams[s] : stream;
-                if ((err = packed ? launchTileFwd16(sub[s], st) : launchTileFwd32(sub[s], st)) != hipSuccess)
+                if ((err = op([=] { return packed ? launchTileFwd16(t, st) : launchTileFwd32(t, st); })) !=
+                    hipSuccess)
                     return hipFail(ctx, err, "launching long-pair forward");
             }
         }
         if (ns > 1) {
-            for (int s = 1; s < ns; ++s)
-                if ((err = hipEventRecord(ctx->longJoin[s], ctx->longStreams[s])) != hipSuccess ||
-                    (err = hipStreamWaitEvent(stream, ctx->longJoin[s], 0)) != hipSuccess)
+            for (int s = 1; s < ns; ++s) {
+                hipEvent_t join = ctx->longJoin[s];
+                hipStream_t side = ctx->longStreams[s];
+                if ((err = op([=] {
+                         hipError_t e = hipEventRecord(join, side);
+                         return e != hipSuccess ? e : hipStreamWaitEvent(stream, join, 0);
+                     })) != hipSuccess)
                     return hipFail(ctx, err, "long-pair join");
+            }
         }
         if (dCounts) {
             a.trace = static_cast<uint2 *>(ctx->b.longTrace.p);
@@ -901,14 +985,17 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
             a.ops = rows.ops;
             a.opOff = rows.opOff;
             a.opLen = rows.opLen;
-            if ((err = launchTileWalkInit(a, stream)) != hipSuccess)
+            const TileArgs ta = a;
+            if ((err = op([=] { return launchTileWalkInit(ta, stream); })) != hipSuccess)
                 return hipFail(ctx, err, "launching long-pair walk init");
             // Each round moves the walk up kRoundBlocks row blocks or left
             // kRoundTiles tiles (or ends it); extra rounds do nothing.
             const int rounds = (maxB + kRoundBlocks - 1) / kRoundBlocks + (maxT + kRoundTiles - 1) / kRoundTiles + 1;
             for (int r = 0; r < rounds; ++r) {
-                if ((err = launchTileTrace32(a, stream)) != hipSuccess ||
-                    (err = launchTileWalk32(a, stream)) != hipSuccess)
+                if ((err = op([=] {
+                         hipError_t e = launchTileTrace32(ta, stream);
+                         return e != hipSuccess ? e : launchTileWalk32(ta, stream);
+                     })) != hipSuccess)
                     return hipFail(ctx, err, "launching long-pair traceback");
             }
         }
@@ -981,6 +1068,7 @@ sqmi_status sqmi_destroy(sqmi_context *ctx)
         return SQMI_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    dropReplay(ctx);
     ctx->b.forEach([](DevBuf &b) { release(b); });
     if (ctx->ev0)
         (void)hipEventDestroy(ctx->ev0);
@@ -1044,6 +1132,63 @@ static sqmi_status alignScore(sqmi_context *ctx, const sqmi_sequences *setH, con
         return fail(ctx, SQMI_ERR_DEVICE, "hipSetDevice failed");
     hipStream_t stream = ctx->stream;
 
+    // Replay: a call whose plan depends on nothing that differs from the
+    // last call's (lengths, scoring, mode, flags, the device addresses the
+    // kernels take, the planner's switches) reuses its plan, which is still
+    // in the context's device buffers, and relaunches its graph; only host
+    // residues and host scores are copied again.
+    const bool replayable = graphsEnabled() && !ctx->noAlloc && !(flags & SQMI_PLAIN_PER_PAIR);
+    uint64_t key = 0;
+    hipError_t err;
+    if (replayable) {
+        const auto tKey = std::chrono::steady_clock::now();
+        key = digest64(setH->offsets, N + 1) ^ mix64(digest64(setV->offsets, N + 1) + 1);
+        std::vector<int64_t> scal = {N, mode, flags, scoring->alphabet_size, scoring->gap_open, scoring->gap_extend,
+                                     ctx->device};
+        for (int k = 0; k < scoring->alphabet_size * scoring->alphabet_size; ++k)
+            scal.push_back(scoring->table[k]);
+        if (flags & SQMI_RESIDUES_ON_DEVICE) {
+            scal.push_back((int64_t)(uintptr_t)setH->residues);
+            scal.push_back((int64_t)(uintptr_t)setV->residues);
+        }
+        if (flags & SQMI_SCORES_ON_DEVICE)
+            scal.push_back((int64_t)(uintptr_t)scores);
+        for (const char *name : {"SQMI_COOP_MIN_BLOCKS", "SQMI_LONG_ALL", "SQMI_LONG_MIN_SIDE", "SQMI_LONG_STREAMS",
+                                 "SQMI_SHAPE_RULE", "SQMI_TRACE_BUDGET_MB"}) {
+            const char *v = std::getenv(name);
+            for (const char *c = v ? v : ""; *c; ++c)
+                scal.push_back(*c);
+            scal.push_back(-1);
+        }
+        key ^= mix64(digest64(scal.data(), (int64_t)scal.size()) + 2);
+        if (ctx->replay.valid && ctx->replay.key == key) {
+            const double keyMs = msSince(tKey);
+            const uint8_t *rH = setH->residues, *rV = setV->residues;
+            const int64_t hTot = setH->offsets[N], vTot = setV->offsets[N];
+            if (!(flags & SQMI_RESIDUES_ON_DEVICE) &&
+                ((err = hipMemcpyAsync(ctx->b.hRes.p, rH, hTot, hipMemcpyHostToDevice, stream)) != hipSuccess ||
+                 (err = hipMemcpyAsync(ctx->b.vRes.p, rV, vTot, hipMemcpyHostToDevice, stream)) != hipSuccess))
+                return hipFail(ctx, err, "copying residues");
+            if ((err = hipEventRecord(ctx->ev0, stream)) != hipSuccess ||
+                (err = hipGraphLaunch(ctx->replay.exec, stream)) != hipSuccess ||
+                (err = hipEventRecord(ctx->ev1, stream)) != hipSuccess)
+                return hipFail(ctx, err, "launching the call's graph");
+            if (!(flags & SQMI_SCORES_ON_DEVICE) &&
+                (err = hipMemcpyAsync(scores, ctx->b.scores.p, 4 * N, hipMemcpyDeviceToHost, stream)) != hipSuccess)
+                return hipFail(ctx, err, "copying scores");
+            if ((err = hipStreamSynchronize(stream)) != hipSuccess)
+                return hipFail(ctx, err, "alignment kernels");
+            ctx->stats = ctx->replay.stats;
+            ctx->stats.host_plan_ms = keyMs;
+            ctx->stats.replayed = 1;
+            float ms = 0;
+            ctx->stats.device_ms = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess ? ms : 0.0;
+            ctx->stats.total_ms = msSince(tStart);
+            return SQMI_OK;
+        }
+    }
+    dropReplay(ctx);
+
     Planned P;
     const auto tPlan = std::chrono::steady_clock::now();
     plan(*setH, *setV, *scoring, mode, flags, P);
@@ -1096,21 +1241,22 @@ static sqmi_status alignScore(sqmi_context *ctx, const sqmi_sequences *setH, con
     }
 
     // ---- device buffers --------------------------------------------------
-    hipError_t err;
     Inputs in;
     if ((st = uploadInputs(ctx, setH, setV, scoring, si, scores, flags, in)) != SQMI_OK)
         return st;
     const uint8_t *dH = in.dH, *dV = in.dV;
     int32_t *dScores = in.dScores;
     const int A = si.alpha;
-    if ((err = hipEventRecord(ctx->ev0, stream)) != hipSuccess)
-        return hipFail(ctx, err, "event");
+    // Kernel launches wait in `launches` until every upload is enqueued.
+    Deferred launches;
 
     // ---- plain form: one launch per pair (baseline for the batched form) --
     // Host data staged by hipMemcpyAsync must outlive the copies: the plain
     // form's lists live at function scope, like the planner's.
     std::vector<int32_t> plainIds;
     std::vector<WorkItem> plainItems;
+    if ((flags & SQMI_PLAIN_PER_PAIR) && (err = hipEventRecord(ctx->ev0, stream)) != hipSuccess)
+        return hipFail(ctx, err, "event");
     if (flags & SQMI_PLAIN_PER_PAIR) {
         std::vector<int32_t> &ids16 = plainIds;
         std::vector<WorkItem> &items = plainItems;
@@ -1198,8 +1344,8 @@ static sqmi_status alignScore(sqmi_context *ctx, const sqmi_sequences *setH, con
             (err = ensure(ctx->b.items, sizeof(WorkItem) * itemOff.back())) != hipSuccess ||
             (err = ensure(ctx->b.queue, 4 * nq)) != hipSuccess || (err = ensure(ctx->b.rowbuf, rbMax)) != hipSuccess)
             return hipFail(ctx, err, "allocating work lists");
-        if ((err = launchZero32(static_cast<uint32_t *>(ctx->b.queue.p), nq, stream)) != hipSuccess)
-            return hipFail(ctx, err, "clearing work-queue counters");
+        uint32_t *dQueues = static_cast<uint32_t *>(ctx->b.queue.p);
+        launches.push_back([=] { return launchZero32(dQueues, nq, stream); });
         for (size_t k = 0; k < plans.size(); ++k) {
             const Plan16 &pl = plans[k];
             const Variant16 &v = pl.var;
@@ -1241,9 +1387,9 @@ static sqmi_status alignScore(sqmi_context *ctx, const sqmi_sequences *setH, con
             a.alpha = A;
             a.tTable = static_cast<const int16_t *>(ctx->b.tTable.p);
             a.dnaProfile = static_cast<const uint32_t *>(ctx->b.dnaProf.p);
-            if ((err = v.G == 1 ? launchLane16(v, a, grids[k], stream) : launchFwd16(v, a, grids[k], stream)) !=
-                hipSuccess)
-                return hipFail(ctx, err, "launching packed kernel");
+            const int grid = grids[k];
+            launches.push_back(
+                [=] { return v.G == 1 ? launchLane16(v, a, grid, stream) : launchFwd16(v, a, grid, stream); });
         }
     }
 
@@ -1273,11 +1419,10 @@ static sqmi_status alignScore(sqmi_context *ctx, const sqmi_sequences *setH, con
         a.mode = mode;
         a.alpha = A;
         a.table = static_cast<const int32_t *>(ctx->b.table32.p);
-        if ((err = launchSimple32(a, stream)) != hipSuccess)
-            return hipFail(ctx, err, "launching int32 kernel");
+        launches.push_back([=] { return launchSimple32(a, stream); });
     }
     std::vector<LongPair> longPairs;
-    if ((st = runLong(ctx, setH, setV, P, in, flags, nullptr, RowsDev{}, longPairs)) != SQMI_OK)
+    if ((st = runLong(ctx, setH, setV, P, in, flags, nullptr, RowsDev{}, longPairs, &launches)) != SQMI_OK)
         return st;
     if (!emptyIds.empty()) {
         if ((err = ensure(ctx->b.emptyIds, 4 * emptyIds.size())) != hipSuccess)
@@ -1286,10 +1431,46 @@ static sqmi_status alignScore(sqmi_context *ctx, const sqmi_sequences *setH, con
                                   stream)) != hipSuccess)
             return hipFail(ctx, err, "uploading empty list");
         const int ne = (int)emptyIds.size();
-        if ((err = launchFillScores(dScores, static_cast<const int32_t *>(ctx->b.emptyIds.p), ne, INT32_MIN,
-                                    stream)) != hipSuccess)
-            return hipFail(ctx, err, "launching fill kernel");
+        const int32_t *dEmpty = static_cast<const int32_t *>(ctx->b.emptyIds.p);
+        launches.push_back([=] { return launchFillScores(dScores, dEmpty, ne, INT32_MIN, stream); });
     }
+
+    // ---- the launches: captured into a graph that an identical next call
+    // relaunches, or run directly (plain form, no-allocation calls, or
+    // SQMI_GRAPH=0); a capture the runtime refuses falls back to running them.
+    if (!(flags & SQMI_PLAIN_PER_PAIR) && (err = hipEventRecord(ctx->ev0, stream)) != hipSuccess)
+        return hipFail(ctx, err, "event");
+    bool ran = false;
+    if (replayable && !launches.empty()) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        err = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+        if (err == hipSuccess) {
+            hipError_t capErr = hipSuccess;
+            for (auto &f : launches)
+                if (capErr == hipSuccess)
+                    capErr = f();
+            err = hipStreamEndCapture(stream, &graph);
+            if (capErr == hipSuccess && err == hipSuccess && graph &&
+                hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess &&
+                (err = hipGraphLaunch(exec, stream)) == hipSuccess) {
+                ran = true;
+                ctx->replay.valid = true;
+                ctx->replay.key = key;
+                ctx->replay.exec = exec;
+                exec = nullptr;
+            }
+            if (graph)
+                (void)hipGraphDestroy(graph);
+            if (exec)
+                (void)hipGraphExecDestroy(exec);
+        }
+        (void)hipGetLastError();   // a refused capture leaves its error behind
+    }
+    if (!ran)
+        for (auto &f : launches)
+            if ((err = f()) != hipSuccess)
+                return hipFail(ctx, err, "launching alignment kernels");
     if ((err = hipEventRecord(ctx->ev1, stream)) != hipSuccess)
         return hipFail(ctx, err, "event");
     if (!(flags & SQMI_SCORES_ON_DEVICE)) {
@@ -1302,6 +1483,8 @@ static sqmi_status alignScore(sqmi_context *ctx, const sqmi_sequences *setH, con
     if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess)
         ctx->stats.device_ms = ms;
     ctx->stats.total_ms = msSince(tStart);
+    if (ctx->replay.valid)
+        ctx->replay.stats = ctx->stats;
     return SQMI_OK;
 }
 
@@ -1326,6 +1509,7 @@ sqmi_status sqmi_align_score_noalloc(sqmi_context *ctx, const sqmi_sequences *se
         return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_score_noalloc: no plain form");
     if (!ctx->scratchPtr)
         return fail(ctx, SQMI_ERR_DEVICE, "sqmi_align_score_noalloc: no device scratch on this context");
+    dropReplay(ctx);
     Buffers own;
     std::swap(own, ctx->b);
     ctx->arena = Arena{static_cast<char *>(ctx->scratchPtr), kNoAllocScratchBytes, 0};
@@ -1370,6 +1554,7 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
     const bool wantRows = ops != nullptr;
     if (!ctx)
         return SQMI_ERR_INVALID_ARGUMENT;
+    dropReplay(ctx);   // this call rewrites the buffers a replay would reuse
     ctx->lastError.clear();
     ctx->stats = sqmi_stats{};
     sqmi_status st;
@@ -1808,6 +1993,7 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
     const auto tStart = std::chrono::steady_clock::now();
     if (!ctx)
         return SQMI_ERR_INVALID_ARGUMENT;
+    dropReplay(ctx);   // this call rewrites the buffers a replay would reuse
     ctx->lastError.clear();
     ctx->stats = sqmi_stats{};
     sqmi_status st;
