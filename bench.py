@@ -10,6 +10,8 @@ residues already resident in HBM.  Prints one JSON line.
 
 Also measured in the same run, each reported beside the headline figure:
   - the plain one-pair-per-launch GPU form on a sample of the same pairs;
+  - config 1: 4,096 DNA pairs of 150 bp, global: batched (device and host
+    buffers, with per-call host time), plain per-pair form, reference CPU;
   - fwd+bwd: score plus traceback counts (sqmi_align_counts) of 65,536 DNA
     pairs of 1 kbp, global, and the forward alone on the same pairs;
   - protein local: 262,144 Smith-Waterman pairs, Blosum62(-1, -11);
@@ -38,6 +40,7 @@ METRIC = "GCUPS exact affine DP, batched vs plain on 1 GPU; fwd+bwd GCUPS; % VAL
 SEED = 0x5EED0002
 N_PAIRS = 1 << 20
 MODE = sa.FREE_TOP | sa.FREE_BOTTOM
+SEED_DNA150, N_DNA150 = 0x5EED0001, 4096
 SEED_PROTEIN, N_PROTEIN = 0x5EED0003, 1 << 18
 SEED_DNA1K, N_DNA1K = 0x5EED0004, 1 << 16
 SEED_DNA100K, N_DNA100K = 0x5EED0005, 64
@@ -99,6 +102,15 @@ def reference_run(op, H, V, count, mode, threads, repeats, gpu_scores, gpu_count
             "parity_vs_gpu": ok}
 
 
+def shape_provenance():
+    """Where the planner's shape table came from (its JSON's provenance)."""
+    try:
+        with open(os.path.join(ROOT, "seqan_amd", "tuning", "shapes.json")) as f:
+            return json.load(f).get("provenance")
+    except (OSError, ValueError):
+        return None
+
+
 def profiled_traffic():
     """HBM bytes per launch of the dominant kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None when absent)."""
@@ -123,6 +135,82 @@ def timed(fn, steps, warmup, label):
     ms = (time.perf_counter() - t0) * 1e3 / steps
     log(f"{label}: {ms:.2f} ms/step")
     return ms
+
+
+def config1_leg(ctx, steps, warmup, cpu):
+    """4,096 DNA pairs of 150 bp, global, Score<int,Simple>(2,-3,-1,-5): the
+    batched call with residues and scores resident in HBM, the same call
+    from host buffers (PCIe copies inside), the plain form (one launch per
+    pair) on the same pairs, and the reference CPU path on the same pairs.
+    Per-call host time (planning and launch) is reported beside the device
+    time: at 9.2e7 cells a call is dominated by it."""
+    H, V = sa.workload(sa.WL_DNA150, SEED_DNA150, 0, N_DNA150)
+    cells = int((H.lengths() * V.lengths()).sum())
+    score = sa.simple_score(2, -3, -1, -5)
+    dh, dv = sa.DeviceBuffer.from_array(H.residues), sa.DeviceBuffer.from_array(V.residues)
+    ds = sa.DeviceBuffer(4 * N_DNA150)
+    dev_ms, plan_ms, call_ms = [], [], []
+
+    def step():
+        t = time.perf_counter()
+        ctx.align_score(H, V, score, 0, h_dev=dh.address, v_dev=dv.address, scores_dev=ds.address)
+        call_ms.append((time.perf_counter() - t) * 1e3)
+        st = ctx.stats()
+        dev_ms.append(st["device_ms"])
+        plan_ms.append(st["host_plan_ms"])
+
+    os.environ["SQMI_GRAPH"] = "0"      # every call plans afresh
+    ms = timed(step, steps, warmup, "config1 batched")
+    scores = ds.to_array(np.int32, N_DNA150)
+    n = steps
+    os.environ.pop("SQMI_GRAPH", None)
+    rep_call = []
+
+    def replay_step():
+        t = time.perf_counter()
+        ctx.align_score(H, V, score, 0, h_dev=dh.address, v_dev=dv.address, scores_dev=ds.address)
+        rep_call.append((time.perf_counter() - t) * 1e3)
+
+    rep_ms = timed(replay_step, steps, 2, "config1 batched, replayed graph")
+    replay = {"ms_per_step": round(rep_ms, 4), "gcups": round(cells / (rep_ms / 1e3) / 1e9, 2),
+              "call_ms": round(float(np.median(rep_call[-steps:])), 4), "replayed": int(ctx.stats()["replayed"]),
+              "device_ms": round(ctx.stats()["device_ms"], 4)}
+    kern = float(np.median(dev_ms[-n:]))
+    achieved = OPS_PER_CELL * cells / (kern / 1e3) / 1e12
+    host_ms = timed(lambda: ctx.align_score(H, V, score, 0), steps, warmup, "config1 batched, host buffers")
+    host_scores = ctx.align_score(H, V, score, 0)
+    # Plain form: one kernel launch per pair, same pairs, same library.
+    t = time.perf_counter()
+    plain_scores = ctx.align_score(H, V, score, 0, flags=sa.PLAIN_PER_PAIR)
+    plain_s = time.perf_counter() - t
+    res = {"value": round(cells / (ms / 1e3) / 1e9, 2), "unit": "GCUPS", "ms_per_step": round(ms, 4),
+           "device_ms": round(kern, 4), "host_plan_ms": round(float(np.median(plan_ms[-n:])), 4),
+           "call_ms": round(float(np.median(call_ms[-n:])), 4),
+           "replay": replay,
+           "host_buffers_ms_per_step": round(host_ms, 4),
+           "host_buffers_gcups": round(cells / (host_ms / 1e3) / 1e9, 2),
+           "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_TOPS, 2),
+                        "unit": "TOP/s", "frac": round(achieved / PEAK_TOPS, 4),
+                        "note": "packed-int16 VALU ops, 8 per cell, over the median HIP-event time of one call"},
+           "plain": {"value": round(cells / plain_s / 1e9, 3), "unit": "GCUPS", "ms_per_call": round(plain_s * 1e3, 2),
+                     "launches": N_DNA150, "batched_over_plain": round((cells / (ms / 1e3)) / (cells / plain_s), 1),
+                     "parity_vs_batched": bool(np.array_equal(plain_scores, scores))},
+           "parity_host_vs_device_buffers": bool(np.array_equal(host_scores, scores)),
+           "workload": "4,096 DNA pairs of 150 bp, 5% subst, 1% single-base indels, global, "
+                       "Score<int,Simple>(2,-3,-1,-5)", "pairs": N_DNA150, "cells": cells, "seed": SEED_DNA150}
+    if cpu:
+        res["cpu_baseline"] = reference_run(0, H, V, N_DNA150, 0, host_cores(), 3, scores, alphabet="dna", match=2,
+                                            mismatch=-3, gap_extend=-1, gap_open=-5)
+        if res["cpu_baseline"] is not None:
+            one = reference_run(0, H, V, N_DNA150, 0, 1, 3, scores, alphabet="dna", match=2, mismatch=-3,
+                                gap_extend=-1, gap_open=-5)
+            res["cpu_baseline"]["one_core"] = {"value": one["value"], "parity_vs_gpu": one["parity_vs_gpu"]}
+            res["cpu_baseline"]["sample"] = ("all 4,096 pairs (the whole workload); warm-up call, best of 3 timed "
+                                             f"calls; {cpu_model()}")
+        log(f"config1 cpu baseline: {res['cpu_baseline']}")
+    for b in (dh, dv, ds):
+        b.free()
+    return res
 
 
 def backward_leg(ctx, steps, warmup, cpu):
@@ -258,14 +346,21 @@ def protein_leg(ctx, steps, warmup, cpu):
     score = sa.blosum62(-1, -11)
     dh, dv = sa.DeviceBuffer.from_array(H.residues), sa.DeviceBuffer.from_array(V.residues)
     ds = sa.DeviceBuffer(4 * N_PROTEIN)
+    os.environ["SQMI_GRAPH"] = "0"
     ms = timed(lambda: ctx.align_score(H, V, score, sa.LOCAL, h_dev=dh.address, v_dev=dv.address,
                                        scores_dev=ds.address), steps, warmup, "protein local")
     st = ctx.stats()
+    os.environ["SQMI_SHAPE_RULE"] = "fixed"
+    ctx.align_score(H, V, score, sa.LOCAL, h_dev=dh.address, v_dev=dv.address, scores_dev=ds.address)
+    fixed_dev = ctx.stats()["device_ms"]
+    os.environ.pop("SQMI_SHAPE_RULE", None)
+    os.environ.pop("SQMI_GRAPH", None)
     scores = ds.to_array(np.int32, N_PROTEIN)
     kern = st["device_ms"]
     achieved = OPS_PER_CELL_LOCAL * cells / (kern / 1e3) / 1e12
     res = {"value": round(cells / (ms / 1e3) / 1e9, 2), "unit": "GCUPS", "ms_per_step": round(ms, 3),
            "device_ms": round(kern, 3), "roofline_frac": round(achieved / PEAK_TOPS, 4),
+           "shape_table": {"tuned_device_ms": round(kern, 3), "fixed_rule_device_ms": round(fixed_dev, 3)},
            "workload": "262,144 protein pairs, query U[50,1000] aa, target U[50,2000] aa, 30% homologs, "
                        "local, Blosum62(-1,-11)",
            "pairs": N_PROTEIN, "cells": cells, "seed": SEED_PROTEIN, "pairs_int32": int(st["pairs_int32"])}
@@ -400,7 +495,7 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the fwd+bwd and protein legs")
-    ap.add_argument("--leg", choices=["fwd_bwd", "protein_local", "long_fwd_bwd", "banded"], default=None,
+    ap.add_argument("--leg", choices=["config1", "fwd_bwd", "protein_local", "long_fwd_bwd", "banded"], default=None,
                     help="run only this extra leg and print its JSON (for profiling)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also copy this many MiB device-to-device (counter calibration)")
@@ -410,8 +505,8 @@ def main():
         sys.exit(2)
 
     if args.leg:
-        legs = {"fwd_bwd": backward_leg, "protein_local": protein_leg, "long_fwd_bwd": long_leg,
-                "banded": banded_leg}
+        legs = {"config1": config1_leg, "fwd_bwd": backward_leg, "protein_local": protein_leg,
+                "long_fwd_bwd": long_leg, "banded": banded_leg}
         print(json.dumps({args.leg: legs[args.leg](sa.Context(0), args.steps, args.warmup, not args.no_cpu)}))
         return
 
@@ -430,6 +525,9 @@ def main():
     def step():
         ctx.align_score(H, V, score, MODE, h_dev=dh.address, v_dev=dv.address, scores_dev=out.address)
 
+    # Every timed step plans afresh (SQMI_GRAPH=0): a replayed call would
+    # skip the host planning; its rate is reported separately below.
+    os.environ["SQMI_GRAPH"] = "0"
     for w in range(args.warmup):
         tw = time.perf_counter()
         step()
@@ -451,6 +549,25 @@ def main():
     st = ctx.stats()
     log(f"batched: {ms_per_step:.1f} ms/step, {gcups:.1f} GCUPS, device {np.mean(dev_ms):.1f} ms, "
         f"plan {np.mean(plan_ms):.1f} ms, stats {st}")
+
+    os.environ.pop("SQMI_GRAPH", None)
+    # The same call replayed: plan kept, launches relaunched as one graph.
+    replay_ms = timed(step, args.steps, 1, "batched, replayed graph")
+    replay = {"ms_per_step": round(replay_ms, 3), "gcups": round(cells / (replay_ms / 1e3) / 1e9, 2),
+              "host_key_ms": round(ctx.stats()["host_plan_ms"], 3), "replayed": int(ctx.stats()["replayed"]),
+              "note": "identical calls: the plan is kept and the launches relaunch the captured HIP graph"}
+    # The shape table against the fixed rule it replaced (device time).
+    os.environ["SQMI_SHAPE_RULE"] = "fixed"
+    os.environ["SQMI_GRAPH"] = "0"
+    step()
+    fixed_dev = ctx.stats()["device_ms"]
+    os.environ.pop("SQMI_SHAPE_RULE", None)
+    step()
+    tuned_dev = ctx.stats()["device_ms"]
+    os.environ.pop("SQMI_GRAPH", None)
+    shape_table = {"tuned_device_ms": round(tuned_dev, 3), "fixed_rule_device_ms": round(fixed_dev, 3),
+                   "table": "seqan_amd/tuning/shapes.json", "provenance": shape_provenance()}
+    log(f"shape table: tuned {tuned_dev:.2f} ms, fixed rule {fixed_dev:.2f} ms")
 
     kern_ms = float(np.mean(dev_ms))
     achieved = OPS_PER_CELL * cells / (kern_ms / 1e3) / 1e12
@@ -506,6 +623,7 @@ def main():
             log(f"cpu baseline, one core: {one}")
     extra = {}
     if not args.no_extra:
+        extra["config1"] = config1_leg(ctx, 20, 3, not args.no_cpu)
         extra["fwd_bwd"] = backward_leg(ctx, 3, 1, not args.no_cpu)
         extra["protein_local"] = protein_leg(ctx, 3, 1, not args.no_cpu)
         extra["long_fwd_bwd"] = long_leg(ctx, 3, 1, not args.no_cpu)
@@ -523,6 +641,8 @@ def main():
         "plain": plain,
         "host_plan_ms": round(float(np.mean(plan_ms)), 2),
         "pairs_int32": int(st["pairs_int32"]),
+        "replay": replay,
+        "shape_table": shape_table,
     }
     res.update(extra)
     print(json.dumps(res))

@@ -190,9 +190,9 @@ void runLocalRowsCase(char const * name, StringSet<String<TAlphabet>> const & se
 // reference's banded function for one pair (global_alignment_banded.h:
 // 152-181, local_alignment_banded.h:242-259), pair by pair.
 template <typename TAlphabet, typename TScore, typename TGpu, typename... TConfig>
-void runBandedRowsCase(char const * name, bool local, StringSet<String<TAlphabet>> const & setH,
-                       StringSet<String<TAlphabet>> const & setV, TScore const & sc, TGpu const & ex, int lo, int up,
-                       TConfig const &... cfg)
+void runBandedRowsCaseImpl(char const * name, bool local, bool viaExecutor, StringSet<String<TAlphabet>> const & setH,
+                           StringSet<String<TAlphabet>> const & setV, TScore const & sc, TGpu const & ex, int lo,
+                           int up, TConfig const &... cfg)
 {
     using TGaps = Gaps<String<TAlphabet> const>;
     StringSet<TGaps> gH, gV, rH, rV;
@@ -204,9 +204,13 @@ void runBandedRowsCase(char const * name, bool local, StringSet<String<TAlphabet
         appendValue(rH, TGaps(setH[i]));
         appendValue(rV, TGaps(setV[i]));
     }
-    for (size_t i = 0; i < length(rH); ++i)
-        appendValue(expect, local ? localAlignment(rH[i], rV[i], sc, lo, up)
-                                  : globalAlignment(rH[i], rV[i], sc, cfg..., lo, up));
+    if (viaExecutor)
+        expect = local ? localAlignment<Parallel, Vectorial>(ex, rH, rV, sc, lo, up)
+                       : globalAlignment<Parallel, Vectorial>(ex, rH, rV, sc, cfg..., lo, up);
+    else
+        for (size_t i = 0; i < length(rH); ++i)
+            appendValue(expect, local ? localAlignment(rH[i], rV[i], sc, lo, up)
+                                      : globalAlignment(rH[i], rV[i], sc, cfg..., lo, up));
     auto same = [](TGaps & a, TGaps & b) {
         return rowText(a) == rowText(b) && clippedBeginPosition(a) == clippedBeginPosition(b) &&
                clippedEndPosition(a) == clippedEndPosition(b) && beginPosition(a) == beginPosition(b) &&
@@ -227,6 +231,17 @@ void runBandedRowsCase(char const * name, bool local, StringSet<String<TAlphabet
     {
         std::printf("case %s %s (%s)\n", name, e.status() == SQMI_ERR_NO_DEVICE ? "nodevice" : "error", e.what());
     }
+}
+
+// Banded rows on a single pair each (the reference's banded function for one
+// pair), or, viaExecutor, on the whole sets through the reference's own
+// ExecutionPolicy<Parallel, Vectorial> executor.
+template <typename TAlphabet, typename TScore, typename TGpu, typename... TConfig>
+void runBandedRowsCase(char const * name, bool local, StringSet<String<TAlphabet>> const & setH,
+                       StringSet<String<TAlphabet>> const & setV, TScore const & sc, TGpu const & ex, int lo, int up,
+                       TConfig const &... cfg)
+{
+    runBandedRowsCaseImpl(name, local, false, setH, setV, sc, ex, lo, up, cfg...);
 }
 
 }  // namespace
@@ -318,15 +333,18 @@ int main()
     runCase("protein_local_banded_blosum62", [&] { return localAlignmentScore(ex, eph, epv, b62, -20, 30); },
             [&] { return localAlignmentScore(cpu, eph, epv, b62, -20, 30); });
     // Bands the library does not take (dp_algorithm_impl.h:192-232, 595-700):
-    // the overloads hand them to the reference's executor.  A one-diagonal
-    // band, a band whose edge is the corner diagonal under free ends, and a
-    // local band that only reaches the corner column.
+    // the overloads hand them to the reference's executor, so they must give
+    // what the reference's own ExecutionPolicy<Parallel, Vectorial> call
+    // gives (its template named explicitly: the overloads are not viable
+    // with these template arguments).  A one-diagonal band, a band whose edge
+    // is the corner diagonal under free ends, and a local band that only
+    // reaches the corner column.
     runCase("dna_banded_one_diagonal", [&] { return globalAlignmentScore(ex, eh, ev, dna, 0, 0); },
-            [&] { return globalAlignmentScore(cpu, eh, ev, dna, 0, 0); });
+            [&] { return globalAlignmentScore<Parallel, Vectorial>(ex, eh, ev, dna, 0, 0); });
     runCase("dna_banded_corner_semiglobal", [&] { return globalAlignmentScore(ex, eh, ev, dna, semi, 0, 12); },
-            [&] { return globalAlignmentScore(cpu, eh, ev, dna, semi, 0, 12); });
+            [&] { return globalAlignmentScore<Parallel, Vectorial>(ex, eh, ev, dna, semi, 0, 12); });
     runCase("dna_local_banded_corner", [&] { return localAlignmentScore(ex, eh, ev, dna, 119, 125); },
-            [&] { return localAlignmentScore(cpu, eh, ev, dna, 119, 125); });
+            [&] { return localAlignmentScore<Parallel, Vectorial>(ex, eh, ev, dna, 119, 125); });
 
     runRowsCase("dna_rows", dh, dv, dna, ex, cpu);
     runRowsCase("protein_rows_blosum62", ph, pv, b62, ex, cpu);
@@ -346,7 +364,7 @@ int main()
     // (No one-diagonal rows case: the reference's own vectorised banded
     // traceback, which these calls fall back to, does not return for a
     // one-diagonal band.)
-    runBandedRowsCase("dna_banded_rows_corner_semiglobal", false, eh, ev, dna, ex, -9, 0, semi);
+    runBandedRowsCaseImpl("dna_banded_rows_corner_semiglobal", false, true, eh, ev, dna, ex, -9, 0, semi);
 
     // |setH| != |setV| raises instead of asserting.
     StringSet<String<Dna>> shorter = dv;
