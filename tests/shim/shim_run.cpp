@@ -248,6 +248,7 @@ void runBandedRowsCase(char const * name, bool local, StringSet<String<TAlphabet
 
 int main()
 {
+    std::setvbuf(stdout, nullptr, _IOLBF, 0);   // one line per case, as it finishes
     ExecutionPolicy<Parallel, Vectorial> ex;
     ExecutionPolicy<Parallel, Serial> cpu;
     setNumThreads(cpu, 4);
@@ -361,10 +362,15 @@ int main()
     runBandedRowsCase("dna_banded_rows_semiglobal", false, eh, ev, dna, ex, -8, 12, semi);
     runBandedRowsCase("dna_local_banded_rows", true, eh, ev, dna, ex, -6, 9);
     runBandedRowsCase("protein_local_banded_rows_blosum62", true, eph, epv, b62, ex, -20, 30);
-    // (No one-diagonal rows case: the reference's own vectorised banded
-    // traceback, which these calls fall back to, does not return for a
-    // one-diagonal band.)
-    runBandedRowsCaseImpl("dna_banded_rows_corner_semiglobal", false, true, eh, ev, dna, ex, -9, 0, semi);
+    // A rows call the library does not take (one H of a single residue):
+    // the overloads hand the whole sets to the reference's executor, so they
+    // must give what its own ExecutionPolicy<Parallel, Vectorial> call gives.
+    // (No corner-band rows case: the reference's vectorised banded traceback,
+    // which such calls fall back to, does not return for a one-diagonal band
+    // and returns only now and then for a band ending on the corner diagonal.)
+    StringSet<String<Dna>> sh = eh;
+    resize(sh[7], 1);
+    runBandedRowsCaseImpl("dna_banded_rows_single_residue_h", false, true, sh, ev, dna, ex, -5, 5);
 
     // |setH| != |setV| raises instead of asserting.
     StringSet<String<Dna>> shorter = dv;

@@ -28,7 +28,7 @@ CASES = ["dna_global_nonconst", "dna_semiglobal_const", "dna_overlap_rvalue", "d
          "dna_banded", "dna_banded_semiglobal", "protein_banded_blosum62", "protein_local_banded_blosum62",
          "dna_banded_rows", "dna_banded_rows_semiglobal", "dna_local_banded_rows",
          "protein_local_banded_rows_blosum62", "dna_banded_one_diagonal", "dna_banded_corner_semiglobal",
-         "dna_local_banded_corner", "dna_banded_rows_corner_semiglobal",
+         "dna_local_banded_corner", "dna_banded_rows_single_residue_h",
          "length_mismatch"]
 
 
