@@ -63,13 +63,15 @@ __device__ __forceinline__ uint32_t signBytes(s16x2 t0, s16x2 t1)
 }
 
 // Lane l of every group of G lanes receives v from lane l-1; lane 0 of the
-// group receives `first`.
-// G = 16: DPP row_shr:1 (lane 0 of each row of 16 keeps `first`).
+// group receives `first`.  One DPP move whose lanes without a source keep
+// the old value (`first`): row_shr:1 for rows of 16 lanes, wave_shr:1 for
+// the whole wavefront.
 template <int G>
 __device__ __forceinline__ uint32_t shiftIn(uint32_t v, uint32_t first)
 {
-    static_assert(G == 16, "the packed family is built for groups of 16 lanes");
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)first, (int)v, 0x111, 0xf, 0xf, false);
+    static_assert(G == 16 || G == 64, "groups of 16 lanes or whole wavefronts");
+    constexpr int ctrl = G == 16 ? 0x111 : 0x138;
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)first, (int)v, ctrl, 0xf, 0xf, false);
 }
 
 template <int G>
@@ -120,11 +122,13 @@ constexpr int kCoopWaves = 4;
 __shared__ uint32_t gProd[kCoopWaves];
 
 // Per-group LDS ring of the profile words of both pairs of a couple, by
-// column: entry x holds column x - 15 (lane l reads entry s - l + 15 at
-// step s).  Refilled every kRefill steps from residues loaded one refill
-// earlier, so the column sweep issues no residue loads.
-constexpr int kRing = 64, kRefill = 32;
-__shared__ uint2 gRing[256 / 16][kRing];
+// column: 4 G entries per group, entry x holds column x - (G - 1) (lane l
+// reads entry s - l + G - 1 at step s).  Refilled every 2 G steps from
+// residues loaded one refill earlier, so the column sweep issues no residue
+// loads.  The columns live during one refill period span 3 G - 1 entries.
+// (256 / G) groups of 4 G entries: the same 8 KB for every G.
+constexpr int kRingWords = 4 * 256;
+__shared__ uint2 gRing[kRingWords];
 
 // Per-lane state of one pair couple (lo = pair A, hi = pair B).
 struct Couple {
@@ -475,8 +479,9 @@ struct Fwd16 {
         // The couple's profile words come from the group's LDS ring.  A
         // refill for steps [s, s + kRefill) writes columns s + 2 l and
         // s + 2 l + 1 of lane l; their residues were loaded a refill earlier.
-        static_assert(G == 16 && kRefill == 2 * G && kRefill % QD == 0, "ring refill covers 2 columns per lane");
-        uint2 *ring = gRing[threadIdx.x / G];
+        constexpr int kRing = 4 * G, kRefill = 2 * G;
+        static_assert(kRefill % QD == 0 && (kRing & (kRing - 1)) == 0, "ring refill covers 2 columns per lane");
+        uint2 *ring = gRing + (threadIdx.x / G) * kRing;
         uint32_t hr[4];
         auto refillLoad = [&](int s) {
             const int j0 = s + 2 * l;
@@ -612,9 +617,14 @@ struct Fwd16 {
                 }
                 coop = true;
             } else {
-                it = 0;
-                if (lane == 0)
-                    it = (int)atomicAdd(a.queue, 1u);
+                // Every lane takes part in the atomic (lane 0 adds 1, the
+                // others 0) and the wave takes lane 0's old value.  A
+                // branch on lane == 0 here could be merged with the
+                // lane-0 store at the end of the loop (the same predicate
+                // when G = 64) into divergent paths around the loop's back
+                // edge, where this readfirstlane would then read a lane
+                // that holds no item index.
+                it = (int)atomicAdd(a.queue, lane == 0 ? 1u : 0u);
                 it = __builtin_amdgcn_readfirstlane(it) + a.nLong;
                 if (it >= a.nItems)
                     break;
@@ -1305,13 +1315,11 @@ const void *kernelTrk(uint32_t track)
     }
 }
 
-// The compiled (G, R) family of this template: groups of 16 lanes (the LDS
-// profile ring assumes them).  An earlier build of this template with G = 64
-// (one pair couple per wavefront) did not complete on gfx950 in one test
-// run; its cause was never established (its loop control, item loads and
-// lane shifts read correct in its assembly), so no G = 64 build exists.  The
-// one-pair-per-lane form is a separate kernel (lane.hip).
-#define SQMI_FOR_EACH_GR(X) X(16, 16) X(16, 8)
+// The compiled (G, R) family of this template: a pair couple per group of
+// 16 lanes, or per wavefront (G = 64: the short-batch shapes, four times the
+// lanes per pair for the same rows).  The one-pair-per-lane form is a
+// separate kernel (lane.hip).
+#define SQMI_FOR_EACH_GR(X) X(16, 16) X(16, 8) X(64, 4) X(64, 8) X(64, 16)
 
 template <bool DNA>
 const void *kernelFor(const Variant16 &v)

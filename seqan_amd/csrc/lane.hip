@@ -284,9 +284,10 @@ struct Lane16 {
         const size_t slot = (size_t)blockIdx.x * (blockDim.x >> 6) + wave;
         uint2 *rb = reinterpret_cast<uint2 *>(a.rowbuf) + slot * (size_t)a.rowCols * 64 + lane;
         for (;;) {
-            int it = 0;
-            if (lane == 0)
-                it = (int)atomicAdd(a.queue, 1u);
+            // Branch-free: every lane takes part (lane 0 adds 1), so no
+            // lane-0 branch can be merged with another around the back edge
+            // and this readfirstlane always runs with lane 0 active.
+            int it = (int)atomicAdd(a.queue, lane == 0 ? 1u : 0u);
             it = __builtin_amdgcn_readfirstlane(it);
             if (it >= a.nItems)
                 break;

@@ -7,7 +7,8 @@ substitution table, global or local), every length bin b (2^(b-1) < |V| + 1
 <= 2^b) and every count class (how many of a call's pairs fall in the bin),
 it times one call of the library on synthetic pairs of that bin with each
 compiled shape forced (G = 16 with R = 16 / 8: a 16-lane group per pair
-couple; G = 1 with R = 16 / 32: one pair couple per lane), checks that all
+couple; G = 1 with R = 16 / 32: one pair couple per lane; G = 64 with
+R = 4 / 8: a wavefront per pair couple), checks that all
 shapes give the same scores, and keeps the fastest.  Measurements bounded
 by --cells (cells per call) cover the classes they can; a class needing more
 cells takes the choice of the nearest measured class below it.  The table it
@@ -35,7 +36,7 @@ from . import (LOCAL, FREE_TOP, FREE_BOTTOM, Context, DeviceBuffer, SeqSet, blos
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning", "shapes.json")
-SHAPES = [(16, 16), (16, 8), (1, 16), (1, 32)]
+SHAPES = [(16, 16), (16, 8), (1, 16), (1, 32), (64, 4), (64, 8)]
 COUNT_CLASSES = [0, 16384, 131072, 524288]
 CLASS_REP = [4096, 32768, 262144, 1048576]   # pairs timed for each class
 CATEGORIES = ["dna", "dna_local", "table", "table_local"]
@@ -144,6 +145,12 @@ def gpu_name():
             arch = val
         if key == "Marketing Name" and arch is not None and name is None:
             name = val
+    if not name:
+        try:   # rocminfo leaves the marketing name blank on some hosts
+            import torch
+            name = torch.cuda.get_device_name(0)
+        except Exception:
+            name = None
     return f"{name or 'unknown'} ({arch or 'unknown'})"
 
 

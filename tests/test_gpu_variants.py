@@ -8,13 +8,9 @@ import seqan_amd as sa
 
 pytestmark = pytest.mark.gpu
 
-G64_SKIP = pytest.mark.skip(
-    reason="G=64 (one pair couple per wavefront) is not built: an earlier build did not complete on gfx950 "
-           "(this test timed out) and its cause was never established; kept off the GPU until one is shown")
-# (16, R): a group of 16 lanes per pair couple (kFwd16); (1, R): one pair
-# couple per lane (kLane16).
-SHAPES = [(16, 16), (16, 8), (1, 16), (1, 32), pytest.param((64, 16), marks=G64_SKIP),
-          pytest.param((64, 4), marks=G64_SKIP)]
+# (16, R): a group of 16 lanes per pair couple; (64, R): a wavefront per
+# pair couple (kFwd16); (1, R): one pair couple per lane (kLane16).
+SHAPES = [(16, 16), (16, 8), (64, 4), (64, 8), (64, 16), (1, 16), (1, 32)]
 
 
 def _random_set(rng, count, lo, hi, alpha):
@@ -125,3 +121,19 @@ def test_lane_shape_device_residues_unaligned(ctx):
     finally:
         dh.free()
         dv.free()
+
+
+@pytest.mark.parametrize("min_blocks", ["2", "0"])
+def test_wavefront_shape_cooperative_and_long_rows(ctx, monkeypatch, min_blocks):
+    # A wavefront per pair couple over pairs of many row blocks (1,500-row
+    # pairs are six 256-row blocks of G = 64, R = 4), swept by one wave or by
+    # the four waves of a workgroup together.
+    monkeypatch.setenv("SQMI_COOP_MIN_BLOCKS", min_blocks)
+    rng = np.random.default_rng(int(min_blocks) + 60)
+    dna = sa.simple_score(2, -3, -1, -5)
+    H, V = _random_set(rng, 120, 1, 1500, 4)
+    for shape in ((64, 4), (64, 8)):
+        for mode in (0, 9, 15, sa.LOCAL):
+            print("wavefront shape", shape, "coop", min_blocks, "mode", mode, flush=True)
+            got = ctx.align_score(H, V, dna, mode, flags=sa.force_shape(*shape))
+            assert np.array_equal(got, _oracle(H, V, dna, mode)), (shape, mode)
