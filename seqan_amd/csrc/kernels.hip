@@ -30,6 +30,7 @@
 // at the -inf sentinel) and the border row emerges from the recurrence
 // itself (TOP free: floored at 0).
 #include <algorithm>
+#include <type_traits>
 
 #include "kernels.hpp"
 #include "packed16.hpp"
@@ -126,8 +127,8 @@ __shared__ uint32_t gProd[kCoopWaves];
 // reads entry s - l + G - 1 at step s).  Refilled every 2 G steps from
 // residues loaded one refill earlier, so the column sweep issues no residue
 // loads.  The columns live during one refill period span 3 G - 1 entries.
-// (256 / G) groups of 4 G entries: the same 8 KB for every G.
-constexpr int kRingWords = 4 * 256;
+// (256 / G) groups of 4 G entries, each followed by a copy of its first 4.
+constexpr int kRingWords = 4 * 256 + (256 / 16) * 4;
 __shared__ uint2 gRing[kRingWords];
 
 // Per-lane state of one pair couple (lo = pair A, hi = pair B).
@@ -197,12 +198,28 @@ struct Fwd16 {
         const int TW = a.alpha + 1;
         const int rowbase = b * BR + l * R;
 
+        // Rows above a pair's row 0 (block 0 only) are virtual.  The trace
+        // kernels floor H of every row of block 0 (rowK = 0 for row 0 under a
+        // free top, else -inf).  The score kernels instead compute block 0
+        // with saturating adds: the virtual rows start at -inf and every term
+        // adds a value <= 0 to them (e, o, and -256 for the row padding
+        // code), so they stay in [-32768, kNeg16] and never wrap; row 0 under
+        // a free top extends its horizontal gap by 0 from E = 0 (rowK = 0,
+        // else e), which keeps E(0, j) = H(0, j) = 0.  Real rows never reach
+        // the saturation bounds (the int16 window), so their values are the
+        // wrapping adds' values.
+        constexpr bool SAT = HEAD && !TB;
+        // ROWT: the score kernels take the bottom-row result (free last row:
+        // the maximum over row m, j <= n; global: cell (m, n)) from the
+        // bottom row that the last block stores like every other block,
+        // after its sweep, instead of tracking it per column.
+        constexpr bool ROWT = !TB && !(TRK & kTrackLocal) && (TRK & (kTrackBottom | kTrackEnd)) != 0;
         uint32_t sel[R];
-        s16x2 Gk[R], Ek[R], floorK[R];
+        s16x2 Gk[R], Ek[R], rowK[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const int r = rowbase + k;
-            int code[2], g0[2], fl[2];
+            int code[2], g0[2], fl[2], e0[2];
 #pragma unroll
             for (int x = 0; x < 2; ++x) {
                 const int i = r - c.p[x];
@@ -214,7 +231,8 @@ struct Fwd16 {
                 code[x] = cc;
                 // G at column 0: virtual rows -inf, border H(0,0)=0, first column.
                 g0[x] = (i < 0) ? kNeg16 : (i == 0) ? o : ((leftFree ? 0 : o + (i - 1) * eo) + o);
-                fl[x] = (i == 0 && topFree) ? 0 : kNeg16;
+                fl[x] = (i == 0 && topFree) ? 0 : (TB ? kNeg16 : e);
+                e0[x] = (SAT && i == 0 && topFree) ? 0 : kNeg16;
             }
             if constexpr (DNA) {
                 const uint32_t sa = code[0] < 4 ? (uint32_t)(code[0] | 0x0c00) : 0x0d0cu;
@@ -224,8 +242,8 @@ struct Fwd16 {
                 sel[k] = (uint32_t)code[0] | ((uint32_t)code[1] << 16);
             }
             Gk[k] = pack(g0[0], g0[1]);
-            Ek[k] = neg2;
-            floorK[k] = pack(fl[0], fl[1]);
+            Ek[k] = pack(e0[0], e0[1]);
+            rowK[k] = pack(fl[0], fl[1]);
         }
 
         // Row-buffer entry for step 0 (read before this block's column-0 write).
@@ -238,7 +256,7 @@ struct Fwd16 {
         }
         if (b > 0)
             rbv = rbLoad(rbIn);
-        if (!last && l == G - 1)
+        if ((!last || ROWT) && l == G - 1)
             rbStore(rbOut, ((uint64_t)asu(neg2) << 32) | asu(Gk[R - 1]));
         if (coop && !last)
             publish(prodOut, progTag(b, 0));
@@ -267,18 +285,34 @@ struct Fwd16 {
         if constexpr (TB)
             trBlock = trItem + (size_t)b * steps * 64;
 
-        // One column of the lane's R rows.  pA / pB: the column's profile
-        // (DNA: t bytes for v = 0..3) or table row offset, for pair A / B.
-        auto step = [&](int s, uint32_t pA, uint32_t pB) {
+        auto add = [](s16x2 x, s16x2 y) -> s16x2 {
+            if constexpr (SAT)
+                return addSat(x, y);
+            else
+                return x + y;
+        };
+
+        // One column of the lane's R rows, step s = s0 + u of a chunk.  pA /
+        // pB: the column's profile (DNA: t bytes for v = 0..3) or table row
+        // offset, for pair A / B.  CHK: the lane's column j is checked
+        // against 1..Nw (the chunks at the skewed ends of the sweep); in the
+        // other chunks every lane's column lies in 1..Nw.  rq: the chunk's
+        // row-buffer entries (column s0 + 1 + u at rq + 2 u); wq: this lane's
+        // bottom-row entries (column s0 + u - l at wq + 2 u).
+        auto step = [&](auto chk, int s, int u, uint32_t pA, uint32_t pB, const uint32_t *rq, uint32_t *wq) {
+            constexpr bool CHK = decltype(chk)::value;
             const int j = s - l;
             const s16x2 upG = as16(shiftIn<G>(botG, (uint32_t)rbv));
             const s16x2 upF = as16(shiftIn<G>(botF, (uint32_t)(rbv >> 32)));
-            // Prefetch the next row-buffer entry (column s+1 for the group head).
-            if (b > 0 && s + 1 <= Nw)
-                rbv = rbLoad(rbIn + 2 * (s + 1));
+            // Prefetch the next row-buffer entry (column s+1 for the group
+            // head; block 0's head reads the constant -inf row).
+            if constexpr (!HEAD) {
+                if (!CHK || s + 1 <= Nw)
+                    rbv = rbLoad(rq + 2 * u);
+            }
             const s16x2 diagTop = upGprev;
             upGprev = upG;
-            if (j >= 1 && j <= Nw) {
+            if (!CHK || (j >= 1 && j <= Nw)) {
                 s16x2 uG = upG, uF = upF;
                 uint32_t accEF0 = 0, accEF1 = 0, accDX0 = 0, accDX1 = 0;
                 // t(h, v) - gapOpen of row k for both pairs.
@@ -294,7 +328,7 @@ struct Fwd16 {
                 // D: the diagonal term of row k, G(i-1, j-1) + t, taken from
                 // the old G of row k - 1 before it is overwritten (no copies
                 // of the column's G values).
-                s16x2 D = diagTop + as16(tOf(0));
+                s16x2 D = add(diagTop, as16(tOf(0)));
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
                     s16x2 F, H;
@@ -319,18 +353,18 @@ struct Fwd16 {
                             asm volatile("v_bfi_b32 %0, %1, %2, %0" : "+v"(accDX1) : "s"(bit), "v"(dx));
                         }
                     } else {
-                        Ek[k] = vmax(Ek[k] + e2, Gk[k]);
-                        F = vmax(uF + e2, uG);
+                        Ek[k] = vmax(add(Ek[k], SAT ? rowK[k] : e2), Gk[k]);
+                        F = vmax(add(uF, e2), uG);
                         H = vmax(vmax(D, Ek[k]), F);
                     }
                     if constexpr (TRK & kTrackLocal) {
                         H = vmax(H, splat(0));
-                    } else if constexpr (HEAD) {
-                        H = vmax(H, floorK[k]);
+                    } else if constexpr (HEAD && TB) {
+                        H = vmax(H, rowK[k]);
                     }
                     if (k + 1 < R)
-                        D = Gk[k] + as16(tOf(k + 1));
-                    const s16x2 Gn = H + o2;
+                        D = add(Gk[k], as16(tOf(k + 1)));
+                    const s16x2 Gn = add(H, o2);
                     Gk[k] = Gn;
                     if constexpr ((TRK & kTrackLocal) && !TB)
                         best = vmax(best, Gn);
@@ -387,12 +421,12 @@ struct Fwd16 {
                         }
                     }
                 }
-                if constexpr (TRK & (kTrackEnd | kTrackBottom | kTrackRight)) {
+                if constexpr ((TRK & kTrackRight) || (!ROWT && (TRK & (kTrackEnd | kTrackBottom)))) {
                     const s16x2 rem = remBase - splat(s);   // n - j per half
                     // eq = 0xffff in halves where rem == 0.
                     const uint16_t r0 = (uint16_t)rem.x, r1 = (uint16_t)rem.y;
                     const uint32_t eq = (r0 == 0 ? 0xffffu : 0u) | (r1 == 0 ? 0xffff0000u : 0u);
-                    if constexpr (TRK & kTrackBottom) {
+                    if constexpr ((TRK & kTrackBottom) && !ROWT) {
                         if (last) {
                             const uint32_t after = asu(rem >> splat(15));   // 0xffff where j > n
                             const s16x2 cand = as16(bfi(after, asu(neg2), asu(uG)));
@@ -416,7 +450,7 @@ struct Fwd16 {
                         }
                     }
                     if constexpr (TRK & (kTrackEnd | kTrackRight)) {
-                        if constexpr (TRK & kTrackEnd) {
+                        if constexpr ((TRK & kTrackEnd) && !ROWT) {
                             if (last && l == G - 1) {
                                 best = as16(bfi(eq, asu(uG), asu(best)));
                                 if constexpr (TB) {
@@ -455,8 +489,8 @@ struct Fwd16 {
                         }
                     }
                 }
-                if (!last && l == G - 1)
-                    rbStore(rbOut + 2 * j, ((uint64_t)botF << 32) | botG);
+                if ((!last || ROWT) && l == G - 1)
+                    rbStore(wq + 2 * u, ((uint64_t)botF << 32) | botG);
                 if constexpr (TB)
                     trBlock[(size_t)s * 64] = make_uint4(accEF0, accEF1, accDX0, accDX1);
             }
@@ -479,9 +513,12 @@ struct Fwd16 {
         // The couple's profile words come from the group's LDS ring.  A
         // refill for steps [s, s + kRefill) writes columns s + 2 l and
         // s + 2 l + 1 of lane l; their residues were loaded a refill earlier.
-        constexpr int kRing = 4 * G, kRefill = 2 * G;
+        // Entries 0 .. QD - 1 are mirrored after the ring, so that a chunk
+        // reads its QD entries from one base without a wrap.
+        constexpr int kRing = 4 * G, kRefill = 2 * G, kStride = kRing + QD;
         static_assert(kRefill % QD == 0 && (kRing & (kRing - 1)) == 0, "ring refill covers 2 columns per lane");
-        uint2 *ring = gRing + (threadIdx.x / G) * kRing;
+        static_assert((256 / G) * kStride <= kRingWords, "ring layout");
+        uint2 *ring = gRing + (threadIdx.x / G) * kStride;
         uint32_t hr[4];
         auto refillLoad = [&](int s) {
             const int j0 = s + 2 * l;
@@ -492,16 +529,24 @@ struct Fwd16 {
         };
         auto refillStore = [&](int s) {
             const int j0 = s + 2 * l;
-            ring[(s + G - 1 + 2 * l) & (kRing - 1)] = make_uint2(prof(hcode(0, j0, hr[0])), prof(hcode(1, j0, hr[2])));
-            ring[(s + G + 2 * l) & (kRing - 1)] =
-                make_uint2(prof(hcode(0, j0 + 1, hr[1])), prof(hcode(1, j0 + 1, hr[3])));
+            const int x0 = (s + G - 1 + 2 * l) & (kRing - 1), x1 = (x0 + 1) & (kRing - 1);
+            const uint2 w0 = make_uint2(prof(hcode(0, j0, hr[0])), prof(hcode(1, j0, hr[2])));
+            const uint2 w1 = make_uint2(prof(hcode(0, j0 + 1, hr[1])), prof(hcode(1, j0 + 1, hr[3])));
+            ring[x0] = w0;
+            ring[x1] = w1;
+            if (x0 < QD)
+                ring[x0 + kRing] = w0;
+            if (x1 < QD)
+                ring[x1 + kRing] = w1;
             __builtin_amdgcn_wave_barrier();
         };
         refillLoad(0);
         refillStore(0);
         refillLoad(kRefill);
         uint2 pp = ring[(G - 1 - l) & (kRing - 1)];
-        for (int s0 = 0; s0 < steps; s0 += QD) {
+        // One chunk of QD steps from s0: the cooperative wait, the ring
+        // refill, the steps, the cooperative progress.
+        auto chunk = [&](auto chk, int s0) {
             if (coop) {
                 // This chunk reads columns up to s0 + QD of block b - 1.
                 const int need = min(s0 + QD, Nw);
@@ -516,11 +561,13 @@ struct Fwd16 {
                 refillStore(s0 + QD);
                 refillLoad(s0 + QD + kRefill);
             }
+            const uint2 *rr = ring + ((s0 + G - l) & (kRing - 1));   // rr[u]: the words of step s0 + u + 1
+            const uint32_t *rq = rbIn + 2 * (s0 + 1);
+            uint32_t *wq = rbOut + 2 * (s0 - l);
 #pragma unroll
             for (int u = 0; u < QD; ++u) {
-                const int s = s0 + u;
-                const uint2 ppN = ring[(s + 1 + G - 1 - l) & (kRing - 1)];
-                step(s, pp.x, pp.y);
+                const uint2 ppN = rr[u];
+                step(chk, s0 + u, u, pp.x, pp.y, rq, wq);
                 pp = ppN;
             }
             if (coop) {
@@ -528,11 +575,47 @@ struct Fwd16 {
                 if (!last && ((s0 / QD) % PUB == PUB - 1 || s0 + QD >= steps))
                     publish(prodOut, progTag(b, max(0, min(s0 + QD - G, Nw))));
             }
+        };
+        // Three loops, so that the steady chunks keep their registers from
+        // chunk to chunk: the head (some lane's column below 1), the steady
+        // chunks (every lane's column in 1..Nw), the tail (some lane past
+        // Nw).  The trace kernels check every chunk.
+        int s0 = 0;
+        if constexpr (!TB) {
+            for (; s0 < min(G, steps); s0 += QD)
+                chunk(std::true_type{}, s0);
+            for (; s0 + QD - 1 <= Nw; s0 += QD)
+                chunk(std::false_type{}, s0);
         }
+        for (; s0 < steps; s0 += QD)
+            chunk(std::true_type{}, s0);
         // Make this block's row-buffer writes visible to the next block.
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        if constexpr (ROWT) {
+            if (last) {
+                // Row m of both pairs: G(m, j) is the low word of row-buffer
+                // entry j (column 0 came from the block's init).
+                if constexpr (TRK & kTrackBottom) {
+                    // The lanes share the columns 1..Nw; a half takes the
+                    // columns up to its pair's n (-32768 beyond).
+                    const s16x2 nn = pack(c.n[0], c.n[1]);
+                    for (int j = 1 + l; j <= Nw; j += G) {
+                        const s16x2 g = as16((uint32_t)rbLoad(rbOut + 2 * j));
+                        const uint32_t lim = asu((nn - splat(j)) >> splat(15)) ^ 0x7fff7fffu;
+                        best = vmax(best, vmin(g, as16(lim)));
+                    }
+                } else {
+                    // The end cell (m, n) of each pair.
+                    if (l == G - 1) {
+                        const uint32_t ga = (uint32_t)rbLoad(rbOut + 2 * c.n[0]);
+                        const uint32_t gb = (uint32_t)rbLoad(rbOut + 2 * c.n[1]);
+                        best = as16((ga & 0xffffu) | (gb & 0xffff0000u));
+                    }
+                }
+            }
+        }
     }
 
     __device__ __forceinline__ static bool wellFormed(const FwdArgs &a, const WorkItem &w)
