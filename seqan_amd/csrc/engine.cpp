@@ -388,7 +388,7 @@ int longStreams()
 // team costs more than it saves.
 constexpr int64_t kParallelMin = 32768;
 
-// Stable LSD radix sort of 26-bit keys in two passes of 13 bits, each pass
+// Stable LSD radix sort of 28-bit keys in two passes of 14 bits, each pass
 // split into contiguous per-thread ranges so that equal digits keep their
 // input order.
 void radixSort(std::vector<Key> &keys)
@@ -401,7 +401,7 @@ void radixSort(std::vector<Key> &keys)
 #ifdef _OPENMP
     T = std::max(1, std::min(omp_get_max_threads(), (int)(n / 16384) + 1));
 #endif
-    constexpr int B = 13, NB = 1 << B;
+    constexpr int B = 14, NB = 1 << B;
     std::vector<uint32_t> hist((size_t)T * NB);
     for (int shift = 0; shift < 2 * B; shift += B) {
         const uint32_t mask = (uint32_t)NB - 1;
@@ -435,15 +435,46 @@ void radixSort(std::vector<Key> &keys)
     }
 }
 
-// Bucket keys (26 bits): class (3 bits: 0..5 int16 shape slot, 6 int32,
-// 7 empty) | 255 - row blocks (8 bits) | 32767 - |H| (15 bits).  Ascending
-// order puts int16 pairs first, grouped by shape and row blocks (most first),
-// widest first, then the int32 and the empty pairs in input order.
-constexpr int kKeyNBits = 15, kKeyBlockBits = 8, kKeyClassShift = kKeyNBits + kKeyBlockBits;
+// Bucket keys (28 bits): class (3 bits: 0..5 int16 shape slot, 6 int32,
+// 7 empty) | 1023 - row units (10 bits) | 32767 - |H| (15 bits).  Ascending
+// order puts int16 pairs first, grouped by shape and row units (most first),
+// widest first, then the int32 and the empty pairs in input order.  A row
+// unit is a quarter of a G*R-row block for the packed kernels with groups
+// of 16 or 64 lanes (their block 0 may be 1..4 quarters tall), a whole
+// block for the one-pair-per-lane kernel and the trace kernel.
+constexpr int kKeyNBits = 15, kKeyUnitBits = 10, kKeyClassShift = kKeyNBits + kKeyUnitBits;
 constexpr uint32_t kClassInt32 = 6, kClassEmpty = 7, kMaxShapeSlots = 6;
+constexpr int64_t kMaxBlocks16 = 255;   // row blocks per pair of a packed work item
 inline uint32_t bucketOf(uint32_t key) { return key >> kKeyNBits; }
 inline int32_t widthOf(uint32_t key) { return (int32_t)((1u << kKeyNBits) - 1 - (key & ((1u << kKeyNBits) - 1))); }
-inline int32_t blocksOf(uint32_t key) { return (int32_t)(255u - ((key >> kKeyNBits) & 255u)); }
+inline int32_t unitsOf(uint32_t key) { return (int32_t)(1023u - ((key >> kKeyNBits) & 1023u)); }
+// Row units of a shape: quarter blocks where block 0 may be shortened.
+inline bool quarterUnits(const Variant16 &v, bool traceShape) { return v.G >= 16 && v.R % 4 == 0 && !traceShape; }
+// Quarter-block shapes: a pair takes just enough quarters (fine), or whole
+// blocks counted as 4 quarters each (coarse).  Fine units cut the rows of
+// block 0 but split the pairs of a length bin over 4 times as many buckets,
+// which widens the spread of |H| inside a work item when the bin holds few
+// pairs; so a bin takes fine units from kFineUnitPairs pairs on
+// (SQMI_ROW_UNITS=block / quarter: always coarse / fine).
+constexpr int64_t kFineUnitPairs = 8192;
+int rowUnitsMode()   // 0 by bin size, 1 coarse, 2 fine
+{
+    const char *env = std::getenv("SQMI_ROW_UNITS");
+    if (env && std::strcmp(env, "block") == 0)
+        return 1;
+    if (env && std::strcmp(env, "quarter") == 0)
+        return 2;
+    return 0;
+}
+inline int64_t unitsFor(const Variant16 &v, int64_t m, bool quarter, bool fine)
+{
+    if (!quarter)
+        return blocksFor(v, m);
+    if (!fine)
+        return 4 * blocksFor(v, m);
+    const int64_t rows = (int64_t)v.G * v.R / 4;
+    return (m + 1 + rows - 1) / rows;
+}
 
 // Routing (empty / int16 / int32) and length bucketing into work items.
 // Pure host code: needs no device.
@@ -500,12 +531,14 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
     std::vector<Variant16> &shapes = P.shapes;
     shapes.clear();
     uint8_t binSlot[16];
+    bool fineBin[16] = {};
     int fallbackSlot = -1;
     const int forcedG = (int)((flags >> 8) & 0xFF), forcedR = (int)((flags >> 16) & 0xFF);
     if (traceShape || (forcedG && forcedR)) {
         shapes.push_back(traceShape ? Variant16{kTraceLanes, kTraceRows / kTraceLanes, si.dna, track}
                                     : Variant16{forcedG, forcedR, si.dna, track});
         std::fill(binSlot, binSlot + 16, (uint8_t)0);
+        std::fill(fineBin, fineBin + 16, rowUnitsMode() == 2);
         if (shapes[0].G == 1) {
             // Pairs taller than 255 blocks of the forced lane shape.
             shapes.push_back(Variant16{16, 16, si.dna, track});
@@ -543,6 +576,9 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
             binSlot[b] = (uint8_t)(binCount[b] ? slotOf(v) : 0);
         }
         fallbackSlot = slotOf(Variant16{16, 16, si.dna, track});
+        const int unitsMode = rowUnitsMode();
+        for (int b = 0; b < 16; ++b)
+            fineBin[b] = unitsMode == 2 || (unitsMode == 0 && binCount[b] >= kFineUnitPairs);
     }
 #pragma omp parallel for schedule(static) reduction(+ : cells) if (N > kParallelMin)
     for (int64_t i = 0; i < N; ++i) {
@@ -555,13 +591,16 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
             if (route16(n, m)) {
                 uint32_t slot = binSlot[lengthBin(m)];
                 int64_t nblocks = blocksFor(shapes[slot], m);
-                if (nblocks > 255 && fallbackSlot >= 0) {
+                if (nblocks > kMaxBlocks16 && fallbackSlot >= 0) {
                     slot = (uint32_t)fallbackSlot;
                     nblocks = blocksFor(shapes[slot], m);
                 }
-                if (nblocks <= 255)
-                    key = (slot << kKeyClassShift) | ((255u - (uint32_t)nblocks) << kKeyNBits) |
+                if (nblocks <= kMaxBlocks16) {
+                    const int64_t units =
+                        unitsFor(shapes[slot], m, quarterUnits(shapes[slot], traceShape), fineBin[lengthBin(m)]);
+                    key = (slot << kKeyClassShift) | ((1023u - (uint32_t)units) << kKeyNBits) |
                           ((1u << kKeyNBits) - 1 - (uint32_t)n);
+                }
             }
         }
         keys[i] = Key{key, (int32_t)i};
@@ -650,6 +689,7 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
         const size_t p0 = starts[b0], p1 = starts[b1];
         const size_t i0 = itemOff[b0];
         const int PP = pairsPerWave(pl.var);
+        const bool quarter = quarterUnits(pl.var, traceShape);
         pl.order.resize(p1 - p0);
         pl.items.resize(itemOff[b1] - i0);
         int32_t maxN = 0;
@@ -662,11 +702,14 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
 #pragma omp parallel for schedule(dynamic, 8) if (p1 - p0 > (size_t)kParallelMin)
         for (int64_t b = (int64_t)b0; b < (int64_t)b1; ++b) {
             size_t it = itemOff[b] - i0;
+            const int32_t units = unitsOf(keys16[starts[b]].key);
+            const int32_t nblocks = quarter ? (units + 3) / 4 : units;
+            const int32_t headQ = quarter ? units - 4 * (nblocks - 1) : 4;   // quarters in block 0
             for (size_t k = starts[b]; k < starts[b + 1]; k += PP, ++it) {
                 const size_t e = std::min(starts[b + 1], k + PP);
                 // Widest first inside a bucket: the first pair has the item's width.
-                pl.items[it] = WorkItem{(int32_t)(k - p0), (int32_t)(e - k), widthOf(keys16[k].key),
-                                        blocksOf(keys16[k].key)};
+                pl.items[it] = WorkItem{(int32_t)(k - p0), (int32_t)(e - k) | ((headQ & 3) << 16),
+                                        widthOf(keys16[k].key), nblocks};
             }
         }
     }
@@ -686,8 +729,10 @@ void fillStats(const Planned &P, sqmi_stats &st)
         items += (int64_t)pl.items.size();
         if (pl.var.G == 1)
             st.pairs_lane += (int64_t)pl.order.size();
-        for (const WorkItem &w : pl.items)
-            padded += 2LL * (64 / pl.var.G) * (int64_t)w.maxN * w.nblocks * pl.var.G * pl.var.R;
+        for (const WorkItem &w : pl.items) {
+            const int64_t BR = (int64_t)pl.var.G * pl.var.R, headQ = ((w.count >> 16) & 3) ? ((w.count >> 16) & 3) : 4;
+            padded += 2LL * (64 / pl.var.G) * (int64_t)w.maxN * ((w.nblocks - 1) * BR + headQ * BR / 4);
+        }
     }
     st.work_items = items;
     st.padded_cells = padded;
@@ -1154,7 +1199,7 @@ static sqmi_status alignScore(sqmi_context *ctx, const sqmi_sequences *setH, con
         if (flags & SQMI_SCORES_ON_DEVICE)
             scal.push_back((int64_t)(uintptr_t)scores);
         for (const char *name : {"SQMI_COOP_MIN_BLOCKS", "SQMI_LONG_ALL", "SQMI_LONG_MIN_SIDE", "SQMI_LONG_STREAMS",
-                                 "SQMI_SHAPE_RULE", "SQMI_TRACE_BUDGET_MB"}) {
+                                 "SQMI_SHAPE_RULE", "SQMI_TRACE_BUDGET_MB", "SQMI_ROW_UNITS"}) {
             const char *v = std::getenv(name);
             for (const char *c = v ? v : ""; *c; ++c)
                 scal.push_back(*c);

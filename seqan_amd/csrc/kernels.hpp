@@ -25,7 +25,9 @@ enum Track : uint32_t {
 // row blocks; processed together by one wavefront.
 struct WorkItem {
     int32_t first;      // index of the first pair in `order`
-    int32_t count;      // number of pairs in the item
+    int32_t count;      // bits 0..15: number of pairs in the item; bits 16..17
+                        // (packed kernels with G = 16 / 64): quarters of a block
+                        // held by block 0 (0: a whole block)
     int32_t maxN;       // max |H| over the item's pairs
     int32_t nblocks;    // row blocks of G*R rows (same for all pairs)
 };

@@ -137,3 +137,25 @@ def test_wavefront_shape_cooperative_and_long_rows(ctx, monkeypatch, min_blocks)
             print("wavefront shape", shape, "coop", min_blocks, "mode", mode, flush=True)
             got = ctx.align_score(H, V, dna, mode, flags=sa.force_shape(*shape))
             assert np.array_equal(got, _oracle(H, V, dna, mode)), (shape, mode)
+
+
+@pytest.mark.parametrize("shape", [(16, 16), (16, 8), (64, 4), (64, 8), (64, 16)])
+def test_fine_row_units_match_oracle(ctx, monkeypatch, shape):
+    # Every pair in quarter-block row units: block 0 of an item holds 1..4
+    # quarters of the shape's rows (R / 4 .. R rows per lane), chosen by the
+    # items' tallest pair; every quarter count occurs among these lengths.
+    monkeypatch.setenv("SQMI_ROW_UNITS", "quarter")
+    G, R = shape
+    rng = np.random.default_rng(G * 7 + R)
+    dna = sa.simple_score(2, -3, -1, -5)
+    H, V = _random_set(rng, 300, 1, 1100, 4)
+    for mode in (0, 9, 15, 4, 2, sa.LOCAL):
+        print("fine units", shape, "dna mode", mode, flush=True)
+        got = ctx.align_score(H, V, dna, mode, flags=sa.force_shape(G, R))
+        assert np.array_equal(got, _oracle(H, V, dna, mode)), (shape, mode)
+    P, Q = _random_set(rng, 80, 1, 500, 24)
+    b62 = sa.blosum62(-1, -11)
+    for mode in (16, 0, 9):
+        print("fine units", shape, "protein mode", mode, flush=True)
+        got = ctx.align_score(P, Q, b62, mode, flags=sa.force_shape(G, R))
+        assert np.array_equal(got, _oracle(P, Q, b62, mode)), (shape, mode)
