@@ -306,7 +306,20 @@ struct Fwd16 {
         // other chunks every lane's column lies in 1..Nw.  rq: the chunk's
         // row-buffer entries (column s0 + 1 + u at rq + 2 u); wq: this lane's
         // bottom-row entries (column s0 + u - l at wq + 2 u).
-        auto step = [&](auto chk, int s, int u, uint32_t pA, uint32_t pB, const uint32_t *rq, uint32_t *wq) {
+        // Table scores of row k for column words a / b (pair A / B): two LDS
+        // reads into the halves of one register.
+        auto look = [&](uint32_t a, uint32_t b, int k) -> uint32_t {
+            s16x2 r;
+            r.x = sT[(int)a + (int)(sel[k] & 0xffffu)];
+            r.y = sT[(int)b + (int)(sel[k] >> 16)];
+            return asu(r);
+        };
+        // Table kernels, steady chunks: the rows' scores of the next column
+        // (words nA / nB) are read one step ahead into tq, so that the LDS
+        // latency of the lookups is hidden behind a column of arithmetic.
+        uint32_t tq[DNA ? 1 : RB];
+        auto step = [&](auto chk, int s, int u, uint32_t pA, uint32_t pB, uint32_t nA, uint32_t nB,
+                        const uint32_t *rq, uint32_t *wq) {
             constexpr bool CHK = decltype(chk)::value;
             const int j = s - l;
             const s16x2 upG = as16(shiftIn<G>(botG, (uint32_t)rbv));
@@ -326,10 +339,12 @@ struct Fwd16 {
                 auto tOf = [&](int k) -> uint32_t {
                     if constexpr (DNA) {
                         return __builtin_amdgcn_perm(pB, pA, sel[k]);
+                    } else if constexpr (!CHK) {
+                        const uint32_t t = tq[k];
+                        tq[k] = look(nA, nB, k);
+                        return t;
                     } else {
-                        const uint32_t lo = (uint16_t)sT[(int)pA + (int)(sel[k] & 0xffffu)];
-                        const uint32_t hi = (uint16_t)sT[(int)pB + (int)(sel[k] >> 16)];
-                        return lo | (hi << 16);
+                        return look(pA, pB, k);
                     }
                 };
                 // D: the diagonal term of row k, G(i-1, j-1) + t, taken from
@@ -574,7 +589,7 @@ struct Fwd16 {
 #pragma unroll
             for (int u = 0; u < QD; ++u) {
                 const uint2 ppN = rr[u];
-                step(chk, s0 + u, u, pp.x, pp.y, rq, wq);
+                step(chk, s0 + u, u, pp.x, pp.y, ppN.x, ppN.y, rq, wq);
                 pp = ppN;
             }
             if (coop) {
@@ -591,6 +606,13 @@ struct Fwd16 {
         if constexpr (!TB) {
             for (; s0 < min(G, steps); s0 += QD)
                 chunk(std::true_type{}, s0);
+            if constexpr (!DNA) {
+                if (s0 + QD - 1 <= Nw) {
+#pragma unroll
+                    for (int k = 0; k < RB; ++k)
+                        tq[k] = look(pp.x, pp.y, k);   // the first steady step's scores
+                }
+            }
             for (; s0 + QD - 1 <= Nw; s0 += QD)
                 chunk(std::false_type{}, s0);
         }
@@ -878,10 +900,15 @@ struct Fwd16 {
     }
 };
 
-// Three waves per SIMD: the register budget (168 VGPRs) the R = 16 shapes
-// need without spilling inside the column loop.
+// Waves per SIMD the register budget is set for: three (168 VGPRs) for the
+// DNA kernels and the R = 8 table kernels; two (256 VGPRs) for the R = 16
+// table kernels, whose table-row offsets and scores read one column ahead
+// would spill at three.
+template <int R, bool DNA>
+constexpr int kFwdWaves = (!DNA && R >= 16) ? 2 : 3;
+
 template <int G, int R, bool DNA, uint32_t TRK>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kFwd16(FwdArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kFwdWaves<R, DNA>))) void kFwd16(FwdArgs a)
 {
     Fwd16<G, R, DNA, TRK>::run(a);
 }
