@@ -268,12 +268,15 @@ bool fixedShapeRule()
     return env && std::strcmp(env, "fixed") == 0;
 }
 
-Variant16 tableShape(int category, int bin, int64_t pairsInBin, bool dna, uint32_t track)
+// The shape of length bin `bin` for a call holding pairs16 pairs on the
+// packed route (all bins: the shapes' launches run side by side, so the
+// whole call is what fills the GPU).
+Variant16 tableShape(int category, int bin, int64_t pairs16, bool dna, uint32_t track)
 {
     if (fixedShapeRule())
         return Variant16{16, bin <= 7 ? 8 : 16, dna, track};
     int cls = 0;
-    while (cls + 1 < kTabNClasses && pairsInBin >= kTabCountClasses[cls + 1])
+    while (cls + 1 < kTabNClasses && pairs16 >= kTabCountClasses[cls + 1])
         ++cls;
     const int idx = kTabChoice[category][bin][cls];
     return Variant16{kTabShapes[idx][0], kTabShapes[idx][1], dna, track};
@@ -569,8 +572,29 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
             shapes.push_back(v);
             return (int)shapes.size() - 1;
         };
+        int64_t pairs16 = 0;
+        for (int b = 0; b < 16; ++b)
+            pairs16 += binCount[b];
+        // SQMI_BIN_SHAPES="b:G:R,...": the shape of bin b, in place of the
+        // table's (the shape tuner times whole calls with it).
+        int overG[16] = {}, overR[16] = {};
+        if (const char *env = std::getenv("SQMI_BIN_SHAPES")) {
+            for (const char *c = env; *c;) {
+                int b = -1, g = 0, r = 0, used = 0;
+                if (std::sscanf(c, "%d:%d:%d%n", &b, &g, &r, &used) != 3)
+                    break;
+                if (b >= 0 && b < 16) {
+                    overG[b] = g;
+                    overR[b] = r;
+                }
+                c += used;
+                if (*c == ',')
+                    ++c;
+            }
+        }
         for (int b = 0; b < 16; ++b) {
-            Variant16 v = tableShape(cat, b, binCount[b], si.dna, track);
+            Variant16 v = overG[b] ? Variant16{overG[b], overR[b], si.dna, track}
+                                   : tableShape(cat, b, pairs16, si.dna, track);
             if (!shapeSupported(v))
                 v = Variant16{16, 16, si.dna, track};
             binSlot[b] = (uint8_t)(binCount[b] ? slotOf(v) : 0);
@@ -1199,7 +1223,7 @@ static sqmi_status alignScore(sqmi_context *ctx, const sqmi_sequences *setH, con
         if (flags & SQMI_SCORES_ON_DEVICE)
             scal.push_back((int64_t)(uintptr_t)scores);
         for (const char *name : {"SQMI_COOP_MIN_BLOCKS", "SQMI_LONG_ALL", "SQMI_LONG_MIN_SIDE", "SQMI_LONG_STREAMS",
-                                 "SQMI_SHAPE_RULE", "SQMI_TRACE_BUDGET_MB", "SQMI_ROW_UNITS"}) {
+                                 "SQMI_SHAPE_RULE", "SQMI_TRACE_BUDGET_MB", "SQMI_ROW_UNITS", "SQMI_BIN_SHAPES"}) {
             const char *v = std::getenv(name);
             for (const char *c = v ? v : ""; *c; ++c)
                 scal.push_back(*c);
