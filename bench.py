@@ -321,6 +321,18 @@ def long_leg(ctx, steps, warmup, cpu):
            "pairs": N_DNA100K, "pairs_long": int(st["pairs_long"]),
            "pairs_long_packed": int(st_f["pairs_long_packed"]), "cells": cells, "seed": SEED_DNA100K,
            "counts_form_alignments": ident}
+    # The same pairs' scores under a free-end AlignConfig and local alignment
+    # (the long route's int32 forward and its result pass).
+    modes = {}
+    for name, mode in (("semi_global_1001", sa.FREE_TOP | sa.FREE_BOTTOM), ("local", sa.LOCAL)):
+        ms = timed(lambda: ctx.align_score(H, V, score, mode, scores_dev=ds.address, **dev), 1, 1,
+                   f"dna100k forward, {name}")
+        stm = ctx.stats()
+        ach = OPS_PER_CELL * cells / (stm["device_ms"] / 1e3) / 1e12
+        modes[name] = {"ms_per_step": round(ms, 3), "gcups": round(cells / (ms / 1e3) / 1e9, 2),
+                       "device_ms": round(stm["device_ms"], 3), "pairs_long": int(stm["pairs_long"]),
+                       "frac_of_int32_roof": round(ach / PEAK_TOPS_INT32, 4)}
+    res["forward_other_modes"] = modes
     if cpu:
         # The reference runs 16 pairs per SIMD batch on one thread; full-length
         # pairs would take minutes, so the sample is the first 16 pairs of the

@@ -53,7 +53,7 @@ struct Buffers
     // Traceback (sqmi_align_counts).
     DevBuf trace, traceOff, locs, endEF, endIJ, counts;
     // Long pairs (tiled int32 kernels).
-    DevBuf longCk, longPairs, longWalk, longTrace;
+    DevBuf longCk, longPairs, longWalk, longTrace, longBest;
     // Alignment rows (sqmi_align_rows, sqmi_align_local_rows).
     DevBuf ops, opOff, opLen, begins;
     // Banded scores (sqmi_align_score_banded).
@@ -64,8 +64,8 @@ struct Buffers
     {
         for (DevBuf *b : {&hRes, &vRes, &hOff, &vOff, &order, &items, &queue, &rowbuf, &scores, &tTable, &dnaProf,
                           &ids32, &scratchOff32, &scratch32, &table32, &emptyIds, &trace, &traceOff, &locs, &endEF,
-                          &endIJ, &counts, &longCk, &longPairs, &longWalk, &longTrace, &ops, &opOff, &opLen, &begins,
-                          &bandIds, &bandScratch})
+                          &endIJ, &counts, &longCk, &longPairs, &longWalk, &longTrace, &longBest, &ops, &opOff, &opLen,
+                          &begins, &bandIds, &bandScratch})
             f(*b);
     }
 };
@@ -329,6 +329,7 @@ struct Planned
     ScoringInfo si;
     uint32_t flags = 0;
     bool local = false;
+    uint32_t mode = 0;
     uint32_t track = 0;
     std::vector<int32_t> emptyIds, ids32;
     std::vector<int32_t> idsLong;   // int32 pairs for the tiled long-pair kernels
@@ -518,6 +519,7 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
             track = kTrackEnd;
     }
     P.track = track;
+    P.mode = mode;
     // Route every pair and give it a bucket key (see bucketOf).  The int16
     // pairs' shape slots: the trace shape, a forced shape, or per length bin
     // the shape table's choice for the number of the call's pairs in that
@@ -525,7 +527,9 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
     // takes G = 16, R = 16).
     std::vector<Key> keys((size_t)N);
     int64_t cells = 0;
-    const bool allLong = longAll() && si.dna && mode == 0 && !noLong;
+    // The long route: every AlignConfig and local for scores, global only
+    // for the traceback.
+    const bool allLong = longAll() && si.dna && (mode == 0 || !traceShape) && !noLong;
     const int64_t longMin0 = longMinSide();
     auto route16 = [&](int64_t n, int64_t m) {
         return !(allLong && n >= longMin0 && m >= longMin0) && costsOk16 && fitsInt16(si, n, m, local) &&
@@ -644,7 +648,7 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
     while (e32 < keys.size() && (keys[e32].key >> kKeyClassShift) == kClassInt32)
         ++e32;
     const int64_t longMin = longMinSide();
-    const bool longOk = si.dna && mode == 0 && !noLong;
+    const bool longOk = si.dna && (mode == 0 || !traceShape) && !noLong;
     for (size_t k = n16; k < e32; ++k) {
         const int32_t i = keys[k].id;
         const int64_t n = H.offsets[i + 1] - H.offsets[i];
@@ -745,7 +749,7 @@ void fillStats(const Planned &P, sqmi_stats &st)
     st.cells = P.cells;
     st.pairs_int32 = (int64_t)(P.ids32.size() + P.idsLong.size());
     st.pairs_long = (int64_t)P.idsLong.size();
-    st.pairs_long_packed = longPacked(P.si, P.flags) ? st.pairs_long : 0;
+    st.pairs_long_packed = (longPacked(P.si, P.flags) && P.mode == 0) ? st.pairs_long : 0;
     st.pairs_int16 = (int64_t)P.keys16.size();
     int64_t items = 0, padded = 0;
     st.pairs_lane = 0;
@@ -925,7 +929,9 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
     const int C = kLongTileC;
     const int64_t budget = traceBudget(ctx);
     const int64_t N = setH->count;
-    const bool packed = longPacked(P.si, flags);
+    // The packed forward is global only; free end gaps and local take the
+    // int32 forward and a result pass over its checkpoints.
+    const bool packed = longPacked(P.si, flags) && P.mode == 0;
     // Longest first, so that the couples of the packed forward pair similar shapes.
     std::vector<int32_t> ids(P.idsLong);
     std::stable_sort(ids.begin(), ids.end(), [&](int32_t x, int32_t y) {
@@ -935,7 +941,7 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
     });
     // Descriptors with checkpoint offsets relative to the pair's chunk.
     std::vector<size_t> chunkEnd;
-    int64_t run = 0, maxRun = 0;
+    int64_t run = 0, maxRun = 0, bestRun = 0;
     pairs.clear();
     for (int32_t id : ids) {
         LongPair lp{};
@@ -952,6 +958,8 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
         }
         lp.rowck = run;
         lp.colck = run + (int64_t)lp.nblocks * (lp.n + 1) * 2;
+        lp.bestck = bestRun;
+        bestRun += lp.nblocks;
         run += need;
         maxRun = std::max(maxRun, run);
         pairs.push_back(lp);
@@ -961,7 +969,8 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
     hipError_t err;
     if ((err = ensure(ctx->b.longCk, (size_t)maxRun * 4)) != hipSuccess ||
         (err = ensure(ctx->b.longPairs, sizeof(LongPair) * np)) != hipSuccess ||
-        (err = ensure(ctx->b.endEF, (size_t)8 * N)) != hipSuccess)
+        (err = ensure(ctx->b.endEF, (size_t)8 * N)) != hipSuccess ||
+        (P.local && (err = ensure(ctx->b.longBest, (size_t)4 * std::max<int64_t>(bestRun, 1))) != hipSuccess))
         return hipFail(ctx, err, "allocating long-pair buffers");
     if (dCounts && ((err = ensure(ctx->b.longWalk, (size_t)4 * kWalkStateInts * np)) != hipSuccess ||
                     (err = ensure(ctx->b.longTrace, sizeof(uint2) * (size_t)kTileRecSteps * 16 * kRoundSlots * np)) !=
@@ -986,6 +995,8 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
         a.dnaProfile = static_cast<const uint32_t *>(ctx->b.dnaProf.p);
         a.scores = in.dScores;
         a.endEF = static_cast<int32_t *>(ctx->b.endEF.p);
+        a.mode = P.mode;
+        a.best = static_cast<int32_t *>(ctx->b.longBest.p);
         a.alpha = P.si.alpha;
         int maxB = 0, maxT = 0;
         for (size_t k = b; k < e; ++k) {
@@ -1046,6 +1057,11 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
                      })) != hipSuccess)
                     return hipFail(ctx, err, "long-pair join");
             }
+        }
+        if (P.mode != 0) {
+            const TileArgs ta = a;
+            if ((err = op([=] { return launchLongResult(ta, stream); })) != hipSuccess)
+                return hipFail(ctx, err, "launching long-pair results");
         }
         if (dCounts) {
             a.trace = static_cast<uint2 *>(ctx->b.longTrace.p);

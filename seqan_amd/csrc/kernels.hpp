@@ -253,6 +253,7 @@ struct LongPair {
     int32_t p;
     int64_t rowck;      // int32 offset of the row checkpoints: nblocks x (n + 1) x (G, F)
     int64_t colck;      // int32 offset of the column checkpoints: nblocks x ntiles x kTileRows x (G, E)
+    int64_t bestck;     // local scores: offset of the pair's nblocks maxima (one per row block)
 };
 struct TileArgs {
     const uint8_t *hRes;
@@ -270,6 +271,8 @@ struct TileArgs {
     const uint32_t *dnaProfile;
     int32_t *scores;            // per pair id
     int32_t *endEF;             // per pair id: E, F of cell (m, n)
+    uint32_t mode;              // SQMI_FREE_* / SQMI_LOCAL (int32 forward; the packed one is global only)
+    int32_t *best;              // local: per row block of each pair (LongPair::bestck), the maximum G so far
     // traceback
     uint2 *trace;               // per long pair: kRoundSlots tiles x kTileRecSteps steps x 16 lanes records
     int32_t *walk;              // per long pair: walk state (kWalkStateInts ints)
@@ -295,6 +298,9 @@ hipError_t launchTileFwd32(const TileArgs &args, hipStream_t stream);
 constexpr int kLong16Span = 800;
 hipError_t launchTileFwd16(const TileArgs &args, hipStream_t stream);
 hipError_t launchTileTrace32(const TileArgs &args, hipStream_t stream);
+// Scores of long pairs under free end gaps or local alignment, from the
+// forward's checkpoints (free last row / column) or its running maximum.
+hipError_t launchLongResult(const TileArgs &args, hipStream_t stream);
 hipError_t launchTileWalkInit(const TileArgs &args, hipStream_t stream);
 hipError_t launchTileWalk32(const TileArgs &args, hipStream_t stream);
 

@@ -60,12 +60,18 @@ struct TileGeom {
 
 // One tile sweep for the lane's group.  TRACE: write trace records instead
 // of the forward outputs.  HEAD: the tile may hold rows above the pair's row
-// 0 (block 0), which are floored at -inf.
-template <bool HEAD, bool TRACE>
+// 0 (block 0), which are floored at -inf (row 0 at 0 under a free top).
+// LOCAL: Smith-Waterman (every H floored at 0, free borders), the maximum G
+// of the tile's cells folded into the block's slot of a.best.  The other AlignConfigs
+// (forward only): a free left border starts every row at H = 0; the free
+// last row and column are reduced from the checkpoints afterwards.
+template <bool HEAD, bool TRACE, bool LOCAL = false>
 __device__ __forceinline__ void tileSweep(const TileArgs &a, const LongPair &P, const TileGeom &g, int l,
                                           const uint32_t *sProf, GroupRings *rg, uint2 *rec)
 {
     const int o = a.gapOpen, e = a.gapExtend, eo = max(o, e);
+    const bool leftFree = !TRACE && (LOCAL || (a.mode & SQMI_FREE_LEFT));
+    const bool topFree = !TRACE && (a.mode & SQMI_FREE_TOP);
     const int C = a.tileC;
     const uint8_t *h = a.hRes + a.hOff[P.pid];
     const uint8_t *v = a.vRes + a.vOff[P.pid];
@@ -79,6 +85,7 @@ __device__ __forceinline__ void tileSweep(const TileArgs &a, const LongPair &P, 
 
     uint32_t sel[TR];
     int Gk[TR], Ek[TR];
+    int floorK[HEAD && !LOCAL ? TR : 1];
 #pragma unroll
     for (int k = 0; k < TR; ++k) {
         const int i = rowbase + k - P.p;
@@ -86,10 +93,12 @@ __device__ __forceinline__ void tileSweep(const TileArgs &a, const LongPair &P, 
         if (i >= 1 && i <= m)
             code = min((int)v[i - 1], 4);
         sel[k] = code < 4 ? (0x0c0c0c00u | (uint32_t)code) : 0x0d0d0d0cu;   // pad: t = -256
+        if constexpr (HEAD && !LOCAL)
+            floorK[k] = (i == 0 && topFree) ? 0 : kNegL;
         if (g.t == 0) {
-            // Column 0 of a global alignment: -inf above row 0, H(0,0) = 0,
-            // then the gap chain growing by max(o, e) per row.
-            Gk[k] = (i < 0) ? kNegL : (i == 0) ? o : (o + (i - 1) * eo) + o;
+            // Column 0: -inf above row 0 (local: 0), H(0,0) = 0, then the gap
+            // chain growing by max(o, e) per row (free left border: H = 0).
+            Gk[k] = (i < 0) ? (LOCAL ? o : kNegL) : (i == 0 || leftFree) ? o : (o + (i - 1) * eo) + o;
             Ek[k] = kNegL;
         } else if (g.active) {
             Gk[k] = ck[colIn + 2 * k];
@@ -111,6 +120,7 @@ __device__ __forceinline__ void tileSweep(const TileArgs &a, const LongPair &P, 
 
     int botG = Gk[TR - 1], botF = kNegL;
     int upGprev = kNegL;
+    int bestG = o;   // local: H = 0, the empty alignment
     const int width = g.c1 - g.c0 + 1;
     const int steps = ((width + TG + TQD - 1) / TQD) * TQD;
     // Ring refills: the refill for steps [s, s + TK) covers columns
@@ -175,12 +185,16 @@ __device__ __forceinline__ void tileSweep(const TileArgs &a, const LongPair &P, 
                 } else {
                     H = max(D, max(E, F));
                 }
-                if constexpr (HEAD)
-                    H = max(H, kNegL);
+                if constexpr (LOCAL)
+                    H = max(H, 0);
+                else if constexpr (HEAD)
+                    H = max(H, floorK[k]);
                 if (k + 1 < TR)
                     D = Gleft + (int)__builtin_amdgcn_perm(0u, pr, sel[k + 1]);
                 Ek[k] = E;
                 Gk[k] = H + o;
+                if constexpr (LOCAL)
+                    bestG = max(bestG, Gk[k]);
                 uG = Gk[k];
                 uF = F;
             }
@@ -192,7 +206,9 @@ __device__ __forceinline__ void tileSweep(const TileArgs &a, const LongPair &P, 
                 } else {
                     if (l == TG - 1)
                         *reinterpret_cast<int2 *>(ck + rowOut + 2 * jj) = make_int2(botG, botF);
-                    if (jj == g.c1 && g.t + 1 < P.ntiles) {
+                    // The right column (the last tile's too: a free last
+                    // column is reduced from it).
+                    if (jj == g.c1) {
 #pragma unroll
                         for (int k = 0; k < TR; ++k)
                             *reinterpret_cast<int2 *>(ck + colOut + 2 * k) = make_int2(Gk[k], Ek[k]);
@@ -222,6 +238,17 @@ __device__ __forceinline__ void tileSweep(const TileArgs &a, const LongPair &P, 
             step(s, prof, topv);
             prof = prN;
             topv = topN;
+        }
+    }
+    if constexpr (LOCAL) {
+        // The group's maximum into the block's slot: one tile of a block per
+        // launch, so the slot has no other writer (its first tile starts it).
+#pragma unroll
+        for (int d = 1; d < TG; d <<= 1)
+            bestG = max(bestG, __shfl_xor(bestG, d, TG));
+        if (g.active && l == 0) {
+            int32_t *slot = a.best + P.bestck + g.b;
+            *slot = g.t == 0 ? bestG : max(*slot, bestG);
         }
     }
 }
@@ -259,10 +286,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
     g.c0 = g.t * a.tileC + 1;
     g.c1 = min((g.t + 1) * a.tileC, P.n);
-    if (b0 == 0)
+    if (a.mode & SQMI_LOCAL) {
+        tileSweep<false, false, true>(a, P, g, l, sProf, &sRings[threadIdx.x / TG], nullptr);
+    } else if (b0 == 0) {
         tileSweep<true, false>(a, P, g, l, sProf, &sRings[threadIdx.x / TG], nullptr);
-    else
+    } else {
         tileSweep<false, false>(a, P, g, l, sProf, &sRings[threadIdx.x / TG], nullptr);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -775,6 +805,62 @@ __global__ __launch_bounds__(kWalkWGThreads) void kTileWalk32(TileArgs a)
 }
 
 }  // namespace
+
+// Results of long pairs under free end gaps or local alignment, one
+// workgroup per pair: the maximum G over the cells the AlignConfig tracks
+// (dp_scout.h:169-210; the reference's own scouts, restated in the oracle):
+// the last row (free bottom: the last block's bottom-row checkpoint, columns
+// 0 .. n), the last column (free right: the last tile's right-column
+// checkpoints, rows 0 .. m), or every cell (local: the maxima of the row
+// blocks the tile sweeps kept).  Score = G - gapOpen.
+__global__ __launch_bounds__(256) void kLongResult(TileArgs a)
+{
+    __shared__ int red[256 / 64];
+    const int pi = blockIdx.x;
+    if (pi >= a.nPairs)
+        return;
+    const LongPair P = a.pairs[pi];
+    const int32_t *ck = a.ck;
+    const bool local = (a.mode & SQMI_LOCAL) != 0;
+    int best = kNegL;
+    if (local) {
+        for (int b = threadIdx.x; b < P.nblocks; b += blockDim.x)
+            best = max(best, a.best[P.bestck + b]);
+    }
+    if (!local && (a.mode & SQMI_FREE_BOTTOM)) {
+        const int64_t row = P.rowck + (int64_t)(P.nblocks - 1) * (P.n + 1) * 2;
+        for (int j = threadIdx.x; j <= P.n; j += blockDim.x)
+            best = max(best, ck[row + 2 * j]);
+    }
+    if (!local && (a.mode & SQMI_FREE_RIGHT)) {
+        for (int r = threadIdx.x; r < P.nblocks * kTileRows; r += blockDim.x) {
+            const int i = r - P.p;
+            if (i >= 0 && i <= P.m)
+                best = max(best, ck[P.colck + (((int64_t)(r / kTileRows) * P.ntiles + (P.ntiles - 1)) * kTileRows +
+                                               r % kTileRows) * 2]);
+        }
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1)
+        best = max(best, __shfl_xor(best, d, 64));
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+            best = max(best, red[w]);
+        if (local || (a.mode & (SQMI_FREE_BOTTOM | SQMI_FREE_RIGHT)))
+            a.scores[P.pid] = best - a.gapOpen;
+    }
+}
+
+hipError_t launchLongResult(const TileArgs &args, hipStream_t stream)
+{
+    if (args.nPairs <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kLongResult, dim3(args.nPairs), dim3(256), 0, stream, args);
+    return hipGetLastError();
+}
 
 hipError_t launchTileFwd32(const TileArgs &args, hipStream_t stream)
 {

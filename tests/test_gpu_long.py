@@ -144,8 +144,9 @@ def test_long_edges(ctx, route):
 
 
 def test_long_mixed_with_packed_routes(ctx, route):
-    # Long pairs beside short ones, an empty pair and a local-mode call
-    # (local never takes the long route).
+    # Long pairs beside short ones, an empty pair, and score calls under
+    # free end gaps and local alignment (the long route's int32 forward; its
+    # packed forward is global only; the traceback is global only).
     rng = np.random.default_rng(12)
     Hl, Vl = _mutated_pairs(rng, 6, 600, 1200)
     hs = [Hl[i] for i in range(6)] + [rng.integers(0, 4, 100).astype(np.uint8), np.zeros(0, np.uint8)]
@@ -156,12 +157,14 @@ def test_long_mixed_with_packed_routes(ctx, route):
     _check_route(ctx, 6, route)
     ws, wc = _oracle_counts(H, V, score)
     assert np.array_equal(s, ws) and np.array_equal(c, wc)
-    for mode in (0, 9, 15):
+    for mode in (0, 9, 15, sa.LOCAL):
         print("mixed mode", mode, flush=True)
         want = po.score_batch(H.residues, H.offsets, V.residues, V.offsets,
                               po.Scoring(score.table, score.gap_open, score.gap_extend), mode)
         assert np.array_equal(ctx.align_score(H, V, score, mode, flags=route), want)
-        assert ctx.stats()["pairs_long"] == (6 if mode == 0 else 0)
+        st = ctx.stats()
+        assert st["pairs_long"] == 6
+        assert st["pairs_long_packed"] == (6 if mode == 0 and not route & F32 else 0)
 
 
 def test_long_chunked_checkpoints(ctx, route, monkeypatch):
@@ -228,3 +231,48 @@ def test_long_full_100kbp_workload(ctx):
     print("100 kbp backward stats", ctx.stats(), flush=True)
     assert np.array_equal(s, s2)
     _check_invariants(H, V, score, s2, c)
+
+
+@pytest.mark.parametrize("lo,hi", [(256, 700), (900, 2600)])
+def test_long_scores_every_alignconfig_and_local(ctx, monkeypatch, lo, hi):
+    # Scores of long pairs under every AlignConfig and local alignment run
+    # the tiled int32 forward (free left / top borders in the sweep; the free
+    # last row / column reduced from the checkpoints; local with every H
+    # floored at 0 and the running maximum of every tile).
+    monkeypatch.setenv("SQMI_LONG_MIN_SIDE", "256")
+    rng = np.random.default_rng(5 * lo + hi)
+    H, V = _mutated_pairs(rng, 40, lo, hi)
+    for score in SCORES[:3]:
+        sc = po.Scoring(score.table, score.gap_open, score.gap_extend)
+        for mode in list(range(16)) + [sa.LOCAL]:
+            print("long mode", mode, lo, hi, score.table[0, :2], score.gap_extend, score.gap_open, flush=True)
+            s = ctx.align_score(H, V, score, mode, flags=F32)
+            assert ctx.stats()["pairs_long"] == len(H), mode
+            want = po.score_batch(H.residues, H.offsets, V.residues, V.offsets, sc, mode)
+            bad = np.nonzero(s != want)[0]
+            assert len(bad) == 0, (mode, bad[:5], s[bad[:3]], want[bad[:3]])
+
+
+def test_long_local_and_free_ends_edges(ctx, monkeypatch):
+    # Unrelated pairs (small local scores), identical pairs, very unequal
+    # sides, sides at the block / tile edges, under local and the free-end
+    # configs that reduce the last row / column.
+    monkeypatch.setenv("SQMI_LONG_MIN_SIDE", "256")
+    rng = np.random.default_rng(23)
+    shapes = [(256, 256), (511, 257), (513, 767), (3000, 260), (260, 3000), (1024, 1023)]
+    hs, vs = [], []
+    for n, m in shapes:
+        hs.append(rng.integers(0, 4, n).astype(np.uint8))
+        vs.append(rng.integers(0, 4, m).astype(np.uint8))
+    same = rng.integers(0, 4, 1500).astype(np.uint8)
+    hs.append(same)
+    vs.append(same.copy())
+    H, V = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+    for score in SCORES[:2]:
+        sc = po.Scoring(score.table, score.gap_open, score.gap_extend)
+        for mode in (sa.LOCAL, 2, 4, 6, 9, 15):
+            print("long edges mode", mode, flush=True)
+            s = ctx.align_score(H, V, score, mode, flags=F32)
+            assert ctx.stats()["pairs_long"] == len(H)
+            want = po.score_batch(H.residues, H.offsets, V.residues, V.offsets, sc, mode)
+            assert np.array_equal(s, want), mode
