@@ -500,7 +500,8 @@ void plan(const sqmi_sequences &H, const sqmi_sequences &V, const sqmi_scoring &
     const bool local = (mode & SQMI_LOCAL) != 0;
     P.local = local;
     const bool costsOk16 = si.o <= 0 && si.e <= 0 && si.o >= -1024 && si.e >= -1024 && si.sMax <= 1024 &&
-                           si.sMin >= -1024 && !(flags & SQMI_FORCE_INT32);
+                           si.sMin >= -1024 && !(flags & SQMI_FORCE_INT32) &&
+                           (si.alpha <= 4 || si.alpha + 1 <= kTabMaxCodes);
     // DNA byte profile: t(h, v) - gapOpen in 0..255.  Its padding column
     // (past n) is 0, i.e. t = gapOpen <= 0, so under local alignment the
     // cells past n stay at or below the maximum over columns 1..n, and an

@@ -135,6 +135,24 @@ __shared__ uint32_t gProd[kCoopWaves];
 constexpr int kRingWords = 4 * 256 + (256 / 16) * 4;
 __shared__ uint2 gRing[kRingWords];
 
+// Substitution table of the table kernels in LDS, codes h, v < kTabMaxCodes:
+// word (h, v / 2) holds the two v of the word in its halves; with C copies,
+// copy c of the word sits at byte ((h * VW + v / 2) * C + c) * 4 (VW words
+// per h).  Lane c of a half-wave reads copy c mod C, so with C = 32 every
+// lane reads its own bank and the lookups are free of bank conflicts.
+constexpr int kTabWordsPerRow = (kTabMaxCodes + 1) / 2;
+constexpr int tabBytes(int C) { return kTabMaxCodes * kTabWordsPerRow * 4 * C; }
+template <int C>
+__device__ __forceinline__ uint32_t tabColOffset(int h, int TW)
+{
+    return (uint32_t)(h * ((TW + 1) / 2) * 4 * C);
+}
+template <int C>
+__device__ __forceinline__ uint32_t tabRowOffset(int v, int lane)
+{
+    return (uint32_t)((v >> 1) * 4 * C + (v & 1) * 2 + 4 * (lane & (C - 1)));
+}
+
 // Per-lane state of one pair couple (lo = pair A, hi = pair B).
 struct Couple {
     const uint8_t *h[2];
@@ -161,6 +179,9 @@ struct Fwd16 {
     static constexpr int BR = G * R;
     static constexpr int GROUPS = 64 / G;
     static constexpr int QD = 4;   // residue prefetch depth (steps)
+    // Table copies in LDS: one per bank for the R = 16 table kernels (two
+    // waves per SIMD, so the 50 KB fit twice per CU); one for the others.
+    static constexpr int TC = (!DNA && R >= 16) ? 32 : 1;
     static_assert(!TB || R == 16, "trace records hold 16 rows");
 
     __host__ __device__ __forceinline__ static int stepsFor(int Nw) { return ((Nw + G + QD - 1) / QD) * QD; }
@@ -190,7 +211,7 @@ struct Fwd16 {
     __device__ __forceinline__ static void block(const FwdArgs &a, const Couple &c, int b, int Nw, int l,
                                                  const bool last, const bool coop, const uint32_t *rbIn,
                                                  uint32_t *rbOut, uint32_t *prodIn, uint32_t *prodOut,
-                                                 const uint32_t *sProf, const int16_t *sT, s16x2 &best,
+                                                 const uint32_t *sProf, const char *sT, s16x2 &best,
                                                  uint4 *trItem, s16x2 &endE, s16x2 &endF, EndTrack &et)
     {
         const int o = a.gapOpen, e = a.gapExtend;
@@ -246,7 +267,10 @@ struct Fwd16 {
                 const uint32_t sb = code[1] < 4 ? (uint32_t)((code[1] + 4) | 0x0c00) : 0x0d0cu;
                 sel[k] = sa | (sb << 16);
             } else {
-                sel[k] = (uint32_t)code[0] | ((uint32_t)code[1] << 16);
+                // Byte offsets of the row codes in the table, this lane's
+                // copy included.
+                const int lane = (int)threadIdx.x;
+                sel[k] = tabRowOffset<TC>(code[0], lane) | (tabRowOffset<TC>(code[1], lane) << 16);
             }
             Gk[k] = pack(g0[0], g0[1]);
             Ek[k] = pack(e0[0], e0[1]);
@@ -310,8 +334,8 @@ struct Fwd16 {
         // reads into the halves of one register.
         auto look = [&](uint32_t a, uint32_t b, int k) -> uint32_t {
             s16x2 r;
-            r.x = sT[(int)a + (int)(sel[k] & 0xffffu)];
-            r.y = sT[(int)b + (int)(sel[k] >> 16)];
+            r.x = *reinterpret_cast<const int16_t *>(sT + a + (sel[k] & 0xffffu));
+            r.y = *reinterpret_cast<const int16_t *>(sT + b + (sel[k] >> 16));
             return asu(r);
         };
         // Table kernels, steady chunks: the rows' scores of the next column
@@ -530,7 +554,7 @@ struct Fwd16 {
             if constexpr (DNA)
                 return sProf[code];
             else
-                return (uint32_t)(code * TW);
+                return tabColOffset<TC>(code, TW);
         };
         // The couple's profile words come from the group's LDS ring.  A
         // refill for steps [s, s + kRefill) writes columns s + 2 l and
@@ -684,16 +708,21 @@ struct Fwd16 {
     __device__ __forceinline__ static void run(const FwdArgs &a)
     {
         __shared__ uint32_t sProf[8];
-        __shared__ int16_t sT[33 * 33];
+        __shared__ __align__(16) char sT[DNA ? 16 : tabBytes(TC)];
         __shared__ uint32_t sBest[NW][GROUPS];
         __shared__ int sItem;
         if constexpr (DNA) {
             if (threadIdx.x < 5)
                 sProf[threadIdx.x] = a.dnaProfile[threadIdx.x];
         } else {
-            const int nT = (a.alpha + 1) * (a.alpha + 1);
-            for (int i = threadIdx.x; i < nT; i += blockDim.x)
-                sT[i] = a.tTable[i];
+            // Word (h, v / 2) of copy cc at byte ((h * VW + v / 2) * TC + cc) * 4.
+            const int TW = a.alpha + 1, VW = (TW + 1) / 2;
+            for (int i = threadIdx.x; i < TW * VW * TC; i += blockDim.x) {
+                const int w = i / TC, h = w / VW, v0 = 2 * (w % VW);
+                const uint32_t lo = (uint16_t)a.tTable[h * TW + v0];
+                const uint32_t hi = v0 + 1 < TW ? (uint16_t)a.tTable[h * TW + v0 + 1] : 0u;
+                *reinterpret_cast<uint32_t *>(sT + 4 * i) = lo | (hi << 16);
+            }
         }
         __syncthreads();
 

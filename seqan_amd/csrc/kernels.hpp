@@ -11,6 +11,9 @@ namespace sqmi {
 // value of a pair routed to them is proven (on the host) to stay within
 // [kNeg16 + kGuard16, 32767 - kGuard16].
 constexpr int kNeg16 = -28672;
+// Substitution tables of the packed forward: at most this many codes
+// (alphabet + padding code) per side, held in LDS replicated per bank.
+constexpr int kTabMaxCodes = 28;
 constexpr int kGuard16 = 2048;
 
 // Tracking kinds (which cells define the result), from the mode bits.
