@@ -182,6 +182,14 @@ struct Fwd16 {
     // Table copies in LDS: one per bank for the R = 16 table kernels (two
     // waves per SIMD, so the 50 KB fit twice per CU); one for the others.
     static constexpr int TC = (!DNA && R >= 16) ? 32 : 1;
+    // LFL (local scores): every value of the sweep is held shifted by
+    // -32768 - gapOpen, so that G = gapOpen (H = 0) is -32768, and all adds
+    // saturate.  Then G = H + gapOpen, saturated, is G of max(H, 0): the
+    // local floor costs no instruction.  In local alignment every E, F and G
+    // is >= gapOpen and a D below 0 only matters through the floor, so
+    // values clamped at -32768 (-inf included) give the exact result; the
+    // shift leaves the int16 range below the window's bound unused.
+    static constexpr bool LFL = (TRK & kTrackLocal) && !TB;
     static_assert(!TB || R == 16, "trace records hold 16 rows");
 
     __host__ __device__ __forceinline__ static int stepsFor(int Nw) { return ((Nw + G + QD - 1) / QD) * QD; }
@@ -215,7 +223,7 @@ struct Fwd16 {
                                                  uint4 *trItem, s16x2 &endE, s16x2 &endF, EndTrack &et)
     {
         const int o = a.gapOpen, e = a.gapExtend;
-        const s16x2 o2 = splat(o), e2 = splat(e), neg2 = splat(kNeg16);
+        const s16x2 o2 = splat(o), e2 = splat(e), neg2 = splat(LFL ? -32768 : kNeg16);
         const bool leftFree = (a.mode & (SQMI_FREE_LEFT | SQMI_LOCAL)) != 0;
         const bool topFree = (a.mode & SQMI_FREE_TOP) != 0;
         const int eo = max(e, o);   // first-column gap chain grows by max(e, o) per row
@@ -261,6 +269,10 @@ struct Fwd16 {
                 g0[x] = (i < 0) ? kNeg16 : (i == 0) ? o : ((leftFree ? 0 : o + (i - 1) * eo) + o);
                 fl[x] = (i == 0 && topFree) ? 0 : (TB ? kNeg16 : e);
                 e0[x] = (SAT && i == 0 && topFree) ? 0 : kNeg16;
+                if constexpr (LFL) {
+                    g0[x] = -32768;   // G = gapOpen (H = 0) on column 0 and every -inf
+                    e0[x] = -32768;
+                }
             }
             if constexpr (DNA) {
                 const uint32_t sa = code[0] < 4 ? (uint32_t)(code[0] | 0x0c00) : 0x0d0cu;
@@ -317,7 +329,7 @@ struct Fwd16 {
             trBlock = trItem + (size_t)b * steps * 64;
 
         auto add = [](s16x2 x, s16x2 y) -> s16x2 {
-            if constexpr (SAT)
+            if constexpr (SAT || LFL)
                 return addSat(x, y);
             else
                 return x + y;
@@ -403,13 +415,14 @@ struct Fwd16 {
                         F = vmax(add(uF, e2), uG);
                         H = vmax(vmax(D, Ek[k]), F);
                     }
-                    if constexpr (TRK & kTrackLocal) {
+                    if constexpr ((TRK & kTrackLocal) && TB) {
                         H = vmax(H, splat(0));
                     } else if constexpr (HEAD && TB) {
                         H = vmax(H, rowK[k]);
                     }
                     if (k + 1 < RB)
                         D = add(Gk[k], as16(tOf(k + 1)));
+                    // LFL: the saturating add floors G at -32768, i.e. H at 0.
                     const s16x2 Gn = add(H, o2);
                     Gk[k] = Gn;
                     if constexpr ((TRK & kTrackLocal) && !TB)
@@ -775,7 +788,7 @@ struct Fwd16 {
             const WorkItem w = a.items[it];
             const bool ok = wellFormed(a, w);
             const Couple c = couple(a, w, grp);
-            s16x2 best = (TRK & kTrackLocal) ? splat(a.gapOpen) : splat(kNeg16);
+            s16x2 best = LFL ? splat(-32768) : (TRK & kTrackLocal) ? splat(a.gapOpen) : splat(kNeg16);
             s16x2 endE = splat(kNeg16), endF = splat(kNeg16);
             EndTrack et;
             et.bV = et.bE = et.bF = et.rV = et.rE = et.rF = splat(kNeg16);
@@ -920,10 +933,12 @@ struct Fwd16 {
                 __syncthreads();
             }
             if (ok && l == 0 && (!coop || wave == 0)) {
+                // H = G - gapOpen; LFL: G = stored + 32768 + gapOpen.
+                const int toH = LFL ? 32768 : -a.gapOpen;
                 if (c.pid[0] >= 0)
-                    a.scores[c.pid[0]] = (int)best.x - a.gapOpen;
+                    a.scores[c.pid[0]] = (int)best.x + toH;
                 if (c.pid[1] >= 0)
-                    a.scores[c.pid[1]] = (int)best.y - a.gapOpen;
+                    a.scores[c.pid[1]] = (int)best.y + toH;
             }
         }
     }
