@@ -205,6 +205,13 @@ struct ScoringInfo
     bool dna = false;   // v_perm profile path possible
 };
 
+// Local table kernels fuse the two lookups and the diagonal add into one
+// 32-bit add when every table value s - gapOpen is >= 0.
+int32_t tNonNeg(const ScoringInfo &si, uint32_t mode)
+{
+    return (mode & SQMI_LOCAL) != 0 && !si.dna && si.sMin >= si.o ? 1 : 0;
+}
+
 // Conservative range of every value a pair can produce inside the DP (H, E, F
 // and H + gapOpen), from the sizes alone.  H <= max(0, sMax) * min(n, m) since
 // gap costs are <= 0.  H(i,j) >= the score of the all-gap path along row 0
@@ -1390,6 +1397,7 @@ static sqmi_status alignScore(sqmi_context *ctx, const sqmi_sequences *setH, con
             a.alpha = A;
             a.tTable = static_cast<const int16_t *>(ctx->b.tTable.p);
             a.dnaProfile = static_cast<const uint32_t *>(ctx->b.dnaProf.p);
+            a.tNonNeg = tNonNeg(si, mode);
             if ((err = launchFwd16(v, a, 1, stream, 64)) != hipSuccess)
                 return hipFail(ctx, err, "launching plain kernel");
         }
@@ -1473,6 +1481,7 @@ static sqmi_status alignScore(sqmi_context *ctx, const sqmi_sequences *setH, con
             a.alpha = A;
             a.tTable = static_cast<const int16_t *>(ctx->b.tTable.p);
             a.dnaProfile = static_cast<const uint32_t *>(ctx->b.dnaProf.p);
+            a.tNonNeg = tNonNeg(si, mode);
             const int grid = grids[k];
             launches.push_back(
                 [=] { return v.G == 1 ? launchLane16(v, a, grid, stream) : launchFwd16(v, a, grid, stream); });

@@ -190,6 +190,16 @@ struct Fwd16 {
     // values clamped at -32768 (-inf included) give the exact result; the
     // shift leaves the int16 range below the window's bound unused.
     static constexpr bool LFL = (TRK & kTrackLocal) && !TB;
+    // U3 (local table kernels with every t >= 0): the table holds t << 16 in
+    // 32-bit words, TU copies; pair A reads the high half (t, zero-extended),
+    // pair B the word, and D = G + tA + (tB << 16) is one 32-bit add of
+    // three operands.  The sum never carries between the halves: G + t is a
+    // D of the sweep, below 0 in the shifted representation (the int16
+    // window), and the padding codes' t is 0 (the cells they reach stay at
+    // or below a real cell's value, so the local maximum is unchanged).
+    static constexpr bool U3 = !DNA && LFL && (TRK & kLocalAdd3) != 0;
+    static constexpr int TU = R >= 16 ? 16 : 1;
+    static constexpr int kTabLds = DNA ? 16 : U3 ? kTabMaxCodes * kTabMaxCodes * 4 * TU : tabBytes(TC);
     static_assert(!TB || R == 16, "trace records hold 16 rows");
 
     __host__ __device__ __forceinline__ static int stepsFor(int Nw) { return ((Nw + G + QD - 1) / QD) * QD; }
@@ -282,7 +292,14 @@ struct Fwd16 {
                 // Byte offsets of the row codes in the table, this lane's
                 // copy included.
                 const int lane = (int)threadIdx.x;
-                sel[k] = tabRowOffset<TC>(code[0], lane) | (tabRowOffset<TC>(code[1], lane) << 16);
+                if constexpr (U3) {
+                    // Word (h, v) copy c at ((h * TW + v) * TU + c) * 4; pair A
+                    // reads its high half.
+                    const uint32_t cp = 4u * (uint32_t)(lane & (TU - 1));
+                    sel[k] = ((uint32_t)code[0] * 4 * TU + cp + 2) | (((uint32_t)code[1] * 4 * TU + cp) << 16);
+                } else {
+                    sel[k] = tabRowOffset<TC>(code[0], lane) | (tabRowOffset<TC>(code[1], lane) << 16);
+                }
             }
             Gk[k] = pack(g0[0], g0[1]);
             Ek[k] = pack(e0[0], e0[1]);
@@ -353,7 +370,17 @@ struct Fwd16 {
         // Table kernels, steady chunks: the rows' scores of the next column
         // (words nA / nB) are read one step ahead into tq, so that the LDS
         // latency of the lookups is hidden behind a column of arithmetic.
-        uint32_t tq[DNA ? 1 : RB];
+        uint32_t tq[DNA || U3 ? 1 : RB];
+        // U3: the lookups run DL rows ahead, through DL slots (tA, tB << 16);
+        // the last rows of a steady step read the next column's first rows.
+        constexpr int DL = RB % 4 == 0 ? 4 : RB % 2 == 0 ? 2 : 1;
+        uint32_t qa[U3 ? DL : 1], qb[U3 ? DL : 1];
+        auto ldA = [&](uint32_t col, int k) -> uint32_t {
+            return *reinterpret_cast<const uint16_t *>(sT + col + (sel[k] & 0xffffu));
+        };
+        auto ldB = [&](uint32_t col, int k) -> uint32_t {
+            return *reinterpret_cast<const uint32_t *>(sT + col + (sel[k] >> 16));
+        };
         auto step = [&](auto chk, int s, int u, uint32_t pA, uint32_t pB, uint32_t nA, uint32_t nB,
                         const uint32_t *rq, uint32_t *wq) {
             constexpr bool CHK = decltype(chk)::value;
@@ -375,7 +402,7 @@ struct Fwd16 {
                 auto tOf = [&](int k) -> uint32_t {
                     if constexpr (DNA) {
                         return __builtin_amdgcn_perm(pB, pA, sel[k]);
-                    } else if constexpr (!CHK) {
+                    } else if constexpr (!CHK && !U3) {
                         const uint32_t t = tq[k];
                         tq[k] = look(nA, nB, k);
                         return t;
@@ -383,10 +410,29 @@ struct Fwd16 {
                         return look(pA, pB, k);
                     }
                 };
+                // The diagonal term of row k from G(i-1, j-1).
+                auto dOf = [&](s16x2 g, int k) -> s16x2 {
+                    if constexpr (U3) {
+                        uint32_t ta, tb;
+                        if constexpr (!CHK) {
+                            ta = qa[k % DL];
+                            tb = qb[k % DL];
+                            const int kn = k + DL;
+                            qa[k % DL] = kn < RB ? ldA(pA, kn) : ldA(nA, kn - RB);
+                            qb[k % DL] = kn < RB ? ldB(pB, kn) : ldB(nB, kn - RB);
+                        } else {
+                            ta = ldA(pA, k);
+                            tb = ldB(pB, k);
+                        }
+                        return as16(asu(g) + ta + tb);
+                    } else {
+                        return add(g, as16(tOf(k)));
+                    }
+                };
                 // D: the diagonal term of row k, G(i-1, j-1) + t, taken from
                 // the old G of row k - 1 before it is overwritten (no copies
                 // of the column's G values).
-                s16x2 D = add(diagTop, as16(tOf(0)));
+                s16x2 D = dOf(diagTop, 0);
 #pragma unroll
                 for (int k = 0; k < RB; ++k) {
                     s16x2 F, H;
@@ -421,7 +467,7 @@ struct Fwd16 {
                         H = vmax(H, rowK[k]);
                     }
                     if (k + 1 < RB)
-                        D = add(Gk[k], as16(tOf(k + 1)));
+                        D = dOf(Gk[k], k + 1);
                     // LFL: the saturating add floors G at -32768, i.e. H at 0.
                     const s16x2 Gn = add(H, o2);
                     Gk[k] = Gn;
@@ -566,6 +612,8 @@ struct Fwd16 {
         auto prof = [&](int code) -> uint32_t {
             if constexpr (DNA)
                 return sProf[code];
+            else if constexpr (U3)
+                return (uint32_t)(code * TW * 4 * TU);
             else
                 return tabColOffset<TC>(code, TW);
         };
@@ -643,7 +691,15 @@ struct Fwd16 {
         if constexpr (!TB) {
             for (; s0 < min(G, steps); s0 += QD)
                 chunk(std::true_type{}, s0);
-            if constexpr (!DNA) {
+            if constexpr (U3) {
+                if (s0 + QD - 1 <= Nw) {
+#pragma unroll
+                    for (int k = 0; k < DL; ++k) {
+                        qa[k] = ldA(pp.x, k);   // the first steady step's first rows
+                        qb[k] = ldB(pp.y, k);
+                    }
+                }
+            } else if constexpr (!DNA) {
                 if (s0 + QD - 1 <= Nw) {
 #pragma unroll
                     for (int k = 0; k < RB; ++k)
@@ -721,12 +777,21 @@ struct Fwd16 {
     __device__ __forceinline__ static void run(const FwdArgs &a)
     {
         __shared__ uint32_t sProf[8];
-        __shared__ __align__(16) char sT[DNA ? 16 : tabBytes(TC)];
+        __shared__ __align__(16) char sT[kTabLds];
         __shared__ uint32_t sBest[NW][GROUPS];
         __shared__ int sItem;
         if constexpr (DNA) {
             if (threadIdx.x < 5)
                 sProf[threadIdx.x] = a.dnaProfile[threadIdx.x];
+        } else if constexpr (U3) {
+            // Word (h, v) of copy cc at byte ((h * TW + v) * TU + cc) * 4:
+            // t << 16, t = 0 for the padding code.
+            const int TW = a.alpha + 1;
+            for (int i = threadIdx.x; i < TW * TW * TU; i += blockDim.x) {
+                const int w = i / TU, h = w / TW, v = w % TW;
+                const uint32_t t = (h == a.alpha || v == a.alpha) ? 0u : (uint16_t)a.tTable[w];
+                *reinterpret_cast<uint32_t *>(sT + 4 * i) = t << 16;
+            }
         } else {
             // Word (h, v / 2) of copy cc at byte ((h * VW + v / 2) * TC + cc) * 4.
             const int TW = a.alpha + 1, VW = (TW + 1) / 2;
@@ -984,6 +1049,12 @@ hipError_t launchTrk(const Variant16 &v, const FwdArgs &args, int gridBlocks, hi
             hipLaunchKernelGGL((kFwd16<G, R, DNA, kTrackBottom | kTrackRight>), grid, block, 0, stream, args);
             break;
         case kTrackLocal:
+            if constexpr (!DNA) {
+                if (args.tNonNeg) {
+                    hipLaunchKernelGGL((kFwd16<G, R, DNA, kTrackLocal | kLocalAdd3>), grid, block, 0, stream, args);
+                    break;
+                }
+            }
             hipLaunchKernelGGL((kFwd16<G, R, DNA, kTrackLocal>), grid, block, 0, stream, args);
             break;
         default:

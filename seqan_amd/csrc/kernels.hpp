@@ -21,7 +21,11 @@ enum Track : uint32_t {
     kTrackEnd = 1,      // global: cell (m, n)
     kTrackBottom = 2,   // last row free: max over row m, j <= n
     kTrackRight = 4,    // last column free: max over column n, i <= m
-    kTrackLocal = 8     // local: max over all cells
+    kTrackLocal = 8,    // local: max over all cells
+    // Kernel variant flag beside kTrackLocal (table kernels): every table
+    // value t = s - gapOpen is >= 0, so the two lookups and the diagonal
+    // add fuse into one 32-bit add (FwdArgs::tNonNeg).
+    kLocalAdd3 = 16
 };
 
 // A work item: up to `pairs per wave` pairs that share the same number of
@@ -56,6 +60,7 @@ struct FwdArgs {
     const int16_t *tTable;      // (alpha+1)^2 entries: s(h,v) - gapOpen, pad row/col
     const uint32_t *dnaProfile; // dwords 0..4: bytes v=0..3 of s(h,v)-gapOpen, h=0..3, pad;
                                 // dwords 8..12: bytes h=0..3 of s(h,v)-gapOpen, v=0..3, pad
+    int32_t tNonNeg;            // local table kernels: every s(h,v) - gapOpen >= 0
     // Trace variant only:
     uint4 *trace;               // 16-byte trace records (64 per step)
     const int64_t *traceOff;    // per item: first record (units of 64 records)

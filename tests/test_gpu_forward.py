@@ -160,6 +160,39 @@ def test_protein_local_window_vs_oracle(ctx):
     assert np.array_equal(got, _oracle(H, V, sa.blosum62(-1, -11), sa.LOCAL))
 
 
+@pytest.mark.parametrize("gaps", [(-1, -4), (-1, -1), (-2, -11)])
+def test_table_local_lookup_forms(ctx, gaps):
+    """Local scores on the table kernels with every s - gapOpen >= 0 (gap
+    open -4 is Blosum62's minimum: t = 0 entries) and with some below 0
+    (linear gaps -1): the two lookup forms of the packed sweep.  Pairs of 1 to
+    1,100 residues (several row blocks, skewed ends, padding rows and
+    columns), and identical runs of W (11 per residue) whose scores reach the
+    top of the int16 window."""
+    e, o = gaps
+    sc = sa.blosum62(e, o)
+    rng = np.random.default_rng(11 - o)
+    hs, vs = [], []
+    for i in range(600):
+        h = rng.integers(0, 24, int(rng.integers(1, 1100))).astype(np.uint8)
+        if i % 3 == 0:
+            v = rng.integers(0, 24, int(rng.integers(1, 1100))).astype(np.uint8)
+        else:
+            v = h.copy()
+            mut = rng.random(len(v)) < 0.2
+            v[mut] = rng.integers(0, 24, int(mut.sum()))
+        hs.append(h)
+        vs.append(v)
+    w = sa.AMINO_ACIDS.index("W")
+    for n in (2000, 2700):
+        hs.append(np.full(n, w, np.uint8))
+        vs.append(np.full(n + 7, w, np.uint8))
+    H, V = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+    _say("table local", gaps, len(hs))
+    got = ctx.align_score(H, V, sc, sa.LOCAL)
+    assert got[-1] == 2700 * 11 and ctx.stats()["pairs_int16"] == len(hs)
+    assert np.array_equal(got, _oracle(H, V, sc, sa.LOCAL))
+
+
 def test_device_resident_inputs(ctx):
     H, V = sa.workload(sa.WL_DNA_LOGNORMAL, 0x5EED0002, 0, 3000)
     dh = sa.DeviceBuffer.from_array(H.residues)
