@@ -471,8 +471,17 @@ struct Fwd16 {
                     // LFL: the saturating add floors G at -32768, i.e. H at 0.
                     const s16x2 Gn = add(H, o2);
                     Gk[k] = Gn;
-                    if constexpr ((TRK & kTrackLocal) && !TB)
+                    if constexpr ((TRK & kTrackLocal) && !TB && !U3) {
+                        // Taken in the row, as a statement: left to itself the
+                        // compiler defers a chunk's 4 x RB maxima to its end
+                        // and keeps every G of the chunk live for them (the
+                        // DNA R = 16 kernel then spills in its loops).
+                        uint32_t bw = asu(best);
+                        asm volatile("v_pk_max_i16 %0, %0, %1" : "+v"(bw) : "v"(asu(Gn)));
+                        best = as16(bw);
+                    } else if constexpr ((TRK & kTrackLocal) && !TB) {
                         best = vmax(best, Gn);
+                    }
                     uG = Gn;
                     uF = F;
                 }
