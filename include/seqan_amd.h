@@ -56,14 +56,15 @@
 extern "C" {
 #endif
 
-#define SQMI_ABI_VERSION 6   /* 2: sqmi_stats.pairs_long, pairs_long_packed; sqmi_align_rows
+#define SQMI_ABI_VERSION 7   /* 2: sqmi_stats.pairs_long, pairs_long_packed; sqmi_align_rows
                                 3: sqmi_align_local_rows; SQMI_LOCAL in sqmi_align_counts;
                                    sqmi_align_score_banded
                                 4: sqmi_align_rows_banded
                                 5: sqmi_stats.pairs_lane; one-pair-per-lane shapes (G = 1);
                                    sqmi_align_score_noalloc
                                 6: sqmi_stats.replayed: identical sqmi_align_score calls relaunch
-                                   the previous call's HIP graph */
+                                   the previous call's HIP graph
+                                7: SQMI_ROWS_PACKED */
 
 typedef enum sqmi_status {
     SQMI_OK = 0,
@@ -90,7 +91,8 @@ enum {
     SQMI_RESIDUES_ON_DEVICE = 1,  /* residues pointers are device addresses   */
     SQMI_SCORES_ON_DEVICE = 2,    /* scores/counts pointers are device addresses */
     SQMI_FORCE_INT32 = 4,         /* skip the packed int16 kernels             */
-    SQMI_PLAIN_PER_PAIR = 8       /* one kernel launch per pair (baseline form) */
+    SQMI_PLAIN_PER_PAIR = 8,      /* one kernel launch per pair (baseline form) */
+    SQMI_ROWS_PACKED = 0x1000000  /* rows calls: rows back to back in pair order */
 };
 /* Tuning knob: force the packed kernel shape instead of the planner's choice
  * (0 means automatic): lanes per pair group G and rows per lane R, G = 16
@@ -187,7 +189,10 @@ sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *set_h,
  * op_offsets[i+1] and start at op_offsets[i+1] - op_lengths[i] (the bytes
  * before them are unspecified).  op_offsets: count + 1 int64 in host memory;
  * ops and op_lengths (count int64) are host memory, or device memory with
- * SQMI_SCORES_ON_DEVICE.  An empty pair gets INT32_MIN, zero counts and no
+ * SQMI_SCORES_ON_DEVICE.  With SQMI_ROWS_PACKED the rows are instead written
+ * back to back in pair order (packed on the device): row i starts at
+ * op_offsets[0] + op_lengths[0] + ... + op_lengths[i-1]; op_offsets still
+ * bound the space as above.  An empty pair gets INT32_MIN, zero counts and no
  * rows.  Global modes as sqmi_align_counts (SQMI_LOCAL: sqmi_align_local_rows). */
 sqmi_status sqmi_align_rows(sqmi_context *ctx, const sqmi_sequences *set_h,
                             const sqmi_sequences *set_v, const sqmi_scoring *scoring,

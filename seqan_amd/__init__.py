@@ -27,6 +27,7 @@ LIB_PATH = os.path.join(_HERE, "libseqan_amd.so")
 
 FREE_TOP, FREE_LEFT, FREE_RIGHT, FREE_BOTTOM, LOCAL = 1, 2, 4, 8, 16
 RESIDUES_ON_DEVICE, SCORES_ON_DEVICE, FORCE_INT32, PLAIN_PER_PAIR = 1, 2, 4, 8
+ROWS_PACKED = 0x1000000   # rows back to back in pair order, packed on the device
 
 
 def force_shape(G, R):
@@ -75,7 +76,7 @@ class Stats(ctypes.Structure):
 
 
 # SQMI_ABI_VERSION of include/seqan_amd.h that the structures below mirror.
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 # Every symbol the C ABI declares (include/seqan_amd.h).
 ABI_SYMBOLS = (
@@ -407,9 +408,9 @@ class Context:
         lens = np.zeros(n, np.int64)
         st = self._L.sqmi_align_rows(self._h, ctypes.byref(ch), ctypes.byref(cv), ctypes.byref(sc), mode,
                                      scores.ctypes.data, counts.ctypes.data, buf.ctypes.data, seg.ctypes.data,
-                                     lens.ctypes.data, flags)
+                                     lens.ctypes.data, flags | ROWS_PACKED)
         self._check(st, "sqmi_align_rows")
-        return scores, counts, *_compact_rows(buf, seg, lens)
+        return scores, counts, *_packed_rows(buf, lens)
 
     def align_local_rows(self, set_h: SeqSet, set_v: SeqSet, score: Score, flags=0):
         """Local alignment: scores, counts and the rows of every local window.
@@ -430,9 +431,10 @@ class Context:
         begins = np.zeros((n, 2), np.int64)
         st = self._L.sqmi_align_local_rows(self._h, ctypes.byref(ch), ctypes.byref(cv), ctypes.byref(sc),
                                            scores.ctypes.data, counts.ctypes.data, buf.ctypes.data,
-                                           seg.ctypes.data, lens.ctypes.data, begins.ctypes.data, flags)
+                                           seg.ctypes.data, lens.ctypes.data, begins.ctypes.data,
+                                           flags | ROWS_PACKED)
         self._check(st, "sqmi_align_local_rows")
-        return (scores, counts, *_compact_rows(buf, seg, lens), begins)
+        return (scores, counts, *_packed_rows(buf, lens), begins)
 
     def align_rows_banded(self, set_h: SeqSet, set_v: SeqSet, score: Score, lower_diag: int, upper_diag: int,
                           mode=0, flags=0, rows=True):
@@ -462,19 +464,17 @@ class Context:
         st = self._L.sqmi_align_rows_banded(self._h, ctypes.byref(ch), ctypes.byref(cv), ctypes.byref(sc), mode,
                                             lower_diag, upper_diag, scores.ctypes.data, counts.ctypes.data,
                                             buf.ctypes.data, seg.ctypes.data, lens.ctypes.data,
-                                            begins.ctypes.data if local else None, flags)
+                                            begins.ctypes.data if local else None, flags | ROWS_PACKED)
         self._check(st, "sqmi_align_rows_banded")
-        out = (scores, counts, *_compact_rows(buf, seg, lens))
+        out = (scores, counts, *_packed_rows(buf, lens))
         return (*out, begins) if local else out
 
 
-def _compact_rows(buf, seg, lens):
-    """(op_off, ops) from rows written to the ends of the segments seg."""
-    n = len(lens)
-    op_off = np.zeros(n + 1, np.int64)
-    op_off[1:] = np.cumsum(lens)
-    ops = np.concatenate([buf[seg[i + 1] - lens[i]:seg[i + 1]] for i in range(n)]) if n else buf[:0]
-    return op_off, ops
+def _packed_rows(buf, lens):
+    """(op_off, ops) from rows the library wrote back to back (ROWS_PACKED)."""
+    op_off = np.zeros(len(lens) + 1, np.int64)
+    np.cumsum(lens, out=op_off[1:])
+    return op_off, buf[:op_off[-1]]
 
 
 # ---------------------------------------------------------------------------

@@ -55,7 +55,7 @@ struct Buffers
     // Long pairs (tiled int32 kernels).
     DevBuf longCk, longPairs, longWalk, longTrace, longBest;
     // Alignment rows (sqmi_align_rows, sqmi_align_local_rows).
-    DevBuf ops, opOff, opLen, begins;
+    DevBuf ops, opOff, opLen, begins, opsPacked, opStart;
     // Banded scores (sqmi_align_score_banded).
     DevBuf bandIds, bandScratch;
 
@@ -65,7 +65,7 @@ struct Buffers
         for (DevBuf *b : {&hRes, &vRes, &hOff, &vOff, &order, &items, &queue, &rowbuf, &scores, &tTable, &dnaProf,
                           &ids32, &scratchOff32, &scratch32, &table32, &emptyIds, &trace, &traceOff, &locs, &endEF,
                           &endIJ, &counts, &longCk, &longPairs, &longWalk, &longTrace, &longBest, &ops, &opOff, &opLen,
-                          &begins, &bandIds, &bandScratch})
+                          &begins, &opsPacked, &opStart, &bandIds, &bandScratch})
             f(*b);
     }
 };
@@ -2016,6 +2016,23 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
                                     INT32_MIN, stream)) != hipSuccess)
             return hipFail(ctx, err, "launching fill kernel");
     }
+    const uint8_t *rowsOut = rows.ops;
+    if (wantRows && (flags & SQMI_ROWS_PACKED)) {
+        // Rows back to back in pair order from ops + op_offsets[0].
+        const int64_t opBytes = opOffsets[N];
+        if ((err = ensure(ctx->b.opsPacked, (size_t)std::max<int64_t>(opBytes, 1))) != hipSuccess ||
+            (err = ensure(ctx->b.opStart, 8 * (N + 1))) != hipSuccess)
+            return hipFail(ctx, err, "allocating packed rows");
+        uint8_t *packed = static_cast<uint8_t *>(ctx->b.opsPacked.p);
+        if ((err = launchPackRows(rows.ops, rows.opOff, rows.opLen, static_cast<int64_t *>(ctx->b.opStart.p), packed,
+                                  N, stream)) != hipSuccess)
+            return hipFail(ctx, err, "launching row packing");
+        if ((flags & SQMI_SCORES_ON_DEVICE) &&
+            (err = hipMemcpyAsync(ops + opOffsets[0], packed + opOffsets[0], (size_t)(opBytes - opOffsets[0]),
+                                  hipMemcpyDeviceToDevice, stream)) != hipSuccess)
+            return hipFail(ctx, err, "copying packed rows");
+        rowsOut = packed;
+    }
     if ((err = hipEventRecord(ctx->ev1, stream)) != hipSuccess)
         return hipFail(ctx, err, "event");
     if (!(flags & SQMI_SCORES_ON_DEVICE)) {
@@ -2023,7 +2040,7 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             (err = hipMemcpyAsync(counts, dCounts, (size_t)4 * S * N, hipMemcpyDeviceToHost, stream)) != hipSuccess)
             return hipFail(ctx, err, "copying results");
         if (wantRows &&
-            ((err = hipMemcpyAsync(ops + opOffsets[0], rows.ops + opOffsets[0], (size_t)(opOffsets[N] - opOffsets[0]),
+            ((err = hipMemcpyAsync(ops + opOffsets[0], rowsOut + opOffsets[0], (size_t)(opOffsets[N] - opOffsets[0]),
                                    hipMemcpyDeviceToHost, stream)) != hipSuccess ||
              (err = hipMemcpyAsync(opLengths, rows.opLen, 8 * N, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
              (begins && (err = hipMemcpyAsync(begins, rows.begins, 16 * N, hipMemcpyDeviceToHost, stream)) !=
@@ -2231,8 +2248,9 @@ sqmi_status sqmi_align_rows_banded(sqmi_context *ctx, const sqmi_sequences *setH
     sqmi_status st;
     if ((st = validate(ctx, setH, setV, scoring, mode)) != SQMI_OK)
         return st;
-    if (flags & ~(uint32_t)(SQMI_RESIDUES_ON_DEVICE | SQMI_SCORES_ON_DEVICE))
-        return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_rows_banded: flags other than the *_ON_DEVICE ones");
+    if (flags & ~(uint32_t)(SQMI_RESIDUES_ON_DEVICE | SQMI_SCORES_ON_DEVICE | SQMI_ROWS_PACKED))
+        return fail(ctx, SQMI_ERR_UNSUPPORTED,
+                    "sqmi_align_rows_banded: flags other than the *_ON_DEVICE ones and SQMI_ROWS_PACKED");
     if (lower_diag > upper_diag)
         return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "lower_diag > upper_diag");
     const bool local = (mode & SQMI_LOCAL) != 0;

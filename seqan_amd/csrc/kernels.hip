@@ -109,6 +109,49 @@ __global__ void kZero32(uint32_t *p, size_t n)
         p[t] = 0;
 }
 
+// Exclusive prefix sums of the row lengths (one workgroup of 1024 threads:
+// thread t sums a contiguous chunk, the chunk sums are scanned in LDS, and
+// each thread writes its chunk's prefixes).  start[n] is the total.
+__global__ __launch_bounds__(1024) void kScanLengths(const int64_t *len, int64_t *start, int64_t n)
+{
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t chunk = (n + 1023) / 1024, b = min(n, t * chunk), e = min(n, b + chunk);
+    int64_t sum = 0;
+    for (int64_t i = b; i < e; ++i)
+        sum += len[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int64_t add = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    int64_t run = part[t] - sum;
+    for (int64_t i = b; i < e; ++i) {
+        start[i] = run;
+        run += len[i];
+    }
+    if (t == 1023)
+        start[n] = part[1023];
+}
+
+// Rows from the ends of their segments (row i ends at src + off[i + 1]) to
+// dst + off[0] + start[i], back to back: one wavefront per row.
+__global__ __launch_bounds__(256) void kPackRows(const uint8_t *src, const int64_t *off, const int64_t *len,
+                                                 const int64_t *start, uint8_t *dst, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n)
+        return;
+    const int64_t L = len[i];
+    const uint8_t *s = src + off[i + 1] - L;
+    uint8_t *d = dst + off[0] + start[i];
+    for (int64_t k = threadIdx.x & 63; k < L; k += 64)
+        d[k] = s[k];
+}
+
 // ---------------------------------------------------------------------------
 // Traceback.  The walk follows _computeTraceback / _doTracebackGoDiagonal /
 // _doTracebackGoVertical / _doTracebackGoHorizontal (dp_traceback_impl.h:
@@ -602,6 +645,17 @@ hipError_t launchZero32(uint32_t *p, size_t n, hipStream_t stream)
         return hipSuccess;
     const size_t blocks = std::min<size_t>((n + 255) / 256, 1024);
     hipLaunchKernelGGL(kZero32, dim3((unsigned)blocks), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launchPackRows(const uint8_t *src, const int64_t *off, const int64_t *len, int64_t *start, uint8_t *dst,
+                          int64_t n, hipStream_t stream)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kScanLengths, dim3(1), dim3(1024), 0, stream, len, start, n);
+    hipLaunchKernelGGL(kPackRows, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, stream, src, off, len,
+                       (const int64_t *)start, dst, n);
     return hipGetLastError();
 }
 

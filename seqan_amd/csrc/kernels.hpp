@@ -236,6 +236,11 @@ struct BandedWalkArgs {
 hipError_t launchBandedWalk(const BandedWalkArgs &args, hipStream_t stream);
 // scores[ids[k]] = value for k < n.
 hipError_t launchFillScores(int32_t *scores, const int32_t *ids, int n, int32_t value, hipStream_t stream);
+// Rows written at the ends of their segments [off[i], off[i + 1]) of src,
+// len[i] bytes each, copied back to back to dst + off[0]; start receives the
+// n + 1 exclusive prefix sums of len (device memory throughout).
+hipError_t launchPackRows(const uint8_t *src, const int64_t *off, const int64_t *len, int64_t *start, uint8_t *dst,
+                          int64_t n, hipStream_t stream);
 // p[0 .. n) = 0 by a kernel (no memset node, so the call can be captured in a
 // HIP graph).
 hipError_t launchZero32(uint32_t *p, size_t n, hipStream_t stream);

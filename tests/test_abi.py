@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_status_strings_and_version():
     L = sa.lib()
-    assert L.sqmi_abi_version() == 6 == sa.ABI_VERSION
+    assert L.sqmi_abi_version() == 7 == sa.ABI_VERSION
     assert L.sqmi_status_string(2) == b"sequence sets differ in length"
 
 

@@ -378,3 +378,54 @@ def test_local_rows_packed_ties_multi_block(ctx, scoring):
     assert np.array_equal(s, ws) and np.array_equal(c, wc)
     assert np.array_equal(off, woff) and np.array_equal(ops, wops)
     assert np.array_equal(begins, wbeg)
+
+
+@pytest.mark.parametrize("where", ["host", "device"])
+def test_rows_segment_and_packed_layouts(ctx, where):
+    """The two layouts of sqmi_align_rows: each row at the end of the
+    caller's segment (the default, which the SeqAn Gaps adapter reads) and
+    back to back from op_offsets[0] (SQMI_ROWS_PACKED, packed on the device),
+    with the outputs in host memory or in device memory."""
+    import ctypes
+    fx = "ref_dna_short_counts.npz"
+    z = _load(fx)
+    score = _score_of(z, fx)
+    Hs, Vs = sa.SeqSet(z["h_res"], z["h_off"]), sa.SeqSet(z["v_res"], z["v_off"])
+    n, A = len(Hs.offsets) - 1, score.alphabet_size
+    ch, _kh = sa._seqs(Hs)
+    cv, _kv = sa._seqs(Vs)
+    sc = score._c()
+    seg = np.zeros(n + 1, np.int64)
+    seg[1:] = np.cumsum(Hs.lengths() + Vs.lengths() + 3)
+    seg += 5   # op_offsets[0] > 0 and slack in every segment
+    total = int(seg[-1])
+    got = {}
+    for flag in (0, sa.ROWS_PACKED):
+        print("rows layout", where, hex(flag), flush=True)
+        scores = np.empty(n, np.int32)
+        counts = np.empty((n, A * A + 2), np.int32)
+        if where == "host":
+            buf = np.zeros(total, np.uint8)
+            lens = np.zeros(n, np.int64)
+            st = ctx._L.sqmi_align_rows(ctx._h, ctypes.byref(ch), ctypes.byref(cv), ctypes.byref(sc), 0,
+                                        scores.ctypes.data, counts.ctypes.data, buf.ctypes.data, seg.ctypes.data,
+                                        lens.ctypes.data, flag)
+        else:
+            dev = [sa.DeviceBuffer(4 * n), sa.DeviceBuffer(counts.nbytes), sa.DeviceBuffer(total),
+                   sa.DeviceBuffer(8 * n)]
+            st = ctx._L.sqmi_align_rows(ctx._h, ctypes.byref(ch), ctypes.byref(cv), ctypes.byref(sc), 0,
+                                        dev[0].address, dev[1].address, dev[2].address, seg.ctypes.data,
+                                        dev[3].address, flag | sa.SCORES_ON_DEVICE)
+            scores = dev[0].to_array(np.int32, n)
+            buf = dev[2].to_array(np.uint8, total)
+            lens = dev[3].to_array(np.int64, n)
+            for b in dev:
+                b.free()
+        assert st == 0
+        assert np.array_equal(scores, z["scores"])
+        got[flag] = (buf, lens)
+    (buf0, lens0), (buf1, lens1) = got[0], got[sa.ROWS_PACKED]
+    assert np.array_equal(lens0, lens1) and np.array_equal(lens0, np.diff(z["op_off"]))
+    at_ends = np.concatenate([buf0[seg[i + 1] - lens0[i]:seg[i + 1]] for i in range(n)])
+    packed = buf1[seg[0]:seg[0] + int(lens1.sum())]
+    assert np.array_equal(at_ends, z["ops"]) and np.array_equal(packed, z["ops"])
