@@ -2112,8 +2112,9 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
     if ((st = validate(ctx, setH, setV, scoring, mode)) != SQMI_OK)
         return st;
     const bool local = (mode & SQMI_LOCAL) != 0;
-    if (flags & ~(uint32_t)(SQMI_RESIDUES_ON_DEVICE | SQMI_SCORES_ON_DEVICE))
-        return fail(ctx, SQMI_ERR_UNSUPPORTED, "sqmi_align_score_banded: flags other than the *_ON_DEVICE ones");
+    if (flags & ~(uint32_t)(SQMI_RESIDUES_ON_DEVICE | SQMI_SCORES_ON_DEVICE | SQMI_FORCE_INT32))
+        return fail(ctx, SQMI_ERR_UNSUPPORTED,
+                    "sqmi_align_score_banded: flags other than the *_ON_DEVICE ones and SQMI_FORCE_INT32");
     if (lower_diag > upper_diag)
         return fail(ctx, SQMI_ERR_INVALID_ARGUMENT, "lower_diag > upper_diag");
     const int64_t N = setH->count;
@@ -2125,9 +2126,27 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
         return fail(ctx, SQMI_ERR_DEVICE, "hipSetDevice failed");
     hipStream_t stream = ctx->stream;
 
-    // Route: empty pairs (INT32_MIN), narrow bands (<= 64 diagonals), wide bands.
+    // Route: empty pairs (INT32_MIN), the packed int16 kernel (global
+    // alignment, values inside the int16 window, at least 8 diagonals past
+    // the first), narrow bands (<= 64 diagonals), wide bands.
     const auto tPlan = std::chrono::steady_clock::now();
-    std::vector<int32_t> narrow, wide, rejected;
+    ScoringInfo si{};
+    si.alpha = scoring->alphabet_size;
+    si.o = scoring->gap_open;
+    si.e = scoring->gap_extend;
+    si.sMax = INT_MIN;
+    si.sMin = INT_MAX;
+    for (int k = 0; k < si.alpha * si.alpha; ++k) {
+        si.sMax = std::max(si.sMax, scoring->table[k]);
+        si.sMin = std::min(si.sMin, scoring->table[k]);
+    }
+    si.dna = si.alpha <= 4 && si.sMin - si.o >= 0 && si.sMax - si.o <= 255;
+    const int64_t W1 = upper_diag - lower_diag;
+    const int bandR = W1 >= 24 ? 16 : 8;
+    const bool packedOk = mode == 0 && !(flags & SQMI_FORCE_INT32) && W1 >= bandR && si.o <= 0 && si.e <= 0 &&
+                          si.o >= -1024 && si.e >= -1024 && si.sMax <= 1024 && si.sMin >= -1024 && si.alpha + 1 <= 33 &&
+                          lower_diag > -30000 && upper_diag < 30000;
+    std::vector<int32_t> narrow, wide, rejected, packed;
     int64_t wNarrow = 1, wWide = 0, cells = 0;
     for (int64_t i = 0; i < N; ++i) {
         const int64_t n = setH->offsets[i + 1] - setH->offsets[i], m = setV->offsets[i + 1] - setV->offsets[i];
@@ -2141,8 +2160,12 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
                             (local ? ": needs |H| >= 2, two diagonals, lower_diag <= |H| - 2 and upper_diag >= 2 - |V|"
                                    : ": needs |H| >= 2 and lower_diag < min(0, |H| - |V|) < max(0, |H| - |V|) "
                                      "< upper_diag"));
-        const int64_t w = std::min<int64_t>(upper_diag, n) - std::max<int64_t>(lower_diag, -m) + 1;
         cells += n * m;
+        if (packedOk && fitsInt16(si, n, m, false)) {
+            packed.push_back((int32_t)i);
+            continue;
+        }
+        const int64_t w = std::min<int64_t>(upper_diag, n) - std::max<int64_t>(lower_diag, -m) + 1;
         if (w <= 64) {
             narrow.push_back((int32_t)i);
             wNarrow = std::max(wNarrow, w);
@@ -2154,15 +2177,42 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
     int seg = 1;
     while (seg < wNarrow)
         seg <<= 1;
+    // Packed items: up to 128 pairs of one |V| (counting sort by |V|).
+    std::vector<int32_t> bandOrder;
+    std::vector<WorkItem> bandItems;
+    int bandMaxN = 1;
+    if (!packed.empty()) {
+        int maxM = 0;
+        for (int32_t id : packed)
+            maxM = std::max(maxM, (int)(setV->offsets[id + 1] - setV->offsets[id]));
+        std::vector<int32_t> start((size_t)maxM + 2, 0);
+        for (int32_t id : packed)
+            ++start[(size_t)(setV->offsets[id + 1] - setV->offsets[id]) + 1];
+        for (int mm = 0; mm <= maxM; ++mm)
+            start[(size_t)mm + 1] += start[(size_t)mm];
+        bandOrder.resize(packed.size());
+        {
+            std::vector<int32_t> pos(start.begin(), start.end() - 1);
+            for (int32_t id : packed)
+                bandOrder[(size_t)pos[(size_t)(setV->offsets[id + 1] - setV->offsets[id])]++] = id;
+        }
+        for (int mm = 1; mm <= maxM; ++mm) {
+            for (int32_t f = start[(size_t)mm]; f < start[(size_t)mm + 1]; f += 128) {
+                const int32_t cnt = std::min<int32_t>(128, start[(size_t)mm + 1] - f);
+                int mxN = 1;
+                for (int32_t q = f; q < f + cnt; ++q)
+                    mxN = std::max(mxN, (int)(setH->offsets[bandOrder[q] + 1] - setH->offsets[bandOrder[q]]));
+                bandMaxN = std::max(bandMaxN, mxN);
+                bandItems.push_back(WorkItem{f, cnt | (mm << 16), mxN, (mm + bandR) / bandR});
+            }
+        }
+    }
     ctx->stats.pairs = N;
     ctx->stats.cells = cells;
     ctx->stats.pairs_int32 = (int64_t)(narrow.size() + wide.size());
+    ctx->stats.pairs_int16 = (int64_t)packed.size();
     ctx->stats.host_plan_ms = msSince(tPlan);
 
-    ScoringInfo si{};
-    si.alpha = scoring->alphabet_size;
-    si.o = scoring->gap_open;
-    si.e = scoring->gap_extend;
     Inputs in;
     if ((st = uploadInputs(ctx, setH, setV, scoring, si, scores, flags, in)) != SQMI_OK)
         return st;
@@ -2180,8 +2230,51 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
     if (!all.empty() &&
         (err = hipMemcpyAsync(dIds, all.data(), 4 * all.size(), hipMemcpyHostToDevice, stream)) != hipSuccess)
         return hipFail(ctx, err, "uploading banded pair lists");
+    int bandGrid = 0, bandRowColsN = 0;
+    if (!bandItems.empty()) {
+        const int occ = occupancyBand16(bandR, si.dna);
+        const int64_t nIt = (int64_t)bandItems.size();
+        bandGrid = std::max(1, (int)std::min<int64_t>((int64_t)ctx->numCUs * occ, (nIt + 3) / 4));
+        bandRowColsN = bandRowCols(bandMaxN);
+        if ((err = ensure(ctx->b.order, 4 * bandOrder.size())) != hipSuccess ||
+            (err = ensure(ctx->b.items, sizeof(WorkItem) * bandItems.size())) != hipSuccess ||
+            (err = ensure(ctx->b.queue, 64)) != hipSuccess ||
+            (err = ensure(ctx->b.rowbuf, (size_t)bandGrid * 4 * 64 * bandRowColsN * 8)) != hipSuccess)
+            return hipFail(ctx, err, "allocating packed banded buffers");
+        if ((err = hipMemcpyAsync(ctx->b.order.p, bandOrder.data(), 4 * bandOrder.size(), hipMemcpyHostToDevice,
+                                  stream)) != hipSuccess ||
+            (err = hipMemcpyAsync(ctx->b.items.p, bandItems.data(), sizeof(WorkItem) * bandItems.size(),
+                                  hipMemcpyHostToDevice, stream)) != hipSuccess)
+            return hipFail(ctx, err, "uploading packed banded work");
+    }
     if ((err = hipEventRecord(ctx->ev0, stream)) != hipSuccess)
         return hipFail(ctx, err, "event");
+    if (!bandItems.empty()) {
+        if ((err = launchZero32(static_cast<uint32_t *>(ctx->b.queue.p), 16, stream)) != hipSuccess)
+            return hipFail(ctx, err, "launching queue reset");
+        FwdArgs f{};
+        f.hRes = in.dH;
+        f.vRes = in.dV;
+        f.hOff = static_cast<const int64_t *>(ctx->b.hOff.p);
+        f.vOff = static_cast<const int64_t *>(ctx->b.vOff.p);
+        f.order = static_cast<const int32_t *>(ctx->b.order.p);
+        f.items = static_cast<const WorkItem *>(ctx->b.items.p);
+        f.nItems = (int32_t)bandItems.size();
+        f.queue = static_cast<uint32_t *>(ctx->b.queue.p);
+        f.rowbuf = static_cast<uint32_t *>(ctx->b.rowbuf.p);
+        f.rowCols = bandRowColsN;
+        f.scores = in.dScores;
+        f.gapOpen = si.o;
+        f.gapExtend = si.e;
+        f.mode = mode;
+        f.alpha = si.alpha;
+        f.tTable = static_cast<const int16_t *>(ctx->b.tTable.p);
+        f.dnaProfile = static_cast<const uint32_t *>(ctx->b.dnaProf.p);
+        f.bandLo = (int32_t)lower_diag;
+        f.bandUp = (int32_t)upper_diag;
+        if ((err = launchBand16(bandR, si.dna, f, bandGrid, stream)) != hipSuccess)
+            return hipFail(ctx, err, "launching packed banded kernel");
+    }
     BandedArgs a{};
     a.hRes = in.dH;
     a.vRes = in.dV;

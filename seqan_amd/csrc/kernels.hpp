@@ -34,7 +34,8 @@ struct WorkItem {
     int32_t first;      // index of the first pair in `order`
     int32_t count;      // bits 0..15: number of pairs in the item; bits 16..17
                         // (packed kernels with G = 16 / 64): quarters of a block
-                        // held by block 0 (0: a whole block)
+                        // held by block 0 (0: a whole block); banded kernels:
+                        // bits 16..31 the pairs' common |V|
     int32_t maxN;       // max |H| over the item's pairs
     int32_t nblocks;    // row blocks of G*R rows (same for all pairs)
 };
@@ -61,6 +62,7 @@ struct FwdArgs {
     const uint32_t *dnaProfile; // dwords 0..4: bytes v=0..3 of s(h,v)-gapOpen, h=0..3, pad;
                                 // dwords 8..12: bytes h=0..3 of s(h,v)-gapOpen, v=0..3, pad
     int32_t tNonNeg;            // local table kernels: every s(h,v) - gapOpen >= 0
+    int32_t bandLo, bandUp;     // banded kernels: cells with bandLo <= j - i <= bandUp
     // Trace variant only:
     uint4 *trace;               // 16-byte trace records (64 per step)
     const int64_t *traceOff;    // per item: first record (units of 64 records)
@@ -175,6 +177,14 @@ hipError_t launchSimple32(const Simple32Args &args, hipStream_t stream);
 // x 64 lanes of (G, F) pairs (8 bytes each).
 bool laneSupported(const Variant16 &v);
 int laneRowCols(int maxN, int R);
+// Packed banded global alignment (band16.hip): one pair couple per lane, R in
+// {8, 16} rows per block, bandUp - bandLo >= R; work items hold up to 128
+// pairs of one length m (WorkItem.count = pairs | m << 16, nblocks =
+// (m + R) / R).  Row buffer per wave slot: bandRowCols(maxN) columns x 64
+// lanes x 8 bytes.
+int bandRowCols(int maxN);
+int occupancyBand16(int R, bool dna);
+hipError_t launchBand16(int R, bool dna, const FwdArgs &args, int gridBlocks, hipStream_t stream);
 int occupancyLane16(const Variant16 &v);
 hipError_t launchLane16(const Variant16 &v, const FwdArgs &args, int gridBlocks, hipStream_t stream);
 

@@ -71,6 +71,47 @@ def test_banded_matches_oracle(ctx, width):
             assert np.array_equal(got, want), (mode, np.nonzero(got != want)[0][:5])
 
 
+@pytest.mark.parametrize("band", [(-5, 4), (-13, 12), (-16, 16), (-40, 30), (-3, 120)])
+@pytest.mark.parametrize("alphabet", ["dna", "protein"])
+def test_banded_packed_route_matches_int32_and_oracle(ctx, band, alphabet):
+    """Global bands of at least 9 diagonals run on the packed int16 kernel
+    (band16.hip; 16 rows per block from 25 diagonals on, else 8): its scores
+    equal the int32 kernels' (SQMI_FORCE_INT32) and the oracle's, on pairs of
+    many |V| (several per wavefront and one alone, |V| below, at and above a
+    block's rows), |H| spread over the band, and 300 pairs of one length."""
+    lo, up = band
+    rng = np.random.default_rng(1000 + up - lo)
+    A = 4 if alphabet == "dna" else 24
+    score = sa.simple_score(2, -3, -1, -5) if alphabet == "dna" else sa.blosum62(-1, -11)
+    hs, vs = [], []
+    for k in range(900):
+        m = int(rng.integers(1, 300)) if k < 600 else 150
+        if k == 599:
+            m = 1
+        d = int(rng.integers(lo + 1, up))          # |H| - |V| strictly inside the band
+        n = max(2, m + d)
+        if not (lo < min(0, n - m) and max(0, n - m) < up):
+            n = m + (1 if up > 1 and lo < 0 else 0)
+            if n < 2 or not (lo < min(0, n - m) and max(0, n - m) < up):
+                continue
+        v = rng.integers(0, A, m).astype(np.uint8)
+        h = v[:n].copy() if n <= m else np.concatenate([v, rng.integers(0, A, n - m).astype(np.uint8)])
+        f = rng.random(n) < 0.15
+        h[f] = rng.integers(0, A, int(f.sum()))
+        hs.append(h)
+        vs.append(v)
+    Hs, Vs = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+    print("packed banded", band, alphabet, len(hs), flush=True)
+    got = ctx.align_score_banded(Hs, Vs, score, lo, up)
+    assert ctx.stats()["pairs_int16"] == len(hs)
+    ref32 = ctx.align_score_banded(Hs, Vs, score, lo, up, flags=sa.FORCE_INT32)
+    assert ctx.stats()["pairs_int16"] == 0
+    assert np.array_equal(got, ref32), np.nonzero(got != ref32)[0][:5]
+    sc = po.Scoring(score.table, score.gap_open, score.gap_extend)
+    want = np.array([po.score_banded(h, v, sc, 0, lo, up) for h, v in zip(hs, vs)], np.int32)
+    assert np.array_equal(got, want), np.nonzero(got != want)[0][:5]
+
+
 def test_banded_full_band_equals_unbanded(ctx):
     H, V = sa.workload(sa.WL_DNA150, 0x5EED0401, 0, 2048)
     score = sa.simple_score(2, -3, -1, -5)
