@@ -2142,7 +2142,9 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
     }
     si.dna = si.alpha <= 4 && si.sMin - si.o >= 0 && si.sMax - si.o <= 255;
     const int64_t W1 = upper_diag - lower_diag;
-    const int bandR = W1 >= 24 ? 16 : 8;
+    int bandR = W1 >= 24 ? 16 : 8;
+    if (const char *env = std::getenv("SQMI_BAND_ROWS"))   // tuning knob: 8 or 16 rows per block
+        bandR = std::atoi(env) == 8 ? 8 : std::atoi(env) == 16 && W1 >= 16 ? 16 : bandR;
     const bool packedOk = mode == 0 && !(flags & SQMI_FORCE_INT32) && W1 >= bandR && si.o <= 0 && si.e <= 0 &&
                           si.o >= -1024 && si.e >= -1024 && si.sMax <= 1024 && si.sMin >= -1024 && si.alpha + 1 <= 33 &&
                           lower_diag > -30000 && upper_diag < 30000;
