@@ -53,7 +53,7 @@ struct Buffers
     // Traceback (sqmi_align_counts).
     DevBuf trace, traceOff, locs, endEF, endIJ, counts;
     // Long pairs (tiled int32 kernels).
-    DevBuf longCk, longPairs, longWalk, longTrace, longBest;
+    DevBuf longCk, longPairs, longWalk, longTrace, longBest, longSched;
     // Alignment rows (sqmi_align_rows, sqmi_align_local_rows).
     DevBuf ops, opOff, opLen, begins, opsPacked, opStart;
     // Banded scores (sqmi_align_score_banded).
@@ -64,7 +64,7 @@ struct Buffers
     {
         for (DevBuf *b : {&hRes, &vRes, &hOff, &vOff, &order, &items, &queue, &rowbuf, &scores, &tTable, &dnaProf,
                           &ids32, &scratchOff32, &scratch32, &table32, &emptyIds, &trace, &traceOff, &locs, &endEF,
-                          &endIJ, &counts, &longCk, &longPairs, &longWalk, &longTrace, &longBest, &ops, &opOff, &opLen,
+                          &endIJ, &counts, &longCk, &longPairs, &longWalk, &longTrace, &longBest, &longSched, &ops, &opOff, &opLen,
                           &begins, &opsPacked, &opStart, &bandIds, &bandScratch})
             f(*b);
     }
@@ -87,6 +87,11 @@ struct sqmi_context
     static constexpr int kMaxLongStreams = 4;
     hipStream_t longStreams[kMaxLongStreams] = {};
     hipEvent_t longFork = nullptr, longJoin[kMaxLongStreams] = {};
+    // Streaming long-pair forward: its item ranges (kept until the next
+    // call, as a pending upload may read them) and whether the call used it
+    // (then the call checks the kernel's wait-bound flag after its sync).
+    std::vector<int4> longRanges;
+    bool longStreamed = false;
     Buffers b;
     // The no-allocation entry point: this device's scratch array (its device
     // address, from hipGetSymbolAddress at creation) and, during such a
@@ -103,6 +108,7 @@ struct sqmi_context
         uint64_t key = 0;
         hipGraphExec_t exec = nullptr;
         sqmi_stats stats{};
+        bool longStreamed = false;   // the graph holds the streaming long-pair forward
     } replay;
 };
 
@@ -380,6 +386,33 @@ int64_t longMinSide()
     if (env && *env && std::atoll(env) > 0)
         return std::atoll(env);
     return 2048;
+}
+
+// The packed long-pair forward as one persistent launch over every tile
+// (SQMI_LONG_STREAM=1), or one launch per tile anti-diagonal (default).  On
+// 64 pairs of 100 kbp the persistent form measured 382 ms against 161 ms:
+// 323 ms even without its agent-scope fences, so the dataflow schedule
+// itself, not only the cross-XCD coherence, costs more than the launches.
+bool longStreaming()
+{
+    const char *env = std::getenv("SQMI_LONG_STREAM");
+    return env && *env == '1';
+}
+
+// After a call's final synchronisation: the streaming forward's waits are
+// bounded, and one that ran past its bound leaves a flag (never expected:
+// every wait is for tiles that running waves hold).
+sqmi_status longStreamCheck(sqmi_context *ctx)
+{
+    if (!ctx->longStreamed)
+        return SQMI_OK;
+    ctx->longStreamed = false;
+    uint32_t flag = 0;
+    if (hipMemcpy(&flag, static_cast<const uint32_t *>(ctx->b.longSched.p) + 1, 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(ctx, SQMI_ERR_DEVICE, "reading the long-pair forward's wait flag");
+    if (flag)
+        return fail(ctx, SQMI_ERR_DEVICE, "long-pair forward: a tile wait ran past its bound; scores are not valid");
+    return SQMI_OK;
 }
 
 // Streams the long-pair forward spreads its pair couples over (1..4).  On
@@ -1012,6 +1045,63 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
             maxT = std::max(maxT, pairs[k].ntiles);
         }
         a.wavesPerPair = (maxB + 3) / 4;
+        // (One chunk only: the schedule's upload happens now, and with
+        // deferred launches a second chunk's would overwrite the first's.)
+        if (packed && longStreaming() && chunkEnd.size() == 1 && maxB + maxT < 32768) {
+            // One persistent launch: items = groups of four row blocks of a
+            // couple on one anti-diagonal, in anti-diagonal order (ranges of
+            // consecutive groups of a couple).
+            const int couples = (int)((e - b + 1) / 2);
+            std::vector<int> nbC(couples), ntC(couples);
+            int stride = 1, maxD = 0;
+            for (int c = 0; c < couples; ++c) {
+                const LongPair &p0 = pairs[b + 2 * c];
+                const LongPair &p1 = b + 2 * c + 1 < e ? pairs[b + 2 * c + 1] : p0;
+                nbC[c] = std::max(p0.nblocks, p1.nblocks);
+                ntC[c] = std::max(p0.ntiles, p1.ntiles);
+                stride = std::max(stride, nbC[c]);
+                maxD = std::max(maxD, nbC[c] + ntC[c] - 1);
+            }
+            std::vector<int4> &ranges = ctx->longRanges;
+            ranges.clear();
+            int64_t items = 0;
+            for (int d = 0; d < maxD; ++d)
+                for (int c = 0; c < couples; ++c) {
+                    const int blo = std::max(0, d - ntC[c] + 1), bhi = std::min(d, nbC[c] - 1);
+                    if (blo > bhi)
+                        continue;
+                    ranges.push_back(make_int4((int)items, c, d, blo / 4));
+                    items += bhi / 4 - blo / 4 + 1;
+                }
+            if (items >= INT32_MAX)
+                return fail(ctx, SQMI_ERR_UNSUPPORTED, "long pairs: too many tiles for one launch");
+            // longSched: [0] item counter, [1] wait flag, [16 ..) progress, then the ranges.
+            const size_t progWords = (size_t)couples * stride;
+            const size_t rangeOff = ((16 + progWords) * 4 + 15) / 16 * 16;
+            if ((err = ensure(ctx->b.longSched, rangeOff + sizeof(int4) * ranges.size())) != hipSuccess)
+                return hipFail(ctx, err, "allocating the long-pair schedule");
+            char *sched = static_cast<char *>(ctx->b.longSched.p);
+            if ((err = hipMemcpyAsync(sched + rangeOff, ranges.data(), sizeof(int4) * ranges.size(),
+                                      hipMemcpyHostToDevice, stream)) != hipSuccess)
+                return hipFail(ctx, err, "uploading the long-pair schedule");
+            TileArgs t = a;
+            t.ranges = reinterpret_cast<const int4 *>(sched + rangeOff);
+            t.nRanges = (int32_t)ranges.size();
+            t.nItems = (int32_t)items;
+            t.queue = reinterpret_cast<uint32_t *>(sched);
+            t.error = reinterpret_cast<uint32_t *>(sched) + 1;
+            t.progress = reinterpret_cast<uint32_t *>(sched) + 16;
+            t.progStride = stride;
+            const int grid = std::max(1, ctx->numCUs * occupancyTileFwd16Stream());
+            uint32_t *words = reinterpret_cast<uint32_t *>(sched);
+            const size_t nWords = 16 + progWords;
+            if ((err = op([=] {
+                     hipError_t e2 = launchZero32(words, nWords, stream);
+                     return e2 != hipSuccess ? e2 : launchTileFwd16Stream(t, grid, stream);
+                 })) != hipSuccess)
+                return hipFail(ctx, err, "launching long-pair forward");
+            ctx->longStreamed = true;
+        } else {
         // Every launch sweeps one tile anti-diagonal and ends with a barrier
         // for its whole grid.  Groups of pair couples on separate streams do
         // not wait for each other's slowest tiles, so their launches fill
@@ -1065,6 +1155,7 @@ sqmi_status runLong(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_se
                      })) != hipSuccess)
                     return hipFail(ctx, err, "long-pair join");
             }
+        }
         }
         if (P.mode != 0) {
             const TileArgs ta = a;
@@ -1271,6 +1362,9 @@ static sqmi_status alignScore(sqmi_context *ctx, const sqmi_sequences *setH, con
                 return hipFail(ctx, err, "copying scores");
             if ((err = hipStreamSynchronize(stream)) != hipSuccess)
                 return hipFail(ctx, err, "alignment kernels");
+            ctx->longStreamed = ctx->replay.longStreamed;
+            if ((st = longStreamCheck(ctx)) != SQMI_OK)
+                return st;
             ctx->stats = ctx->replay.stats;
             ctx->stats.host_plan_ms = keyMs;
             ctx->stats.replayed = 1;
@@ -1574,6 +1668,10 @@ static sqmi_status alignScore(sqmi_context *ctx, const sqmi_sequences *setH, con
     }
     if ((err = hipStreamSynchronize(stream)) != hipSuccess)
         return hipFail(ctx, err, "alignment kernels");
+    if (ctx->replay.valid)
+        ctx->replay.longStreamed = ctx->longStreamed;
+    if ((st = longStreamCheck(ctx)) != SQMI_OK)
+        return st;
     float ms = 0;
     if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess)
         ctx->stats.device_ms = ms;
@@ -2049,6 +2147,8 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
     }
     if ((err = hipStreamSynchronize(stream)) != hipSuccess)
         return hipFail(ctx, err, "traceback kernels");
+    if ((st = longStreamCheck(ctx)) != SQMI_OK)
+        return st;
     float ms = 0;
     if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess)
         ctx->stats.device_ms = ms;

@@ -304,6 +304,17 @@ struct TileArgs {
     uint8_t *ops;               // rows, as in WalkArgs (optional)
     const int64_t *opOff;
     int64_t *opLen;
+    // Streaming packed forward (kTileFwd16S): items = row-block groups of a
+    // couple on an anti-diagonal, claimed in anti-diagonal order.  ranges[r]
+    // = (first item, couple, diagonal, first group) of the consecutive
+    // groups of one couple on one diagonal.
+    const int4 *ranges;
+    int32_t nRanges;
+    int32_t nItems;
+    uint32_t *queue;            // item counter (zeroed before the launch)
+    uint32_t *progress;         // per couple x row block: tiles done (zeroed)
+    int32_t progStride;         // row blocks per couple in `progress`
+    uint32_t *error;            // set when a wait runs past its bound
 };
 constexpr int kTileSlack = 24;        // steps per tile beyond its width (skew + prefetch)
 constexpr int kWalkStateInts = 8;
@@ -316,6 +327,10 @@ constexpr int kLongTileC = 128;
 constexpr int kTileRecSteps = kLongTileC + kTileSlack;
 constexpr int kRoundBlocks = 8, kRoundTiles = 16, kRoundSlots = kRoundBlocks * kRoundTiles;
 hipError_t launchTileFwd32(const TileArgs &args, hipStream_t stream);
+// One persistent launch of the packed forward over every tile of the
+// chunk's pairs (kTileFwd16S); gridBlocks of 256 threads.
+hipError_t launchTileFwd16Stream(const TileArgs &args, int gridBlocks, hipStream_t stream);
+int occupancyTileFwd16Stream();
 // Packed variant: pair couples (2 c, 2 c + 1), values relative to a per-tile
 // base; valid when kLong16Span x (max|s| + |o| + |e|) fits the int16 window.
 constexpr int kLong16Span = 800;

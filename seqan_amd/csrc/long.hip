@@ -570,6 +570,82 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     tileSweep16(a, P, act, bq, tq, l, sProf, &sRings[threadIdx.x / TG]);
 }
 
+// Streaming packed forward: one persistent launch for every tile of the
+// chunk.  Waves claim items (four consecutive row blocks of a couple on one
+// tile anti-diagonal) through one counter, in anti-diagonal order, and a
+// tile (b, t) waits until tiles (b - 1, t) and (b, t - 1) are done: those
+// lie on the previous anti-diagonal, so their items were claimed earlier by
+// waves that are running, and the unfinished claimed item with the lowest
+// index always has all its inputs.  No wave waits on an item that no
+// running wave holds, whatever the residency.  A tile's checkpoints are
+// published by an agent-scope release fence and a relaxed agent-scope store
+// of the row block's progress (tiles done); a consumer reads the progress
+// with relaxed agent-scope loads and then fences with acquire, so the
+// checkpoints written on another XCD are visible.  A wait is bounded: past
+// kSpinLimit polls (about a second) the wave sets a.error (the host reports
+// a device error) and goes on, and once a.error is set no wave waits any
+// more, so the launch drains at once.
+constexpr int kSpinLimit = 1 << 20;
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void kTileFwd16S(TileArgs a)
+{
+    __shared__ uint32_t sProf[8];
+    __shared__ PairRings sRings[256 / TG];
+    loadProfile(a, sProf);
+    const int lane = threadIdx.x & 63, l = lane % TG, grp = lane / TG;
+    for (;;) {
+        // Branch-free claim: every lane takes part (lane 0 adds 1).
+        int it = (int)atomicAdd(a.queue, lane == 0 ? 1u : 0u);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= a.nItems)
+            break;
+        // The range holding item `it` (last range starting at or before it).
+        int lo = 0, hi = a.nRanges - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.ranges[mid].x <= it)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int4 rg = a.ranges[lo];
+        const int cp = rg.y, d = rg.z, wb = rg.w + (it - rg.x);
+        const bool hasB = 2 * cp + 1 < a.nPairs;
+        const LongPair P[2] = {a.pairs[2 * cp], a.pairs[hasB ? 2 * cp + 1 : 2 * cp]};
+        const int b = wb * 4 + grp, t = d - b;
+        bool act[2];
+        act[0] = b < P[0].nblocks && t >= 0 && t < P[0].ntiles;
+        act[1] = hasB && b < P[1].nblocks && t >= 0 && t < P[1].ntiles;
+        const bool any = act[0] || act[1];
+        uint32_t *prog = a.progress + (size_t)cp * a.progStride;
+        for (int spin = 0;; ++spin) {
+            bool ready = true;
+            if (any && b > 0)
+                ready = __hip_atomic_load(prog + b - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= (uint32_t)(t + 1);
+            if (any && t > 0)
+                ready = ready && __hip_atomic_load(prog + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= (uint32_t)t;
+            if (__all(ready))
+                break;
+            if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+                break;
+            if (spin >= kSpinLimit) {
+                if (lane == 0)
+                    atomicOr(a.error, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(4);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // Idle groups sweep a clamped tile of pair A with both halves inactive.
+        const int bq = any ? b : min(b, P[0].nblocks - 1);
+        const int tq = any ? t : min(max(t, 0), P[0].ntiles - 1);
+        tileSweep16(a, P, act, bq, tq, l, sProf, &sRings[threadIdx.x / TG]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (any && l == 0)
+            __hip_atomic_store(prog + b, (uint32_t)(t + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // Walk state of one long pair: the cell (i, j) reached, the walk mode
 // (0: choose from the cell's H decision, 1 / 2: inside a vertical /
 // horizontal gap), the last operation, gap opens / extensions, done flag.
@@ -878,6 +954,22 @@ hipError_t launchTileFwd16(const TileArgs &args, hipStream_t stream)
         return hipSuccess;
     hipLaunchKernelGGL(kTileFwd16, dim3((waves + 3) / 4), dim3(256), 0, stream, args);
     return hipGetLastError();
+}
+
+hipError_t launchTileFwd16Stream(const TileArgs &args, int gridBlocks, hipStream_t stream)
+{
+    if (args.nItems <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kTileFwd16S, dim3(gridBlocks), dim3(256), 0, stream, args);
+    return hipGetLastError();
+}
+
+int occupancyTileFwd16Stream()
+{
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void *)&kTileFwd16S, 256, 0) != hipSuccess)
+        return 1;
+    return blocks > 0 ? blocks : 1;
 }
 
 hipError_t launchTileTrace32(const TileArgs &args, hipStream_t stream)

@@ -276,3 +276,28 @@ def test_long_local_and_free_ends_edges(ctx, monkeypatch):
             assert ctx.stats()["pairs_long"] == len(H)
             want = po.score_batch(H.residues, H.offsets, V.residues, V.offsets, sc, mode)
             assert np.array_equal(s, want), mode
+
+
+def test_long_streaming_forward_matches_diagonal_launches(ctx, monkeypatch):
+    """The packed long-pair forward as one persistent launch (SQMI_LONG_STREAM=1:
+    tiles claimed in anti-diagonal order, each waiting for its two producer
+    tiles) against the same forward launched once per tile anti-diagonal
+    (the default) and the oracle: an odd number of pairs (the last couple
+    holds one), sides from 256 to 3,000 and couples of unequal shapes; then
+    the counts after the persistent forward."""
+    monkeypatch.setenv("SQMI_LONG_MIN_SIDE", "256")
+    monkeypatch.setenv("SQMI_LONG_ALL", "1")
+    rng = np.random.default_rng(77)
+    H, V = _mutated_pairs(rng, 37, 256, 3000)
+    score = SCORES[0]
+    print("long streaming: 37 pairs", flush=True)
+    per_diag = ctx.align_score(H, V, score, 0)
+    monkeypatch.setenv("SQMI_LONG_STREAM", "1")
+    got = ctx.align_score(H, V, score, 0)
+    _check_route(ctx, 37, 0)
+    assert np.array_equal(got, per_diag)
+    assert np.array_equal(got, _oracle_scores(H, V, score))
+    s, c = ctx.align_counts(H, V, score)
+    ws, wc = _oracle_counts(H, V, score)
+    assert np.array_equal(s, got) and np.array_equal(s, ws)
+    assert np.array_equal(c, wc)
