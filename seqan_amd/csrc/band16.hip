@@ -112,7 +112,13 @@ struct Band16 {
         }
         // The diagonal input of the first swept column, and column 0 of this
         // block's bottom row for the next block.
-        s16x2 upGprev = first ? neg2 : as16(rb[(jA - 1) * 64].x);
+        // The row buffer is a ring of a.bandCols columns (a power of two of at
+        // least W - 1 + R + 10): the columns a block reads from the block
+        // above (jA' - 1 .. its last band column) lie within W + 9 of each
+        // other, and no column the block writes before reading it shares a
+        // slot with one still to be read.
+        const int cm = a.bandCols - 1;
+        s16x2 upGprev = first ? neg2 : as16(rb[((jA - 1) & cm) * 64].x);
         if (!last && jA == 1)
             rb[0] = make_uint2(asu(Gk[R - 1]), asu(neg2));
 
@@ -120,7 +126,7 @@ struct Band16 {
         uint2 rq[QD];
 #pragma unroll
         for (int u = 0; u < QD; ++u)
-            rq[u] = first ? make_uint2(asu(neg2), asu(neg2)) : rb[(jA + u) * 64];
+            rq[u] = first ? make_uint2(asu(neg2), asu(neg2)) : rb[((jA + u) & cm) * 64];
         const Residues ra = residues(c.h[0], c.n[0]), rbs = residues(c.h[1], c.n[1]);
         const int g0 = (jA - 1) / QD;
         uint32_t a0 = ra.words[min(g0, ra.lastW)], a1 = ra.words[min(g0 + 1, ra.lastW)];
@@ -133,14 +139,21 @@ struct Band16 {
         const uint32_t lastMask = last ? 0xffffffffu : 0u;
         // One group of QD columns from s0; EDGE: the group meets a band
         // boundary (some column with u < R or u >= W - 1).
+        // The next group's row-buffer entries are read at the start of a
+        // group and the group's bottom-row entries stored at its end, so that
+        // waiting for a group's loads (vmcnt counts loads and stores in
+        // order) never waits for stores issued just before them.
         auto group = [&](auto edge, int s0, uint32_t curA, uint32_t curB) {
             constexpr bool EDGE = decltype(edge)::value;
+            uint2 nx[QD], outv[QD];
+#pragma unroll
+            for (int u = 0; u < QD; ++u)
+                nx[u] = rb[((s0 + QD + u) & cm) * 64];   // read before this block writes them, next group
 #pragma unroll
             for (int u = 0; u < QD; ++u) {
                 const int j = s0 + u;
                 const int uu = j - jS;
                 const uint2 up = make_uint2(bfi(above, rq[u].x, asu(neg2)), bfi(above, rq[u].y, asu(neg2)));
-                rq[u] = rb[(j + QD) * 64];   // read before this block writes it, QD columns from now
                 uint32_t colSel = 0, pA = 0, pB = 0;
                 if constexpr (DNA) {
                     // [h_A, 0x0c, h_B + 4, 0x0c]: byte h_A of the pair-A row
@@ -184,23 +197,33 @@ struct Band16 {
                     uF = F;
                 }
                 upGprev = upG;
-                // The last block's stores are never read as values inside the band.
-                rb[j * 64] = make_uint2(asu(uG), asu(uF));
+                outv[u] = make_uint2(asu(uG), asu(uF));
                 // The end cell (m, n), last block: 0xffff in the halves where j == n.
                 const s16x2 rem = nHalf - splat(j);
                 const uint32_t eq = ((uint16_t)rem.x == 0 ? 0xffffu : 0u) | ((uint16_t)rem.y == 0 ? 0xffff0000u : 0u);
                 best = as16(bfi(eq & lastMask, asu(uG), asu(best)));
             }
+            // The last block's stores are never read as values inside the band.
+#pragma unroll
+            for (int u = 0; u < QD; ++u) {
+                rb[((s0 + u) & cm) * 64] = outv[u];
+                rq[u] = nx[u];
+            }
         };
         int s0 = jA, g = g0;
+        // The group's residue bytes; the next group's dwords are loaded
+        // before a group and shifted after it, so the loop head waits for
+        // nothing (a wait there would also cover the last group's stores).
+        uint32_t curA = __builtin_amdgcn_alignbyte(a1, a0, ra.mis);
+        uint32_t curB = __builtin_amdgcn_alignbyte(b1, b0, rbs.mis);
         auto next = [&](auto edge) {
-            const uint32_t curA = __builtin_amdgcn_alignbyte(a1, a0, ra.mis);
-            const uint32_t curB = __builtin_amdgcn_alignbyte(b1, b0, rbs.mis);
             a0 = ra.words[min(g + 1, ra.lastW)];
             a1 = ra.words[min(g + 2, ra.lastW)];
             b0 = rbs.words[min(g + 1, rbs.lastW)];
             b1 = rbs.words[min(g + 2, rbs.lastW)];
             group(edge, s0, curA, curB);
+            curA = __builtin_amdgcn_alignbyte(a1, a0, ra.mis);
+            curB = __builtin_amdgcn_alignbyte(b1, b0, rbs.mis);
             s0 += QD;
             ++g;
         };
@@ -223,8 +246,9 @@ struct Band16 {
     __device__ __forceinline__ static bool wellFormed(const FwdArgs &a, const WorkItem &w)
     {
         const int pairs = w.count & 0xffff, m = (int)((uint32_t)w.count >> 16);
-        return !(pairs < 1 || pairs > 128 || w.maxN < 1 || rowColsFor(w.maxN) > a.rowCols || m < 1 ||
-                 w.nblocks != (m + R) / R);
+        return !(pairs < 1 || pairs > 128 || w.maxN < 1 || a.bandCols < 1 || (a.bandCols & (a.bandCols - 1)) != 0 ||
+                 a.bandCols > a.rowCols || (a.bandCols < rowColsFor(w.maxN) && a.bandCols < a.bandUp - a.bandLo + R + 10) ||
+                 m < 1 || w.nblocks != (m + R) / R);
     }
 
     __device__ __forceinline__ static void run(const FwdArgs &a)
@@ -302,6 +326,18 @@ const void *bandKernel(int R, bool dna)
 }  // namespace
 
 int bandRowCols(int maxN) { return Band16<8, true>::rowColsFor(maxN); }
+
+int bandRingCols(int maxN, int64_t lower, int64_t upper, int R)
+{
+    // The smallest power of two of at least W - 1 + R + 10 columns; when that
+    // reaches the pair's whole width the buffer is linear (a power of two
+    // no smaller than rowColsFor).
+    const int64_t need = std::min<int64_t>(upper - lower + R + 10, (int64_t)bandRowCols(maxN));
+    int c = 1;
+    while (c < need)
+        c <<= 1;
+    return c;
+}
 
 int occupancyBand16(int R, bool dna)
 {

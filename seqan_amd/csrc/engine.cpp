@@ -2337,7 +2337,7 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
         const int occ = occupancyBand16(bandR, si.dna);
         const int64_t nIt = (int64_t)bandItems.size();
         bandGrid = std::max(1, (int)std::min<int64_t>((int64_t)ctx->numCUs * occ, (nIt + 3) / 4));
-        bandRowColsN = bandRowCols(bandMaxN);
+        bandRowColsN = bandRingCols(bandMaxN, lower_diag, upper_diag, bandR);
         if ((err = ensure(ctx->b.order, 4 * bandOrder.size())) != hipSuccess ||
             (err = ensure(ctx->b.items, sizeof(WorkItem) * bandItems.size())) != hipSuccess ||
             (err = ensure(ctx->b.queue, 64)) != hipSuccess ||
@@ -2365,6 +2365,7 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
         f.queue = static_cast<uint32_t *>(ctx->b.queue.p);
         f.rowbuf = static_cast<uint32_t *>(ctx->b.rowbuf.p);
         f.rowCols = bandRowColsN;
+        f.bandCols = bandRowColsN;
         f.scores = in.dScores;
         f.gapOpen = si.o;
         f.gapExtend = si.e;

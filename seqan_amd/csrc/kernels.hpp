@@ -63,6 +63,7 @@ struct FwdArgs {
                                 // dwords 8..12: bytes h=0..3 of s(h,v)-gapOpen, v=0..3, pad
     int32_t tNonNeg;            // local table kernels: every s(h,v) - gapOpen >= 0
     int32_t bandLo, bandUp;     // banded kernels: cells with bandLo <= j - i <= bandUp
+    int32_t bandCols;           // banded kernels: row-buffer ring columns (a power of two <= rowCols)
     // Trace variant only:
     uint4 *trace;               // 16-byte trace records (64 per step)
     const int64_t *traceOff;    // per item: first record (units of 64 records)
@@ -180,9 +181,10 @@ int laneRowCols(int maxN, int R);
 // Packed banded global alignment (band16.hip): one pair couple per lane, R in
 // {8, 16} rows per block, bandUp - bandLo >= R; work items hold up to 128
 // pairs of one length m (WorkItem.count = pairs | m << 16, nblocks =
-// (m + R) / R).  Row buffer per wave slot: bandRowCols(maxN) columns x 64
-// lanes x 8 bytes.
+// (m + R) / R).  Row buffer per wave slot: a ring of bandRingCols(...)
+// columns x 64 lanes x 8 bytes.
 int bandRowCols(int maxN);
+int bandRingCols(int maxN, int64_t lower, int64_t upper, int R);
 int occupancyBand16(int R, bool dna);
 hipError_t launchBand16(int R, bool dna, const FwdArgs &args, int gridBlocks, hipStream_t stream);
 int occupancyLane16(const Variant16 &v);
