@@ -2045,6 +2045,7 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
         wa.nIds = ba.nIds;
         wa.segLanes = bandLanes;
         wa.lower = band[0];
+        wa.upper = band[1];
         wa.trace = ba.trace;
         wa.traceOff = ba.traceOff;
         wa.scores = in.dScores;

@@ -339,12 +339,16 @@ struct BandBits {
     const uint8_t *trace;
     int S;
     int lo;
+    int w;   // diagonals of the clipped band (<= S)
+    int n;
     __device__ __forceinline__ uint32_t operator()(int i, int j) const
     {
-        // An optimal path stays inside the band (cells outside are -inf);
-        // the guard keeps any read inside the pair's rows regardless.
+        // An optimal path stays inside the band (cells outside are -inf).
+        // Cells outside the clipped band or past column n read as a stop,
+        // so that no walk can step onto trace bytes the trace kernel did
+        // not write (the lanes of a segment beyond the band's width).
         const int c = j - i - lo;
-        if (c < 0 || c >= S)
+        if (c < 0 || c >= w || j > n)
             return kBitStop;
         return trace[(int64_t)(i - 1) * S + c];
     }
@@ -361,7 +365,8 @@ __global__ __launch_bounds__(64) void kBandedWalk(BandedWalkArgs a)
     const int n = (int)(a.hOff[pid + 1] - ho), m = (int)(a.vOff[pid + 1] - vo);
     int32_t *cnt = a.counts + (int64_t)pid * S;
     const int lo = (int)(a.lower > -(int64_t)m ? a.lower : -(int64_t)m);   // the band clipped to the matrix
-    BandBits bits{a.trace + a.traceOff[t], a.segLanes, lo};
+    const int up = (int)(a.upper < (int64_t)n ? a.upper : (int64_t)n);
+    BandBits bits{a.trace + a.traceOff[t], a.segLanes, lo, min(up - lo + 1, a.segLanes), n};
     walkCounts(n, m, a.hRes + ho, a.vRes + vo, A, a.scores[pid], a.endEF[2 * pid], a.endEF[2 * pid + 1], bits, cnt,
                a.ops ? a.ops + a.opOff[pid + 1] : nullptr, a.ops ? a.opLen + pid : nullptr, a.endIJ[2 * pid],
                a.endIJ[2 * pid + 1], a.local != 0, a.begins ? a.begins + 2 * pid : nullptr, a.linear != 0);

@@ -231,6 +231,7 @@ struct BandedWalkArgs {
     int32_t nIds;
     int32_t segLanes;
     int64_t lower;
+    int64_t upper;
     const uint8_t *trace;
     const int64_t *traceOff;
     const int32_t *scores;
