@@ -1980,10 +1980,13 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
     }
     // Banded pairs whose band, clipped to the matrix, has at most 64
     // diagonals: the lane-per-diagonal trace kernel (S trace bytes per row,
-    // S = the widest band rounded up to a power of two) and a walk.
+    // S = the widest band rounded up to a power of two) and a walk, in
+    // chunks whose trace bytes fit the trace budget (offsets restart at 0 in
+    // every chunk; the chunks run one after another on the call's stream).
     // SQMI_BANDED_TRACE_INT32=1 keeps them on the int32 kernel.
     int bandLanes = 0;
     std::vector<int64_t> bandOff;
+    std::vector<size_t> bandChunkEnd;
     int64_t bandBytes = 0;
     if (band && !P.ids32.empty() && !envFlag("SQMI_BANDED_TRACE_INT32")) {
         int64_t wmax = 1;
@@ -1996,13 +1999,19 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             while (bandLanes < wmax)
                 bandLanes <<= 1;
             bandOff.resize(P.ids32.size());
+            int64_t run = 0;
             for (size_t k = 0; k < P.ids32.size(); ++k) {
                 const int32_t id = P.ids32[k];
-                bandOff[k] = bandBytes;
-                bandBytes += (setV->offsets[id + 1] - setV->offsets[id]) * bandLanes;
+                const int64_t need = (setV->offsets[id + 1] - setV->offsets[id]) * bandLanes;
+                if (run > 0 && run + need > budget) {
+                    bandChunkEnd.push_back(k);
+                    run = 0;
+                }
+                bandOff[k] = run;
+                run += need;
+                bandBytes = std::max(bandBytes, run);
             }
-            if (bandBytes > budget)
-                bandLanes = 0;
+            bandChunkEnd.push_back(P.ids32.size());
         }
     }
     if (bandLanes > 0) {
@@ -2015,13 +2024,15 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             (err = hipMemcpyAsync(ctx->b.scratchOff32.p, bandOff.data(), 8 * n32, hipMemcpyHostToDevice, stream)) !=
                 hipSuccess)
             return hipFail(ctx, err, "uploading banded traceback lists");
+        size_t cb = 0;
+        for (size_t ce : bandChunkEnd) {
         BandedArgs ba{};
         ba.hRes = in.dH;
         ba.vRes = in.dV;
         ba.hOff = static_cast<const int64_t *>(ctx->b.hOff.p);
         ba.vOff = static_cast<const int64_t *>(ctx->b.vOff.p);
-        ba.ids = static_cast<const int32_t *>(ctx->b.ids32.p);
-        ba.nIds = (int32_t)n32;
+        ba.ids = static_cast<const int32_t *>(ctx->b.ids32.p) + cb;
+        ba.nIds = (int32_t)(ce - cb);
         ba.scores = in.dScores;
         ba.gapOpen = si.o;
         ba.gapExtend = si.e;
@@ -2031,7 +2042,7 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
         ba.lower = band[0];
         ba.upper = band[1];
         ba.trace = static_cast<uint8_t *>(ctx->b.scratch32.p);
-        ba.traceOff = static_cast<const int64_t *>(ctx->b.scratchOff32.p);
+        ba.traceOff = static_cast<const int64_t *>(ctx->b.scratchOff32.p) + cb;
         ba.endIJ = static_cast<int32_t *>(ctx->b.endIJ.p);
         ba.endEF = static_cast<int32_t *>(ctx->b.endEF.p);
         if ((err = launchBandedTrace(ba, bandLanes, stream)) != hipSuccess)
@@ -2061,6 +2072,8 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
         wa.begins = rows.begins;
         if ((err = launchBandedWalk(wa, stream)) != hipSuccess)
             return hipFail(ctx, err, "launching banded walk kernel");
+        cb = ce;
+        }
     } else if (!P.ids32.empty()) {
         const size_t n32 = P.ids32.size();
         if ((err = ensure(ctx->b.ids32, 4 * n32)) != hipSuccess || (err = ensure(ctx->b.scratchOff32, 8 * n32)) != hipSuccess ||

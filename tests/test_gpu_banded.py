@@ -393,3 +393,30 @@ def test_banded_rows_narrow_global_bands(ctx, band):
         want = po.rows_batch(Hs.residues, Hs.offsets, Vs.residues, Vs.offsets, sc, mode, band=(lo, up))
         for a, b in zip(out, want):
             assert np.array_equal(a, b), (band, mode)
+
+
+def test_banded_rows_chunked_trace_budget(ctx, monkeypatch):
+    """Banded rows and counts with a 1 MB trace budget: the lane-per-diagonal
+    trace and walk run in chunks whose trace bytes fit it (3,000 pairs of
+    ~150 bp, band [-16, 16]: 64 trace bytes per row, ~29 chunks), and give
+    what one chunk gives."""
+    rng = np.random.default_rng(5)
+    hs, vs = [], []
+    for _ in range(3000):
+        n = int(rng.integers(140, 160))
+        h = rng.integers(0, 4, n).astype(np.uint8)
+        v = h.copy()
+        f = rng.random(n) < 0.1
+        v[f] = rng.integers(0, 4, int(f.sum()))
+        v = v[:max(2, n - int(rng.integers(0, 6)))]
+        hs.append(h)
+        vs.append(v)
+    Hs, Vs = sa.SeqSet.from_list(hs), sa.SeqSet.from_list(vs)
+    score = sa.simple_score(2, -3, -1, -5)
+    print("banded rows, one chunk", flush=True)
+    want = ctx.align_rows_banded(Hs, Vs, score, -16, 16)
+    monkeypatch.setenv("SQMI_TRACE_BUDGET_MB", "1")
+    print("banded rows, 1 MB trace budget", flush=True)
+    got = ctx.align_rows_banded(Hs, Vs, score, -16, 16)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
