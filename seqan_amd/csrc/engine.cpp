@@ -265,10 +265,11 @@ bool shapeSupported(const Variant16 &v) { return v.G == 1 ? laneSupported(v) : v
 
 int lengthBin(int64_t m)
 {
-    int b = 0;
-    while (b < 15 && (int64_t(1) << b) < m + 1)
-        ++b;
-    return b;
+    // The smallest b <= 15 with 2^b >= m + 1.
+    if (m < 1)
+        return 0;
+    const int b = 64 - __builtin_clzll((unsigned long long)m);
+    return b < 15 ? b : 15;
 }
 
 int shapeCategory(bool dna, bool local) { return (dna ? 0 : 2) + (local ? 1 : 0); }
@@ -296,7 +297,12 @@ Variant16 tableShape(int category, int bin, int64_t pairs16, bool dna, uint32_t 
 }
 
 // Row blocks of a pair with m rows under shape v (at most 255 fit the keys).
-int64_t blocksFor(const Variant16 &v, int64_t m) { return (m + 1 + (int64_t)v.G * v.R - 1) / ((int64_t)v.G * v.R); }
+// Row blocks of G * R rows (a power of two for every compiled shape).
+int64_t blocksFor(const Variant16 &v, int64_t m)
+{
+    const int64_t br = (int64_t)v.G * v.R;
+    return (br & (br - 1)) == 0 ? (m + br) >> __builtin_ctzll((unsigned long long)br) : (m + br) / br;
+}
 
 // One pair per launch: one wavefront, whose first 16-lane group sweeps the
 // pair (the other pair slots of the wavefront stay empty).
@@ -335,6 +341,10 @@ struct Key
 {
     uint32_t key;
     int32_t id;
+    // Left uninitialised: the planner's key arrays (8 MB for a million
+    // pairs) are written in full, in parallel, right after allocation.
+    Key() {}
+    Key(uint32_t k, int32_t i) : key(k), id(i) {}
 };
 
 struct Planned
@@ -517,7 +527,7 @@ inline int64_t unitsFor(const Variant16 &v, int64_t m, bool quarter, bool fine)
     if (!fine)
         return 4 * blocksFor(v, m);
     const int64_t rows = (int64_t)v.G * v.R / 4;
-    return (m + 1 + rows - 1) / rows;
+    return (rows & (rows - 1)) == 0 ? (m + rows) >> __builtin_ctzll((unsigned long long)rows) : (m + rows) / rows;
 }
 
 // Routing (empty / int16 / int32) and length bucketing into work items.
