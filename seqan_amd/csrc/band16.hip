@@ -23,8 +23,9 @@
 // neighbour lies outside), and a diagonal step stays on its diagonal.  Those
 // two boundaries cross the block only in its first and last R columns: there
 // row k is forced when k >= u (E; u = j - i0 - lower) or k <= u - (W - 1)
-// (F), one select per row on a wavefront-uniform condition; the columns
-// between are plain sweeps.  The saturating adds keep the values outside the
+// (F), one bit-select per row with a wavefront-uniform mask (the masks
+// depend on k minus the column's place in its group only, so a group makes
+// them once); the columns between are plain sweeps.  The saturating adds keep the values outside the
 // band bounded, and the row buffer entries a block reads outside the columns
 // the block above wrote only feed cells outside the band.
 #include "kernels.hpp"
@@ -85,11 +86,35 @@ struct Band16 {
         // The rows' residues, all loads first (an empty lane reads the set's
         // first residue; rows above row 1 are padding): one wait per block.
         int raw[R][2];
+        if (i0 >= 1) {
+            // Rows i0 .. i0 + R - 1 read bytes i0 - 1 .. i0 + R - 2: R / 4 + 1
+            // aligned dwords per pair (the last one clamped to the dword of
+            // the last byte needed), funnel-shifted into place.
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
+            for (int x = 0; x < 2; ++x) {
+                const uint8_t *p = c.v[x] + (i0 - 1);
+                const uint32_t mv = (uint32_t)(reinterpret_cast<uintptr_t>(p) & 3u);
+                const uint32_t *w = reinterpret_cast<const uint32_t *>(p - mv);
+                uint32_t wv[R / 4 + 1];
 #pragma unroll
-            for (int x = 0; x < 2; ++x)
-                raw[k][x] = c.v[x][max(i0 + k - 1, 0)];
+                for (int q = 0; q < R / 4; ++q)
+                    wv[q] = w[q];
+                wv[R / 4] = w[min(R / 4, (int)((mv + R - 1) >> 2))];
+#pragma unroll
+                for (int q = 0; q < R / 4; ++q) {
+                    const uint32_t al = __builtin_amdgcn_alignbyte(wv[q + 1], wv[q], mv);
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        raw[4 * q + b][x] = (int)((al >> (8 * b)) & 0xffu);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+                    raw[k][x] = c.v[x][max(i0 + k - 1, 0)];
+            }
         }
 #pragma unroll
         for (int k = 0; k < R; ++k) {
@@ -146,13 +171,24 @@ struct Band16 {
         auto group = [&](auto edge, int s0, uint32_t curA, uint32_t curB) {
             constexpr bool EDGE = decltype(edge)::value;
             uint2 nx[QD], outv[QD];
+            // Row k of column s0 + u is forced by a wave-uniform condition on
+            // c = k - u alone: one mask per c and boundary for the group.
+            constexpr int NC = EDGE ? QD + R - 1 : 1;
+            uint32_t fE[NC], fF[NC];
+            if constexpr (EDGE) {
+                const int d0 = s0 - jS;
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    fE[c] = (c - (QD - 1) >= d0) ? 0xffffffffu : 0u;        // k >= u + d0: j - i <= lower
+                    fF[c] = (c - (QD - 1) <= d0 - W1) ? 0xffffffffu : 0u;   // k <= u + d0 - (W - 1): j - i >= upper
+                }
+            }
 #pragma unroll
             for (int u = 0; u < QD; ++u)
                 nx[u] = rb[((s0 + QD + u) & cm) * 64];   // read before this block writes them, next group
 #pragma unroll
             for (int u = 0; u < QD; ++u) {
                 const int j = s0 + u;
-                const int uu = j - jS;
                 const uint2 up = make_uint2(bfi(above, rq[u].x, asu(neg2)), bfi(above, rq[u].y, asu(neg2)));
                 uint32_t colSel = 0, pA = 0, pB = 0;
                 if constexpr (DNA) {
@@ -183,10 +219,8 @@ struct Band16 {
                     Ek[k] = vmax(addSat(Ek[k], e2), Gk[k]);
                     s16x2 F = vmax(addSat(uF, e2), uG);
                     if constexpr (EDGE) {
-                        if (k >= uu)
-                            Ek[k] = neg2;   // j - i <= lower
-                        if (k <= uu - W1)
-                            F = neg2;       // j - i >= upper
+                        Ek[k] = as16(bfi(fE[k - u + QD - 1], asu(neg2), asu(Ek[k])));
+                        F = as16(bfi(fF[k - u + QD - 1], asu(neg2), asu(F)));
                     }
                     const s16x2 H = vmax(vmax(D, Ek[k]), F);
                     if (k + 1 < R)
@@ -216,14 +250,15 @@ struct Band16 {
         // nothing (a wait there would also cover the last group's stores).
         uint32_t curA = __builtin_amdgcn_alignbyte(a1, a0, ra.mis);
         uint32_t curB = __builtin_amdgcn_alignbyte(b1, b0, rbs.mis);
+        // Consecutive groups share a dword: one new dword per pair and group.
         auto next = [&](auto edge) {
-            a0 = ra.words[min(g + 1, ra.lastW)];
-            a1 = ra.words[min(g + 2, ra.lastW)];
-            b0 = rbs.words[min(g + 1, rbs.lastW)];
-            b1 = rbs.words[min(g + 2, rbs.lastW)];
+            a0 = ra.words[min(g + 2, ra.lastW)];
+            b0 = rbs.words[min(g + 2, rbs.lastW)];
             group(edge, s0, curA, curB);
-            curA = __builtin_amdgcn_alignbyte(a1, a0, ra.mis);
-            curB = __builtin_amdgcn_alignbyte(b1, b0, rbs.mis);
+            curA = __builtin_amdgcn_alignbyte(a0, a1, ra.mis);
+            curB = __builtin_amdgcn_alignbyte(b0, b1, rbs.mis);
+            a1 = a0;
+            b1 = b0;
             s0 += QD;
             ++g;
         };

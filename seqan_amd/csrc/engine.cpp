@@ -78,6 +78,7 @@ struct sqmi_context
     hipStream_t stream = nullptr;
     hipStream_t ownStream = nullptr;
     int numCUs = 0;
+    int64_t l2PerCU = 0;    // bytes of L2 per CU (one L2 per XCD of 32 CUs on gfx950)
     int64_t totalMem = 0;   // device memory, bytes
     std::string lastError;
     sqmi_stats stats{};
@@ -1238,6 +1239,7 @@ sqmi_status sqmi_create(int device, sqmi_context **out)
         return SQMI_ERR_OUT_OF_MEMORY;
     ctx->device = device;
     ctx->numCUs = prop.multiProcessorCount;
+    ctx->l2PerCU = (int64_t)prop.l2CacheSize / 32;
     ctx->totalMem = (int64_t)prop.totalGlobalMem;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->ownStream) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
@@ -2358,10 +2360,17 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
         return hipFail(ctx, err, "uploading banded pair lists");
     int bandGrid = 0, bandRowColsN = 0;
     if (!bandItems.empty()) {
-        const int occ = occupancyBand16(bandR, si.dna);
+        // Workgroups of 4 waves per CU: the occupancy limit, but no more than
+        // keep the row-buffer rings of a CU's waves within its share of L2
+        // (at least one).  On 1 M pairs of 150 bp, band [-16, 16] (rings of
+        // 32 KB per wave), 1, 2 and 3 workgroups per CU ran 1.92, 2.09 and
+        // 2.61 ms: the rings of more waves than L2 holds go to memory.
+        bandRowColsN = bandRingCols(bandMaxN, lower_diag, upper_diag, bandR);
+        const int64_t ringBytes = (int64_t)64 * bandRowColsN * 8;   // per wave
+        const int occ = (int)std::max<int64_t>(
+            1, std::min<int64_t>(occupancyBand16(bandR, si.dna), ctx->l2PerCU / (4 * ringBytes)));
         const int64_t nIt = (int64_t)bandItems.size();
         bandGrid = std::max(1, (int)std::min<int64_t>((int64_t)ctx->numCUs * occ, (nIt + 3) / 4));
-        bandRowColsN = bandRingCols(bandMaxN, lower_diag, upper_diag, bandR);
         if ((err = ensure(ctx->b.order, 4 * bandOrder.size())) != hipSuccess ||
             (err = ensure(ctx->b.items, sizeof(WorkItem) * bandItems.size())) != hipSuccess ||
             (err = ensure(ctx->b.queue, 64)) != hipSuccess ||
