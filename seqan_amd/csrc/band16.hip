@@ -35,7 +35,9 @@
 namespace sqmi {
 namespace {
 
-template <int R, bool DNA>
+// LR: the row-buffer ring lives in LDS (dynamic shared memory, one ring per
+// wave) instead of global memory.
+template <int R, bool DNA, bool LR = false>
 struct Band16 {
     static constexpr int QD = 4;   // columns per group: one dword of residues per pair
 
@@ -303,7 +305,13 @@ struct Band16 {
         const int lane = threadIdx.x & 63;
         const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
         const size_t slot = (size_t)blockIdx.x * (blockDim.x >> 6) + wave;
-        uint2 *rb = reinterpret_cast<uint2 *>(a.rowbuf) + slot * (size_t)a.rowCols * 64 + lane;
+        uint2 *rb;
+        if constexpr (LR) {
+            extern __shared__ uint2 sRing[];
+            rb = sRing + (size_t)wave * a.bandCols * 64 + lane;
+        } else {
+            rb = reinterpret_cast<uint2 *>(a.rowbuf) + slot * (size_t)a.rowCols * 64 + lane;
+        }
         for (;;) {
             // Branch-free fetch: every lane takes part (lane 0 adds 1).
             int it = (int)atomicAdd(a.queue, lane == 0 ? 1u : 0u);
@@ -343,18 +351,24 @@ struct Band16 {
     }
 };
 
-template <int R, bool DNA>
+template <int R, bool DNA, bool LR>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R >= 16 ? 3 : 4))) void kBand16(FwdArgs a)
 {
-    Band16<R, DNA>::run(a);
+    Band16<R, DNA, LR>::run(a);
 }
 
-const void *bandKernel(int R, bool dna)
+const void *bandKernel(int R, bool dna, bool lds)
 {
-    if (R == 16)
-        return dna ? (const void *)&kBand16<16, true> : (const void *)&kBand16<16, false>;
-    if (R == 8)
-        return dna ? (const void *)&kBand16<8, true> : (const void *)&kBand16<8, false>;
+    if (R == 16) {
+        if (lds)
+            return dna ? (const void *)&kBand16<16, true, true> : (const void *)&kBand16<16, false, true>;
+        return dna ? (const void *)&kBand16<16, true, false> : (const void *)&kBand16<16, false, false>;
+    }
+    if (R == 8) {
+        if (lds)
+            return dna ? (const void *)&kBand16<8, true, true> : (const void *)&kBand16<8, false, true>;
+        return dna ? (const void *)&kBand16<8, true, false> : (const void *)&kBand16<8, false, false>;
+    }
     return nullptr;
 }
 
@@ -374,25 +388,38 @@ int bandRingCols(int maxN, int64_t lower, int64_t upper, int R)
     return c;
 }
 
-int occupancyBand16(int R, bool dna)
+int bandLdsRingBytes(int R, bool dna, int bandCols)
 {
-    const void *k = bandKernel(R, dna);
+    // Four waves' rings of bandCols columns x 64 lanes x 8 bytes, when they
+    // leave room for the static tables in the 160 KB of LDS per CU.
+    const int64_t bytes = (int64_t)4 * 64 * bandCols * 8;
+    if (bytes > 128 * 1024)
+        return 0;
+    const void *k = bandKernel(R, dna, true);
+    if (!k || hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+        return 0;
+    return (int)bytes;
+}
+
+int occupancyBand16(int R, bool dna, int ringLdsBytes)
+{
+    const void *k = bandKernel(R, dna, ringLdsBytes > 0);
     if (!k)
         return 0;
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 256, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 256, (size_t)ringLdsBytes) != hipSuccess)
         return 1;
     return blocks > 0 ? blocks : 1;
 }
 
-hipError_t launchBand16(int R, bool dna, const FwdArgs &args, int gridBlocks, hipStream_t stream)
+hipError_t launchBand16(int R, bool dna, const FwdArgs &args, int gridBlocks, int ringLdsBytes, hipStream_t stream)
 {
-    const void *k = bandKernel(R, dna);
+    const void *k = bandKernel(R, dna, ringLdsBytes > 0);
     if (!k)
         return hipErrorInvalidValue;
     FwdArgs copy = args;
     void *params[] = {&copy};
-    return hipLaunchKernel(k, dim3(gridBlocks), dim3(256), params, 0, stream);
+    return hipLaunchKernel(k, dim3(gridBlocks), dim3(256), params, (size_t)ringLdsBytes, stream);
 }
 
 }  // namespace sqmi

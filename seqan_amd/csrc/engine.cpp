@@ -2358,23 +2358,29 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
     if (!all.empty() &&
         (err = hipMemcpyAsync(dIds, all.data(), 4 * all.size(), hipMemcpyHostToDevice, stream)) != hipSuccess)
         return hipFail(ctx, err, "uploading banded pair lists");
-    int bandGrid = 0, bandRowColsN = 0;
+    int bandGrid = 0, bandRowColsN = 0, bandLds = 0;
     if (!bandItems.empty()) {
         // Workgroups of 4 waves per CU: the occupancy limit, but no more than
         // keep the row-buffer rings of a CU's waves within its share of L2
         // (at least one).  On 1 M pairs of 150 bp, band [-16, 16] (rings of
         // 32 KB per wave), 1, 2 and 3 workgroups per CU ran 1.92, 2.09 and
         // 2.61 ms: the rings of more waves than L2 holds go to memory.
+        // Rings of up to 64 columns (128 KB per workgroup) live in LDS
+        // (SQMI_BAND_LDS=0 keeps them in global memory), which also limits
+        // the CU to the workgroups whose rings fit.
         bandRowColsN = bandRingCols(bandMaxN, lower_diag, upper_diag, bandR);
+        const char *ldsEnv = std::getenv("SQMI_BAND_LDS");
+        bandLds = (ldsEnv && ldsEnv[0] == '0') ? 0 : bandLdsRingBytes(bandR, si.dna, bandRowColsN);
         const int64_t ringBytes = (int64_t)64 * bandRowColsN * 8;   // per wave
-        const int occ = (int)std::max<int64_t>(
-            1, std::min<int64_t>(occupancyBand16(bandR, si.dna), ctx->l2PerCU / (4 * ringBytes)));
+        const int occ = bandLds ? occupancyBand16(bandR, si.dna, bandLds)
+                                : (int)std::max<int64_t>(1, std::min<int64_t>(occupancyBand16(bandR, si.dna, 0),
+                                                                              ctx->l2PerCU / (4 * ringBytes)));
         const int64_t nIt = (int64_t)bandItems.size();
         bandGrid = std::max(1, (int)std::min<int64_t>((int64_t)ctx->numCUs * occ, (nIt + 3) / 4));
         if ((err = ensure(ctx->b.order, 4 * bandOrder.size())) != hipSuccess ||
             (err = ensure(ctx->b.items, sizeof(WorkItem) * bandItems.size())) != hipSuccess ||
             (err = ensure(ctx->b.queue, 64)) != hipSuccess ||
-            (err = ensure(ctx->b.rowbuf, (size_t)bandGrid * 4 * 64 * bandRowColsN * 8)) != hipSuccess)
+            (err = ensure(ctx->b.rowbuf, bandLds ? 64 : (size_t)bandGrid * 4 * 64 * bandRowColsN * 8)) != hipSuccess)
             return hipFail(ctx, err, "allocating packed banded buffers");
         if ((err = hipMemcpyAsync(ctx->b.order.p, bandOrder.data(), 4 * bandOrder.size(), hipMemcpyHostToDevice,
                                   stream)) != hipSuccess ||
@@ -2408,7 +2414,7 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
         f.dnaProfile = static_cast<const uint32_t *>(ctx->b.dnaProf.p);
         f.bandLo = (int32_t)lower_diag;
         f.bandUp = (int32_t)upper_diag;
-        if ((err = launchBand16(bandR, si.dna, f, bandGrid, stream)) != hipSuccess)
+        if ((err = launchBand16(bandR, si.dna, f, bandGrid, bandLds, stream)) != hipSuccess)
             return hipFail(ctx, err, "launching packed banded kernel");
     }
     BandedArgs a{};

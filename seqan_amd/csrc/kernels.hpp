@@ -185,8 +185,10 @@ int laneRowCols(int maxN, int R);
 // columns x 64 lanes x 8 bytes.
 int bandRowCols(int maxN);
 int bandRingCols(int maxN, int64_t lower, int64_t upper, int R);
-int occupancyBand16(int R, bool dna);
-hipError_t launchBand16(int R, bool dna, const FwdArgs &args, int gridBlocks, hipStream_t stream);
+// LDS bytes of the rings when they fit in LDS (the kernel's LDS form), else 0.
+int bandLdsRingBytes(int R, bool dna, int bandCols);
+int occupancyBand16(int R, bool dna, int ringLdsBytes);
+hipError_t launchBand16(int R, bool dna, const FwdArgs &args, int gridBlocks, int ringLdsBytes, hipStream_t stream);
 int occupancyLane16(const Variant16 &v);
 hipError_t launchLane16(const Variant16 &v, const FwdArgs &args, int gridBlocks, hipStream_t stream);
 
