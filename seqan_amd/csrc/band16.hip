@@ -252,15 +252,19 @@ struct Band16 {
         // nothing (a wait there would also cover the last group's stores).
         uint32_t curA = __builtin_amdgcn_alignbyte(a1, a0, ra.mis);
         uint32_t curB = __builtin_amdgcn_alignbyte(b1, b0, rbs.mis);
-        // Consecutive groups share a dword: one new dword per pair and group.
+        // Consecutive groups share a dword: one new dword per pair and group,
+        // loaded two groups before its use.
+        uint32_t a2 = ra.words[min(g0 + 2, ra.lastW)], b2 = rbs.words[min(g0 + 2, rbs.lastW)];
         auto next = [&](auto edge) {
-            a0 = ra.words[min(g + 2, ra.lastW)];
-            b0 = rbs.words[min(g + 2, rbs.lastW)];
+            const uint32_t a3 = ra.words[min(g + 3, ra.lastW)];
+            const uint32_t b3 = rbs.words[min(g + 3, rbs.lastW)];
             group(edge, s0, curA, curB);
-            curA = __builtin_amdgcn_alignbyte(a0, a1, ra.mis);
-            curB = __builtin_amdgcn_alignbyte(b0, b1, rbs.mis);
-            a1 = a0;
-            b1 = b0;
+            curA = __builtin_amdgcn_alignbyte(a2, a1, ra.mis);
+            curB = __builtin_amdgcn_alignbyte(b2, b1, rbs.mis);
+            a1 = a2;
+            b1 = b2;
+            a2 = a3;
+            b2 = b3;
             s0 += QD;
             ++g;
         };
