@@ -49,10 +49,11 @@ struct Band16 {
 
     __host__ __device__ static int rowColsFor(int maxN) { return (maxN + QD - 1) / QD * QD + QD + 1; }
 
-    // Residues of a pair as aligned dwords: group g (columns 4g + 1 .. 4g + 4)
-    // is the funnel shift by `mis` of dwords g and g + 1, both clamped to the
-    // dword of the pair's last residue (bytes past n are arbitrary: a column
-    // never feeds the columns left of it, so they cannot reach (m, n)).
+    // Residues of a pair as aligned dwords: columns j .. j + 3 are the funnel
+    // shift by (j - 1 + mis) % 4 of dwords (j - 1 + mis) / 4 and the next,
+    // both clamped to the dword of the pair's last residue (bytes past n are
+    // arbitrary: a column never feeds the columns left of it, so they cannot
+    // reach (m, n)).
     struct Residues {
         const uint32_t *words;
         uint32_t mis;
@@ -76,9 +77,11 @@ struct Band16 {
         const int TW = a.alpha + 1;
         const int W1 = a.bandUp - a.bandLo;          // W - 1
         const int jS = i0 + a.bandLo;                // the block's first band column (u = 0)
-        const int jA = jS <= 1 ? 1 : ((jS - 1) & ~(QD - 1)) + 1;   // sweep start, group aligned
+        // Sweep start: the first band column (LDS rings), else rounded down
+        // to a multiple of QD plus one (measured faster with global rings).
+        const int jA = jS <= 1 ? 1 : LR ? jS : ((jS - 1) & ~(QD - 1)) + 1;
         const int jE = min(i0 + R - 1 + a.bandUp, maxN);           // the block's last band column
-        const int jEnd = (jE + QD - 1) / QD * QD;
+        const int jEnd = LR ? jE : (jE + QD - 1) / QD * QD;   // the last group may run past it
 
         // Per row: DNA the row words of both pairs, else the two row codes;
         // G and E of the column left of the sweep: column 0's border when the
@@ -155,9 +158,17 @@ struct Band16 {
         for (int u = 0; u < QD; ++u)
             rq[u] = first ? make_uint2(asu(neg2), asu(neg2)) : rb[((jA + u) & cm) * 64];
         const Residues ra = residues(c.h[0], c.n[0]), rbs = residues(c.h[1], c.n[1]);
-        const int g0 = (jA - 1) / QD;
-        uint32_t a0 = ra.words[min(g0, ra.lastW)], a1 = ra.words[min(g0 + 1, ra.lastW)];
-        uint32_t b0 = rbs.words[min(g0, rbs.lastW)], b1 = rbs.words[min(g0 + 1, rbs.lastW)];
+        // Column jA is byte jA - 1 of a pair's residues: dword (jA - 1 + mis)
+        // / 4 of its aligned words, at byte (jA - 1 + mis) % 4.  With LDS rings
+        // (any start) that is a per-lane base wA and shift; otherwise jA - 1 is
+        // a multiple of QD and the dword index is the uniform group count gS
+        // (the per-lane base measured 6 % slower there).
+        const int offA = jA - 1 + (int)ra.mis, offB = jA - 1 + (int)rbs.mis;
+        const int wA = LR ? offA >> 2 : 0, wB = LR ? offB >> 2 : 0;
+        const int gS = LR ? 0 : (jA - 1) / QD;
+        const uint32_t shA = LR ? (uint32_t)offA & 3u : ra.mis, shB = LR ? (uint32_t)offB & 3u : rbs.mis;
+        uint32_t a0 = ra.words[min(wA + gS, ra.lastW)], a1 = ra.words[min(wA + gS + 1, ra.lastW)];
+        uint32_t b0 = rbs.words[min(wB + gS, rbs.lastW)], b1 = rbs.words[min(wB + gS + 1, rbs.lastW)];
 
         // The first block reads -inf for the row above; its row-buffer loads
         // still run (in bounds, the values unused), so that no branch splits
@@ -246,21 +257,21 @@ struct Band16 {
                 rq[u] = nx[u];
             }
         };
-        int s0 = jA, g = g0;
+        int s0 = jA, g = gS;
         // The group's residue bytes; the next group's dwords are loaded
         // before a group and shifted after it, so the loop head waits for
         // nothing (a wait there would also cover the last group's stores).
-        uint32_t curA = __builtin_amdgcn_alignbyte(a1, a0, ra.mis);
-        uint32_t curB = __builtin_amdgcn_alignbyte(b1, b0, rbs.mis);
+        uint32_t curA = __builtin_amdgcn_alignbyte(a1, a0, shA);
+        uint32_t curB = __builtin_amdgcn_alignbyte(b1, b0, shB);
         // Consecutive groups share a dword: one new dword per pair and group,
         // loaded two groups before its use.
-        uint32_t a2 = ra.words[min(g0 + 2, ra.lastW)], b2 = rbs.words[min(g0 + 2, rbs.lastW)];
+        uint32_t a2 = ra.words[min(wA + gS + 2, ra.lastW)], b2 = rbs.words[min(wB + gS + 2, rbs.lastW)];
         auto next = [&](auto edge) {
-            const uint32_t a3 = ra.words[min(g + 3, ra.lastW)];
-            const uint32_t b3 = rbs.words[min(g + 3, rbs.lastW)];
+            const uint32_t a3 = ra.words[min(wA + g + 3, ra.lastW)];
+            const uint32_t b3 = rbs.words[min(wB + g + 3, rbs.lastW)];
             group(edge, s0, curA, curB);
-            curA = __builtin_amdgcn_alignbyte(a2, a1, ra.mis);
-            curB = __builtin_amdgcn_alignbyte(b2, b1, rbs.mis);
+            curA = __builtin_amdgcn_alignbyte(a2, a1, shA);
+            curB = __builtin_amdgcn_alignbyte(b2, b1, shB);
             a1 = a2;
             b1 = b2;
             a2 = a3;
