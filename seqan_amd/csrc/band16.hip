@@ -367,7 +367,7 @@ struct Band16 {
 };
 
 template <int R, bool DNA, bool LR>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R >= 16 ? 3 : 4))) void kBand16(FwdArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R >= 12 ? 3 : 4))) void kBand16(FwdArgs a)
 {
     Band16<R, DNA, LR>::run(a);
 }
@@ -378,6 +378,11 @@ const void *bandKernel(int R, bool dna, bool lds)
         if (lds)
             return dna ? (const void *)&kBand16<16, true, true> : (const void *)&kBand16<16, false, true>;
         return dna ? (const void *)&kBand16<16, true, false> : (const void *)&kBand16<16, false, false>;
+    }
+    if (R == 12) {
+        if (lds)
+            return dna ? (const void *)&kBand16<12, true, true> : (const void *)&kBand16<12, false, true>;
+        return dna ? (const void *)&kBand16<12, true, false> : (const void *)&kBand16<12, false, false>;
     }
     if (R == 8) {
         if (lds)

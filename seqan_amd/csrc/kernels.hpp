@@ -179,7 +179,7 @@ hipError_t launchSimple32(const Simple32Args &args, hipStream_t stream);
 bool laneSupported(const Variant16 &v);
 int laneRowCols(int maxN, int R);
 // Packed banded global alignment (band16.hip): one pair couple per lane, R in
-// {8, 16} rows per block, bandUp - bandLo >= R; work items hold up to 128
+// {8, 12, 16} rows per block, bandUp - bandLo >= R; work items hold up to 128
 // pairs of one length m (WorkItem.count = pairs | m << 16, nblocks =
 // (m + R) / R).  Row buffer per wave slot: a ring of bandRingCols(...)
 // columns x 64 lanes x 8 bytes.
