@@ -192,7 +192,9 @@ sqmi_status sqmi_align_counts(sqmi_context *ctx, const sqmi_sequences *set_h,
  * SQMI_SCORES_ON_DEVICE.  With SQMI_ROWS_PACKED the rows are instead written
  * back to back in pair order (packed on the device): row i starts at
  * op_offsets[0] + op_lengths[0] + ... + op_lengths[i-1]; op_offsets still
- * bound the space as above.  An empty pair gets INT32_MIN, zero counts and no
+ * bound the space as above, and the bytes after the last row are
+ * unspecified (host buffers: left as they were).  An empty pair gets
+ * INT32_MIN, zero counts and no
  * rows.  Global modes as sqmi_align_counts (SQMI_LOCAL: sqmi_align_local_rows). */
 sqmi_status sqmi_align_rows(sqmi_context *ctx, const sqmi_sequences *set_h,
                             const sqmi_sequences *set_v, const sqmi_scoring *scoring,

@@ -2164,12 +2164,27 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
             (err = hipMemcpyAsync(counts, dCounts, (size_t)4 * S * N, hipMemcpyDeviceToHost, stream)) != hipSuccess)
             return hipFail(ctx, err, "copying results");
         if (wantRows &&
-            ((err = hipMemcpyAsync(ops + opOffsets[0], rowsOut + opOffsets[0], (size_t)(opOffsets[N] - opOffsets[0]),
-                                   hipMemcpyDeviceToHost, stream)) != hipSuccess ||
-             (err = hipMemcpyAsync(opLengths, rows.opLen, 8 * N, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+            ((err = hipMemcpyAsync(opLengths, rows.opLen, 8 * N, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
              (begins && (err = hipMemcpyAsync(begins, rows.begins, 16 * N, hipMemcpyDeviceToHost, stream)) !=
                             hipSuccess)))
             return hipFail(ctx, err, "copying rows");
+        if (wantRows) {
+            // Packed rows fill only the first sum(op_lengths) bytes: read
+            // that total (the scan's last entry) and copy no more.
+            int64_t opsBytes = opOffsets[N] - opOffsets[0];
+            if (flags & SQMI_ROWS_PACKED) {
+                int64_t total = 0;
+                if ((err = hipMemcpyAsync(&total, static_cast<const int64_t *>(ctx->b.opStart.p) + N, 8,
+                                          hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+                    (err = hipStreamSynchronize(stream)) != hipSuccess)
+                    return hipFail(ctx, err, "reading the packed rows' length");
+                opsBytes = std::min(opsBytes, std::max<int64_t>(total, 0));
+            }
+            if (opsBytes > 0 &&
+                (err = hipMemcpyAsync(ops + opOffsets[0], rowsOut + opOffsets[0], (size_t)opsBytes,
+                                      hipMemcpyDeviceToHost, stream)) != hipSuccess)
+                return hipFail(ctx, err, "copying rows");
+        }
     }
     if ((err = hipStreamSynchronize(stream)) != hipSuccess)
         return hipFail(ctx, err, "traceback kernels");
