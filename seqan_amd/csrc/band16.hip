@@ -143,10 +143,10 @@ struct Band16 {
         // The diagonal input of the first swept column, and column 0 of this
         // block's bottom row for the next block.
         // The row buffer is a ring of a.bandCols columns (a power of two of at
-        // least W - 1 + R + 10): the columns a block reads from the block
-        // above (jA' - 1 .. its last band column) lie within W + 9 of each
-        // other, and no column the block writes before reading it shares a
-        // slot with one still to be read.
+        // least W - 1 + R + 10): the columns a block uses, jA - 1 to the end
+        // of its last group, span at most W + R + 3 (first band column
+        // start) or W + R + 6 (group-aligned start) columns, so no column the
+        // block writes shares a slot with one it still has to read.
         const int cm = a.bandCols - 1;
         s16x2 upGprev = first ? neg2 : as16(rb[((jA - 1) & cm) * 64].x);
         if (!last && jA == 1)
