@@ -25,9 +25,10 @@
 // row k is forced when k >= u (E; u = j - i0 - lower) or k <= u - (W - 1)
 // (F), one bit-select per row with a wavefront-uniform mask (the masks
 // depend on k minus the column's place in its group only, so a group makes
-// them once); the columns between are plain sweeps.  The saturating adds keep the values outside the
-// band bounded, and the row buffer entries a block reads outside the columns
-// the block above wrote only feed cells outside the band.
+// them once); the columns between are plain sweeps.  The saturating adds
+// keep the values outside the band bounded, and the row buffer entries a
+// block reads outside the columns the block above wrote only feed cells
+// outside the band.
 #include "kernels.hpp"
 #include "packed16.hpp"
 #include "seqan_amd.h"
