@@ -461,6 +461,32 @@ def banded_leg(ctx, steps, warmup, cpu):
                    "scores_equal_banded_scores": bool(np.array_equal(rout[0][0], scores[0][:nr])),
                    "workload": f"first {nr:,} pairs of the narrow workload: scores, counts and rows "
                                "(host buffers, rows packed on the device)", "cells": rcells}
+    # The same call with residues and outputs resident in device memory.
+    A = score.alphabet_size
+    seg_bytes = int((Hr.lengths() + Vr.lengths()).sum())
+    bufs = [sa.DeviceBuffer.from_array(Hr.residues), sa.DeviceBuffer.from_array(Vr.residues),
+            sa.DeviceBuffer(4 * nr), sa.DeviceBuffer(4 * (A * A + 2) * nr), sa.DeviceBuffer(seg_bytes),
+            sa.DeviceBuffer(8 * nr)]
+    try:
+        out = tuple(b.address for b in bufs[2:])
+        dms = timed(lambda: ctx.align_rows_banded(Hr, Vr, score, lo, up, h_dev=bufs[0].address,
+                                                  v_dev=bufs[1].address, out_dev=out), steps, warmup,
+                    "banded rows resident")
+        dst = ctx.stats()
+        dev_scores = bufs[2].to_array(np.int32, nr)
+        dev_lens = bufs[5].to_array(np.int64, nr)
+        dev_ops = bufs[4].to_array(np.uint8, int(dev_lens.sum()))
+        res["rows"]["resident"] = {
+            "ms_per_step": round(dms, 3), "device_ms": round(dst["device_ms"], 3),
+            "host_plan_ms": round(dst["host_plan_ms"], 3),
+            "value": round(rcells / (dms / 1e3) / 1e9, 2),
+            "equal_to_host_buffers": bool(np.array_equal(dev_scores, rout[0][0]) and
+                                          np.array_equal(dev_lens, np.diff(rout[0][2])) and
+                                          np.array_equal(dev_ops, rout[0][3])),
+            "note": "residues, scores, counts and packed rows in device memory"}
+    finally:
+        for b in bufs:
+            b.free()
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as po

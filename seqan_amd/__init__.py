@@ -437,16 +437,32 @@ class Context:
         return (scores, counts, *_packed_rows(buf, lens), begins)
 
     def align_rows_banded(self, set_h: SeqSet, set_v: SeqSet, score: Score, lower_diag: int, upper_diag: int,
-                          mode=0, flags=0, rows=True):
+                          mode=0, flags=0, rows=True, h_dev=None, v_dev=None, out_dev=None):
         """Banded alignment (cells with lower_diag <= j - i <= upper_diag):
         (scores, counts, op_off, ops) as align_rows, plus begins as in
         align_local_rows with mode LOCAL; rows=False returns (scores,
-        counts) only."""
-        ch, keep_h = _seqs(set_h)
-        cv, keep_v = _seqs(set_v)
+        counts) only.  h_dev / v_dev: device addresses of the residues;
+        out_dev: device addresses (scores, counts, ops, op_lengths[, begins])
+        of the outputs, rows packed from ops (returns None: the results stay
+        on the device; ops must hold sum(|H| + |V|) bytes)."""
+        ch, keep_h = _seqs(set_h, h_dev)
+        cv, keep_v = _seqs(set_v, v_dev)
+        if h_dev is not None or v_dev is not None:
+            assert h_dev is not None and v_dev is not None
+            flags |= RESIDUES_ON_DEVICE
         sc = score._c()
         n = len(set_h.offsets) - 1
         A = score.alphabet_size
+        if out_dev is not None:
+            seg = np.zeros(n + 1, np.int64)
+            seg[1:] = np.cumsum(set_h.lengths() + set_v.lengths())
+            begins = out_dev[4] if len(out_dev) > 4 else None
+            st = self._L.sqmi_align_rows_banded(self._h, ctypes.byref(ch), ctypes.byref(cv), ctypes.byref(sc), mode,
+                                                lower_diag, upper_diag, out_dev[0], out_dev[1], out_dev[2],
+                                                seg.ctypes.data, out_dev[3], begins,
+                                                flags | SCORES_ON_DEVICE | ROWS_PACKED)
+            self._check(st, "sqmi_align_rows_banded")
+            return None
         scores = np.empty(n, np.int32)
         counts = np.empty((n, A * A + 2), np.int32)
         local = bool(mode & LOCAL)

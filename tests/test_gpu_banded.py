@@ -196,6 +196,34 @@ def test_banded_device_resident(ctx):
             b.free()
 
 
+def test_banded_rows_device_resident(ctx):
+    # Residues and outputs in device memory (rows packed from ops) give the
+    # host-buffer results, local windows included.
+    H, V = sa.workload(sa.WL_DNA150, 0x5EED0404, 0, 2000)
+    score = sa.simple_score(2, -3, -1, -5)
+    n, S = len(H), score.alphabet_size ** 2 + 2
+    for mode in (0, sa.LOCAL):
+        want = ctx.align_rows_banded(H, V, score, -12, 12, mode=mode)
+        bufs = [sa.DeviceBuffer.from_array(H.residues), sa.DeviceBuffer.from_array(V.residues),
+                sa.DeviceBuffer(4 * n), sa.DeviceBuffer(4 * S * n),
+                sa.DeviceBuffer(int((H.lengths() + V.lengths()).sum())), sa.DeviceBuffer(8 * n),
+                sa.DeviceBuffer(16 * n)]
+        try:
+            out = tuple(b.address for b in bufs[2:(7 if mode else 6)])
+            assert ctx.align_rows_banded(H, V, score, -12, 12, mode=mode, h_dev=bufs[0].address,
+                                         v_dev=bufs[1].address, out_dev=out) is None
+            lens = bufs[5].to_array(np.int64, n)
+            assert np.array_equal(bufs[2].to_array(np.int32, n), want[0])
+            assert np.array_equal(bufs[3].to_array(np.int32, n * S).reshape(n, S), want[1])
+            assert np.array_equal(lens, np.diff(want[2]))
+            assert np.array_equal(bufs[4].to_array(np.uint8, int(lens.sum())), want[3])
+            if mode:
+                assert np.array_equal(bufs[6].to_array(np.int64, 2 * n).reshape(n, 2), want[4])
+        finally:
+            for b in bufs:
+                b.free()
+
+
 def test_python_mirror_functions(ctx):
     # The SeqAn-named functions of the Python mirror reach the same entry points.
     hs = [[0, 1, 2, 3, 0, 1, 2, 3], [3, 3, 2, 1, 0, 0, 1]]
