@@ -892,6 +892,22 @@ struct Inputs
     uint32_t prof[16] = {};
 };
 
+// A host thread that is joined when it goes out of scope (early returns
+// included).
+struct JoinedThread {
+    std::thread t;
+    ~JoinedThread()
+    {
+        if (t.joinable())
+            t.join();
+    }
+    void join()
+    {
+        if (t.joinable())
+            t.join();
+    }
+};
+
 // Host residues into the context's buffers (in.dH / in.dV).
 hipError_t uploadResidues(sqmi_context *ctx, const sqmi_sequences *setH, const sqmi_sequences *setV, Inputs &in)
 {
@@ -1808,9 +1824,9 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
     // pageable copy holds its thread until the data is staged).  On 65,536
     // banded pairs of 150 bp: 3.9-4.2 ms per rows call against 4.4-4.7.
     hipError_t upErr = hipSuccess;
-    std::thread up;
+    JoinedThread up;
     if (!(flags & SQMI_RESIDUES_ON_DEVICE) && setH->offsets[N] + setV->offsets[N] >= ((int64_t)4 << 20))
-        up = std::thread([&] {
+        up.t = std::thread([&] {
             upErr = hipSetDevice(ctx->device);
             if (upErr == hipSuccess)
                 upErr = uploadResidues(ctx, setH, setV, in);
@@ -1819,8 +1835,7 @@ static sqmi_status alignBackward(sqmi_context *ctx, const sqmi_sequences *setH, 
     // Banded pairs run on the int32 kernel, band-masked.
     plan(*setH, *setV, *scoring, mode, band ? (flags | SQMI_FORCE_INT32) : flags, P, true, band != nullptr);
     ctx->stats.host_plan_ms = msSince(tPlan);
-    if (up.joinable())
-        up.join();
+    up.join();
     if (upErr != hipSuccess)
         return hipFail(ctx, upErr, "copying residues");
     fillStats(P, ctx->stats);
@@ -2297,6 +2312,18 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
         return fail(ctx, SQMI_ERR_DEVICE, "hipSetDevice failed");
     hipStream_t stream = ctx->stream;
 
+    // Large host residue sets are copied in on a second host thread while
+    // this one routes the pairs (as in the rows calls).
+    Inputs in;
+    hipError_t upErr = hipSuccess;
+    JoinedThread up;
+    if (!(flags & SQMI_RESIDUES_ON_DEVICE) && setH->offsets[N] + setV->offsets[N] >= ((int64_t)4 << 20))
+        up.t = std::thread([&] {
+            upErr = hipSetDevice(ctx->device);
+            if (upErr == hipSuccess)
+                upErr = uploadResidues(ctx, setH, setV, in);
+        });
+
     // Route: empty pairs (INT32_MIN), the packed int16 kernel (global
     // alignment, values inside the int16 window, at least 8 diagonals past
     // the first), narrow bands (<= 64 diagonals), wide bands.
@@ -2393,7 +2420,9 @@ sqmi_status sqmi_align_score_banded(sqmi_context *ctx, const sqmi_sequences *set
     ctx->stats.pairs_int16 = (int64_t)packed.size();
     ctx->stats.host_plan_ms = msSince(tPlan);
 
-    Inputs in;
+    up.join();
+    if (upErr != hipSuccess)
+        return hipFail(ctx, upErr, "copying residues");
     if ((st = uploadInputs(ctx, setH, setV, scoring, si, scores, flags, in)) != SQMI_OK)
         return st;
     hipError_t err;
